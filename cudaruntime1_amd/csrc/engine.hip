@@ -1,0 +1,609 @@
+// engine.hip -- k-NN pipelines behind the C ABI.
+//
+// REF mode (the reference pipeline, e.g. launch_dtw_batch_gpu_knn dtw.cu:
+// 99-234): envelope -> per-point filter score of every pair -> kc best
+// candidates per query -> exact refine -> best k of the kc by exact value.
+// The reference downloads the dense score matrix and selects on the host
+// (dtw.cu:156-176); here selection stays on the device.
+//
+// EXACT mode (no reference counterpart): admissible summary bounds seed each
+// query with its s best-bounded candidates, whose exact values give a
+// threshold tau (the k-th best); one more filter pass compacts every pair
+// whose bound can still reach tau, those are refined, and the top-k is taken
+// over them. The result equals brute force over the whole database.
+//
+// Batched (ts_knn) runs each stage once for a whole batch of queries, with
+// persistent refine kernels draining one work list. Plain (ts_knn_plain)
+// runs the same stages once per query.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ts_internal.h"
+
+namespace ts {
+
+__global__ void k_count_cells(const Item* items, const int32_t* nitems, int64_t max_items, SetView q, SetView db,
+                              int32_t measure, unsigned long long* acc) {
+  __shared__ unsigned long long part[4];
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  int64_t n = max_items;
+  if (nitems) n = min<int64_t>(n, (int64_t)*nitems);
+  unsigned long long c = 0;
+  if (i < n) {
+    const Item it = items[i];
+    if (it.d >= 0) {
+      const int64_t a = q.len[it.q], b = db.len[it.d];
+      c = measure == TS_EUCLIDEAN ? (unsigned long long)a : (unsigned long long)(a * b);
+    }
+  }
+  for (int sh = 32; sh > 0; sh >>= 1) c += __shfl_xor(c, sh);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(acc, part[0] + part[1] + part[2] + part[3]);
+}
+
+__global__ void k_fill_f32(float* p, float v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+enum BufId {
+  B_ENV, B_SCORES, B_KEYS, B_KEYS2, B_SCA, B_SCB, B_CAND, B_CANDIDX, B_ITEMS, B_ITEMS3, B_VALS, B_REGION,
+  B_RVALS, B_COUNTS, B_CTR, B_TAU, B_TOP, B_OIDX, B_OVAL, B_CELLS, B_HOSTIN, B_NBUF
+};
+
+struct Buf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+struct KStat {
+  std::string name;
+  int64_t launches = 0;
+  double ms = 0, bytes = 0, units = 0;
+};
+
+struct Pending {
+  int kid;
+  hipEvent_t a, b;
+  double bytes, units;
+  int cells_slot;  // >= 0: units are read from the cell counter slot
+};
+
+}  // namespace ts
+
+using namespace ts;
+
+struct ts_engine {
+  hipStream_t st = nullptr;
+  int device = 0;
+  bool timing = false;
+  int64_t launches = 0;
+  Buf bufs[B_NBUF];
+  std::vector<KStat> ks;
+  std::map<std::string, int> kidx;
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> evpool;
+  size_t ev_used = 0;
+  hipEvent_t call_a = nullptr, call_b = nullptr;
+  std::vector<unsigned long long> cells_host;
+  int cells_used = 0;
+  int64_t overflow_retries = 0;
+};
+
+namespace ts {
+namespace {
+
+constexpr int kCellSlots = 64;
+
+template <class T>
+ts_status ensure(ts_engine* e, BufId id, int64_t n, T** out) {
+  Buf& b = e->bufs[id];
+  size_t bytes = (size_t)std::max<int64_t>(n, 1) * sizeof(T);
+  if (bytes > b.cap) {
+    TS_HIP(hipStreamSynchronize(e->st));
+    if (b.p) TS_HIP(hipFree(b.p));
+    b.p = nullptr;
+    b.cap = 0;
+    size_t want = std::max(bytes, (size_t)4096);
+    hipError_t err = hipMalloc(&b.p, want);
+    if (err != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(TS_ERR_OOM, "device workspace allocation of " + std::to_string(want) + " bytes failed");
+    }
+    b.cap = want;
+  }
+  *out = (T*)b.p;
+  return TS_OK;
+}
+
+int kernel_id(ts_engine* e, const std::string& name) {
+  auto it = e->kidx.find(name);
+  if (it != e->kidx.end()) return it->second;
+  int id = (int)e->ks.size();
+  e->ks.push_back(KStat{name});
+  e->kidx[name] = id;
+  return id;
+}
+
+hipEvent_t next_event(ts_engine* e) {
+  if (e->ev_used == e->evpool.size()) {
+    hipEvent_t ev;
+    if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+    e->evpool.push_back(ev);
+  }
+  return e->evpool[e->ev_used++];
+}
+
+// Enqueue one launcher, optionally bracketed by timing events.
+template <class F>
+ts_status run(ts_engine* e, const char* name, double bytes, double units, F f, int cells_slot = -1,
+              int nlaunch = 1) {
+  hipEvent_t a = nullptr;
+  if (e->timing) {
+    a = next_event(e);
+    if (a) (void)hipEventRecord(a, e->st);
+  }
+  hipError_t err = f();
+  if (err != hipSuccess) return hip_fail(err, name, __FILE__, __LINE__);
+  e->launches += nlaunch;
+  if (e->timing && a) {
+    hipEvent_t b = next_event(e);
+    if (b) {
+      (void)hipEventRecord(b, e->st);
+      e->pending.push_back(Pending{kernel_id(e, name), a, b, bytes, units, cells_slot});
+    }
+  }
+  return TS_OK;
+}
+
+ts_status resolve_timing(ts_engine* e) {
+  if (!e->pending.empty()) {
+    TS_HIP(hipStreamSynchronize(e->st));
+    if (e->cells_used > 0) {
+      unsigned long long* dc = (unsigned long long*)e->bufs[B_CELLS].p;
+      e->cells_host.resize(e->cells_used);
+      TS_HIP(hipMemcpy(e->cells_host.data(), dc, e->cells_used * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    }
+    for (const Pending& p : e->pending) {
+      float ms = 0;
+      TS_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+      KStat& k = e->ks[p.kid];
+      k.launches += 1;
+      k.ms += ms;
+      k.bytes += p.bytes;
+      k.units += p.cells_slot >= 0 && p.cells_slot < (int)e->cells_host.size() ? (double)e->cells_host[p.cells_slot]
+                                                                               : p.units;
+    }
+  }
+  e->pending.clear();
+  e->ev_used = 0;
+  return TS_OK;
+}
+
+MeasureArgs margs(const ts_params& p) { return MeasureArgs{p.measure, p.band, p.delta, p.epsilon}; }
+
+bool desc_measure(int m) { return m == TS_LCSS; }
+
+ts_status check_params(const ts_params* p, const ts_set* db, const ts_set* qs) {
+  if (!p || !db || !qs) return fail(TS_ERR_INVALID, "null argument");
+  if (p->measure < TS_DTW || p->measure > TS_EUCLIDEAN) return fail(TS_ERR_INVALID, "unknown measure");
+  if (p->mode != TS_MODE_REF && p->mode != TS_MODE_EXACT) return fail(TS_ERR_INVALID, "unknown mode");
+  if (p->k < 1 || p->k > 1000) return fail(TS_ERR_INVALID, "k must be in [1, 1000]");
+  if (p->mode == TS_MODE_REF && (p->kc < p->k || p->kc > 1000))
+    return fail(TS_ERR_INVALID, "REF mode needs k <= kc <= 1000");
+  if (p->band < 0) return fail(TS_ERR_INVALID, "band must be >= 0");
+  if (p->band > 0 && p->measure != TS_DTW) return fail(TS_ERR_UNSUPPORTED, "band is a DTW option");
+  if (p->delta >= 0 && p->measure != TS_LCSS) return fail(TS_ERR_UNSUPPORTED, "delta is an LCSS option");
+  if (p->measure == TS_LCSS && !(p->epsilon > 0.0f)) return fail(TS_ERR_INVALID, "LCSS needs epsilon > 0");
+  if (p->mode == TS_MODE_REF && p->r < 0) return fail(TS_ERR_INVALID, "r must be >= 0");
+  if (db->device != qs->device) return fail(TS_ERR_INVALID, "sets live on different devices");
+  return TS_OK;
+}
+
+// Refine a device item list into vals[slot]. nitems_dev may be null (all
+// max_items valid positions; items with d < 0 are skipped).
+ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_set* qs, const Item* items,
+                 const int32_t* nitems_dev, int64_t max_items, float* vals) {
+  if (max_items == 0) return TS_OK;
+  const SetView qv = qs->view(), dv = db->view();
+  const MeasureArgs ma = margs(p);
+  const int cus = device_cu_count();
+  unsigned long long* cells = nullptr;
+  TS_TRY(ensure(e, B_CELLS, kCellSlots, &cells));
+  const int slot = e->cells_used < kCellSlots ? e->cells_used++ : kCellSlots - 1;
+  TS_TRY(run(e, "count_cells", 0, 0, [&] {
+    hipLaunchKernelGGL(k_count_cells, dim3((unsigned)((max_items + 255) / 256)), dim3(256), 0, e->st, items,
+                       nitems_dev, max_items, qv, dv, p.measure, cells + slot);
+    return hipGetLastError();
+  }));
+  int32_t* ctr = nullptr;
+  TS_TRY(ensure(e, B_CTR, 16, &ctr));
+  if (p.measure == TS_HAUSDORFF) {
+    TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, ctr, 0, 1); }));
+    return run(e, "hausdorff_refine", 0, 0, [&] {
+      return launch_hausdorff(e->st, qv, dv, items, nitems_dev, max_items, ctr, vals, cus * 8);
+    }, slot);
+  }
+  if (p.measure == TS_EUCLIDEAN) {
+    return run(e, "euclidean_refine", 0, 0, [&] {
+      return launch_euclidean(e->st, qv, dv, items, nitems_dev, max_items, vals);
+    }, slot);
+  }
+  // Wavefront DP: split by lanes-per-pair class, one persistent launch each.
+  const int32_t lo = std::min(qs->min_len, db->min_len), hi = std::min(qs->max_len, db->max_len);
+  const int cls_lo = dp_class_for_min_len(hi), cls_hi = dp_class_for_min_len(lo);
+  Item* items3 = nullptr;
+  if (cls_lo == cls_hi) {
+    // Every pair has the same class: no split needed.
+    TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, ctr, 0, 1); }));
+    const int32_t cmax = std::max(dp_lanes_for_class(cls_lo), hi);
+    const char* nm = p.measure == TS_DTW ? "dtw_wavefront" : (p.measure == TS_FRECHET ? "frechet_wavefront" : "lcss_wavefront");
+    return run(e, nm, 0, 0, [&] {
+      return launch_dp(e->st, ma, cls_lo, qv, dv, items, nitems_dev, max_items, ctr, vals, cmax, cus * 8);
+    }, slot);
+  }
+  TS_TRY(ensure(e, B_ITEMS3, 3 * max_items, &items3));
+  TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, ctr, 0, 8); }));
+  TS_TRY(run(e, "items_by_class", 0, 0, [&] {
+    return launch_items_by_class(e->st, items, nitems_dev, max_items, qv, dv, items3, max_items, ctr + 4);
+  }));
+  for (int c = cls_lo; c <= cls_hi; ++c) {
+    const int W = dp_lanes_for_class(c);
+    const int32_t cmax = std::max(W, c == 0 ? hi : (c == 1 ? std::min(hi, 63) : std::min(hi, 31)));
+    const char* nm = p.measure == TS_DTW ? "dtw_wavefront" : (p.measure == TS_FRECHET ? "frechet_wavefront" : "lcss_wavefront");
+    TS_TRY(run(e, nm, 0, 0, [&] {
+      return launch_dp(e->st, ma, c, qv, dv, items3 + (int64_t)c * max_items, ctr + 4 + c, max_items, ctr + c,
+                       vals, cmax, cus * 8);
+    }, slot));
+  }
+  return TS_OK;
+}
+
+ts_status fill_nan(ts_engine* e, float* p, int64_t n) {
+  if (n == 0) return TS_OK;
+  return run(e, "fill", 0, 0, [&] {
+    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->st, p,
+                       __builtin_nanf(""), n);
+    return hipGetLastError();
+  });
+}
+
+ts_status topk(ts_engine* e, const uint64_t* in, int64_t stride, int32_t len, const int32_t* counts, int32_t nseg,
+               int32_t k, uint64_t* out) {
+  int64_t need = select_scratch_elems(len, nseg, k);
+  uint64_t *a = nullptr, *b = nullptr;
+  TS_TRY(ensure(e, B_SCA, need, &a));
+  TS_TRY(ensure(e, B_SCB, need, &b));
+  int nl = 0;
+  return run(e, "topk_select", (double)nseg * len * 8.0, (double)nseg * len, [&] {
+    return select_topk(e->st, in, stride, len, counts, nseg, k, out, a, b, need, &nl);
+  });
+}
+
+// REF pipeline for queries [q0, q0+nq); results into device top keys [nq][k].
+ts_status knn_ref_batch(ts_engine* e, const ts_params& p, const ts_set* db, const ts_set* qs, int32_t q0, int32_t nq,
+                        const float4* env, uint64_t* top) {
+  const int32_t N = db->count;
+  const int32_t kc = p.kc;
+  const bool desc = desc_measure(p.measure);
+  float* scores = nullptr;
+  uint64_t *keys = nullptr, *cand = nullptr, *keys2 = nullptr;
+  int32_t* cand_idx = nullptr;
+  Item* items = nullptr;
+  float* vals = nullptr;
+  TS_TRY(ensure(e, B_SCORES, (int64_t)nq * N, &scores));
+  TS_TRY(ensure(e, B_KEYS, (int64_t)nq * N, &keys));
+  TS_TRY(ensure(e, B_CAND, (int64_t)nq * kc, &cand));
+  TS_TRY(ensure(e, B_CANDIDX, (int64_t)nq * kc, &cand_idx));
+  TS_TRY(ensure(e, B_ITEMS, (int64_t)nq * kc, &items));
+  TS_TRY(ensure(e, B_VALS, (int64_t)nq * kc, &vals));
+  TS_TRY(ensure(e, B_KEYS2, (int64_t)nq * kc, &keys2));
+  const SetView qv = qs->view(), dv = db->view();
+  const double dbpts = (double)db->points;
+  TS_TRY(run(e, "filter_ref", nq * (dbpts * 8.0), nq * dbpts, [&] {
+    return launch_filter_ref(e->st, p.measure, qv, q0, nq, env, dv, p.epsilon, scores);
+  }));
+  TS_TRY(run(e, "keys", 0, 0, [&] { return launch_keys_from_rows(e->st, scores, nq, N, desc, keys); }));
+  TS_TRY(topk(e, keys, N, N, nullptr, nq, kc, cand));
+  TS_TRY(run(e, "items", 0, 0, [&] { return launch_items_from_keys(e->st, cand, nq, kc, q0, items, nullptr); }));
+  TS_TRY(run(e, "decode", 0, 0, [&] { return launch_decode_keys(e->st, cand, (int64_t)nq * kc, desc, cand_idx, nullptr); }));
+  TS_TRY(fill_nan(e, vals, (int64_t)nq * kc));
+  TS_TRY(refine(e, p, db, qs, items, nullptr, (int64_t)nq * kc, vals));
+  TS_TRY(run(e, "keys", 0, 0, [&] {
+    return launch_keys_from_regions(e->st, vals, cand_idx, nullptr, nq, kc, desc, keys2);
+  }));
+  return topk(e, keys2, kc, kc, nullptr, nq, p.k, top);
+}
+
+// EXACT pipeline for queries [q0, q0+nq).
+ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, const ts_set* qs, int32_t q0,
+                          int32_t nq, int32_t cap, uint64_t* top, bool* overflowed) {
+  const int32_t N = db->count;
+  const int32_t k = p.k;
+  const bool desc = desc_measure(p.measure);
+  const int32_t s = std::min<int32_t>(std::max<int32_t>(2 * k, 16), 1024);
+  const int32_t nch = (N + 2047) / 2048;
+  const SetView qv = qs->view(), dv = db->view();
+  const MeasureArgs ma = margs(p);
+  uint64_t *bkeys = nullptr, *seeds = nullptr, *skeys = nullptr, *stop = nullptr, *rkeys = nullptr;
+  int32_t *seed_idx = nullptr, *region = nullptr, *counts = nullptr;
+  Item* items = nullptr;
+  float *vals = nullptr, *tau = nullptr, *rvals = nullptr;
+  TS_TRY(ensure(e, B_KEYS, (int64_t)nq * nch * s, &bkeys));
+  TS_TRY(ensure(e, B_CAND, (int64_t)nq * s, &seeds));
+  TS_TRY(ensure(e, B_CANDIDX, (int64_t)nq * s, &seed_idx));
+  TS_TRY(ensure(e, B_KEYS2, (int64_t)nq * s, &skeys));
+  TS_TRY(ensure(e, B_TOP, (int64_t)nq * k, &stop));
+  TS_TRY(ensure(e, B_TAU, nq, &tau));
+  TS_TRY(ensure(e, B_VALS, (int64_t)nq * s, &vals));
+  TS_TRY(ensure(e, B_ITEMS, std::max<int64_t>((int64_t)nq * s, (int64_t)nq * cap), &items));
+  TS_TRY(ensure(e, B_REGION, (int64_t)nq * cap, &region));
+  TS_TRY(ensure(e, B_RVALS, (int64_t)nq * cap, &rvals));
+  TS_TRY(ensure(e, B_COUNTS, nq + 2 + nq + 1, &counts));
+  TS_TRY(ensure(e, B_SCORES, (int64_t)nq * cap * 2, (float**)&rkeys));
+  const double pairs = (double)nq * N;
+  // 1. seeds: s best bounds per query
+  TS_TRY(run(e, "bound_seed", pairs * 36.0, pairs, [&] {
+    return launch_bound_chunks(e->st, ma, k, qv, q0, nq, dv, s, 2048, bkeys);
+  }));
+  TS_TRY(topk(e, bkeys, (int64_t)nch * s, nch * s, nullptr, nq, s, seeds));
+  TS_TRY(run(e, "items", 0, 0, [&] { return launch_items_from_keys(e->st, seeds, nq, s, q0, items, nullptr); }));
+  TS_TRY(run(e, "decode", 0, 0, [&] { return launch_decode_keys(e->st, seeds, (int64_t)nq * s, desc, seed_idx, nullptr); }));
+  TS_TRY(fill_nan(e, vals, (int64_t)nq * s));
+  TS_TRY(refine(e, p, db, qs, items, nullptr, (int64_t)nq * s, vals));
+  // 2. threshold: k-th best exact value among the seeds
+  TS_TRY(run(e, "keys", 0, 0, [&] { return launch_keys_from_regions(e->st, vals, seed_idx, nullptr, nq, s, desc, skeys); }));
+  TS_TRY(topk(e, skeys, s, s, nullptr, nq, k, stop));
+  TS_TRY(run(e, "threshold", 0, 0, [&] { return launch_kth_threshold(e->st, stop, nq, k, k, desc, tau); }));
+  // 3. compaction of every pair that can still reach tau
+  int32_t* overflow = counts + nq;
+  int32_t* nitems = counts + nq + 2;  // [0] total, [1..nq] offsets
+  TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, counts, 0, nq + 2); }));
+  TS_TRY(run(e, "bound_compact", pairs * 36.0, pairs, [&] {
+    return launch_bound_compact(e->st, ma, k, qv, q0, nq, dv, tau, cap, region, counts, overflow);
+  }));
+  TS_TRY(run(e, "items", 0, 0, [&] { return launch_items_from_regions(e->st, region, counts, nq, cap, q0, items, nitems); }));
+  TS_TRY(fill_nan(e, rvals, (int64_t)nq * cap));
+  TS_TRY(refine(e, p, db, qs, items, nitems, (int64_t)nq * cap, rvals));
+  // 4. top-k over the refined candidates
+  TS_TRY(run(e, "keys", 0, 0, [&] { return launch_keys_from_regions(e->st, rvals, region, counts, nq, cap, desc, rkeys); }));
+  TS_TRY(topk(e, rkeys, cap, cap, counts, nq, k, top));
+  int32_t ov = 0;
+  TS_HIP(hipMemcpyAsync(&ov, overflow, sizeof(int32_t), hipMemcpyDeviceToHost, e->st));
+  TS_HIP(hipStreamSynchronize(e->st));
+  *overflowed = ov != 0;
+  return TS_OK;
+}
+
+ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_params* pp, int32_t* out_idx,
+                   float* out_val, ts_stats* stats, bool plain) {
+  auto t0 = std::chrono::steady_clock::now();
+  if (!e) return fail(TS_ERR_INVALID, "null engine");
+  TS_TRY(check_params(pp, db, qs));
+  if (!out_idx || !out_val) return fail(TS_ERR_INVALID, "null output");
+  const ts_params p = *pp;
+  const int32_t Q = qs->count, N = db->count, k = p.k;
+  const bool desc = desc_measure(p.measure);
+  const int64_t l0 = e->launches;
+  e->cells_used = 0;
+  float4* env = nullptr;
+  if (p.mode == TS_MODE_REF && (p.measure == TS_DTW || p.measure == TS_FRECHET || p.measure == TS_LCSS)) {
+    TS_TRY(ensure(e, B_ENV, qs->points, &env));
+  }
+  int32_t* d_idx = nullptr;
+  float* d_val = nullptr;
+  uint64_t* top = nullptr;
+  TS_TRY(ensure(e, B_OIDX, (int64_t)Q * k, &d_idx));
+  TS_TRY(ensure(e, B_OVAL, (int64_t)Q * k, &d_val));
+  TS_TRY(ensure(e, B_HOSTIN, (int64_t)Q * k, &top));
+  unsigned long long* cells = nullptr;
+  TS_TRY(ensure(e, B_CELLS, kCellSlots, &cells));
+  TS_TRY(run(e, "fill", 0, 0, [&] {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, e->st, cells, kCellSlots);
+    return hipGetLastError();
+  }));
+  if (!e->call_a) {
+    TS_HIP(hipEventCreate(&e->call_a));
+    TS_HIP(hipEventCreate(&e->call_b));
+  }
+  TS_HIP(hipEventRecord(e->call_a, e->st));
+  if (env) {
+    TS_TRY(run(e, "envelope", 0, 0, [&] { return launch_envelope(e->st, qs->view(), qs->max_len, p.r, env); }));
+  }
+  // Queries per batch: bounded by the dense REF score matrix or by the
+  // EXACT candidate regions.
+  int32_t bq;
+  if (plain) {
+    bq = 1;
+  } else if (p.mode == TS_MODE_REF) {
+    bq = (int32_t)std::max<int64_t>(1, std::min<int64_t>(Q, (int64_t)(256ll << 20) / (8ll * std::max(1, N))));
+  } else {
+    int64_t nch = (N + 2047) / 2048;
+    int64_t s = std::min<int64_t>(std::max<int64_t>(2 * k, 16), 1024);
+    bq = (int32_t)std::max<int64_t>(1, std::min<int64_t>({(int64_t)Q, 4096, (int64_t)(512ll << 20) / (8 * nch * s)}));
+  }
+  int32_t cap = 4096;
+  for (int32_t q0 = 0; q0 < Q; q0 += bq) {
+    const int32_t nq = std::min(bq, Q - q0);
+    uint64_t* topq = top + (int64_t)q0 * k;
+    if (p.mode == TS_MODE_REF) {
+      TS_TRY(knn_ref_batch(e, p, db, qs, q0, nq, env, topq));
+    } else {
+      for (;;) {
+        bool ov = false;
+        TS_TRY(knn_exact_batch(e, p, db, qs, q0, nq, std::min(cap, N), topq, &ov));
+        if (!ov || cap >= N) break;
+        cap = (int32_t)std::min<int64_t>((int64_t)cap * 4, N);
+        e->overflow_retries++;
+      }
+    }
+  }
+  TS_TRY(run(e, "decode", 0, 0, [&] { return launch_decode_keys(e->st, top, (int64_t)Q * k, desc, d_idx, d_val); }));
+  TS_HIP(hipEventRecord(e->call_b, e->st));
+  TS_HIP(hipMemcpyAsync(out_idx, d_idx, sizeof(int32_t) * Q * k, hipMemcpyDeviceToHost, e->st));
+  TS_HIP(hipMemcpyAsync(out_val, d_val, sizeof(float) * Q * k, hipMemcpyDeviceToHost, e->st));
+  TS_HIP(hipStreamSynchronize(e->st));
+  if (stats) {
+    std::vector<unsigned long long> hc(kCellSlots);
+    TS_HIP(hipMemcpy(hc.data(), cells, sizeof(unsigned long long) * kCellSlots, hipMemcpyDeviceToHost));
+    float dms = 0;
+    TS_HIP(hipEventElapsedTime(&dms, e->call_a, e->call_b));
+    stats->ms_device = dms;
+    stats->pairs = (int64_t)Q * N;
+    stats->cells = 0;
+    for (int i = 0; i < e->cells_used && i < kCellSlots; ++i) stats->cells += (int64_t)hc[i];
+    stats->refined_pairs = -1;
+    stats->launches = e->launches - l0;
+    stats->overflow_retries = e->overflow_retries;
+  }
+  TS_TRY(resolve_timing(e));
+  if (stats) stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return TS_OK;
+}
+
+}  // namespace
+}  // namespace ts
+
+extern "C" {
+
+ts_status ts_engine_create(ts_engine** out) {
+  if (!out) return fail(TS_ERR_INVALID, "null out");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(TS_ERR_NO_DEVICE, "no HIP device available");
+  auto e = new ts_engine();
+  (void)hipGetDevice(&e->device);
+  hipError_t err = hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking);
+  if (err != hipSuccess) {
+    delete e;
+    return hip_fail(err, "hipStreamCreate", __FILE__, __LINE__);
+  }
+  *out = e;
+  return TS_OK;
+}
+
+ts_status ts_engine_destroy(ts_engine* e) {
+  if (!e) return TS_OK;
+  (void)hipStreamSynchronize(e->st);
+  for (auto& b : e->bufs)
+    if (b.p) (void)hipFree(b.p);
+  for (auto ev : e->evpool) (void)hipEventDestroy(ev);
+  if (e->call_a) (void)hipEventDestroy(e->call_a);
+  if (e->call_b) (void)hipEventDestroy(e->call_b);
+  (void)hipStreamDestroy(e->st);
+  delete e;
+  return TS_OK;
+}
+
+ts_status ts_engine_set_timing(ts_engine* e, int32_t on) {
+  if (!e) return fail(TS_ERR_INVALID, "null engine");
+  e->timing = on != 0;
+  return TS_OK;
+}
+
+ts_status ts_engine_kernel_stats(ts_engine* e, int32_t idx, char* name, int32_t namelen, int64_t* launches,
+                                 double* ms, double* bytes, double* units) {
+  if (!e || idx < 0 || idx >= (int32_t)e->ks.size()) return TS_ERR_INVALID;
+  const KStat& k = e->ks[idx];
+  if (name && namelen > 0) std::snprintf(name, namelen, "%s", k.name.c_str());
+  if (launches) *launches = k.launches;
+  if (ms) *ms = k.ms;
+  if (bytes) *bytes = k.bytes;
+  if (units) *units = k.units;
+  return TS_OK;
+}
+
+ts_status ts_engine_reset_stats(ts_engine* e) {
+  if (!e) return fail(TS_ERR_INVALID, "null engine");
+  for (auto& k : e->ks) k = KStat{k.name};
+  return TS_OK;
+}
+
+ts_status ts_knn(ts_engine* e, const ts_set* db, const ts_set* queries, const ts_params* p, int32_t* out_idx,
+                 float* out_val, ts_stats* stats) {
+  return knn_impl(e, db, queries, p, out_idx, out_val, stats, false);
+}
+
+ts_status ts_knn_plain(ts_engine* e, const ts_set* db, const ts_set* queries, const ts_params* p, int32_t* out_idx,
+                       float* out_val, ts_stats* stats) {
+  return knn_impl(e, db, queries, p, out_idx, out_val, stats, true);
+}
+
+ts_status ts_filter_ref(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_params* p, float* scores) {
+  if (!e || !scores) return fail(TS_ERR_INVALID, "null argument");
+  TS_TRY(check_params(p, db, qs));
+  float4* env = nullptr;
+  float* d = nullptr;
+  TS_TRY(ensure(e, B_ENV, qs->points, &env));
+  TS_TRY(ensure(e, B_SCORES, (int64_t)qs->count * db->count, &d));
+  if (p->measure == TS_DTW || p->measure == TS_FRECHET || p->measure == TS_LCSS)
+    TS_TRY(run(e, "envelope", 0, 0, [&] { return launch_envelope(e->st, qs->view(), qs->max_len, p->r, env); }));
+  TS_TRY(run(e, "filter_ref", 0, 0, [&] {
+    return launch_filter_ref(e->st, p->measure, qs->view(), 0, qs->count, env, db->view(), p->epsilon, d);
+  }));
+  TS_HIP(hipMemcpyAsync(scores, d, sizeof(float) * qs->count * db->count, hipMemcpyDeviceToHost, e->st));
+  TS_HIP(hipStreamSynchronize(e->st));
+  return resolve_timing(e);
+}
+
+ts_status ts_pair_values(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_params* p, const int32_t* qidx,
+                         const int32_t* didx, int64_t npairs, float* out) {
+  if (!e || !qidx || !didx || !out || npairs < 0) return fail(TS_ERR_INVALID, "null argument");
+  TS_TRY(check_params(p, db, qs));
+  if (npairs == 0) return TS_OK;
+  std::vector<Item> h(npairs);
+  for (int64_t i = 0; i < npairs; ++i) {
+    if (qidx[i] < 0 || qidx[i] >= qs->count || didx[i] < 0 || didx[i] >= db->count)
+      return fail(TS_ERR_INVALID, "pair index out of range");
+    h[i] = Item{qidx[i], didx[i], (int32_t)i, 0};
+  }
+  Item* items = nullptr;
+  float* vals = nullptr;
+  TS_TRY(ensure(e, B_ITEMS, npairs, &items));
+  TS_TRY(ensure(e, B_VALS, npairs, &vals));
+  TS_HIP(hipMemcpyAsync(items, h.data(), sizeof(Item) * npairs, hipMemcpyHostToDevice, e->st));
+  e->cells_used = 0;
+  unsigned long long* cells = nullptr;
+  TS_TRY(ensure(e, B_CELLS, kCellSlots, &cells));
+  TS_TRY(refine(e, *p, db, qs, items, nullptr, npairs, vals));
+  TS_HIP(hipMemcpyAsync(out, vals, sizeof(float) * npairs, hipMemcpyDeviceToHost, e->st));
+  TS_HIP(hipStreamSynchronize(e->st));
+  return resolve_timing(e);
+}
+
+ts_status ts_topk_rows(ts_engine* e, const float* scores, int32_t rows, int32_t cols, int32_t k, int32_t descending,
+                       int32_t* out_idx, float* out_val) {
+  if (!e || !scores || !out_idx || !out_val || rows < 0 || cols < 1 || k < 1 || k > 1000)
+    return fail(TS_ERR_INVALID, "ts_topk_rows arguments");
+  if (rows == 0) return TS_OK;
+  float* d = nullptr;
+  uint64_t *keys = nullptr, *top = nullptr;
+  int32_t* di = nullptr;
+  float* dv = nullptr;
+  TS_TRY(ensure(e, B_SCORES, (int64_t)rows * cols, &d));
+  TS_TRY(ensure(e, B_KEYS, (int64_t)rows * cols, &keys));
+  TS_TRY(ensure(e, B_TOP, (int64_t)rows * k, &top));
+  TS_TRY(ensure(e, B_OIDX, (int64_t)rows * k, &di));
+  TS_TRY(ensure(e, B_OVAL, (int64_t)rows * k, &dv));
+  TS_HIP(hipMemcpyAsync(d, scores, sizeof(float) * rows * cols, hipMemcpyHostToDevice, e->st));
+  TS_TRY(run(e, "keys", 0, 0, [&] { return launch_keys_from_rows(e->st, d, rows, cols, descending, keys); }));
+  TS_TRY(topk(e, keys, cols, cols, nullptr, rows, k, top));
+  TS_TRY(run(e, "decode", 0, 0, [&] { return launch_decode_keys(e->st, top, (int64_t)rows * k, descending, di, dv); }));
+  TS_HIP(hipMemcpyAsync(out_idx, di, sizeof(int32_t) * rows * k, hipMemcpyDeviceToHost, e->st));
+  TS_HIP(hipMemcpyAsync(out_val, dv, sizeof(float) * rows * k, hipMemcpyDeviceToHost, e->st));
+  TS_HIP(hipStreamSynchronize(e->st));
+  return resolve_timing(e);
+}
+
+}  // extern "C"

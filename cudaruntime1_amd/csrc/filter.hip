@@ -1,0 +1,258 @@
+// filter.hip -- candidate filters.
+//
+// REF mode reproduces the reference's per-point envelope filters bit for bit:
+// one lane per (query, DB trajectory) pair folding over the DB points in
+// order (dtw.cpp:45-58 sum, frechet.cpp:46-62 max, lcss.cpp:46-63 count), so
+// the f32 rounding sequence is the reference's. The index map v*n/m is
+// advanced incrementally (quotient + remainder) instead of dividing per point.
+//
+// EXACT mode uses admissible summary bounds from 36 B per DB trajectory (MBR,
+// endpoints, length; the query's summary is uniform per block) and either
+// keeps the best s keys per DB chunk (seeding) or compacts every trajectory
+// that can still reach the query's threshold with a wave ballot and one
+// atomic per wave.
+#include "ts_internal.h"
+
+namespace ts {
+
+__device__ __forceinline__ uint32_t ord_bits(float v) {
+  uint32_t b = __float_as_uint(v + 0.0f);  // -0 -> +0, so equal values compare equal
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ uint64_t make_key(float v, int32_t idx, bool desc) {
+  if (!isfinite(v)) return ~0ull;
+  uint32_t o = ord_bits(v);
+  if (desc) o = ~o;
+  return ((uint64_t)o << 32) | (uint32_t)idx;
+}
+
+// Envelope of every query point over [i-r, i+r] (dtw.cpp:20-34).
+__global__ void __launch_bounds__(256) k_envelope(SetView q, int32_t r, float4* env) {
+  const int t = blockIdx.y;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int n = q.len[t];
+  if (i >= n) return;
+  const int64_t o = q.off[t];
+  float mnx = 1e20f, mxx = -1e20f, mny = 1e20f, mxy = -1e20f;
+  const int a = max(0, i - r), b = min(n - 1, i + r);
+  for (int k = a; k <= b; ++k) {
+    float px = q.x[o + k], py = q.y[o + k];
+    mnx = fminf(mnx, px); mxx = fmaxf(mxx, px);
+    mny = fminf(mny, py); mxy = fmaxf(mxy, py);
+  }
+  env[o + i] = make_float4(mnx, mxx, mny, mxy);
+}
+
+hipError_t launch_envelope(hipStream_t st, SetView q, int32_t max_len, int32_t r, float4* env) {
+  hipLaunchKernelGGL(k_envelope, dim3((max_len + 255) / 256, q.count), dim3(256), 0, st, q, r, env);
+  return hipGetLastError();
+}
+
+// REF filter, one lane per (query, DB trajectory).
+template <int MEAS>
+__global__ void __launch_bounds__(256) k_filter_ref(SetView q, int32_t q0, const float4* env, SetView db,
+                                                    float eps, float* scores) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= db.count) return;
+  const int qi = q0 + blockIdx.y;
+  const int n = q.len[qi];
+  const int m = db.len[j];
+  const int64_t co = db.off[j];
+  float out;
+  if (MEAS == TS_HAUSDORFF || MEAS == TS_EUCLIDEAN) {
+    // Squared MBR gap: the R-tree's comparable box distance
+    // (hausdorff.cpp:162, euclidean.cpp:93-96).
+    if (MEAS == TS_EUCLIDEAN && m != n) {
+      out = __int_as_float(0x7f800000);  // not comparable: never a candidate
+    } else {
+      float4 a = q.mbr[qi], b = db.mbr[j];
+      float gx = fmaxf(0.0f, fmaxf(b.x - a.y, a.x - b.y));
+      float gy = fmaxf(0.0f, fmaxf(b.z - a.w, a.z - b.w));
+      out = gx * gx + gy * gy;
+    }
+  } else {
+    const float4* e = env + q.off[qi];
+    const int qd = n / m, rm = n - qd * m;
+    int idx = 0, acc = 0;
+    float s = 0.0f;
+    int cnt = 0;
+    for (int v = 0; v < m; ++v) {
+      const float4 E = e[idx];
+      const float px = db.x[co + v], py = db.y[co + v];
+      if (MEAS == TS_LCSS) {
+        if (px > E.x - eps && px < E.y + eps && py > E.z - eps && py < E.w + eps) ++cnt;
+      } else {
+        float dx = fmaxf(0.0f, fmaxf(E.x - px, px - E.y));
+        float dy = fmaxf(0.0f, fmaxf(E.z - py, py - E.w));
+        float d = sqrtf(dx * dx + dy * dy);
+        if (MEAS == TS_DTW) s += d;
+        else s = fmaxf(s, d);
+      }
+      idx += qd;
+      acc += rm;
+      if (acc >= m) { acc -= m; ++idx; }
+    }
+    out = (MEAS == TS_LCSS) ? (float)cnt / (float)min(n, m) : s;
+  }
+  scores[(int64_t)blockIdx.y * db.count + j] = out;
+}
+
+hipError_t launch_filter_ref(hipStream_t st, int32_t measure, SetView q, int32_t q0, int32_t nq,
+                             const float4* env, SetView db, float eps, float* scores) {
+  dim3 grid((db.count + 255) / 256, nq);
+  switch (measure) {
+    case TS_DTW: hipLaunchKernelGGL(k_filter_ref<TS_DTW>, grid, dim3(256), 0, st, q, q0, env, db, eps, scores); break;
+    case TS_FRECHET: hipLaunchKernelGGL(k_filter_ref<TS_FRECHET>, grid, dim3(256), 0, st, q, q0, env, db, eps, scores); break;
+    case TS_LCSS: hipLaunchKernelGGL(k_filter_ref<TS_LCSS>, grid, dim3(256), 0, st, q, q0, env, db, eps, scores); break;
+    case TS_HAUSDORFF: hipLaunchKernelGGL(k_filter_ref<TS_HAUSDORFF>, grid, dim3(256), 0, st, q, q0, env, db, eps, scores); break;
+    case TS_EUCLIDEAN: hipLaunchKernelGGL(k_filter_ref<TS_EUCLIDEAN>, grid, dim3(256), 0, st, q, q0, env, db, eps, scores); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------- EXACT bounds
+// Admissible bound of one pair from summaries (see oracle.cpp summary_bound
+// for the derivation; the f32 DTW sum is discounted by (L+8)*2^-23).
+// Distances: lower bound; LCSS: upper bound of the similarity.
+// Returns +inf (distances) / -inf (LCSS) when the pair is not comparable.
+struct QSum {
+  float4 mbr, ends;
+  int n;
+  int64_t off;
+};
+
+__device__ __forceinline__ float pair_bound(const MeasureArgs& ma, const QSum& qs, const SetView& q,
+                                            const float4 cm, const float4 ce, int m) {
+  const float4 a = qs.mbr;
+  float gx = fmaxf(0.0f, fmaxf(cm.x - a.y, a.x - cm.y));
+  float gy = fmaxf(0.0f, fmaxf(cm.z - a.w, a.z - cm.w));
+  float g2 = gx * gx + gy * gy;
+  const int n = qs.n;
+  switch (ma.measure) {
+    case TS_DTW: {
+      float dx = qs.ends.x - ce.x, dy = qs.ends.y - ce.y;
+      float e = sqrtf(dx * dx + dy * dy);
+      int L = max(n, m);
+      if (L >= 2) {
+        float ex = qs.ends.z - ce.z, ey = qs.ends.w - ce.w;
+        e = e + sqrtf(ex * ex + ey * ey);
+      }
+      float s = e + (float)max(0, L - 2) * sqrtf(g2);
+      return s * (1.0f - (float)(L + 8) * 1.1920929e-7f);
+    }
+    case TS_FRECHET: {
+      float dx = qs.ends.x - ce.x, dy = qs.ends.y - ce.y;
+      float ex = qs.ends.z - ce.z, ey = qs.ends.w - ce.w;
+      return fmaxf(sqrtf(g2), fmaxf(sqrtf(dx * dx + dy * dy), sqrtf(ex * ex + ey * ey)));
+    }
+    case TS_HAUSDORFF: return sqrtf(g2);
+    case TS_EUCLIDEAN:
+      if (m != n) return __int_as_float(0x7f800000);
+      return (float)(sqrt((double)n * (double)g2) * (1.0 - 1e-6));
+    case TS_LCSS: {
+      // Points of q that can match any point of c lie inside c's
+      // eps-inflated MBR (f32 subtraction is monotone).
+      int cnt = 0;
+      const float eps = ma.eps;
+      for (int i = 0; i < n; ++i) {
+        float px = q.x[qs.off + i], py = q.y[qs.off + i];
+        cnt += (px - cm.y < eps && cm.x - px < eps && py - cm.w < eps && cm.z - py < eps) ? 1 : 0;
+      }
+      int mn = min(n, m);
+      return (float)min(cnt, mn) / (float)mn;
+    }
+  }
+  return 0.0f;
+}
+
+constexpr int kBoundChunk = 2048;
+
+__device__ void bitonic_sort_lds(uint64_t* s, int n, int tid, int nthreads) {
+  for (int k = 2; k <= n; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = tid; i < n; i += nthreads) {
+        int ixj = i ^ j;
+        if (ixj > i) {
+          uint64_t a = s[i], b = s[ixj];
+          bool asc = (i & k) == 0;
+          if ((a > b) == asc) { s[i] = b; s[ixj] = a; }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Best `s` keys per (query, DB chunk of kBoundChunk trajectories).
+__global__ void __launch_bounds__(256) k_bound_chunks(MeasureArgs ma, SetView q, int32_t q0, SetView db,
+                                                      int32_t s, uint64_t* keys) {
+  __shared__ uint64_t sk[kBoundChunk];
+  const int qi = q0 + blockIdx.y;
+  QSum qs{q.mbr[qi], q.ends[qi], q.len[qi], q.off[qi]};
+  const bool desc = ma.measure == TS_LCSS;
+  const int base = blockIdx.x * kBoundChunk;
+  for (int i = threadIdx.x; i < kBoundChunk; i += 256) {
+    int j = base + i;
+    uint64_t key = ~0ull;
+    if (j < db.count) {
+      float b = pair_bound(ma, qs, q, db.mbr[j], db.ends[j], db.len[j]);
+      key = make_key(b, j, desc);
+    }
+    sk[i] = key;
+  }
+  __syncthreads();
+  bitonic_sort_lds(sk, kBoundChunk, threadIdx.x, 256);
+  uint64_t* out = keys + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * s;
+  for (int t = threadIdx.x; t < s; t += 256) out[t] = sk[t];
+}
+
+hipError_t launch_bound_chunks(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, int32_t nq,
+                               SetView db, int32_t s, int32_t chunk, uint64_t* keys) {
+  (void)k;
+  (void)chunk;
+  dim3 grid((db.count + kBoundChunk - 1) / kBoundChunk, nq);
+  hipLaunchKernelGGL(k_bound_chunks, grid, dim3(256), 0, st, m, q, q0, db, s, keys);
+  return hipGetLastError();
+}
+
+// Threshold compaction. A pair survives when its bound can still reach the
+// query's k-th value tau: distances bound <= tau; LCSS ub >= tau and, because
+// zero-similarity ties are broken by index, (ub > 0 or j < k).
+__global__ void __launch_bounds__(256) k_bound_compact(MeasureArgs ma, int32_t k, SetView q, int32_t q0,
+                                                       SetView db, const float* tau, int32_t cap,
+                                                       int32_t* region_d, int32_t* counts, int32_t* overflow) {
+  const int qi = q0 + blockIdx.y;
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  QSum qs{q.mbr[qi], q.ends[qi], q.len[qi], q.off[qi]};
+  const float t = tau[blockIdx.y];
+  bool pass = false;
+  if (j < db.count) {
+    float b = pair_bound(ma, qs, q, db.mbr[j], db.ends[j], db.len[j]);
+    if (ma.measure == TS_LCSS) pass = (b >= t) && (b > 0.0f || j < k);
+    else pass = isfinite(b) && (b <= t);
+  }
+  // Wave ballot + prefix count: one atomic per wave.
+  const uint64_t bal = __ballot(pass);
+  if (bal == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)bal) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(&counts[blockIdx.y], __popcll(bal));
+  base = __shfl(base, leader);
+  if (pass) {
+    const int pos = base + __popcll(bal & ((1ull << lane) - 1ull));
+    if (pos < cap) region_d[(int64_t)blockIdx.y * cap + pos] = j;
+    else atomicOr(overflow, 1);
+  }
+}
+
+hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, int32_t nq,
+                                SetView db, const float* tau, int32_t cap, int32_t* region_d, int32_t* counts,
+                                int32_t* overflow) {
+  dim3 grid((db.count + 255) / 256, nq);
+  hipLaunchKernelGGL(k_bound_compact, grid, dim3(256), 0, st, m, k, q, q0, db, tau, cap, region_d, counts, overflow);
+  return hipGetLastError();
+}
+
+}  // namespace ts

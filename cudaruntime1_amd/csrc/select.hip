@@ -1,0 +1,293 @@
+// select.hip -- per-query top-k on the device.
+//
+// Candidates are 64-bit keys: the value's order-preserving bits in the high
+// word (inverted for descending measures) and the DB index in the low word,
+// so ascending key order is the reference's candidate order: (value, index)
+// for DTW/Frechet (std::partial_sort on pair<float,int>, dtw.cpp:62) and
+// (value descending, index ascending) for LCSS (std::stable_sort, lcss.cpp:
+// 67-70). Keys are unique, so the top-k set is well defined under ties.
+//
+// Selection is a tree of chunk sorts: each block bitonic-sorts up to 2048
+// keys of one segment in LDS and keeps the best k; stages repeat until one
+// chunk per segment remains.
+#include "ts_internal.h"
+
+namespace ts {
+
+constexpr int kSelChunk = 2048;
+
+__device__ __forceinline__ uint32_t ord_bits_s(float v) {
+  uint32_t b = __float_as_uint(v + 0.0f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ uint64_t key_of(float v, int32_t idx, bool desc) {
+  if (!isfinite(v) || idx < 0) return ~0ull;
+  uint32_t o = ord_bits_s(v);
+  if (desc) o = ~o;
+  return ((uint64_t)o << 32) | (uint32_t)idx;
+}
+__device__ __forceinline__ float value_of(uint64_t key, bool desc) {
+  uint32_t o = (uint32_t)(key >> 32);
+  if (desc) o = ~o;
+  uint32_t b = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+  return __uint_as_float(b);
+}
+
+__global__ void __launch_bounds__(256) k_keys_from_rows(const float* vals, int32_t rows, int32_t cols,
+                                                        int32_t desc, uint64_t* keys) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t n = (int64_t)rows * cols;
+  if (i >= n) return;
+  keys[i] = key_of(vals[i], (int32_t)(i % cols), desc != 0);
+}
+
+hipError_t launch_keys_from_rows(hipStream_t st, const float* vals, int32_t rows, int32_t cols, int32_t descending,
+                                 uint64_t* keys) {
+  int64_t n = (int64_t)rows * cols;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_keys_from_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, vals, rows, cols,
+                     descending, keys);
+  return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) k_topk_stage(const uint64_t* in, int64_t seg_stride, int32_t seg_len,
+                                                    const int32_t* counts, int32_t k, int32_t ch,
+                                                    uint64_t* out) {
+  __shared__ uint64_t s[kSelChunk];
+  const int seg = blockIdx.y, c = blockIdx.x;
+  int len = seg_len;
+  if (counts) len = min(counts[seg], seg_len);
+  const int begin = c * ch;
+  const int nh = max(0, min(ch, len - begin));
+  const uint64_t* src = in + (int64_t)seg * seg_stride + begin;
+  for (int i = threadIdx.x; i < ch; i += 256) s[i] = i < nh ? src[i] : ~0ull;
+  __syncthreads();
+  for (int kk = 2; kk <= ch; kk <<= 1) {
+    for (int j = kk >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < ch; i += 256) {
+        int ixj = i ^ j;
+        if (ixj > i) {
+          uint64_t a = s[i], b = s[ixj];
+          bool asc = (i & kk) == 0;
+          if ((a > b) == asc) { s[i] = b; s[ixj] = a; }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  uint64_t* dst = out + ((int64_t)seg * gridDim.x + c) * k;
+  for (int t = threadIdx.x; t < k; t += 256) dst[t] = t < ch ? s[t] : ~0ull;
+}
+
+static int chunk_for(int32_t len) {
+  int ch = 64;
+  while (ch < len && ch < kSelChunk) ch <<= 1;
+  return ch;
+}
+
+int64_t select_scratch_elems(int32_t seg_len, int32_t nseg, int32_t k) {
+  int64_t nch = (seg_len + kSelChunk - 1) / kSelChunk;
+  if (nch <= 1) return 1;
+  return (int64_t)nseg * nch * k;
+}
+
+hipError_t select_topk(hipStream_t st, const uint64_t* in, int64_t seg_stride, int32_t seg_len,
+                       const int32_t* counts, int32_t nseg, int32_t k, uint64_t* out, uint64_t* scratch_a,
+                       uint64_t* scratch_b, int64_t scratch_elems, int* launches) {
+  if (nseg == 0) return hipSuccess;
+  if (k > kSelChunk / 2) return hipErrorInvalidValue;
+  const uint64_t* cur = in;
+  int64_t stride = seg_stride;
+  int32_t len = seg_len;
+  const int32_t* cnt = counts;
+  uint64_t* bufs[2] = {scratch_a, scratch_b};
+  int which = 0;
+  for (;;) {
+    int32_t nch = (len + kSelChunk - 1) / kSelChunk;
+    if (nch < 1) nch = 1;
+    int ch = nch == 1 ? chunk_for(len) : kSelChunk;
+    uint64_t* dst = nch == 1 ? out : bufs[which];
+    if (nch > 1 && (int64_t)nseg * nch * k > scratch_elems) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_topk_stage, dim3(nch, nseg), dim3(256), 0, st, cur, stride, len, cnt, k, ch, dst);
+    if (launches) ++*launches;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || nch == 1) return e;
+    cur = dst;
+    stride = (int64_t)nch * k;
+    len = nch * k;
+    cnt = nullptr;
+    which ^= 1;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_decode(const uint64_t* keys, int64_t n, int32_t desc, int32_t* idx,
+                                                float* val) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  uint64_t k = keys[i];
+  if (k == ~0ull) {
+    if (idx) idx[i] = -1;
+    if (val) val[i] = 0.0f;
+  } else {
+    if (idx) idx[i] = (int32_t)(uint32_t)k;
+    if (val) val[i] = value_of(k, desc != 0);
+  }
+}
+
+hipError_t launch_decode_keys(hipStream_t st, const uint64_t* keys, int64_t n, int32_t descending, int32_t* out_idx,
+                              float* out_val) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_decode, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys, n, descending, out_idx,
+                     out_val);
+  return hipGetLastError();
+}
+
+__global__ void k_kth(const uint64_t* keys, int32_t nseg, int32_t s, int32_t k, int32_t desc, float* tau) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nseg) return;
+  uint64_t key = keys[(int64_t)i * s + (k - 1)];
+  float inf = __int_as_float(0x7f800000);
+  tau[i] = key == ~0ull ? (desc ? -inf : inf) : value_of(key, desc != 0);
+}
+
+hipError_t launch_kth_threshold(hipStream_t st, const uint64_t* keys, int32_t nseg, int32_t s, int32_t k,
+                                int32_t descending, float* tau) {
+  if (nseg == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_kth, dim3((nseg + 255) / 256), dim3(256), 0, st, keys, nseg, s, k, descending, tau);
+  return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) k_items_from_keys(const uint64_t* keys, int32_t rows, int32_t s, int32_t q0,
+                                                         Item* items) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (int64_t)rows * s) return;
+  uint64_t k = keys[i];
+  Item it;
+  it.q = q0 + (int32_t)(i / s);
+  it.d = k == ~0ull ? -1 : (int32_t)(uint32_t)k;
+  it.slot = (int32_t)i;
+  it.pad = 0;
+  items[i] = it;
+}
+
+hipError_t launch_items_from_keys(hipStream_t st, const uint64_t* keys, int32_t rows, int32_t s, int32_t q0,
+                                  Item* items, int32_t* nitems) {
+  int64_t n = (int64_t)rows * s;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_items_from_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys, rows, s, q0,
+                     items);
+  (void)nitems;
+  return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) k_keys_from_regions(const float* vals, const int32_t* region_d,
+                                                           const int32_t* counts, int32_t nq, int32_t cap,
+                                                           int32_t desc, uint64_t* keys) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (int64_t)nq * cap) return;
+  const int q = (int)(i / cap), t = (int)(i % cap);
+  const int c = counts ? counts[q] : cap;
+  keys[i] = t < c ? key_of(vals[i], region_d[i], desc != 0) : ~0ull;
+}
+
+hipError_t launch_keys_from_regions(hipStream_t st, const float* vals, const int32_t* region_d,
+                                    const int32_t* counts, int32_t nq, int32_t cap, int32_t descending,
+                                    uint64_t* keys) {
+  int64_t n = (int64_t)nq * cap;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_keys_from_regions, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, vals, region_d,
+                     counts, nq, cap, descending, keys);
+  return hipGetLastError();
+}
+
+__global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill_i32(hipStream_t st, int32_t* p, int32_t v, int64_t n) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
+  return hipGetLastError();
+}
+
+// Refine class of a pair: lanes per pair of the wavefront DP, chosen so that
+// the shorter side (the DP's column count) covers the lanes.
+__host__ __device__ inline int class_of_min_len(int32_t mn) { return mn >= 64 ? 0 : (mn >= 32 ? 1 : 2); }
+int dp_class_for_min_len(int32_t min_len) { return class_of_min_len(min_len); }
+
+__global__ void __launch_bounds__(256) k_items_by_class(const Item* items, const int32_t* nitems, int64_t max_items,
+                                                        SetView q, SetView db, Item* out3, int64_t cap_each,
+                                                        int32_t* counts3) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  int64_t n = nitems ? (int64_t)*nitems : max_items;
+  if (n > max_items) n = max_items;
+  if (i >= n) return;
+  Item it = items[i];
+  if (it.d < 0) return;
+  int c = class_of_min_len(min(q.len[it.q], db.len[it.d]));
+  int pos = atomicAdd(&counts3[c], 1);
+  if (pos < cap_each) out3[c * cap_each + pos] = it;
+}
+
+hipError_t launch_items_by_class(hipStream_t st, const Item* items, const int32_t* nitems, int64_t max_items,
+                                 SetView q, SetView db, Item* out3, int64_t cap_each, int32_t* counts3) {
+  if (max_items == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_items_by_class, dim3((unsigned)((max_items + 255) / 256)), dim3(256), 0, st, items, nitems,
+                     max_items, q, db, out3, cap_each, counts3);
+  return hipGetLastError();
+}
+
+// Dense item list from per-query regions: one block scans the counts
+// (exclusive prefix), then every region entry is scattered.
+__global__ void __launch_bounds__(1024) k_region_scan(const int32_t* counts, int32_t nq, int32_t cap, int32_t* offs,
+                                                      int32_t* total) {
+  __shared__ int32_t part[1024];
+  const int per = (nq + 1023) / 1024;
+  const int b = threadIdx.x * per;
+  int32_t s = 0;
+  for (int i = b; i < min(nq, b + per); ++i) s += min(counts[i], cap);
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    int32_t v = threadIdx.x >= d ? part[threadIdx.x - d] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int32_t run = part[threadIdx.x] - s;
+  for (int i = b; i < min(nq, b + per); ++i) {
+    offs[i] = run;
+    run += min(counts[i], cap);
+  }
+  if (threadIdx.x == 1023) *total = part[1023];
+}
+
+__global__ void __launch_bounds__(256) k_region_scatter(const int32_t* region_d, const int32_t* counts,
+                                                        const int32_t* offs, int32_t nq, int32_t cap, int32_t q0,
+                                                        Item* items) {
+  const int q = blockIdx.y;
+  const int c = min(counts[q], cap);
+  for (int t = blockIdx.x * 256 + threadIdx.x; t < c; t += gridDim.x * 256) {
+    Item it;
+    it.q = q0 + q;
+    it.d = region_d[(int64_t)q * cap + t];
+    it.slot = q * cap + t;
+    it.pad = 0;
+    items[offs[q] + t] = it;
+  }
+}
+
+hipError_t launch_items_from_regions(hipStream_t st, const int32_t* region_d, const int32_t* counts, int32_t nq,
+                                     int32_t cap, int32_t q0, Item* items, int32_t* nitems) {
+  if (nq == 0) return hipSuccess;
+  // nitems[0] receives the total; nitems[1 .. nq] hold the offsets.
+  hipLaunchKernelGGL(k_region_scan, dim3(1), dim3(1024), 0, st, counts, nq, cap, nitems + 1, nitems);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_region_scatter, dim3(std::min(64, (cap + 255) / 256), nq), dim3(256), 0, st, region_d, counts,
+                     nitems + 1, nq, cap, q0, items);
+  return hipGetLastError();
+}
+
+}  // namespace ts

@@ -1,0 +1,278 @@
+// set.hip -- runtime context, errors, device-resident trajectory sets and the
+// synthetic random-walk generator.
+//
+// A set stores its points SoA (x[], y[]), each trajectory contiguous at
+// off[i], plus per-trajectory summaries used by the filters: MBR (the
+// reference's get_trajectory_mbr, hausdorff.cpp:124-134), first/last point
+// and length. The reference instead re-uploads two AoS batches of fixed
+// length on every call (dtw.cu:125-138).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <omp.h>
+
+#include "ts_internal.h"
+
+namespace ts {
+
+namespace {
+thread_local std::string g_err;
+int g_device = -1;
+std::mutex g_init_mu;
+}  // namespace
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+ts_status fail(ts_status s, const std::string& msg) {
+  set_error(msg);
+  return s;
+}
+
+ts_status hip_fail(hipError_t e, const char* what, const char* file, int line) {
+  std::string m = std::string(what) + " failed at " + file + ":" + std::to_string(line) + ": " +
+                  hipGetErrorString(e);
+  set_error(m);
+  return e == hipErrorOutOfMemory ? TS_ERR_OOM : TS_ERR_HIP;
+}
+
+int device_cu_count() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) == hipSuccess) cus = p.multiProcessorCount;
+    if (cus <= 0) cus = 256;
+  }
+  return cus;
+}
+
+// One wave per trajectory: MBR by wave min/max, endpoints from the first and
+// last point.
+__global__ void __launch_bounds__(256) k_summaries(SetView s, float4* mbr, float4* ends) {
+  const int lane = threadIdx.x & 63;
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= s.count) return;
+  const int64_t o = s.off[t];
+  const int n = s.len[t];
+  float mnx = s.x[o], mxx = mnx, mny = s.y[o], mxy = mny;
+  for (int i = lane; i < n; i += 64) {
+    float px = s.x[o + i], py = s.y[o + i];
+    mnx = fminf(mnx, px); mxx = fmaxf(mxx, px);
+    mny = fminf(mny, py); mxy = fmaxf(mxy, py);
+  }
+  for (int sh = 32; sh > 0; sh >>= 1) {
+    mnx = fminf(mnx, __shfl_xor(mnx, sh));
+    mxx = fmaxf(mxx, __shfl_xor(mxx, sh));
+    mny = fminf(mny, __shfl_xor(mny, sh));
+    mxy = fmaxf(mxy, __shfl_xor(mxy, sh));
+  }
+  if (lane == 0) {
+    mbr[t] = make_float4(mnx, mxx, mny, mxy);
+    ends[t] = make_float4(s.x[o], s.y[o], s.x[o + n - 1], s.y[o + n - 1]);
+  }
+}
+
+hipError_t launch_summaries(hipStream_t st, SetView s, float4* mbr, float4* ends) {
+  if (s.count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_summaries, dim3((s.count + 3) / 4), dim3(256), 0, st, s, mbr, ends);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ generator
+// SplitMix64 finaliser over a (seed, trajectory, point, lane) counter.
+static inline uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+static inline uint64_t ctr_hash(uint64_t seed, uint64_t a, uint64_t b) {
+  return mix64(mix64(mix64(seed) ^ a) ^ (b * 0xD1B54A32D192ED03ull));
+}
+// Uniform in [0,1) with 53 random bits.
+static inline double u01(uint64_t h) { return (double)(h >> 11) * (1.0 / 9007199254740992.0); }
+
+}  // namespace ts
+
+using namespace ts;
+
+extern "C" {
+
+const char* ts_status_string(ts_status s) {
+  switch (s) {
+    case TS_OK: return "ok";
+    case TS_ERR_INVALID: return "invalid argument";
+    case TS_ERR_HIP: return "HIP runtime error";
+    case TS_ERR_OOM: return "out of device memory";
+    case TS_ERR_UNSUPPORTED: return "unsupported";
+    case TS_ERR_NO_DEVICE: return "no device";
+    case TS_ERR_BUSY: return "busy";
+    case TS_ERR_TIMEOUT: return "timeout";
+    case TS_ERR_CANCELLED: return "cancelled";
+  }
+  return "unknown status";
+}
+
+const char* ts_last_error(void) { return g_err.c_str(); }
+
+void ts_params_default(ts_params* p, int32_t measure, int32_t mode) {
+  if (!p) return;
+  p->measure = measure;
+  p->mode = mode;
+  p->k = 10;
+  p->kc = measure == TS_EUCLIDEAN ? 50 : 10;  // euclidean.cpp:90, dtw.cpp:36, hausdorff.cuh:9
+  p->r = 5;                                    // dtw.cpp:17
+  p->band = 0;
+  p->delta = -1;
+  p->epsilon = 0.5f;                           // main.cu:170
+}
+
+ts_status ts_init(int32_t device) {
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0) return fail(TS_ERR_NO_DEVICE, "no HIP device available");
+  if (device < 0 || device >= n) return fail(TS_ERR_INVALID, "device index out of range");
+  TS_HIP(hipSetDevice(device));
+  TS_HIP(hipFree(nullptr));
+  g_device = device;
+  return TS_OK;
+}
+
+ts_status ts_device_name(char* buf, int32_t buflen) {
+  if (!buf || buflen <= 0) return fail(TS_ERR_INVALID, "buffer");
+  int dev = 0;
+  TS_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t p;
+  TS_HIP(hipGetDeviceProperties(&p, dev));
+  std::snprintf(buf, buflen, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+  return TS_OK;
+}
+
+ts_status ts_set_create(const ts_point* xy, const int64_t* offsets, int32_t count, ts_set** out) {
+  if (!out || !offsets || count <= 0) return fail(TS_ERR_INVALID, "ts_set_create: need count > 0 and offsets");
+  *out = nullptr;
+  if (offsets[0] != 0) return fail(TS_ERR_INVALID, "ts_set_create: offsets[0] must be 0");
+  auto s = new ts_set();
+  s->count = count;
+  s->h_off.assign(offsets, offsets + count + 1);
+  s->h_len.resize(count);
+  int32_t mn = INT32_MAX, mx = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    int64_t l = offsets[i + 1] - offsets[i];
+    if (l < 1 || l > (1 << 24)) {
+      delete s;
+      return fail(TS_ERR_INVALID, "ts_set_create: trajectory " + std::to_string(i) + " has length " +
+                                      std::to_string(l) + " (need 1..2^24)");
+    }
+    s->h_len[i] = (int32_t)l;
+    mn = std::min<int32_t>(mn, (int32_t)l);
+    mx = std::max<int32_t>(mx, (int32_t)l);
+  }
+  s->min_len = mn;
+  s->max_len = mx;
+  s->points = offsets[count];
+  if (!xy) { delete s; return fail(TS_ERR_INVALID, "ts_set_create: null points"); }
+  (void)hipGetDevice(&s->device);
+  const int64_t P = s->points;
+  std::vector<float> hx(P), hy(P);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < P; ++i) { hx[i] = xy[i].x; hy[i] = xy[i].y; }
+  auto cleanup = [&](ts_status st) {
+    ts_set_destroy(s);
+    return st;
+  };
+  hipError_t e;
+#define TS_ALLOC(ptr, nbytes)                                                    \
+  e = hipMalloc((void**)&(ptr), (nbytes));                                       \
+  if (e != hipSuccess) return cleanup(hip_fail(e, "hipMalloc(set)", __FILE__, __LINE__)); \
+  s->bytes += (int64_t)(nbytes);
+  TS_ALLOC(s->x, P * sizeof(float));
+  TS_ALLOC(s->y, P * sizeof(float));
+  TS_ALLOC(s->off, (count + 1) * sizeof(int64_t));
+  TS_ALLOC(s->len, count * sizeof(int32_t));
+  TS_ALLOC(s->mbr, count * sizeof(float4));
+  TS_ALLOC(s->ends, count * sizeof(float4));
+#undef TS_ALLOC
+  if ((e = hipMemcpy(s->x, hx.data(), P * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(s->y, hy.data(), P * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(s->off, offsets, (count + 1) * sizeof(int64_t), hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(s->len, s->h_len.data(), count * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess)
+    return cleanup(hip_fail(e, "hipMemcpy(set)", __FILE__, __LINE__));
+  if ((e = launch_summaries(nullptr, s->view(), s->mbr, s->ends)) != hipSuccess ||
+      (e = hipDeviceSynchronize()) != hipSuccess)
+    return cleanup(hip_fail(e, "summaries", __FILE__, __LINE__));
+  *out = s;
+  return TS_OK;
+}
+
+ts_status ts_set_destroy(ts_set* s) {
+  if (!s) return TS_OK;
+  (void)hipFree(s->x); (void)hipFree(s->y); (void)hipFree(s->off); (void)hipFree(s->len);
+  (void)hipFree(s->mbr); (void)hipFree(s->ends);
+  delete s;
+  return TS_OK;
+}
+
+ts_status ts_set_info(const ts_set* s, int32_t* count, int64_t* points, int32_t* min_len, int32_t* max_len) {
+  if (!s) return fail(TS_ERR_INVALID, "null set");
+  if (count) *count = s->count;
+  if (points) *points = s->points;
+  if (min_len) *min_len = s->min_len;
+  if (max_len) *max_len = s->max_len;
+  return TS_OK;
+}
+
+ts_status ts_set_bytes(const ts_set* s, int64_t* bytes) {
+  if (!s || !bytes) return fail(TS_ERR_INVALID, "null argument");
+  *bytes = s->bytes;
+  return TS_OK;
+}
+
+ts_status ts_gen_lengths(uint64_t seed, int32_t count, int32_t lo, int32_t hi, int32_t log_uniform,
+                         int32_t* out_len) {
+  if (!out_len || count < 0 || lo < 1 || hi < lo) return fail(TS_ERR_INVALID, "ts_gen_lengths");
+#pragma omp parallel for schedule(static)
+  for (int32_t i = 0; i < count; ++i) {
+    double u = u01(ctr_hash(seed, 0xFFFFFFFFull, (uint64_t)i));
+    int64_t l;
+    if (log_uniform) {
+      double a = std::log((double)lo), b = std::log((double)hi + 1.0);
+      l = (int64_t)std::floor(std::exp(a + u * (b - a)));
+    } else {
+      l = lo + (int64_t)std::floor(u * (double)(hi - lo + 1));
+    }
+    out_len[i] = (int32_t)std::min<int64_t>(hi, std::max<int64_t>(lo, l));
+  }
+  return TS_OK;
+}
+
+ts_status ts_gen_walks(uint64_t seed, int32_t count, const int64_t* offsets, double sigma, ts_point* out_xy) {
+  if (!offsets || !out_xy || count < 0) return fail(TS_ERR_INVALID, "ts_gen_walks");
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int32_t t = 0; t < count; ++t) {
+    const int64_t o = offsets[t], n = offsets[t + 1] - offsets[t];
+    double x = 0, y = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      uint64_t h1 = ctr_hash(seed, (uint64_t)t, (uint64_t)i * 2);
+      uint64_t h2 = ctr_hash(seed, (uint64_t)t, (uint64_t)i * 2 + 1);
+      if (i == 0) {
+        x = u01(h1);
+        y = u01(h2);
+      } else {
+        // Box-Muller: two independent N(0,1) from two uniforms.
+        double u1 = 1.0 - u01(h1), u2 = u01(h2);
+        double rad = std::sqrt(-2.0 * std::log(u1));
+        x += sigma * rad * std::cos(6.283185307179586 * u2);
+        y += sigma * rad * std::sin(6.283185307179586 * u2);
+      }
+      out_xy[o + i].x = (float)x;
+      out_xy[o + i].y = (float)y;
+    }
+  }
+  return TS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,150 @@
+// ts_internal.h -- shared declarations of the trajsim HIP engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/trajsim.h"
+
+namespace ts {
+
+// DP border value: the reference's "infinity" (dtw.cpp:68-69, dtw.cu:59).
+constexpr float kInf = 1e20f;
+
+// Thread-local error message behind ts_last_error().
+void set_error(const std::string& msg);
+ts_status fail(ts_status s, const std::string& msg);
+ts_status hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define TS_HIP(call)                                                          \
+  do {                                                                        \
+    hipError_t _e = (call);                                                   \
+    if (_e != hipSuccess) return ::ts::hip_fail(_e, #call, __FILE__, __LINE__); \
+  } while (0)
+
+#define TS_TRY(call)                    \
+  do {                                  \
+    ts_status _s = (call);              \
+    if (_s != TS_OK) return _s;         \
+  } while (0)
+
+// Read-only device view of a trajectory set, passed to kernels by value.
+struct SetView {
+  const float* x;        // [points]
+  const float* y;        // [points]
+  const int64_t* off;    // [count + 1]
+  const int32_t* len;    // [count]
+  const float4* mbr;     // [count] (minx, maxx, miny, maxy)
+  const float4* ends;    // [count] (x0, y0, x_last, y_last)
+  int32_t count;
+};
+
+// Work item of the refine kernels: one (query, database) pair and the slot
+// its value is written to.
+struct Item {
+  int32_t q, d, slot, pad;
+};
+
+// Launch-time parameters of the measures.
+struct MeasureArgs {
+  int32_t measure, band, delta;
+  float eps;
+};
+
+}  // namespace ts
+
+// Device-resident set (the opaque ts_set of the C ABI).
+struct ts_set {
+  int32_t count = 0;
+  int64_t points = 0;
+  int32_t min_len = 0, max_len = 0;
+  int device = 0;
+  float* x = nullptr;
+  float* y = nullptr;
+  int64_t* off = nullptr;
+  int32_t* len = nullptr;
+  float4* mbr = nullptr;
+  float4* ends = nullptr;
+  std::vector<int64_t> h_off;
+  std::vector<int32_t> h_len;
+  int64_t bytes = 0;
+  ts::SetView view() const { return {x, y, off, len, mbr, ends, count}; }
+};
+
+namespace ts {
+
+// ------------------------------------------------------------ launchers
+// Each launcher enqueues its kernel(s) on `st` and returns the launch status.
+
+// set.hip
+hipError_t launch_summaries(hipStream_t st, SetView s, float4* mbr, float4* ends);
+
+// filter.hip
+hipError_t launch_envelope(hipStream_t st, SetView q, int32_t max_len, int32_t r, float4* env);
+// REF filter scores[q * N + j] for queries [q0, q0+nq).
+hipError_t launch_filter_ref(hipStream_t st, int32_t measure, SetView q, int32_t q0, int32_t nq,
+                             const float4* env, SetView db, float eps, float* scores);
+// Admissible summary bounds (EXACT mode) of queries [q0,q0+nq) against the
+// whole DB, reduced per DB chunk to the best `s` keys per query:
+// keys[(q * nchunks + chunk) * s + t].
+hipError_t launch_bound_chunks(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0,
+                               int32_t nq, SetView db, int32_t s, int32_t chunk, uint64_t* keys);
+// Threshold compaction: every DB trajectory whose bound can still reach the
+// query's threshold tau[q] becomes an Item, appended per query region
+// [q*cap, q*cap+cap) with counts[q]; overflow raises *overflow.
+hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0,
+                                int32_t nq, SetView db, const float* tau, int32_t cap,
+                                int32_t* region_d, int32_t* counts, int32_t* overflow);
+
+// select.hip
+// Key order: ascending u64. enc_asc(v,i) = ordered(v)<<32|i, desc uses ~ordered(v).
+hipError_t launch_keys_from_rows(hipStream_t st, const float* vals, int32_t rows, int32_t cols,
+                                 int32_t descending, uint64_t* keys);
+// Segment top-k: segments of seg_len keys (or counts[s] if counts != null,
+// clamped to seg_len) starting at in + s*seg_stride; writes the k smallest
+// keys of each segment (sorted, padded with ~0ull) to out + s*k.
+hipError_t select_topk(hipStream_t st, const uint64_t* in, int64_t seg_stride, int32_t seg_len,
+                       const int32_t* counts, int32_t nseg, int32_t k, uint64_t* out,
+                       uint64_t* scratch_a, uint64_t* scratch_b, int64_t scratch_elems, int* launches);
+int64_t select_scratch_elems(int32_t seg_len, int32_t nseg, int32_t k);
+// Decode sorted keys into (idx, value) with optional index remap.
+hipError_t launch_decode_keys(hipStream_t st, const uint64_t* keys, int64_t n, int32_t descending,
+                              int32_t* out_idx, float* out_val);
+// Threshold per segment: value of key t (the k-th smallest), or +inf / -inf
+// if the segment has fewer keys.
+hipError_t launch_kth_threshold(hipStream_t st, const uint64_t* keys, int32_t nseg, int32_t s,
+                                int32_t k, int32_t descending, float* tau);
+// Build Items (q0+row, d=decoded idx, slot=row*s+t) from per-row sorted keys.
+hipError_t launch_items_from_keys(hipStream_t st, const uint64_t* keys, int32_t rows, int32_t s,
+                                  int32_t q0, Item* items, int32_t* nitems);
+// Keys of refined values: key(value[slot], d) for every slot of the query
+// regions, ~0 for empty slots.
+hipError_t launch_keys_from_regions(hipStream_t st, const float* vals, const int32_t* region_d,
+                                    const int32_t* counts, int32_t nq, int32_t cap, int32_t descending,
+                                    uint64_t* keys);
+hipError_t launch_fill_i32(hipStream_t st, int32_t* p, int32_t v, int64_t n);
+// Split items by refine class (W lanes per pair) into three lists.
+hipError_t launch_items_by_class(hipStream_t st, const Item* items, const int32_t* nitems, int64_t max_items,
+                                 SetView q, SetView db, Item* out3, int64_t cap_each, int32_t* counts3);
+hipError_t launch_items_from_regions(hipStream_t st, const int32_t* region_d, const int32_t* counts,
+                                     int32_t nq, int32_t cap, int32_t q0, Item* items, int32_t* nitems);
+
+// dp.hip -- persistent wavefront DP over a device item list.
+// cls: 0 -> 64 lanes per pair, 1 -> 32, 2 -> 16.
+int dp_lanes_for_class(int cls);
+int dp_class_for_min_len(int32_t min_len);
+hipError_t launch_dp(hipStream_t st, MeasureArgs m, int cls, SetView q, SetView db, const Item* items,
+                     const int32_t* nitems, int64_t max_items, int32_t* next, float* out, int32_t cmax,
+                     int grid_waves);
+
+// reduce.hip -- Hausdorff (wave per pair) and lock-step Euclidean (lane per pair).
+hipError_t launch_hausdorff(hipStream_t st, SetView q, SetView db, const Item* items, const int32_t* nitems,
+                            int64_t max_items, int32_t* next, float* out, int grid_waves);
+hipError_t launch_euclidean(hipStream_t st, SetView q, SetView db, const Item* items, const int32_t* nitems,
+                            int64_t max_items, float* out);
+
+int device_cu_count();
+
+}  // namespace ts

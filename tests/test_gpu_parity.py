@@ -1,0 +1,194 @@
+"""GPU parity: every HIP kernel path of libtrajsim.so against the CPU oracle
+on the same seeded inputs. Integer/index results and f32 values are compared
+bit for bit (the oracle restates the reference's f32 arithmetic; the kernels
+keep its operation order, no FMA contraction, correctly rounded sqrtf)."""
+import ctypes
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import cudaruntime1_amd as ts
+import oracle
+from tests.helpers import planted, uniform_lengths, walks
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+LENGTHS = [1, 2, 3, 15, 16, 17, 31, 32, 33, 63, 64, 65, 100, 127, 128, 129, 257]
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def eng():
+    ts.init(0)
+    say("device", ts.device_name())
+    e = ts.Engine()
+    yield e
+    e.close()
+
+
+@pytest.fixture(scope="module")
+def shapes():
+    db_xy, db_off = walks(21, LENGTHS, 2e-2)
+    q_xy, q_off = walks(22, LENGTHS[::-1], 2e-2)
+    db = ts.TrajSet(db_xy, db_off)
+    qs = ts.TrajSet(q_xy, q_off)
+    n = len(LENGTHS)
+    qi, di = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    return dict(db=db, qs=qs, db_xy=db_xy, db_off=db_off, q_xy=q_xy, q_off=q_off, qi=qi.ravel(), di=di.ravel())
+
+
+def check_pairs(eng, s, measure, **kw):
+    p = ts.params(measure, **kw)
+    prm = oracle.params(p.measure, p.mode, k=p.k, kc=p.kc, r=p.r, band=p.band, delta=p.delta, epsilon=p.epsilon)
+    got = eng.pair_values(s["db"], s["qs"], p, s["qi"], s["di"])
+    want = oracle.pair_values(prm, s["db_xy"], s["db_off"], s["q_xy"], s["q_off"], s["qi"], s["di"])
+    bad = np.nonzero(got.view(np.int32) != want.view(np.int32))[0]
+    assert len(bad) == 0, [(int(s["qs"].lengths[s["qi"][b]]), int(s["db"].lengths[s["di"][b]]), got[b], want[b])
+                           for b in bad[:8]]
+
+
+@pytest.mark.parametrize("measure", ["dtw", "frechet", "lcss", "hausdorff"])
+def test_pair_values_bit_exact_all_shapes(eng, shapes, measure):
+    say("pairs", measure, len(shapes["qi"]))
+    check_pairs(eng, shapes, measure, epsilon=3e-2)
+
+
+@pytest.mark.parametrize("band", [1, 3, 10, 40])
+def test_dtw_band_bit_exact(eng, shapes, band):
+    say("dtw band", band)
+    check_pairs(eng, shapes, "dtw", band=band)
+
+
+@pytest.mark.parametrize("delta", [0, 5, 16, 64])
+def test_lcss_window_bit_exact(eng, shapes, delta):
+    say("lcss delta", delta)
+    check_pairs(eng, shapes, "lcss", delta=delta, epsilon=3e-2)
+
+
+def test_euclidean_pairs_bit_exact(eng):
+    say("euclidean pairs")
+    xy, off = walks(23, [128] * 40 + [7] * 10)
+    s = ts.TrajSet(xy, off)
+    qi = np.repeat(np.arange(50), 50)
+    di = np.tile(np.arange(50), 50)
+    keep = s.lengths[qi] == s.lengths[di]
+    qi, di = qi[keep], di[keep]
+    p = ts.params("euclidean")
+    got = eng.pair_values(s, s, p, qi, di)
+    want = oracle.pair_values(oracle.params(oracle.EUCLIDEAN), xy, off, xy, off, qi, di)
+    assert np.array_equal(got.view(np.int32), want.view(np.int32))
+
+
+def test_long_pairs_bit_exact(eng):
+    say("long pairs")
+    db_xy, db_off = walks(24, [2048, 3000, 700])
+    q_xy, q_off = walks(25, [2500, 900])
+    db, qs = ts.TrajSet(db_xy, db_off), ts.TrajSet(q_xy, q_off)
+    qi, di = np.array([0, 0, 1, 1, 0]), np.array([0, 1, 0, 2, 2])
+    for measure, kw in [("dtw", {}), ("dtw", {"band": 205}), ("frechet", {}), ("lcss", {"epsilon": 2e-3, "delta": 64})]:
+        p = ts.params(measure, **kw)
+        prm = oracle.params(p.measure, band=p.band, delta=p.delta, epsilon=p.epsilon)
+        got = eng.pair_values(db, qs, p, qi, di)
+        want = oracle.pair_values(prm, db_xy, db_off, q_xy, q_off, qi, di)
+        assert np.array_equal(got.view(np.int32), want.view(np.int32)), (measure, got, want)
+
+
+@pytest.mark.parametrize("measure", ["dtw", "frechet", "lcss", "hausdorff", "euclidean"])
+def test_filter_ref_bit_exact(eng, measure):
+    say("filter_ref", measure)
+    L = uniform_lengths(26, 700, 32, 128)
+    if measure == "euclidean":
+        L[::3] = 64
+    db_xy, db_off = walks(26, L)
+    q_xy, q_off = planted(db_xy, db_off, np.arange(0, 700, 70), L[:10] if measure != "euclidean" else [64] * 10,
+                          1e-3, 2)
+    db, qs = ts.TrajSet(db_xy, db_off), ts.TrajSet(q_xy, q_off)
+    p = ts.params(measure, epsilon=2e-3)
+    got = eng.filter_ref(db, qs, p)
+    if measure in ("dtw", "frechet", "lcss"):
+        want = oracle.filter_ref(oracle.params(p.measure, epsilon=p.epsilon), db_xy, db_off, q_xy, q_off)
+        assert np.array_equal(got.view(np.int32), want.view(np.int32))
+    else:
+        for qi in range(10):
+            mq = oracle.mbr(q_xy[q_off[qi]:q_off[qi + 1]])
+            for j in range(0, 700, 7):
+                c = db_xy[db_off[j]:db_off[j + 1]]
+                if measure == "euclidean" and len(c) != q_off[qi + 1] - q_off[qi]:
+                    assert np.isinf(got[qi, j])
+                else:
+                    assert got[qi, j] == oracle.box_box_d2(mq, oracle.mbr(c))
+
+
+def test_topk_rows_order_and_ties(eng):
+    say("topk rows")
+    rng = np.random.default_rng(27)
+    for rows, cols, k in [(3, 5, 5), (7, 1000, 10), (4, 5000, 100), (2, 70000, 37), (5, 64, 1)]:
+        s = np.round(rng.random((rows, cols)) * 50).astype(np.float32)  # many ties
+        for desc in (False, True):
+            idx, val = eng.topk_rows(s, k, desc)
+            for r in range(rows):
+                order = np.lexsort((np.arange(cols), -s[r] if desc else s[r]))[:k]
+                assert np.array_equal(idx[r, :min(k, cols)], order), (rows, cols, k, desc)
+                assert np.array_equal(val[r, :min(k, cols)], s[r, order])
+
+
+def knn_case(measure, n_db=1500, n_q=12, lo=32, hi=128, seed=30):
+    L = uniform_lengths(seed, n_db, lo, hi)
+    if measure == "euclidean":
+        L[::2] = 96
+    db_xy, db_off = walks(seed, L)
+    picks = np.arange(1, n_db, n_db // n_q)[:n_q]
+    qL = L[picks] if measure != "euclidean" else np.full(n_q, 96)
+    if measure == "euclidean":
+        picks = picks - (picks % 2)
+    q_xy, q_off = planted(db_xy, db_off, picks, qL, 5e-4, seed)
+    return db_xy, db_off, q_xy, q_off
+
+
+@pytest.mark.parametrize("measure", ["dtw", "frechet", "lcss", "hausdorff", "euclidean"])
+@pytest.mark.parametrize("mode", [ts.REF, ts.EXACT])
+def test_knn_batched_and_plain_match_oracle(eng, measure, mode):
+    say("knn", measure, "mode", mode)
+    db_xy, db_off, q_xy, q_off = knn_case(measure)
+    db, qs = ts.TrajSet(db_xy, db_off), ts.TrajSet(q_xy, q_off)
+    k = 10 if mode == ts.EXACT else 5
+    p = ts.params(measure, mode=mode, k=k, epsilon=2e-3)
+    prm = oracle.params(p.measure, p.mode, k=p.k, kc=p.kc, r=p.r, band=p.band, delta=p.delta, epsilon=p.epsilon)
+    wi, wv, _ = oracle.knn(prm, db_xy, db_off, q_xy, q_off)
+    for fn in (eng.knn, eng.knn_plain):
+        gi, gv, st = fn(db, qs, p)
+        assert np.array_equal(gi, wi), (fn.__name__, gi[:2], wi[:2])
+        assert np.array_equal(gv.view(np.int32), wv.view(np.int32)), (fn.__name__, gv[:2], wv[:2])
+
+
+def test_compat_reproduces_reference_fixtures(eng):
+    """The re-exported reference signatures on the GPU give exactly the
+    h_results of the reference's own CPU build (tests/golden)."""
+    lib = ctypes.CDLL(ts.LIB_PATH)
+    fns = {oracle.DTW: lib._Z24launch_dtw_batch_gpu_knnPK5PointS1_PfiiiRf,
+           oracle.FRECHET: lib._Z34launch_frechet_batch_gpu_wavefrontPK5PointS1_PfiiiRf,
+           oracle.LCSS: lib._Z35launch_lcss_batch_gpu_wavefront_knnPK5PointS1_PfiiifRf}
+    files = sorted(f for f in os.listdir(GOLDEN) if f.endswith(".npz"))
+    assert files
+    for f in files:
+        say("compat", f)
+        z = np.load(os.path.join(GOLDEN, f))
+        num_t, n, m, meas = int(z["num_t"]), int(z["n"]), int(z["m"]), int(z["measure"])
+        t1 = np.ascontiguousarray(z["t1"], np.float32)
+        t2 = np.ascontiguousarray(z["t2"], np.float32)
+        out = np.zeros(num_t, np.float32)
+        ms = ctypes.c_float()
+        P = ctypes.c_void_p
+        if meas == oracle.LCSS:
+            fns[meas](P(t1.ctypes.data), P(t2.ctypes.data), P(out.ctypes.data), num_t, n, m,
+                      ctypes.c_float(float(z["eps"])), ctypes.byref(ms))
+        else:
+            fns[meas](P(t1.ctypes.data), P(t2.ctypes.data), P(out.ctypes.data), num_t, n, m, ctypes.byref(ms))
+        assert np.array_equal(out.view(np.int32), z["h_results"].view(np.int32)), f
