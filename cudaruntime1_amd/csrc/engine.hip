@@ -231,9 +231,8 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   int32_t* ctr = nullptr;
   TS_TRY(ensure(e, B_CTR, 16, &ctr));
   if (p.measure == TS_HAUSDORFF) {
-    TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, ctr, 0, 1); }));
     return run(e, "hausdorff_refine", 0, 0, [&] {
-      return launch_hausdorff(e->st, qv, dv, items, nitems_dev, max_items, ctr, vals, cus * 8);
+      return launch_hausdorff(e->st, qv, dv, items, nitems_dev, max_items, vals);
     }, slot);
   }
   if (p.measure == TS_EUCLIDEAN) {

@@ -4,8 +4,8 @@
 // max_b min_a |a-b|^2) with |a-b|^2 = dx*dx + dy*dy in f32, result sqrtf.
 // min/max do not round, so any evaluation order gives the reference value;
 // the reference's grid accelerator and early prune are value-preserving.
-// One wave per pair, pulled from an atomic work counter: lanes own points of
-// the source side, the target side streams through uniform (scalar) loads.
+// One wave per pair: lanes own points of the source side, the target side
+// streams through uniform (scalar) loads.
 //
 // Euclidean (euclidean.cpp:27-40, :113): sqrt of a sequential double sum of
 // dx*dx + dy*dy over equal-length trajectories, then cast to float. One lane
@@ -40,31 +40,37 @@ __device__ float directed_h2_wave(const float* ax, const float* ay, int na, cons
   return best;
 }
 
+// Wave-uniform copies: the pair's index, lengths and offsets are the same on
+// every lane; readfirstlane moves them to scalar registers so the target loops
+// are scalar loops over scalar loads of the target points.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uni64(int64_t v) {
+  return (int64_t)(((uint64_t)(uint32_t)uni((int)(v >> 32)) << 32) | (uint32_t)uni((int)v));
+}
+
+// One wave per work item (no fetch loop): wave w of the grid takes item w.
 __global__ void __launch_bounds__(256) k_hausdorff(SetView q, SetView db, const Item* items, const int32_t* nitems,
-                                                   int64_t max_items, int32_t* next, float* out) {
+                                                   int64_t max_items, float* out) {
   const int lane = threadIdx.x & 63;
+  const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   int64_t n_items = max_items;
   if (nitems) n_items = min<int64_t>(n_items, (int64_t)*nitems);
-  for (;;) {
-    int it_idx = 0;
-    if (lane == 0) it_idx = atomicAdd(next, 1);
-    it_idx = __shfl(it_idx, 0);
-    if (it_idx >= n_items) break;
-    const Item it = items[it_idx];
-    if (it.d < 0) continue;
-    const int n = q.len[it.q], m = db.len[it.d];
-    const int64_t qo = q.off[it.q], dof = db.off[it.d];
-    const float h1 = directed_h2_wave(q.x + qo, q.y + qo, n, db.x + dof, db.y + dof, m, lane);
-    const float h2 = directed_h2_wave(db.x + dof, db.y + dof, m, q.x + qo, q.y + qo, n, lane);
-    if (lane == 0) out[it.slot] = sqrtf(fmaxf(h1, h2));
-  }
+  if (w >= n_items) return;
+  const int64_t it_idx = uni64(w);
+  const int iq = uni(items[it_idx].q), id = uni(items[it_idx].d), slot = uni(items[it_idx].slot);
+  if (id < 0) return;
+  const int n = uni(q.len[iq]), m = uni(db.len[id]);
+  const int64_t qo = uni64(q.off[iq]), dof = uni64(db.off[id]);
+  const float h1 = directed_h2_wave(q.x + qo, q.y + qo, n, db.x + dof, db.y + dof, m, lane);
+  const float h2 = directed_h2_wave(db.x + dof, db.y + dof, m, q.x + qo, q.y + qo, n, lane);
+  if (lane == 0) out[slot] = sqrtf(fmaxf(h1, h2));
 }
 
 hipError_t launch_hausdorff(hipStream_t st, SetView q, SetView db, const Item* items, const int32_t* nitems,
-                            int64_t max_items, int32_t* next, float* out, int grid_waves) {
-  int blocks = (grid_waves + 3) / 4;
-  if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(k_hausdorff, dim3(blocks), dim3(256), 0, st, q, db, items, nitems, max_items, next, out);
+                            int64_t max_items, float* out) {
+  if (max_items == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_hausdorff, dim3((unsigned)((max_items + 3) / 4)), dim3(256), 0, st, q, db, items, nitems,
+                     max_items, out);
   return hipGetLastError();
 }
 

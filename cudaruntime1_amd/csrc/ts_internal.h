@@ -141,7 +141,7 @@ hipError_t launch_dp(hipStream_t st, MeasureArgs m, int cls, SetView q, SetView 
 
 // reduce.hip -- Hausdorff (wave per pair) and lock-step Euclidean (lane per pair).
 hipError_t launch_hausdorff(hipStream_t st, SetView q, SetView db, const Item* items, const int32_t* nitems,
-                            int64_t max_items, int32_t* next, float* out, int grid_waves);
+                            int64_t max_items, float* out);
 hipError_t launch_euclidean(hipStream_t st, SetView q, SetView db, const Item* items, const int32_t* nitems,
                             int64_t max_items, float* out);
 
