@@ -1,0 +1,189 @@
+"""Benchmark of the trajectory k-NN hot path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 64 query
+random walks against a resident database of 100,000 random walks, lengths
+uniform in [64, 256], start U[0,1)^2, Gaussian steps sigma=1e-3 (DB seed
+0xDB5EED, query seed 0x9E3779B9); DTW top-10 with the bounding-box/endpoint
+lower-bound filter (EXACT mode: the true top-10 over the whole DB).
+
+One step = one batched k-NN call (ts_knn) over the 64 queries. The line also
+carries Frechet, the plain one-launch-sequence-per-query method, per-query
+latency p50/p99, the dominant kernel's roofline (HIP events over the timed
+region) and the CPU oracle timed on a bounded sample of the same workload on
+this host's cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "DTW/Fréchet pairs/s & cell-updates/s, batched vs plain; p50/p99 query latency"
+DB_SEED, Q_SEED = 0xDB5EED, 0x9E3779B9
+# FP32 VALU peak (non-packed): 256 CUs x 64 lanes x 2.4 GHz.
+VALU_PEAK_TOPS = 39.3
+HBM_PEAK_GBS = 8000.0
+# Source-level f32 operations per unit of work (DP cell or pair).
+OPS = {"dtw_wavefront": 9.0, "frechet_wavefront": 9.0, "lcss_wavefront": 7.0, "bound_seed": 24.0,
+       "bound_compact": 24.0}
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def build_workload(ts, n_db, n_q, lo, hi):
+    db_len = ts.gen_lengths(DB_SEED, n_db, lo, hi, False)
+    db_xy, db_off = ts.gen_walks(DB_SEED, db_len)
+    q_len = ts.gen_lengths(Q_SEED, n_q, lo, hi, False)
+    q_xy, q_off = ts.gen_walks(Q_SEED, q_len)
+    return db_xy, db_off, q_xy, q_off
+
+
+def timed(fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = fn()
+    return (time.perf_counter() - t0) * 1e3 / steps, out
+
+
+def roofline_of(kstats, pmc):
+    """Dominant kernel by total time; achieved = algorithmic ops per launch /
+    mean launch duration (HIP events on the engine's stream)."""
+    name = max((k for k in kstats if k in OPS), key=lambda k: kstats[k]["ms"])
+    s = kstats[name]
+    per_launch_units = s["units"] / s["launches"]
+    mean_ms = s["ms"] / s["launches"]
+    achieved = per_launch_units * OPS[name] / (mean_ms * 1e-3) / 1e12
+    traffic = None
+    if pmc and name in pmc:
+        traffic = pmc[name].get("hbm_bytes_per_launch")
+    return {"bound": "valu", "kernel": name, "achieved": round(achieved, 4), "peak": VALU_PEAK_TOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": traffic,
+            "units_per_launch": per_launch_units, "ms_per_launch": round(mean_ms, 4),
+            "ops_per_unit": OPS[name]}
+
+
+def cpu_baseline(db_xy, db_off, q_xy, q_off, n_sample):
+    import oracle
+    prm = oracle.params(oracle.DTW, oracle.EXACT, k=10)
+    sub_off = q_off[:n_sample + 1]
+    sub_xy = q_xy[:sub_off[-1]]
+    t0 = time.perf_counter()
+    idx, val, refined = oracle.knn(prm, db_xy, db_off, sub_xy, sub_off)
+    dt = time.perf_counter() - t0
+    n_db = len(db_off) - 1
+    return {"value": round(n_sample * n_db / dt, 1), "unit": "pairs/s", "cores": oracle.num_threads(),
+            "kind": "port",
+            "sample": "%d of the 64 queries against the full %d-trajectory DB, EXACT DTW top-10 (oracle best-first "
+                      "search with the same bounds), %.2f s" % (n_sample, n_db, dt)}, idx, val
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n-db", type=int, default=100_000)
+    ap.add_argument("--n-q", type=int, default=64)
+    ap.add_argument("--cpu-sample", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="skip plain/latency/frechet legs (profiling runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (one GPU per box); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import cudaruntime1_amd as ts
+    ts.init(0)
+    dev = ts.device_name()
+    log("device", dev)
+    t0 = time.time()
+    db_xy, db_off, q_xy, q_off = build_workload(ts, args.n_db, args.n_q, 64, 256)
+    db = ts.TrajSet(db_xy, db_off)
+    qs = ts.TrajSet(q_xy, q_off)
+    log("workload ready in %.1fs: %d DB trajectories (%d points), %d queries" %
+        (time.time() - t0, db.count, len(db_xy), qs.count))
+    Q, N = qs.count, db.count
+    eng = ts.Engine()
+    p_dtw = ts.params("dtw", mode=ts.EXACT, k=10)
+    p_fr = ts.params("frechet", mode=ts.EXACT, k=10)
+
+    # Timed region: K batched DTW steps with per-kernel HIP events.
+    for _ in range(args.warmup):
+        eng.knn(db, qs, p_dtw)
+    eng.set_timing(True)
+    eng.reset_stats()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        idx, val, st = eng.knn(db, qs, p_dtw)
+    ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    kstats = eng.kernel_stats()
+    eng.set_timing(False)
+    value = Q * N / (ms_step * 1e-3)
+    cells = st["cells"]
+    log("DTW batched: %.3f ms/step, %.3e pairs/s, %.3e cells/step" % (ms_step, value, cells))
+
+    pmc = None
+    pmc_path = os.path.join(ROOT, "profiles", "pmc_traffic_c2.json")
+    if os.path.exists(pmc_path):
+        pmc = json.load(open(pmc_path))
+    roof = roofline_of(kstats, pmc)
+    extra = {"device": dev, "dtw_ms_per_step": round(ms_step, 4), "dtw_cells_per_step": cells,
+             "dtw_cell_updates_per_s": cells / (ms_step * 1e-3),
+             "dtw_device_ms": round(st["ms_device"], 4), "launches_per_step": st["launches"],
+             "kernels": {k: {"launches": v["launches"], "ms_mean": round(v["ms"] / max(v["launches"], 1), 5)}
+                         for k, v in kstats.items()}}
+
+    if not args.quick:
+        ms_fr, (fi, fv, fst) = timed(lambda: eng.knn(db, qs, p_fr), args.steps, 1)
+        extra.update({"frechet_pairs_per_s": Q * N / (ms_fr * 1e-3), "frechet_ms_per_step": round(ms_fr, 4),
+                      "frechet_cell_updates_per_s": fst["cells"] / (ms_fr * 1e-3)})
+        ms_plain, (pi, pv, _) = timed(lambda: eng.knn_plain(db, qs, p_dtw), max(1, args.steps // 3), 1)
+        same = bool(np.array_equal(pi, idx) and np.array_equal(pv, val))
+        extra.update({"dtw_plain_pairs_per_s": Q * N / (ms_plain * 1e-3), "dtw_plain_ms_per_step": round(ms_plain, 4),
+                      "batched_vs_plain": round(ms_plain / ms_step, 3), "plain_equals_batched": same})
+        # Per-query latency of a single-query request on the resident DB.
+        lat = []
+        singles = [ts.TrajSet(q_xy[q_off[i]:q_off[i + 1]], np.array([0, q_off[i + 1] - q_off[i]], np.int64))
+                   for i in range(Q)]
+        for s in singles[:4]:
+            eng.knn(db, s, p_dtw)
+        for s in singles:
+            t1 = time.perf_counter()
+            eng.knn(db, s, p_dtw)
+            lat.append((time.perf_counter() - t1) * 1e3)
+        extra.update({"query_latency_ms_p50": round(float(np.percentile(lat, 50)), 4),
+                      "query_latency_ms_p99": round(float(np.percentile(lat, 99)), 4),
+                      "latency_note": "single-query DTW top-10 calls, serial, DB resident"})
+        log("frechet %.3f ms, plain %.3f ms, p50 %.3f ms" % (ms_fr, ms_plain, extra["query_latency_ms_p50"]))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu, ci, cv = cpu_baseline(db_xy, db_off, q_xy, q_off, args.cpu_sample)
+        extra["cpu_matches_gpu"] = bool(np.array_equal(ci, idx[:args.cpu_sample]) and
+                                        np.array_equal(cv, val[:args.cpu_sample]))
+        log("cpu baseline", cpu)
+
+    line = {"metric": METRIC, "value": round(value, 1), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "DTW top-10 k-NN, %d query random walks vs %d-trajectory DB (lengths U[64,256], "
+                                   "sigma 1e-3), bounding-box/endpoint lower-bound filter, batched" % (Q, N),
+                       "queries": Q, "db_trajectories": N, "k": 10, "measure": "dtw", "mode": "exact"},
+            "roofline": roof, "cpu_baseline": cpu, "extra": extra}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -60,8 +60,8 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
   const int nwaves = blockDim.x >> 6;
   const int g = lane / W, lg = lane & (W - 1);
   extern __shared__ int lds_dyn[];
-  volatile int* slot_item = lds_dyn + wv * G + g;
-  volatile float* bnd = (volatile float*)(lds_dyn + ((nwaves * G + 3) & ~3)) + (size_t)(wv * G + g) * a.cmax;
+  int* slot_item = lds_dyn + wv * G + g;
+  float* bnd = (float*)(lds_dyn + ((nwaves * G + 3) & ~3)) + (size_t)(wv * G + g) * a.cmax;
   const int cmax = a.cmax;
 
   int64_t n_items = a.max_items;
@@ -75,8 +75,17 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
   float out_prev = 0.0f, up_prev = 0.0f;
   int64_t bd = 0, bw = 0;
   bool live = false;  // current pair is a real pair (not a skipped slot)
+  // Column operands: (cbx, cby) for the current step, (nbx, nby) for the
+  // next one; loads are issued two steps before their use.
+  float cbx = Bx[0], cby = By[0], nbx = cbx, nby = cby, pbv = 0.0f;
+  // Point of the row this lane starts next (loaded one row ahead, so no load
+  // result is needed in the step that issues it).
+  float nax = 0.0f, nay = 0.0f;
 
   for (;;) {
+    // LDS is shared by the lanes of this wave only; the barrier keeps the
+    // compiler from moving halo/slot accesses across steps.
+    __builtin_amdgcn_wave_barrier();
     // ---- next pair (lane 0 of the group fetches, the others follow it)
     if (!done && col == 0 && row == lg) {
       int it_idx;
@@ -106,24 +115,30 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
         Cp = max(C, W);
         SW = ((R + W - 1) / W) * W;
         if (BAND) bw = (int64_t)a.band * R;
+        cbx = Bx[0];
+        cby = By[0];
+        nbx = Bx[min(1, C - 1)];
+        nby = By[min(1, C - 1)];
+        nax = Ax[min(lg, R - 1)];
+        nay = Ay[min(lg, R - 1)];
       }
     }
     if (__all(done)) break;
 
     // ---- start of a row
     if (col == 0 && !done) {
-      const int rr = min(row, R - 1);
-      ax = Ax[rr];
-      ay = Ay[rr];
+      ax = nax;
+      ay = nay;
+      const int nr = min(row + W, R - 1);
+      nax = Ax[nr];
+      nay = Ay[nr];
       if (BAND) bd = (int64_t)row * C;
     }
 
     // ---- one cell
-    const int cc = min(max(col, 0), C - 1);
-    const float bx = Bx[cc], by = By[cc];
+    const float bx = cbx, by = cby;
     const float up_sh = shift_from_prev_lane(out_prev);
-    const float bval = bnd[min(max(col, 0), cmax - 1)];
-    const float up = (lg == 0) ? ((row == 0) ? BND : bval) : up_sh;
+    const float up = (lg == 0) ? ((row == 0) ? BND : pbv) : up_sh;
     const float diag = (col == 0) ? ((row == 0) ? 0.0f : BND) : up_prev;
     const float left = (col == 0) ? BND : out_prev;
     const float dx = ax - bx, dy = ay - by;
@@ -163,6 +178,19 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
       row += W;
       if (row >= SW) row = lg;
     }
+    // ---- operands of the coming steps. Within a pair the column point of
+    // a step is B[col]; a new pair reloads both stages after its fetch. The
+    // halo value of column c was written by lane W-1 at least one step ago
+    // (columns >= W) or earlier in this step, which precedes this read in
+    // program order.
+    int col2 = col + 1;
+    if (col2 == Cp) col2 = 0;
+    const int ncc = min(max(col2, 0), C - 1);
+    cbx = nbx;
+    cby = nby;
+    nbx = Bx[ncc];
+    nby = By[ncc];
+    pbv = bnd[min(max(col, 0), cmax - 1)];
   }
 }
 

@@ -250,7 +250,7 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
     const int32_t cmax = std::max(dp_lanes_for_class(cls_lo), hi);
     const char* nm = p.measure == TS_DTW ? "dtw_wavefront" : (p.measure == TS_FRECHET ? "frechet_wavefront" : "lcss_wavefront");
     return run(e, nm, 0, 0, [&] {
-      return launch_dp(e->st, ma, cls_lo, qv, dv, items, nitems_dev, max_items, ctr, vals, cmax, cus * 8);
+      return launch_dp(e->st, ma, cls_lo, qv, dv, items, nitems_dev, max_items, ctr, vals, cmax, cus * 32);
     }, slot);
   }
   TS_TRY(ensure(e, B_ITEMS3, 3 * max_items, &items3));
@@ -264,7 +264,7 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
     const char* nm = p.measure == TS_DTW ? "dtw_wavefront" : (p.measure == TS_FRECHET ? "frechet_wavefront" : "lcss_wavefront");
     TS_TRY(run(e, nm, 0, 0, [&] {
       return launch_dp(e->st, ma, c, qv, dv, items3 + (int64_t)c * max_items, ctr + 4 + c, max_items, ctr + c,
-                       vals, cmax, cus * 8);
+                       vals, cmax, cus * 32);
     }, slot));
   }
   return TS_OK;
