@@ -7,8 +7,8 @@ silently run without the native library.
 
 Measures and modes mirror the reference's entry points
 (1li2yang3/CudaRuntime1: dtw.cuh, frechet.cuh, lcss.cuh, hausdorff.cuh,
-euclidean.cuh): see `knn`, and `reference_api` for the 13 original
-signatures with numpy arrays.
+euclidean.cuh); the 13 original C++ signatures themselves are exported by
+libtrajsim.so (include/trajsim_compat.hpp).
 """
 import ctypes
 import os
@@ -266,6 +266,55 @@ class Engine:
     def close(self):
         if self._h:
             _lib.ts_engine_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching k-NN server over a resident database (ts_server_*)."""
+
+    def __init__(self, db, streams=4, max_batch=256, max_queue=65536, mem_reserve=256 << 20, fake_free_mem=0,
+                 batch_wait_us=200):
+        cfg = ServerConfig()
+        _lib.ts_server_config_default(ctypes.byref(cfg))
+        cfg.streams, cfg.max_batch, cfg.max_queue = streams, max_batch, max_queue
+        cfg.mem_reserve, cfg.fake_free_mem, cfg.batch_wait_us = mem_reserve, fake_free_mem, batch_wait_us
+        self._db = db  # keep the set alive while the server runs
+        self._h = ctypes.c_void_p()
+        _check(_lib.ts_server_create(db.handle, ctypes.byref(cfg), ctypes.byref(self._h)), "ts_server_create")
+
+    def submit(self, q, p):
+        """Queue one query (float32 [len,2]); returns its ticket. Raises TrajsimError
+        (status TS_ERR_BUSY = 6) when admission refuses it."""
+        q = np.ascontiguousarray(q, np.float32).reshape(-1, 2)
+        t = ctypes.c_uint64()
+        _check(_lib.ts_server_submit(self._h, _ptr(q), len(q), ctypes.byref(p), ctypes.byref(t)), "ts_server_submit")
+        return t.value
+
+    def wait(self, ticket, k, timeout_ms=-1):
+        idx = np.empty(k, np.int32)
+        val = np.empty(k, np.float32)
+        lat = ctypes.c_double()
+        _check(_lib.ts_server_wait(self._h, ticket, timeout_ms, _ptr(idx), _ptr(val), ctypes.byref(lat)),
+               "ts_server_wait")
+        return idx, val, lat.value
+
+    def cancel(self, ticket):
+        _check(_lib.ts_server_cancel(self._h, ticket), "ts_server_cancel")
+
+    def counters(self):
+        c = np.zeros(5, np.int64)
+        _check(_lib.ts_server_counters(self._h, _ptr(c)), "ts_server_counters")
+        return dict(zip(["admitted", "refused", "completed", "batches", "cancelled"], c.tolist()))
+
+    def close(self):
+        if self._h:
+            _lib.ts_server_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

@@ -94,6 +94,60 @@ static inline uint64_t ctr_hash(uint64_t seed, uint64_t a, uint64_t b) {
 // Uniform in [0,1) with 53 random bits.
 static inline double u01(uint64_t h) { return (double)(h >> 11) * (1.0 / 9007199254740992.0); }
 
+ts_status set_load(ts_set* s, const ts_point* xy, const int64_t* offsets, int32_t count, hipStream_t st) {
+  if (!offsets || !xy || count <= 0) return fail(TS_ERR_INVALID, "set: need count > 0, offsets and points");
+  if (offsets[0] != 0) return fail(TS_ERR_INVALID, "set: offsets[0] must be 0");
+  std::vector<int32_t> lens(count);
+  int32_t mn = INT32_MAX, mx = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    const int64_t l = offsets[i + 1] - offsets[i];
+    if (l < 1 || l > (1 << 24))
+      return fail(TS_ERR_INVALID, "set: trajectory " + std::to_string(i) + " has length " + std::to_string(l) +
+                                      " (need 1..2^24)");
+    lens[i] = (int32_t)l;
+    mn = std::min(mn, lens[i]);
+    mx = std::max(mx, lens[i]);
+  }
+  const int64_t P = offsets[count];
+  if (P > s->cap_points || count > s->cap_count) {
+    if (st) TS_HIP(hipStreamSynchronize(st));
+    (void)hipFree(s->x); (void)hipFree(s->y); (void)hipFree(s->off); (void)hipFree(s->len);
+    (void)hipFree(s->mbr); (void)hipFree(s->ends);
+    s->x = s->y = nullptr; s->off = nullptr; s->len = nullptr; s->mbr = s->ends = nullptr;
+    s->bytes = 0; s->cap_points = 0; s->cap_count = 0;
+    hipError_t e;
+#define TS_ALLOC(ptr, nbytes)                                                        \
+    e = hipMalloc((void**)&(ptr), (nbytes));                                         \
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(set)", __FILE__, __LINE__);    \
+    s->bytes += (int64_t)(nbytes);
+    TS_ALLOC(s->x, P * sizeof(float));
+    TS_ALLOC(s->y, P * sizeof(float));
+    TS_ALLOC(s->off, (count + 1) * sizeof(int64_t));
+    TS_ALLOC(s->len, count * sizeof(int32_t));
+    TS_ALLOC(s->mbr, count * sizeof(float4));
+    TS_ALLOC(s->ends, count * sizeof(float4));
+#undef TS_ALLOC
+    s->cap_points = P;
+    s->cap_count = count;
+  }
+  s->count = count;
+  s->points = P;
+  s->min_len = mn;
+  s->max_len = mx;
+  s->h_off.assign(offsets, offsets + count + 1);
+  s->h_len = lens;
+  std::vector<float> hx(P), hy(P);
+#pragma omp parallel for schedule(static) if (P > (1 << 16))
+  for (int64_t i = 0; i < P; ++i) { hx[i] = xy[i].x; hy[i] = xy[i].y; }
+  TS_HIP(hipMemcpyAsync(s->x, hx.data(), P * sizeof(float), hipMemcpyHostToDevice, st));
+  TS_HIP(hipMemcpyAsync(s->y, hy.data(), P * sizeof(float), hipMemcpyHostToDevice, st));
+  TS_HIP(hipMemcpyAsync(s->off, offsets, (count + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  TS_HIP(hipMemcpyAsync(s->len, s->h_len.data(), count * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  TS_HIP(launch_summaries(st, s->view(), s->mbr, s->ends));
+  TS_HIP(hipStreamSynchronize(st));
+  return TS_OK;
+}
+
 }  // namespace ts
 
 using namespace ts;
@@ -152,58 +206,15 @@ ts_status ts_device_name(char* buf, int32_t buflen) {
 }
 
 ts_status ts_set_create(const ts_point* xy, const int64_t* offsets, int32_t count, ts_set** out) {
-  if (!out || !offsets || count <= 0) return fail(TS_ERR_INVALID, "ts_set_create: need count > 0 and offsets");
+  if (!out) return fail(TS_ERR_INVALID, "ts_set_create: null out");
   *out = nullptr;
-  if (offsets[0] != 0) return fail(TS_ERR_INVALID, "ts_set_create: offsets[0] must be 0");
   auto s = new ts_set();
-  s->count = count;
-  s->h_off.assign(offsets, offsets + count + 1);
-  s->h_len.resize(count);
-  int32_t mn = INT32_MAX, mx = 0;
-  for (int32_t i = 0; i < count; ++i) {
-    int64_t l = offsets[i + 1] - offsets[i];
-    if (l < 1 || l > (1 << 24)) {
-      delete s;
-      return fail(TS_ERR_INVALID, "ts_set_create: trajectory " + std::to_string(i) + " has length " +
-                                      std::to_string(l) + " (need 1..2^24)");
-    }
-    s->h_len[i] = (int32_t)l;
-    mn = std::min<int32_t>(mn, (int32_t)l);
-    mx = std::max<int32_t>(mx, (int32_t)l);
-  }
-  s->min_len = mn;
-  s->max_len = mx;
-  s->points = offsets[count];
-  if (!xy) { delete s; return fail(TS_ERR_INVALID, "ts_set_create: null points"); }
   (void)hipGetDevice(&s->device);
-  const int64_t P = s->points;
-  std::vector<float> hx(P), hy(P);
-#pragma omp parallel for schedule(static)
-  for (int64_t i = 0; i < P; ++i) { hx[i] = xy[i].x; hy[i] = xy[i].y; }
-  auto cleanup = [&](ts_status st) {
+  ts_status st = set_load(s, xy, offsets, count, nullptr);
+  if (st != TS_OK) {
     ts_set_destroy(s);
     return st;
-  };
-  hipError_t e;
-#define TS_ALLOC(ptr, nbytes)                                                    \
-  e = hipMalloc((void**)&(ptr), (nbytes));                                       \
-  if (e != hipSuccess) return cleanup(hip_fail(e, "hipMalloc(set)", __FILE__, __LINE__)); \
-  s->bytes += (int64_t)(nbytes);
-  TS_ALLOC(s->x, P * sizeof(float));
-  TS_ALLOC(s->y, P * sizeof(float));
-  TS_ALLOC(s->off, (count + 1) * sizeof(int64_t));
-  TS_ALLOC(s->len, count * sizeof(int32_t));
-  TS_ALLOC(s->mbr, count * sizeof(float4));
-  TS_ALLOC(s->ends, count * sizeof(float4));
-#undef TS_ALLOC
-  if ((e = hipMemcpy(s->x, hx.data(), P * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(s->y, hy.data(), P * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(s->off, offsets, (count + 1) * sizeof(int64_t), hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(s->len, s->h_len.data(), count * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess)
-    return cleanup(hip_fail(e, "hipMemcpy(set)", __FILE__, __LINE__));
-  if ((e = launch_summaries(nullptr, s->view(), s->mbr, s->ends)) != hipSuccess ||
-      (e = hipDeviceSynchronize()) != hipSuccess)
-    return cleanup(hip_fail(e, "summaries", __FILE__, __LINE__));
+  }
   *out = s;
   return TS_OK;
 }

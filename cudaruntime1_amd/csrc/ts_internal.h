@@ -70,6 +70,9 @@ struct ts_set {
   std::vector<int64_t> h_off;
   std::vector<int32_t> h_len;
   int64_t bytes = 0;
+  // Capacity of the device arrays (sets reloaded in place by the server).
+  int64_t cap_points = 0;
+  int32_t cap_count = 0;
   ts::SetView view() const { return {x, y, off, len, mbr, ends, count}; }
 };
 
@@ -80,6 +83,9 @@ namespace ts {
 
 // set.hip
 hipError_t launch_summaries(hipStream_t st, SetView s, float4* mbr, float4* ends);
+// (Re)load a set in place on `st`: grows the device arrays only when the new
+// contents do not fit, so a warmed-up set reloads without hipMalloc/hipFree.
+ts_status set_load(ts_set* s, const ts_point* xy, const int64_t* offsets, int32_t count, hipStream_t st);
 
 // filter.hip
 hipError_t launch_envelope(hipStream_t st, SetView q, int32_t max_len, int32_t r, float4* env);
