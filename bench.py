@@ -31,8 +31,8 @@ DB_SEED, Q_SEED = 0xDB5EED, 0x9E3779B9
 VALU_PEAK_TOPS = 39.3
 HBM_PEAK_GBS = 8000.0
 # Source-level f32 operations per unit of work (DP cell or pair).
-OPS = {"dtw_wavefront": 9.0, "frechet_wavefront": 9.0, "lcss_wavefront": 7.0, "bound_seed": 24.0,
-       "bound_compact": 24.0}
+OPS = {"dtw_wavefront": 9.0, "frechet_wavefront": 9.0, "lcss_wavefront": 7.0, "dtw_wavefront_seed": 9.0,
+       "frechet_wavefront_seed": 9.0, "lcss_wavefront_seed": 7.0, "bound_seed": 24.0, "bound_compact": 24.0}
 
 
 def log(*a):

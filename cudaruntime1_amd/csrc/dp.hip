@@ -13,22 +13,32 @@
 //
 // Execution model (no counterpart in the reference, whose kernels use one
 // block per pair and two block barriers per anti-diagonal, dtw.cu:66-93):
-//  * A wave64 is split into 64/W lane groups (W = 64, 32 or 16). Each group
-//    runs one pair at a time; lane l of the group owns DP rows l, l+W, ...
-//    and sweeps its row left to right one cell per step, one step behind
-//    lane l-1, so the anti-diagonal lives in registers.
-//  * The cell above arrives from lane l-1 by a DPP wave shift (v_mov_dpp
-//    wave_shr:1) of the value it produced on the previous step; the diagonal
-//    is the value that arrived on the step before. Lane 0 of a group reads
-//    the row above from an LDS row written by lane W-1 during the previous
-//    strip of W rows (the pair's "halo" row).
-//  * Rows run along the longer trajectory and columns along the shorter, so
-//    the halo delay (columns - W + 1 steps) is positive; pairs shorter than
-//    W columns are padded with idle columns.
+//  * A wave64 is split into 64/W lane groups (W = 16, 32 or 64).
+//    Each group runs one pair at a time; lane l of the group owns DP rows
+//    l, l+W, ... along the longer trajectory and sweeps its row U cells per
+//    step, one step behind lane l-1, so the anti-diagonal lives in registers.
+//  * The U cells above arrive from lane l-1 by DPP shifts (row_shr:1 for
+//    W = 16, wave_shr:1 otherwise) of the U values it produced on the
+//    previous step; the first cell's diagonal is the last value that arrived
+//    on the step before. Lane 0 of a group reads the row above from an LDS
+//    halo row written by lane W-1 during the previous strip of W rows.
+//  * Columns run along the shorter trajectory, U per step. When a group's
+//    lane 0 starts a pair, the group copies that pair's column trajectory
+//    into one of two LDS buffers (sets pad every trajectory to a multiple of
+//    4 points, so the copy is aligned 16-byte vectors); per-step operands
+//    are then LDS reads. Rows of 1024+ columns read global memory instead.
+//    A row has at least W chunks so the halo is always written at least one
+//    step before it is read; shorter rows are padded with idle chunks.
 //  * Strips and pairs follow each other without draining the pipeline: after
 //    its last row a lane starts its first row of the next pair, which the
 //    group's lane 0 fetched from a device-wide atomic work counter. The
 //    kernel is persistent: it drains the whole work list in one launch.
+//  * Work lists are split into buckets by the shorter length C. Each bucket
+//    uses the widest group that needs no idle chunks (C/U >= W), which
+//    minimises a pair's latency (R*C/(W*U) steps) when pairs are few, and
+//    its own halo length, so short pairs do not pay for long rows' LDS.
+#include <algorithm>
+
 #include "ts_internal.h"
 
 namespace ts {
@@ -46,12 +56,44 @@ struct DpArgs {
   float eps;
 };
 
-// Value of lane-1 (lane 0 receives 0; it is overridden by the caller).
+// Value of the previous lane (lane 0 receives 0). Lane 0 of every group
+// overrides what it receives, so one wave shift serves every group width;
+// W = 16 uses row_shr:1, whose 16-lane rows are exactly the groups.
+template <int W>
 __device__ __forceinline__ float shift_from_prev_lane(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138 /* wave_shr:1 */, 0xf, 0xf, true));
+  constexpr int ctrl = (W == 16) ? 0x111 /* row_shr:1 */ : 0x138 /* wave_shr:1 */;
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xf, 0xf, true));
 }
 
-template <int MEAS, int W, bool BAND, bool DELTA>
+template <int U>
+__device__ __forceinline__ void load_vec(const float* p, float (&v)[U]) {
+  if constexpr (U == 4) {
+    const float4 t = *(const float4*)p;
+    v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+  } else if constexpr (U == 2) {
+    const float2 t = *(const float2*)p;
+    v[0] = t.x; v[1] = t.y;
+  } else {
+    v[0] = *p;
+  }
+}
+
+template <int U>
+__device__ __forceinline__ void store_vec(float* p, const float (&v)[U]) {
+  if constexpr (U == 4) {
+    *(float4*)p = make_float4(v[0], v[1], v[2], v[3]);
+  } else if constexpr (U == 2) {
+    *(float2*)p = make_float2(v[0], v[1]);
+  } else {
+    *p = v[0];
+  }
+}
+
+// LDS per lane group, in floats: the halo row, and with STAGE two buffers
+// each of the column trajectory's x and y (double-buffered per pair).
+__host__ __device__ constexpr int dp_group_floats(int cmax, bool stage) { return stage ? 5 * cmax : cmax; }
+
+template <int MEAS, int W, int U, bool BAND, bool DELTA, bool STAGE>
 __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
   constexpr int G = 64 / W;
   constexpr float BND = (MEAS == TS_LCSS) ? 0.0f : kInf;
@@ -59,42 +101,66 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
   const int wv = threadIdx.x >> 6;
   const int nwaves = blockDim.x >> 6;
   const int g = lane / W, lg = lane & (W - 1);
-  extern __shared__ int lds_dyn[];
-  int* slot_item = lds_dyn + wv * G + g;
-  float* bnd = (float*)(lds_dyn + ((nwaves * G + 3) & ~3)) + (size_t)(wv * G + g) * a.cmax;
   const int cmax = a.cmax;
+  extern __shared__ float4 lds_dyn4[];
+  int* slot_item = (int*)lds_dyn4 + wv * G + g;
+  // Group areas start on 16-byte boundaries (cmax is a multiple of 4).
+  float* gbase = (float*)lds_dyn4 + (((nwaves * G + 3) >> 2) << 2) +
+                 (size_t)(wv * G + g) * dp_group_floats(cmax, STAGE);
+  float* bnd = gbase;
+  float* sbx = gbase + cmax;      // [2][cmax]
+  float* sby = gbase + 3 * cmax;  // [2][cmax]
 
   int64_t n_items = a.max_items;
   if (a.nitems) n_items = min<int64_t>(n_items, (int64_t)*a.nitems);
 
-  int row = lg, col = -lg;
-  bool done = false;
-  int R = 1, C = 1, Cp = W, SW = W, slot = 0;
+  // rem: steps left in this lane's current pair (its first pair starts after
+  // lg idle steps). Lane 0 of the group starts each pair exactly lg steps
+  // before this lane does, i.e. on the step where rem == lg.
+  int row = lg, ch = -lg, rem = lg;
+  bool done = false, live = false;
+  int R = 1, C = 1, Cc = W, SW = W, slot = 0, lastch = 0;
+  int par = 0, gpar = 0;  // buffer of this lane's pair / of the group's next pair
   const float *Ax = a.q.x, *Ay = a.q.y, *Bx = a.q.x, *By = a.q.y;
-  float ax = 0.0f, ay = 0.0f;
-  float out_prev = 0.0f, up_prev = 0.0f;
-  int64_t bd = 0, bw = 0;
-  bool live = false;  // current pair is a real pair (not a skipped slot)
-  // Column operands: (cbx, cby) for the current step, (nbx, nby) for the
-  // next one; loads are issued two steps before their use.
-  float cbx = Bx[0], cby = By[0], nbx = cbx, nby = cby, pbv = 0.0f;
-  // Point of the row this lane starts next (loaded one row ahead, so no load
-  // result is needed in the step that issues it).
-  float nax = 0.0f, nay = 0.0f;
+  float ax = 0.0f, ay = 0.0f, nax = 0.0f, nay = 0.0f;
+  int bd = 0, bw = 0;
+  float o[U], cx[U], cy[U], hv[U];
+#pragma unroll
+  for (int j = 0; j < U; ++j) { o[j] = 0.0f; hv[j] = 0.0f; cx[j] = 0.0f; cy[j] = 0.0f; }
+  float up_last = 0.0f;
 
   for (;;) {
     // LDS is shared by the lanes of this wave only; the barrier keeps the
-    // compiler from moving halo/slot accesses across steps.
+    // compiler from moving LDS accesses across steps.
     __builtin_amdgcn_wave_barrier();
-    // ---- next pair (lane 0 of the group fetches, the others follow it)
-    if (!done && col == 0 && row == lg) {
-      int it_idx;
-      if (lg == 0) {
-        it_idx = atomicAdd(a.next, 1);
-        *slot_item = it_idx;
-      } else {
-        it_idx = *slot_item;
+    // ---- the group's lane 0 starts a pair: fetch it, and with STAGE load its
+    // column trajectory into the group's spare buffer. Lanes still on the
+    // previous pair read the other buffer; the one written here was last
+    // used two pairs ago, which every lane has finished (a pair lasts at
+    // least W steps and lanes trail lane 0 by at most W-1).
+    if (!done && rem == lg) {
+      if (lg == 0) *slot_item = atomicAdd(a.next, 1);
+      __builtin_amdgcn_wave_barrier();
+      const int it_idx = *slot_item;
+      if (STAGE && it_idx < n_items) {
+        const Item it = a.items[it_idx];
+        if (it.d >= 0) {
+          const int n = a.q.len[it.q], m = a.db.len[it.d];
+          const float *gx, *gy;
+          if (n >= m) { gx = a.db.x + a.db.off[it.d]; gy = a.db.y + a.db.off[it.d]; }
+          else { gx = a.q.x + a.q.off[it.q]; gy = a.q.y + a.q.off[it.q]; }
+          const int c4 = (min(n, m) + 3) >> 2;
+          for (int c = lg; c < c4; c += W) {
+            *(float4*)(sbx + gpar * cmax + 4 * c) = *(const float4*)(gx + 4 * c);
+            *(float4*)(sby + gpar * cmax + 4 * c) = *(const float4*)(gy + 4 * c);
+          }
+        }
       }
+      gpar ^= 1;
+    }
+    // ---- this lane starts the pair its group's lane 0 fetched lg steps ago
+    if (!done && rem == 0) {
+      const int it_idx = *slot_item;
       if (it_idx >= n_items) {
         done = true;
       } else {
@@ -112,126 +178,167 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
         } else {
           R = 1; C = 1;
         }
-        Cp = max(C, W);
+        par = gpar ^ 1;
+        lastch = (C - 1) / U;
+        Cc = max(lastch + 1, W);
         SW = ((R + W - 1) / W) * W;
-        if (BAND) bw = (int64_t)a.band * R;
-        cbx = Bx[0];
-        cby = By[0];
-        nbx = Bx[min(1, C - 1)];
-        nby = By[min(1, C - 1)];
+        rem = (SW / W) * Cc;
+        row = lg;
+        ch = 0;
+        if (BAND) bw = a.band * R;
+        if (STAGE) {
+          load_vec<U>(sbx + par * cmax, cx);
+          load_vec<U>(sby + par * cmax, cy);
+        } else {
+          load_vec<U>(Bx, cx);
+          load_vec<U>(By, cy);
+        }
         nax = Ax[min(lg, R - 1)];
         nay = Ay[min(lg, R - 1)];
       }
     }
     if (__all(done)) break;
 
-    // ---- start of a row
-    if (col == 0 && !done) {
+    // ---- start of a row (its point was loaded one row ahead)
+    if (ch == 0 && !done) {
       ax = nax;
       ay = nay;
       const int nr = min(row + W, R - 1);
       nax = Ax[nr];
       nay = Ay[nr];
-      if (BAND) bd = (int64_t)row * C;
+      if (BAND) bd = row * C;
     }
 
-    // ---- one cell
-    const float bx = cbx, by = cby;
-    const float up_sh = shift_from_prev_lane(out_prev);
-    const float up = (lg == 0) ? ((row == 0) ? BND : pbv) : up_sh;
-    const float diag = (col == 0) ? ((row == 0) ? 0.0f : BND) : up_prev;
-    const float left = (col == 0) ? BND : out_prev;
-    const float dx = ax - bx, dy = ay - by;
-    float o;
-    if (MEAS == TS_DTW) {
-      const float cost = sqrtf(dx * dx + dy * dy);
-      o = cost + fminf(fminf(up, left), diag);
-      if (BAND) {
-        const int64_t ad = bd < 0 ? -bd : bd;
-        if (ad > bw) o = kInf;
-      }
-    } else if (MEAS == TS_FRECHET) {
-      const float d2 = dx * dx + dy * dy;
-      o = fmaxf(d2, fminf(fminf(up, left), diag));
-    } else {
-      bool match = fabsf(dx) < a.eps && fabsf(dy) < a.eps;
-      if (DELTA) match = match && abs(row - col) <= a.delta;
-      o = match ? diag + 1.0f : fmaxf(up, left);
+    // ---- U cells
+    float up[U], on[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const float sh = shift_from_prev_lane<W>(o[j]);
+      up[j] = (lg == 0) ? ((row == 0) ? BND : hv[j]) : sh;
     }
+    float d = (ch == 0) ? ((row == 0) ? 0.0f : BND) : up_last;
+    float l = (ch == 0) ? BND : o[U - 1];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const float dx = ax - cx[j], dy = ay - cy[j];
+      float v;
+      if (MEAS == TS_DTW) {
+        v = sqrtf(dx * dx + dy * dy) + fminf(fminf(up[j], l), d);
+        if (BAND) {
+          const int bj = bd - j * R;
+          if ((bj < 0 ? -bj : bj) > bw) v = kInf;
+        }
+      } else if (MEAS == TS_FRECHET) {
+        v = fmaxf(dx * dx + dy * dy, fminf(fminf(up[j], l), d));
+      } else {
+        bool match = fabsf(dx) < a.eps && fabsf(dy) < a.eps;
+        if (DELTA) {
+          const int dd = row - (ch * U + j);
+          match = match && (dd < 0 ? -dd : dd) <= a.delta;
+        }
+        v = match ? d + 1.0f : fmaxf(up[j], l);
+      }
+      on[j] = v;
+      d = up[j];
+      l = v;
+    }
+    up_last = up[U - 1];
+#pragma unroll
+    for (int j = 0; j < U; ++j) o[j] = on[j];
 
     // ---- halo row for the next strip, and the pair's result
-    if (lg == W - 1 && col >= 0 && !done) bnd[min(col, cmax - 1)] = o;
-    if (!done && live && row == R - 1 && col == C - 1) {
-      float v = o;
-      if (MEAS == TS_FRECHET) v = sqrtf(o);
-      if (MEAS == TS_LCSS) v = o / (float)C;
+    if (lg == W - 1 && ch >= 0 && !done) store_vec<U>(bnd + min(ch, Cc - 1) * U, on);
+    if (!done && live && row == R - 1 && ch == lastch) {
+      const int rem_c = (C - 1) - ch * U;
+      float v = on[0];
+#pragma unroll
+      for (int j = 1; j < U; ++j)
+        if (rem_c == j) v = on[j];
+      if (MEAS == TS_FRECHET) v = sqrtf(v);
+      if (MEAS == TS_LCSS) v = v / (float)C;
       a.out[slot] = v;
     }
 
     // ---- advance
-    up_prev = up;
-    out_prev = o;
-    ++col;
-    if (BAND) bd -= R;
-    if (col == Cp) {
-      col = 0;
+    --rem;
+    ++ch;
+    if (BAND) bd -= U * R;
+    if (ch == Cc) {
+      ch = 0;
       row += W;
-      if (row >= SW) row = lg;
     }
-    // ---- operands of the coming steps. Within a pair the column point of
-    // a step is B[col]; a new pair reloads both stages after its fetch. The
-    // halo value of column c was written by lane W-1 at least one step ago
-    // (columns >= W) or earlier in this step, which precedes this read in
-    // program order.
-    int col2 = col + 1;
-    if (col2 == Cp) col2 = 0;
-    const int ncc = min(max(col2, 0), C - 1);
-    cbx = nbx;
-    cby = nby;
-    nbx = Bx[ncc];
-    nby = By[ncc];
-    pbv = bnd[min(max(col, 0), cmax - 1)];
+    // ---- operands of the next step: columns of chunk ch (from the staged
+    // buffer, or from global memory for the longest rows), and the halo
+    // chunk ch, written by lane W-1 at least one step ago (a row has at
+    // least W chunks) or earlier in this step, which precedes this read.
+    const int lc = U * min(max(ch, 0), lastch);
+    if (STAGE) {
+      load_vec<U>(sbx + par * cmax + lc, cx);
+      load_vec<U>(sby + par * cmax + lc, cy);
+    } else {
+      load_vec<U>(Bx + lc, cx);
+      load_vec<U>(By + lc, cy);
+    }
+    load_vec<U>(bnd + min(max(ch, 0), Cc - 1) * U, hv);
   }
 }
 
-int dp_lanes_for_class(int cls) { return cls == 0 ? 64 : (cls == 1 ? 32 : 16); }
+// Buckets by the pair's shorter length C (dp_bucket_of): lanes per pair W,
+// cells per lane step U, largest C. A row has max(ceil(C/U), W) chunks.
+struct Bucket {
+  int W, U, cmax_cap;
+};
+constexpr Bucket kBuckets[kDpBuckets] = {{16, 1, 31},  {16, 2, 63},   {16, 4, 127},
+                                          {32, 4, 255}, {64, 4, 1023}, {64, 4, 1 << 24}};
 
-template <int MEAS, int W, bool BAND, bool DELTA>
+// Halo length (floats) a bucket needs when the longest C in it is cmax_c.
+int32_t dp_halo_len(int b, int32_t cmax_c) {
+  const Bucket& k = kBuckets[b];
+  const int32_t c = std::min<int32_t>(cmax_c, k.cmax_cap);
+  const int32_t chunks = std::max((c + k.U - 1) / k.U, k.W);
+  return ((chunks * k.U + 3) / 4) * 4;
+}
+
+template <int MEAS, int W, int U, bool BAND, bool DELTA, bool STAGE>
 static hipError_t launch_dp_t(hipStream_t st, const DpArgs& a, int grid_waves) {
   constexpr int G = 64 / W;
-  // Waves per block bounded so that the halo rows fit in 64 KiB of LDS.
-  const size_t per_wave = (size_t)G * a.cmax * sizeof(float) + G * sizeof(int);
+  // Waves per block bounded so that the group areas fit in 64 KiB of LDS.
+  const size_t per_wave = (size_t)G * dp_group_floats(a.cmax, STAGE) * sizeof(float) + G * sizeof(int);
   int wpb = 4;
   while (wpb > 1 && per_wave * wpb + 64 > 65536) --wpb;
   if (per_wave + 64 > 65536) return hipErrorInvalidValue;
-  const size_t lds = (size_t)((wpb * G + 3) & ~3) * sizeof(int) + (size_t)wpb * G * a.cmax * sizeof(float);
+  const size_t lds = (size_t)(((wpb * G + 3) >> 2) << 2) * sizeof(int) +
+                     (size_t)wpb * G * dp_group_floats(a.cmax, STAGE) * sizeof(float);
   int blocks = (grid_waves + wpb - 1) / wpb;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL((k_dp<MEAS, W, BAND, DELTA>), dim3(blocks), dim3(64 * wpb), lds, st, a);
+  hipLaunchKernelGGL((k_dp<MEAS, W, U, BAND, DELTA, STAGE>), dim3(blocks), dim3(64 * wpb), lds, st, a);
   return hipGetLastError();
 }
 
 template <int MEAS, bool BAND, bool DELTA>
-static hipError_t launch_dp_w(hipStream_t st, const DpArgs& a, int cls, int grid_waves) {
-  switch (cls) {
-    case 0: return launch_dp_t<MEAS, 64, BAND, DELTA>(st, a, grid_waves);
-    case 1: return launch_dp_t<MEAS, 32, BAND, DELTA>(st, a, grid_waves);
-    default: return launch_dp_t<MEAS, 16, BAND, DELTA>(st, a, grid_waves);
+static hipError_t launch_dp_b(hipStream_t st, const DpArgs& a, int bucket, int grid_waves) {
+  switch (bucket) {
+    case 0: return launch_dp_t<MEAS, 16, 1, BAND, DELTA, true>(st, a, grid_waves);
+    case 1: return launch_dp_t<MEAS, 16, 2, BAND, DELTA, true>(st, a, grid_waves);
+    case 2: return launch_dp_t<MEAS, 16, 4, BAND, DELTA, true>(st, a, grid_waves);
+    case 3: return launch_dp_t<MEAS, 32, 4, BAND, DELTA, true>(st, a, grid_waves);
+    case 4: return launch_dp_t<MEAS, 64, 4, BAND, DELTA, true>(st, a, grid_waves);
+    default: return launch_dp_t<MEAS, 64, 4, BAND, DELTA, false>(st, a, grid_waves);
   }
 }
 
-hipError_t launch_dp(hipStream_t st, MeasureArgs m, int cls, SetView q, SetView db, const Item* items,
+hipError_t launch_dp(hipStream_t st, MeasureArgs m, int bucket, SetView q, SetView db, const Item* items,
                      const int32_t* nitems, int64_t max_items, int32_t* next, float* out, int32_t cmax,
                      int grid_waves) {
   DpArgs a{q, db, items, nitems, max_items, next, out, cmax, m.band, m.delta, m.eps};
   switch (m.measure) {
     case TS_DTW:
-      return m.band > 0 ? launch_dp_w<TS_DTW, true, false>(st, a, cls, grid_waves)
-                        : launch_dp_w<TS_DTW, false, false>(st, a, cls, grid_waves);
-    case TS_FRECHET: return launch_dp_w<TS_FRECHET, false, false>(st, a, cls, grid_waves);
+      return m.band > 0 ? launch_dp_b<TS_DTW, true, false>(st, a, bucket, grid_waves)
+                        : launch_dp_b<TS_DTW, false, false>(st, a, bucket, grid_waves);
+    case TS_FRECHET: return launch_dp_b<TS_FRECHET, false, false>(st, a, bucket, grid_waves);
     case TS_LCSS:
-      return m.delta >= 0 ? launch_dp_w<TS_LCSS, false, true>(st, a, cls, grid_waves)
-                          : launch_dp_w<TS_LCSS, false, false>(st, a, cls, grid_waves);
+      return m.delta >= 0 ? launch_dp_b<TS_LCSS, false, true>(st, a, bucket, grid_waves)
+                          : launch_dp_b<TS_LCSS, false, false>(st, a, bucket, grid_waves);
     default: return hipErrorInvalidValue;
   }
 }

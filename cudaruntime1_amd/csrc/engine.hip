@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -97,6 +98,9 @@ struct ts_engine {
   std::vector<hipEvent_t> evpool;
   size_t ev_used = 0;
   hipEvent_t call_a = nullptr, call_b = nullptr;
+  // Fork/join: per DP bucket a stream, so bucket launches overlap.
+  hipStream_t aux[kDpBuckets] = {};
+  hipEvent_t fork = nullptr, join[kDpBuckets] = {};
   std::vector<unsigned long long> cells_host;
   int cells_used = 0;
   int64_t overflow_retries = 0;
@@ -148,12 +152,12 @@ hipEvent_t next_event(ts_engine* e) {
 
 // Enqueue one launcher, optionally bracketed by timing events.
 template <class F>
-ts_status run(ts_engine* e, const char* name, double bytes, double units, F f, int cells_slot = -1,
-              int nlaunch = 1) {
+ts_status run_on(ts_engine* e, hipStream_t st, const char* name, double bytes, double units, F f,
+                 int cells_slot = -1, int nlaunch = 1) {
   hipEvent_t a = nullptr;
   if (e->timing) {
     a = next_event(e);
-    if (a) (void)hipEventRecord(a, e->st);
+    if (a) (void)hipEventRecord(a, st);
   }
   hipError_t err = f();
   if (err != hipSuccess) return hip_fail(err, name, __FILE__, __LINE__);
@@ -161,11 +165,17 @@ ts_status run(ts_engine* e, const char* name, double bytes, double units, F f, i
   if (e->timing && a) {
     hipEvent_t b = next_event(e);
     if (b) {
-      (void)hipEventRecord(b, e->st);
+      (void)hipEventRecord(b, st);
       e->pending.push_back(Pending{kernel_id(e, name), a, b, bytes, units, cells_slot});
     }
   }
   return TS_OK;
+}
+
+template <class F>
+ts_status run(ts_engine* e, const char* name, double bytes, double units, F f, int cells_slot = -1,
+              int nlaunch = 1) {
+  return run_on(e, e->st, name, bytes, units, f, cells_slot, nlaunch);
 }
 
 ts_status resolve_timing(ts_engine* e) {
@@ -192,6 +202,19 @@ ts_status resolve_timing(ts_engine* e) {
   return TS_OK;
 }
 
+// Persistent DP grid: waves per CU (tuned per measure; TS_DP_WAVES_PER_CU
+// overrides for sweeps).
+int dp_waves_per_cu(ts_engine* e, int measure) {
+  static int env = [] {
+    const char* v = std::getenv("TS_DP_WAVES_PER_CU");
+    return v ? std::atoi(v) : 0;
+  }();
+  if (env > 0) return env;
+  (void)e;
+  (void)measure;
+  return 32;
+}
+
 MeasureArgs margs(const ts_params& p) { return MeasureArgs{p.measure, p.band, p.delta, p.epsilon}; }
 
 bool desc_measure(int m) { return m == TS_LCSS; }
@@ -215,7 +238,7 @@ ts_status check_params(const ts_params* p, const ts_set* db, const ts_set* qs) {
 // Refine a device item list into vals[slot]. nitems_dev may be null (all
 // max_items valid positions; items with d < 0 are skipped).
 ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_set* qs, const Item* items,
-                 const int32_t* nitems_dev, int64_t max_items, float* vals) {
+                 const int32_t* nitems_dev, int64_t max_items, float* vals, bool seed = false) {
   if (max_items == 0) return TS_OK;
   const SetView qv = qs->view(), dv = db->view();
   const MeasureArgs ma = margs(p);
@@ -231,41 +254,50 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   int32_t* ctr = nullptr;
   TS_TRY(ensure(e, B_CTR, 16, &ctr));
   if (p.measure == TS_HAUSDORFF) {
-    return run(e, "hausdorff_refine", 0, 0, [&] {
+    return run(e, seed ? "hausdorff_refine_seed" : "hausdorff_refine", 0, 0, [&] {
       return launch_hausdorff(e->st, qv, dv, items, nitems_dev, max_items, vals);
     }, slot);
   }
   if (p.measure == TS_EUCLIDEAN) {
-    return run(e, "euclidean_refine", 0, 0, [&] {
+    return run(e, seed ? "euclidean_refine_seed" : "euclidean_refine", 0, 0, [&] {
       return launch_euclidean(e->st, qv, dv, items, nitems_dev, max_items, vals);
     }, slot);
   }
-  // Wavefront DP: split by lanes-per-pair class, one persistent launch each.
+  // Wavefront DP: split by bucket of the shorter length, one persistent
+  // launch per bucket that can occur for these two sets.
   const int32_t lo = std::min(qs->min_len, db->min_len), hi = std::min(qs->max_len, db->max_len);
-  const int cls_lo = dp_class_for_min_len(hi), cls_hi = dp_class_for_min_len(lo);
-  Item* items3 = nullptr;
-  if (cls_lo == cls_hi) {
-    // Every pair has the same class: no split needed.
+  const int b_lo = dp_bucket_of(lo), b_hi = dp_bucket_of(hi);
+  if (p.measure == TS_DTW && p.band > 0 && (int64_t)std::max(qs->max_len, db->max_len) * std::max(qs->max_len, db->max_len) >= (1ll << 31))
+    return fail(TS_ERR_UNSUPPORTED, "band needs trajectory lengths below 46341");
+  static const char* kNames[2][3] = {{"dtw_wavefront", "frechet_wavefront", "lcss_wavefront"},
+                                     {"dtw_wavefront_seed", "frechet_wavefront_seed", "lcss_wavefront_seed"}};
+  const char* nm = kNames[seed ? 1 : 0][p.measure == TS_DTW ? 0 : (p.measure == TS_FRECHET ? 1 : 2)];
+  const int waves = cus * dp_waves_per_cu(e, p.measure);
+  if (b_lo == b_hi) {
     TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, ctr, 0, 1); }));
-    const int32_t cmax = std::max(dp_lanes_for_class(cls_lo), hi);
-    const char* nm = p.measure == TS_DTW ? "dtw_wavefront" : (p.measure == TS_FRECHET ? "frechet_wavefront" : "lcss_wavefront");
+    const int32_t cmax = dp_halo_len(b_lo, hi);
     return run(e, nm, 0, 0, [&] {
-      return launch_dp(e->st, ma, cls_lo, qv, dv, items, nitems_dev, max_items, ctr, vals, cmax, cus * 32);
+      return launch_dp(e->st, ma, b_lo, qv, dv, items, nitems_dev, max_items, ctr, vals, cmax, waves);
     }, slot);
   }
-  TS_TRY(ensure(e, B_ITEMS3, 3 * max_items, &items3));
-  TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, ctr, 0, 8); }));
-  TS_TRY(run(e, "items_by_class", 0, 0, [&] {
-    return launch_items_by_class(e->st, items, nitems_dev, max_items, qv, dv, items3, max_items, ctr + 4);
+  Item* itemsb = nullptr;
+  TS_TRY(ensure(e, B_ITEMS3, (int64_t)kDpBuckets * max_items, &itemsb));
+  TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, ctr, 0, 2 * kDpBuckets); }));
+  TS_TRY(run(e, "items_by_bucket", 0, 0, [&] {
+    return launch_items_by_bucket(e->st, items, nitems_dev, max_items, qv, dv, itemsb, max_items, ctr + kDpBuckets);
   }));
-  for (int c = cls_lo; c <= cls_hi; ++c) {
-    const int W = dp_lanes_for_class(c);
-    const int32_t cmax = std::max(W, c == 0 ? hi : (c == 1 ? std::min(hi, 63) : std::min(hi, 31)));
-    const char* nm = p.measure == TS_DTW ? "dtw_wavefront" : (p.measure == TS_FRECHET ? "frechet_wavefront" : "lcss_wavefront");
-    TS_TRY(run(e, nm, 0, 0, [&] {
-      return launch_dp(e->st, ma, c, qv, dv, items3 + (int64_t)c * max_items, ctr + 4 + c, max_items, ctr + c,
-                       vals, cmax, cus * 32);
+  // Buckets run concurrently on their own streams (fork/join by events), so
+  // the latency-bound tail of one overlaps the others.
+  TS_HIP(hipEventRecord(e->fork, e->st));
+  for (int b = b_lo; b <= b_hi; ++b) {
+    const int32_t cmax = dp_halo_len(b, hi);
+    TS_HIP(hipStreamWaitEvent(e->aux[b], e->fork, 0));
+    TS_TRY(run_on(e, e->aux[b], nm, 0, 0, [&] {
+      return launch_dp(e->aux[b], ma, b, qv, dv, itemsb + (int64_t)b * max_items, ctr + kDpBuckets + b, max_items,
+                       ctr + b, vals, cmax, waves);
     }, slot));
+    TS_HIP(hipEventRecord(e->join[b], e->aux[b]));
+    TS_HIP(hipStreamWaitEvent(e->st, e->join[b], 0));
   }
   return TS_OK;
 }
@@ -361,7 +393,7 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   TS_TRY(run(e, "items", 0, 0, [&] { return launch_items_from_keys(e->st, seeds, nq, s, q0, items, nullptr); }));
   TS_TRY(run(e, "decode", 0, 0, [&] { return launch_decode_keys(e->st, seeds, (int64_t)nq * s, desc, seed_idx, nullptr); }));
   TS_TRY(fill_nan(e, vals, (int64_t)nq * s));
-  TS_TRY(refine(e, p, db, qs, items, nullptr, (int64_t)nq * s, vals));
+  TS_TRY(refine(e, p, db, qs, items, nullptr, (int64_t)nq * s, vals, true));
   // 2. threshold: k-th best exact value among the seeds
   TS_TRY(run(e, "keys", 0, 0, [&] { return launch_keys_from_regions(e->st, vals, seed_idx, nullptr, nq, s, desc, skeys); }));
   TS_TRY(topk(e, skeys, s, s, nullptr, nq, k, stop));
@@ -399,7 +431,7 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
   e->cells_used = 0;
   float4* env = nullptr;
   if (p.mode == TS_MODE_REF && (p.measure == TS_DTW || p.measure == TS_FRECHET || p.measure == TS_LCSS)) {
-    TS_TRY(ensure(e, B_ENV, qs->points, &env));
+    TS_TRY(ensure(e, B_ENV, qs->dev_points, &env));
   }
   int32_t* d_idx = nullptr;
   float* d_val = nullptr;
@@ -484,8 +516,13 @@ ts_status ts_engine_create(ts_engine** out) {
   auto e = new ts_engine();
   (void)hipGetDevice(&e->device);
   hipError_t err = hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking);
+  for (int b = 0; b < kDpBuckets && err == hipSuccess; ++b) {
+    err = hipStreamCreateWithFlags(&e->aux[b], hipStreamNonBlocking);
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&e->join[b], hipEventDisableTiming);
+  }
+  if (err == hipSuccess) err = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming);
   if (err != hipSuccess) {
-    delete e;
+    ts_engine_destroy(e);
     return hip_fail(err, "hipStreamCreate", __FILE__, __LINE__);
   }
   *out = e;
@@ -494,13 +531,18 @@ ts_status ts_engine_create(ts_engine** out) {
 
 ts_status ts_engine_destroy(ts_engine* e) {
   if (!e) return TS_OK;
-  (void)hipStreamSynchronize(e->st);
+  if (e->st) (void)hipStreamSynchronize(e->st);
   for (auto& b : e->bufs)
     if (b.p) (void)hipFree(b.p);
   for (auto ev : e->evpool) (void)hipEventDestroy(ev);
   if (e->call_a) (void)hipEventDestroy(e->call_a);
   if (e->call_b) (void)hipEventDestroy(e->call_b);
-  (void)hipStreamDestroy(e->st);
+  for (int b = 0; b < kDpBuckets; ++b) {
+    if (e->aux[b]) (void)hipStreamDestroy(e->aux[b]);
+    if (e->join[b]) (void)hipEventDestroy(e->join[b]);
+  }
+  if (e->fork) (void)hipEventDestroy(e->fork);
+  if (e->st) (void)hipStreamDestroy(e->st);
   delete e;
   return TS_OK;
 }
@@ -544,7 +586,7 @@ ts_status ts_filter_ref(ts_engine* e, const ts_set* db, const ts_set* qs, const 
   TS_TRY(check_params(p, db, qs));
   float4* env = nullptr;
   float* d = nullptr;
-  TS_TRY(ensure(e, B_ENV, qs->points, &env));
+  TS_TRY(ensure(e, B_ENV, qs->dev_points, &env));
   TS_TRY(ensure(e, B_SCORES, (int64_t)qs->count * db->count, &d));
   if (p->measure == TS_DTW || p->measure == TS_FRECHET || p->measure == TS_LCSS)
     TS_TRY(run(e, "envelope", 0, 0, [&] { return launch_envelope(e->st, qs->view(), qs->max_len, p->r, env); }));

@@ -166,7 +166,8 @@ __device__ __forceinline__ float pair_bound(const MeasureArgs& ma, const QSum& q
   return 0.0f;
 }
 
-constexpr int kBoundChunk = 2048;
+constexpr int kBoundChunk = 2048;  // DB trajectories per seeding block
+constexpr int kSeedPerThread = kBoundChunk / 256;
 
 __device__ void bitonic_sort_lds(uint64_t* s, int n, int tid, int nthreads) {
   for (int k = 2; k <= n; k <<= 1) {
@@ -184,27 +185,34 @@ __device__ void bitonic_sort_lds(uint64_t* s, int n, int tid, int nthreads) {
   }
 }
 
-// Best `s` keys per (query, DB chunk of kBoundChunk trajectories).
+// Seeds: good candidates per (query, DB chunk of kBoundChunk trajectories).
+// Any real pair's exact value bounds the k-th best from above, so seeds need
+// not be the exact top-s by bound: each thread keeps the best key of its
+// kSeedPerThread trajectories and the block keeps the best `s` of those
+// (s <= 256). Larger s sorts the whole chunk.
+template <bool WHOLE>
 __global__ void __launch_bounds__(256) k_bound_chunks(MeasureArgs ma, SetView q, int32_t q0, SetView db,
                                                       int32_t s, uint64_t* keys) {
-  __shared__ uint64_t sk[kBoundChunk];
+  constexpr int NS = WHOLE ? kBoundChunk : 256;
+  __shared__ uint64_t sk[NS];
   const int qi = q0 + blockIdx.y;
   QSum qs{q.mbr[qi], q.ends[qi], q.len[qi], q.off[qi]};
   const bool desc = ma.measure == TS_LCSS;
   const int base = blockIdx.x * kBoundChunk;
-  for (int i = threadIdx.x; i < kBoundChunk; i += 256) {
-    int j = base + i;
+  uint64_t best = ~0ull;
+  for (int t = 0; t < kSeedPerThread; ++t) {
+    const int i = t * 256 + threadIdx.x;
+    const int j = base + i;
     uint64_t key = ~0ull;
-    if (j < db.count) {
-      float b = pair_bound(ma, qs, q, db.mbr[j], db.ends[j], db.len[j]);
-      key = make_key(b, j, desc);
-    }
-    sk[i] = key;
+    if (j < db.count) key = make_key(pair_bound(ma, qs, q, db.mbr[j], db.ends[j], db.len[j]), j, desc);
+    if (WHOLE) sk[i] = key;
+    else best = key < best ? key : best;
   }
+  if (!WHOLE) sk[threadIdx.x] = best;
   __syncthreads();
-  bitonic_sort_lds(sk, kBoundChunk, threadIdx.x, 256);
+  bitonic_sort_lds(sk, NS, threadIdx.x, 256);
   uint64_t* out = keys + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * s;
-  for (int t = threadIdx.x; t < s; t += 256) out[t] = sk[t];
+  for (int t = threadIdx.x; t < s; t += 256) out[t] = t < NS ? sk[t] : ~0ull;
 }
 
 hipError_t launch_bound_chunks(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, int32_t nq,
@@ -212,46 +220,69 @@ hipError_t launch_bound_chunks(hipStream_t st, MeasureArgs m, int32_t k, SetView
   (void)k;
   (void)chunk;
   dim3 grid((db.count + kBoundChunk - 1) / kBoundChunk, nq);
-  hipLaunchKernelGGL(k_bound_chunks, grid, dim3(256), 0, st, m, q, q0, db, s, keys);
+  if (s <= 256) hipLaunchKernelGGL(k_bound_chunks<false>, grid, dim3(256), 0, st, m, q, q0, db, s, keys);
+  else hipLaunchKernelGGL(k_bound_chunks<true>, grid, dim3(256), 0, st, m, q, q0, db, s, keys);
   return hipGetLastError();
 }
 
-// Threshold compaction. A pair survives when its bound can still reach the
-// query's k-th value tau: distances bound <= tau; LCSS ub >= tau and, because
-// zero-similarity ties are broken by index, (ub > 0 or j < k).
-__global__ void __launch_bounds__(256) k_bound_compact(MeasureArgs ma, int32_t k, SetView q, int32_t q0,
+// Threshold compaction, kCompactQ queries per block and kCompactJ DB
+// trajectories per thread: each DB summary is loaded once and tested against
+// every query of the block. A pair survives when its bound can still reach
+// the query's k-th value tau: distances bound <= tau; LCSS ub >= tau and,
+// because zero-similarity ties are broken by index, (ub > 0 or j < k).
+// Survivors are appended with a wave ballot + prefix count, one atomic per
+// wave and query.
+constexpr int kCompactQ = 8, kCompactJ = 2;
+
+__global__ void __launch_bounds__(256) k_bound_compact(MeasureArgs ma, int32_t k, SetView q, int32_t q0, int32_t nq,
                                                        SetView db, const float* tau, int32_t cap,
                                                        int32_t* region_d, int32_t* counts, int32_t* overflow) {
-  const int qi = q0 + blockIdx.y;
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  QSum qs{q.mbr[qi], q.ends[qi], q.len[qi], q.off[qi]};
-  const float t = tau[blockIdx.y];
-  bool pass = false;
-  if (j < db.count) {
-    float b = pair_bound(ma, qs, q, db.mbr[j], db.ends[j], db.len[j]);
-    if (ma.measure == TS_LCSS) pass = (b >= t) && (b > 0.0f || j < k);
-    else pass = isfinite(b) && (b <= t);
+  __shared__ QSum sq[kCompactQ];
+  __shared__ float st[kCompactQ];
+  const int qb = blockIdx.y * kCompactQ;
+  const int nqb = min(kCompactQ, nq - qb);
+  if (threadIdx.x < nqb) {
+    const int qi = q0 + qb + threadIdx.x;
+    sq[threadIdx.x] = QSum{q.mbr[qi], q.ends[qi], q.len[qi], q.off[qi]};
+    st[threadIdx.x] = tau[qb + threadIdx.x];
   }
-  // Wave ballot + prefix count: one atomic per wave.
-  const uint64_t bal = __ballot(pass);
-  if (bal == 0) return;
+  __syncthreads();
   const int lane = threadIdx.x & 63;
-  const int leader = __ffsll((unsigned long long)bal) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(&counts[blockIdx.y], __popcll(bal));
-  base = __shfl(base, leader);
-  if (pass) {
-    const int pos = base + __popcll(bal & ((1ull << lane) - 1ull));
-    if (pos < cap) region_d[(int64_t)blockIdx.y * cap + pos] = j;
-    else atomicOr(overflow, 1);
+  for (int t = 0; t < kCompactJ; ++t) {
+    const int j = (blockIdx.x * kCompactJ + t) * 256 + threadIdx.x;
+    const bool valid = j < db.count;
+    const int jj = valid ? j : 0;
+    const float4 cm = db.mbr[jj], ce = db.ends[jj];
+    const int m = db.len[jj];
+    for (int u = 0; u < nqb; ++u) {
+      bool pass = false;
+      if (valid) {
+        const float b = pair_bound(ma, sq[u], q, cm, ce, m);
+        const float tq = st[u];
+        if (ma.measure == TS_LCSS) pass = (b >= tq) && (b > 0.0f || j < k);
+        else pass = isfinite(b) && (b <= tq);
+      }
+      const uint64_t bal = __ballot(pass);
+      if (bal == 0) continue;
+      const int leader = __ffsll((unsigned long long)bal) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(&counts[qb + u], __popcll(bal));
+      base = __shfl(base, leader);
+      if (pass) {
+        const int pos = base + __popcll(bal & ((1ull << lane) - 1ull));
+        if (pos < cap) region_d[(int64_t)(qb + u) * cap + pos] = j;
+        else atomicOr(overflow, 1);
+      }
+    }
   }
 }
 
 hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, int32_t nq,
                                 SetView db, const float* tau, int32_t cap, int32_t* region_d, int32_t* counts,
                                 int32_t* overflow) {
-  dim3 grid((db.count + 255) / 256, nq);
-  hipLaunchKernelGGL(k_bound_compact, grid, dim3(256), 0, st, m, k, q, q0, db, tau, cap, region_d, counts, overflow);
+  dim3 grid((db.count + 256 * kCompactJ - 1) / (256 * kCompactJ), (nq + kCompactQ - 1) / kCompactQ);
+  hipLaunchKernelGGL(k_bound_compact, grid, dim3(256), 0, st, m, k, q, q0, nq, db, tau, cap, region_d, counts,
+                     overflow);
   return hipGetLastError();
 }
 

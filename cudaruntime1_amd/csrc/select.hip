@@ -211,30 +211,36 @@ hipError_t launch_fill_i32(hipStream_t st, int32_t* p, int32_t v, int64_t n) {
   return hipGetLastError();
 }
 
-// Refine class of a pair: lanes per pair of the wavefront DP, chosen so that
-// the shorter side (the DP's column count) covers the lanes.
-__host__ __device__ inline int class_of_min_len(int32_t mn) { return mn >= 64 ? 0 : (mn >= 32 ? 1 : 2); }
-int dp_class_for_min_len(int32_t min_len) { return class_of_min_len(min_len); }
-
-__global__ void __launch_bounds__(256) k_items_by_class(const Item* items, const int32_t* nitems, int64_t max_items,
-                                                        SetView q, SetView db, Item* out3, int64_t cap_each,
-                                                        int32_t* counts3) {
+__global__ void __launch_bounds__(256) k_items_by_bucket(const Item* items, const int32_t* nitems,
+                                                         int64_t max_items, SetView q, SetView db, Item* outb,
+                                                         int64_t cap_each, int32_t* countsb) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   int64_t n = nitems ? (int64_t)*nitems : max_items;
   if (n > max_items) n = max_items;
-  if (i >= n) return;
-  Item it = items[i];
-  if (it.d < 0) return;
-  int c = class_of_min_len(min(q.len[it.q], db.len[it.d]));
-  int pos = atomicAdd(&counts3[c], 1);
-  if (pos < cap_each) out3[c * cap_each + pos] = it;
+  Item it{0, -1, 0, 0};
+  if (i < n) it = items[i];
+  const int b = it.d >= 0 ? dp_bucket_of(min(q.len[it.q], db.len[it.d])) : -1;
+  const int lane = threadIdx.x & 63;
+  // One atomic per wave and bucket (wave ballot + prefix count).
+  for (int bb = 0; bb < kDpBuckets; ++bb) {
+    const uint64_t bal = __ballot(b == bb);
+    if (bal == 0) continue;
+    const int leader = __ffsll((unsigned long long)bal) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&countsb[bb], __popcll(bal));
+    base = __shfl(base, leader);
+    if (b == bb) {
+      const int pos = base + __popcll(bal & ((1ull << lane) - 1ull));
+      if (pos < cap_each) outb[bb * cap_each + pos] = it;
+    }
+  }
 }
 
-hipError_t launch_items_by_class(hipStream_t st, const Item* items, const int32_t* nitems, int64_t max_items,
-                                 SetView q, SetView db, Item* out3, int64_t cap_each, int32_t* counts3) {
+hipError_t launch_items_by_bucket(hipStream_t st, const Item* items, const int32_t* nitems, int64_t max_items,
+                                  SetView q, SetView db, Item* outb, int64_t cap_each, int32_t* countsb) {
   if (max_items == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_items_by_class, dim3((unsigned)((max_items + 255) / 256)), dim3(256), 0, st, items, nitems,
-                     max_items, q, db, out3, cap_each, counts3);
+  hipLaunchKernelGGL(k_items_by_bucket, dim3((unsigned)((max_items + 255) / 256)), dim3(256), 0, st, items, nitems,
+                     max_items, q, db, outb, cap_each, countsb);
   return hipGetLastError();
 }
 

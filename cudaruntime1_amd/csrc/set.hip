@@ -108,7 +108,11 @@ ts_status set_load(ts_set* s, const ts_point* xy, const int64_t* offsets, int32_
     mn = std::min(mn, lens[i]);
     mx = std::max(mx, lens[i]);
   }
-  const int64_t P = offsets[count];
+  // Device offsets padded to kPad points per trajectory.
+  std::vector<int64_t> doff(count + 1);
+  doff[0] = 0;
+  for (int32_t i = 0; i < count; ++i) doff[i + 1] = doff[i] + ((lens[i] + kPad - 1) / kPad) * kPad;
+  const int64_t P = doff[count];
   if (P > s->cap_points || count > s->cap_count) {
     if (st) TS_HIP(hipStreamSynchronize(st));
     (void)hipFree(s->x); (void)hipFree(s->y); (void)hipFree(s->off); (void)hipFree(s->len);
@@ -131,17 +135,23 @@ ts_status set_load(ts_set* s, const ts_point* xy, const int64_t* offsets, int32_
     s->cap_count = count;
   }
   s->count = count;
-  s->points = P;
+  s->points = offsets[count];
+  s->dev_points = P;
   s->min_len = mn;
   s->max_len = mx;
   s->h_off.assign(offsets, offsets + count + 1);
   s->h_len = lens;
   std::vector<float> hx(P), hy(P);
-#pragma omp parallel for schedule(static) if (P > (1 << 16))
-  for (int64_t i = 0; i < P; ++i) { hx[i] = xy[i].x; hy[i] = xy[i].y; }
+#pragma omp parallel for schedule(static) if (count > 4096)
+  for (int32_t i = 0; i < count; ++i) {
+    const int64_t src = offsets[i], dst = doff[i];
+    const int32_t n = lens[i];
+    for (int32_t j = 0; j < n; ++j) { hx[dst + j] = xy[src + j].x; hy[dst + j] = xy[src + j].y; }
+    for (int64_t j = n; j < doff[i + 1] - dst; ++j) { hx[dst + j] = xy[src + n - 1].x; hy[dst + j] = xy[src + n - 1].y; }
+  }
   TS_HIP(hipMemcpyAsync(s->x, hx.data(), P * sizeof(float), hipMemcpyHostToDevice, st));
   TS_HIP(hipMemcpyAsync(s->y, hy.data(), P * sizeof(float), hipMemcpyHostToDevice, st));
-  TS_HIP(hipMemcpyAsync(s->off, offsets, (count + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  TS_HIP(hipMemcpyAsync(s->off, doff.data(), (count + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
   TS_HIP(hipMemcpyAsync(s->len, s->h_len.data(), count * sizeof(int32_t), hipMemcpyHostToDevice, st));
   TS_HIP(launch_summaries(st, s->view(), s->mbr, s->ends));
   TS_HIP(hipStreamSynchronize(st));

@@ -30,6 +30,11 @@ ts_status hip_fail(hipError_t e, const char* what, const char* file, int line);
     if (_s != TS_OK) return _s;         \
   } while (0)
 
+// Trajectories start on multiples of kPad points (16 B of x and of y), the
+// tail of each padded with copies of its last point, so DP kernels can load
+// kPad consecutive points with one aligned vector load.
+constexpr int kPad = 4;
+
 // Read-only device view of a trajectory set, passed to kernels by value.
 struct SetView {
   const float* x;        // [points]
@@ -58,7 +63,8 @@ struct MeasureArgs {
 // Device-resident set (the opaque ts_set of the C ABI).
 struct ts_set {
   int32_t count = 0;
-  int64_t points = 0;
+  int64_t points = 0;      // caller's points
+  int64_t dev_points = 0;  // device points including per-trajectory padding
   int32_t min_len = 0, max_len = 0;
   int device = 0;
   float* x = nullptr;
@@ -131,17 +137,23 @@ hipError_t launch_keys_from_regions(hipStream_t st, const float* vals, const int
                                     const int32_t* counts, int32_t nq, int32_t cap, int32_t descending,
                                     uint64_t* keys);
 hipError_t launch_fill_i32(hipStream_t st, int32_t* p, int32_t v, int64_t n);
-// Split items by refine class (W lanes per pair) into three lists.
-hipError_t launch_items_by_class(hipStream_t st, const Item* items, const int32_t* nitems, int64_t max_items,
-                                 SetView q, SetView db, Item* out3, int64_t cap_each, int32_t* counts3);
+// Split items by DP bucket (dp_bucket_of the shorter length) into
+// kDpBuckets lists of cap_each entries.
+hipError_t launch_items_by_bucket(hipStream_t st, const Item* items, const int32_t* nitems, int64_t max_items,
+                                  SetView q, SetView db, Item* outb, int64_t cap_each, int32_t* countsb);
 hipError_t launch_items_from_regions(hipStream_t st, const int32_t* region_d, const int32_t* counts,
                                      int32_t nq, int32_t cap, int32_t q0, Item* items, int32_t* nitems);
 
-// dp.hip -- persistent wavefront DP over a device item list.
-// cls: 0 -> 64 lanes per pair, 1 -> 32, 2 -> 16.
-int dp_lanes_for_class(int cls);
-int dp_class_for_min_len(int32_t min_len);
-hipError_t launch_dp(hipStream_t st, MeasureArgs m, int cls, SetView q, SetView db, const Item* items,
+// dp.hip -- persistent wavefront DP over a device item list, one launch per
+// bucket of the pair's shorter length C: < 32, < 64, < 128, < 256, < 1024,
+// >= 1024.
+constexpr int kDpBuckets = 6;
+__host__ __device__ inline int dp_bucket_of(int32_t c) {
+  return c <= 31 ? 0 : (c <= 63 ? 1 : (c <= 127 ? 2 : (c <= 255 ? 3 : (c <= 1023 ? 4 : 5))));
+}
+// LDS halo length (floats) for bucket b when its longest C is at most cmax_c.
+int32_t dp_halo_len(int b, int32_t cmax_c);
+hipError_t launch_dp(hipStream_t st, MeasureArgs m, int bucket, SetView q, SetView db, const Item* items,
                      const int32_t* nitems, int64_t max_items, int32_t* next, float* out, int32_t cmax,
                      int grid_waves);
 

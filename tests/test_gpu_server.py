@@ -73,10 +73,10 @@ def test_results_same_alone_and_in_mix(world):
 def test_admission_refuses_what_cannot_fit(world):
     db, reqs = world
     say("admission")
-    srv = ts.Server(db, streams=1, fake_free_mem=300_000, mem_reserve=0)
+    srv = ts.Server(db, streams=1, fake_free_mem=50_000, mem_reserve=0)
     q, p = reqs[0]
     with pytest.raises(ts.TrajsimError) as e:
-        srv.submit(q, p)  # EXACT workspace for a 4000-trajectory DB exceeds 300 kB
+        srv.submit(q, p)  # EXACT workspace for a 4000-trajectory DB exceeds 50 kB
     assert e.value.status == 6
     assert srv.counters()["refused"] == 1
     srv.close()
