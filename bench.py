@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip plain/latency/frechet legs (profiling runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 256 MiB device copy (HBM byte-counter calibration for rocprof --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one GPU per box); "
@@ -106,6 +108,14 @@ def main():
 
     import cudaruntime1_amd as ts
     ts.init(0)
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 x 256 MiB device copies")
     dev = ts.device_name()
     log("device", dev)
     t0 = time.time()

@@ -61,7 +61,7 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 
 enum BufId {
   B_ENV, B_SCORES, B_KEYS, B_KEYS2, B_SCA, B_SCB, B_CAND, B_CANDIDX, B_ITEMS, B_ITEMS3, B_VALS, B_REGION,
-  B_RVALS, B_COUNTS, B_CTR, B_TAU, B_TOP, B_OIDX, B_OVAL, B_CELLS, B_HOSTIN, B_NBUF
+  B_RVALS, B_COUNTS, B_CTR, B_TAU, B_TOP, B_OIDX, B_OVAL, B_CELLS, B_HOSTIN, B_ITEMS2, B_NBUF
 };
 
 struct Buf {
@@ -382,7 +382,7 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   TS_TRY(ensure(e, B_ITEMS, std::max<int64_t>((int64_t)nq * s, (int64_t)nq * cap), &items));
   TS_TRY(ensure(e, B_REGION, (int64_t)nq * cap, &region));
   TS_TRY(ensure(e, B_RVALS, (int64_t)nq * cap, &rvals));
-  TS_TRY(ensure(e, B_COUNTS, nq + 2 + nq + 1, &counts));
+  TS_TRY(ensure(e, B_COUNTS, 2 * nq + 4, &counts));
   TS_TRY(ensure(e, B_SCORES, (int64_t)nq * cap * 2, (float**)&rkeys));
   const double pairs = (double)nq * N;
   // 1. seeds: s best bounds per query
@@ -406,8 +406,17 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
     return launch_bound_compact(e->st, ma, k, qv, q0, nq, dv, tau, cap, region, counts, overflow);
   }));
   TS_TRY(run(e, "items", 0, 0, [&] { return launch_items_from_regions(e->st, region, counts, nq, cap, q0, items, nitems); }));
+  // 3b. tighter O(n+m) bound on the survivors
+  Item* items2 = nullptr;
+  TS_TRY(ensure(e, B_ITEMS2, (int64_t)nq * cap, &items2));
+  int32_t* nitems2 = counts + 2 * nq + 3;  // after the region offsets (counts[nq+3 .. 2nq+2])
+  TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, nitems2, 0, 1); }));
+  TS_TRY(run(e, "tight_filter", 0, 0, [&] {
+    return launch_tight_filter(e->st, ma, k, qv, q0, dv, tau, items, nitems, (int64_t)nq * cap, items2, nitems2,
+                               device_cu_count() * 16);
+  }));
   TS_TRY(fill_nan(e, rvals, (int64_t)nq * cap));
-  TS_TRY(refine(e, p, db, qs, items, nitems, (int64_t)nq * cap, rvals));
+  TS_TRY(refine(e, p, db, qs, items2, nitems2, (int64_t)nq * cap, rvals));
   // 4. top-k over the refined candidates
   TS_TRY(run(e, "keys", 0, 0, [&] { return launch_keys_from_regions(e->st, rvals, region, counts, nq, cap, desc, rkeys); }));
   TS_TRY(topk(e, rkeys, cap, cap, counts, nq, k, top));

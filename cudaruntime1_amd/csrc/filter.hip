@@ -286,4 +286,148 @@ hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetVie
   return hipGetLastError();
 }
 
+// ------------------------------------------------------ tight bound stage
+// Second filter over the pairs that survived the summary bound, O(n+m) per
+// pair and one wave per pair. Every warping path / coupling visits every row
+// and every column, and a cell's computed cost is never below its point's
+// computed distance to the other trajectory's MBR (f32 subtraction is
+// monotone). Hence, with d(p, B) the point-to-box distance:
+//   DTW       >= max( d(q0,c0) + sum_{0<i<n-1} d(q_i, B_c) + d(q_n-1,c_m-1),
+//                     the same over c's points against B_q ),
+//             discounted by (L+72)*2^-23 for the f32 sums (tree order here,
+//             running order in the DP);
+//   Frechet   >= max( max_i d(q_i,B_c), max_j d(c_j,B_q), endpoint costs );
+//   Hausdorff >= max( max_i d(q_i,B_c), max_j d(c_j,B_q) );
+//   Euclidean >= sqrt(sum_i d(q_i,B_c)^2) (double, discounted 1e-6);
+//   LCSS      <= min(#q points in c's eps-box, #c points in q's eps-box,
+//                    min(n,m)) / min(n,m).
+// min/max do not round, so the Frechet/Hausdorff/LCSS forms are exact.
+__device__ __forceinline__ float pt_box(float px, float py, float4 b) {
+  const float gx = fmaxf(0.0f, fmaxf(b.x - px, px - b.y));
+  const float gy = fmaxf(0.0f, fmaxf(b.z - py, py - b.w));
+  return gx * gx + gy * gy;  // squared
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+  for (int sh = 32; sh > 0; sh >>= 1) v += __shfl_xor(v, sh);
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+  for (int sh = 32; sh > 0; sh >>= 1) v = fmaxf(v, __shfl_xor(v, sh));
+  return v;
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+  for (int sh = 32; sh > 0; sh >>= 1) v += __shfl_xor(v, sh);
+  return v;
+}
+
+// Sum/max over the points of `a` (length na) of their distance to box `b`,
+// skipping the first and last point when `skip_ends`.
+template <int MEAS>
+__device__ void side_bound(const float* ax, const float* ay, int na, float4 b, float eps, bool skip_ends, float& acc,
+                           double& accd, int& cnt) {
+  const int lane = threadIdx.x & 63;
+  for (int i = lane; i < na; i += 64) {
+    const float px = ax[i], py = ay[i];
+    if (MEAS == TS_LCSS) {
+      cnt += (px - b.y < eps && b.x - px < eps && py - b.w < eps && b.z - py < eps) ? 1 : 0;
+    } else if (MEAS == TS_EUCLIDEAN) {
+      accd += (double)pt_box(px, py, b);
+    } else {
+      const float d2 = pt_box(px, py, b);
+      if (MEAS == TS_DTW) {
+        if (!(skip_ends && (i == 0 || i == na - 1))) acc += sqrtf(d2);
+      } else {
+        acc = fmaxf(acc, d2);
+      }
+    }
+  }
+}
+
+template <int MEAS>
+__global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k, SetView q, int32_t q0, SetView db,
+                                                      const float* tau, const Item* items, const int32_t* nitems,
+                                                      int64_t max_items, Item* out, int32_t* nout) {
+  const int lane = threadIdx.x & 63;
+  int64_t n_items = min<int64_t>(max_items, (int64_t)*nitems);
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  for (int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); w < n_items; w += nw) {
+    const Item it = items[w];
+    if (it.d < 0) continue;
+    const int n = q.len[it.q], m = db.len[it.d];
+    const int64_t qo = q.off[it.q], co = db.off[it.d];
+    const float4 qb = q.mbr[it.q], cb = db.mbr[it.d];
+    const float4 qe = q.ends[it.q], ce = db.ends[it.d];
+    const float t = tau[it.q - q0];
+    bool pass;
+    if (MEAS == TS_LCSS) {
+      int c1 = 0, c2 = 0;
+      float f = 0.0f;
+      double fd = 0.0;
+      side_bound<MEAS>(q.x + qo, q.y + qo, n, cb, ma.eps, false, f, fd, c1);
+      side_bound<MEAS>(db.x + co, db.y + co, m, qb, ma.eps, false, f, fd, c2);
+      for (int sh = 32; sh > 0; sh >>= 1) {
+        c1 += __shfl_xor(c1, sh);
+        c2 += __shfl_xor(c2, sh);
+      }
+      const int mn = min(n, m);
+      const float ub = (float)min(min(c1, c2), mn) / (float)mn;
+      pass = (ub >= t) && (ub > 0.0f || it.d < k);
+    } else if (MEAS == TS_EUCLIDEAN) {
+      float f = 0.0f;
+      double s = 0.0;
+      int c = 0;
+      if (n != m) continue;
+      side_bound<MEAS>(q.x + qo, q.y + qo, n, cb, ma.eps, false, f, s, c);
+      s = wave_sum_d(s);
+      pass = (float)(sqrt(s) * (1.0 - 1e-6)) <= t;
+    } else if (MEAS == TS_DTW) {
+      float a1 = 0.0f, a2 = 0.0f;
+      double fd = 0.0;
+      int c = 0;
+      side_bound<MEAS>(q.x + qo, q.y + qo, n, cb, ma.eps, n >= 2, a1, fd, c);
+      side_bound<MEAS>(db.x + co, db.y + co, m, qb, ma.eps, m >= 2, a2, fd, c);
+      a1 = wave_sum(a1);
+      a2 = wave_sum(a2);
+      const float dx0 = qe.x - ce.x, dy0 = qe.y - ce.y, dx1 = qe.z - ce.z, dy1 = qe.w - ce.w;
+      const float e0 = sqrtf(dx0 * dx0 + dy0 * dy0), e1 = sqrtf(dx1 * dx1 + dy1 * dy1);
+      // Row form: rows 0 and n-1 contain (0,0) and (n-1,m-1); with n == 1 the
+      // single row holds both when m >= 2. Column form likewise.
+      const float r1 = n >= 2 ? a1 + e0 + e1 : (m >= 2 ? e0 + e1 : e0);
+      const float r2 = m >= 2 ? a2 + e0 + e1 : (n >= 2 ? e0 + e1 : e0);
+      const int L = max(n, m);
+      const float b = fmaxf(r1, r2) * (1.0f - (float)(L + 72) * 1.1920929e-7f);
+      pass = b <= t;
+    } else {
+      float a1 = 0.0f, a2 = 0.0f;
+      double fd = 0.0;
+      int c = 0;
+      side_bound<MEAS>(q.x + qo, q.y + qo, n, cb, ma.eps, false, a1, fd, c);
+      side_bound<MEAS>(db.x + co, db.y + co, m, qb, ma.eps, false, a2, fd, c);
+      float b2 = fmaxf(wave_max(a1), wave_max(a2));
+      if (MEAS == TS_FRECHET) {
+        const float dx0 = qe.x - ce.x, dy0 = qe.y - ce.y, dx1 = qe.z - ce.z, dy1 = qe.w - ce.w;
+        b2 = fmaxf(b2, fmaxf(dx0 * dx0 + dy0 * dy0, dx1 * dx1 + dy1 * dy1));
+      }
+      pass = sqrtf(b2) <= t;
+    }
+    if (pass && lane == 0) out[atomicAdd(nout, 1)] = it;
+  }
+}
+
+hipError_t launch_tight_filter(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, SetView db,
+                               const float* tau, const Item* items, const int32_t* nitems, int64_t max_items,
+                               Item* out, int32_t* nout, int grid_waves) {
+  const dim3 grid((grid_waves + 3) / 4);
+  switch (m.measure) {
+    case TS_DTW: hipLaunchKernelGGL(k_tight_filter<TS_DTW>, grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout); break;
+    case TS_FRECHET: hipLaunchKernelGGL(k_tight_filter<TS_FRECHET>, grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout); break;
+    case TS_LCSS: hipLaunchKernelGGL(k_tight_filter<TS_LCSS>, grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout); break;
+    case TS_HAUSDORFF: hipLaunchKernelGGL(k_tight_filter<TS_HAUSDORFF>, grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout); break;
+    case TS_EUCLIDEAN: hipLaunchKernelGGL(k_tight_filter<TS_EUCLIDEAN>, grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 }  // namespace ts

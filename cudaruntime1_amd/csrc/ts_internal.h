@@ -110,6 +110,12 @@ hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetVie
                                 int32_t nq, SetView db, const float* tau, int32_t cap,
                                 int32_t* region_d, int32_t* counts, int32_t* overflow);
 
+// Second, O(n+m) admissible bound over compacted pairs (one wave per pair);
+// survivors are appended to `out` with count *nout (zeroed by the caller).
+hipError_t launch_tight_filter(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, SetView db,
+                               const float* tau, const Item* items, const int32_t* nitems, int64_t max_items,
+                               Item* out, int32_t* nout, int grid_waves);
+
 // select.hip
 // Key order: ascending u64. enc_asc(v,i) = ordered(v)<<32|i, desc uses ~ordered(v).
 hipError_t launch_keys_from_rows(hipStream_t st, const float* vals, int32_t rows, int32_t cols,
