@@ -88,6 +88,24 @@ def cpu_baseline(db_xy, db_off, q_xy, q_off, n_sample):
                       "search with the same bounds), %.2f s" % (n_sample, n_db, dt)}, idx, val
 
 
+def copy_calibration(nbytes=256 << 20, reps=3):
+    """Plain device-to-device copies of known size through the HIP runtime's
+    copy kernel: the reference point that scales rocprof's FETCH_SIZE and
+    WRITE_SIZE counters to bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log("copy calibration: %d x %d MiB device copies" % (reps, nbytes >> 20))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,13 +127,7 @@ def main():
     import cudaruntime1_amd as ts
     ts.init(0)
     if args.copy_calibration:
-        import torch
-        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        log("copy calibration: 3 x 256 MiB device copies")
+        copy_calibration()
     dev = ts.device_name()
     log("device", dev)
     t0 = time.time()
