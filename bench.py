@@ -106,13 +106,61 @@ def copy_calibration(nbytes=256 << 20, reps=3):
     log("copy calibration: %d x %d MiB device copies" % (reps, nbytes >> 20))
 
 
+def run_c3(ts, args):
+    """Uneven-length batching workload: 4,096 queries (log-uniform lengths
+    [16, 2048]) against 1,000,000 DB random walks (log-uniform [16, 1024]),
+    DTW and Frechet top-10. Batched = all queries in one ts_knn call; plain =
+    one launch sequence per query (ts_knn_plain) on a sample of queries, whose
+    top-10 must equal the batched ones."""
+    t0 = time.time()
+    db_len = ts.gen_lengths(DB_SEED, args.n_db, 16, 1024, True)
+    db_xy, db_off = ts.gen_walks(DB_SEED, db_len)
+    q_len = ts.gen_lengths(Q_SEED, args.n_q, 16, 2048, True)
+    q_xy, q_off = ts.gen_walks(Q_SEED, q_len)
+    db = ts.TrajSet(db_xy, db_off)
+    qs = ts.TrajSet(q_xy, q_off)
+    ns = min(args.plain_sample, args.n_q)
+    sample = ts.TrajSet(q_xy[:q_off[ns]], q_off[:ns + 1])
+    log("c3 workload ready in %.1fs: %d DB trajectories (%d points), %d queries" %
+        (time.time() - t0, db.count, len(db_xy), qs.count))
+    eng = ts.Engine()
+    Q, N = qs.count, db.count
+    out = {}
+    for meas in ("dtw", "frechet"):
+        p = ts.params(meas, mode=ts.EXACT, k=10)
+        eng.knn(db, sample, p)  # warm-up (workspace growth)
+        t1 = time.perf_counter()
+        idx, val, st = eng.knn(db, qs, p)
+        tb = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        pi, pv, pst = eng.knn_plain(db, sample, p)
+        tp = time.perf_counter() - t1
+        same = bool(np.array_equal(pi, idx[:ns]) and np.array_equal(pv, val[:ns]))
+        out[meas] = {"batched_pairs_per_s": Q * N / tb, "batched_s": round(tb, 4),
+                     "batched_cell_updates_per_s": st["cells"] / tb, "cells": st["cells"],
+                     "plain_pairs_per_s": ns * N / tp, "plain_s_for_sample": round(tp, 4), "plain_sample": ns,
+                     "batched_vs_plain": round((Q * N / tb) / (ns * N / tp), 2), "plain_topk_equals_batched": same,
+                     "overflow_retries": st["overflow_retries"]}
+        log(meas, out[meas])
+    line = {"metric": METRIC, "value": round(out["dtw"]["batched_pairs_per_s"], 1), "unit": "pairs/s", "n_gpus": 1,
+            "steps": 1, "warmup": 1, "ms_per_step": round(out["dtw"]["batched_s"] * 1e3, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "uneven-length batching: %d queries log-uniform [16,2048] vs %d DB random walks "
+                                   "log-uniform [16,1024], DTW/Frechet top-10, EXACT bounding-box filter" % (Q, N),
+                       "queries": Q, "db_trajectories": N, "k": 10},
+            "extra": out}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n-db", type=int, default=100_000)
-    ap.add_argument("--n-q", type=int, default=64)
+    ap.add_argument("--config", default="c2", choices=["c2", "c3"])
+    ap.add_argument("--n-db", type=int, default=None)
+    ap.add_argument("--n-q", type=int, default=None)
+    ap.add_argument("--plain-sample", type=int, default=256)
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip plain/latency/frechet legs (profiling runs)")
@@ -126,6 +174,12 @@ def main():
 
     import cudaruntime1_amd as ts
     ts.init(0)
+    if args.config == "c3":
+        args.n_db = args.n_db or 1_000_000
+        args.n_q = args.n_q or 4096
+        return run_c3(ts, args)
+    args.n_db = args.n_db or 100_000
+    args.n_q = args.n_q or 64
     if args.copy_calibration:
         copy_calibration()
     dev = ts.device_name()

@@ -61,7 +61,7 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 
 enum BufId {
   B_ENV, B_SCORES, B_KEYS, B_KEYS2, B_SCA, B_SCB, B_CAND, B_CANDIDX, B_ITEMS, B_ITEMS3, B_VALS, B_REGION,
-  B_RVALS, B_COUNTS, B_CTR, B_TAU, B_TOP, B_OIDX, B_OVAL, B_CELLS, B_HOSTIN, B_ITEMS2, B_NBUF
+  B_RVALS, B_COUNTS, B_CTR, B_TAU, B_TOP, B_OIDX, B_OVAL, B_CELLS, B_HOSTIN, B_ITEMS2, B_RITEMS, B_NBUF
 };
 
 struct Buf {
@@ -312,14 +312,14 @@ ts_status fill_nan(ts_engine* e, float* p, int64_t n) {
 }
 
 ts_status topk(ts_engine* e, const uint64_t* in, int64_t stride, int32_t len, const int32_t* counts, int32_t nseg,
-               int32_t k, uint64_t* out) {
+               int32_t k, uint64_t* out, const int64_t* seg_off = nullptr) {
   int64_t need = select_scratch_elems(len, nseg, k);
   uint64_t *a = nullptr, *b = nullptr;
   TS_TRY(ensure(e, B_SCA, need, &a));
   TS_TRY(ensure(e, B_SCB, need, &b));
   int nl = 0;
   return run(e, "topk_select", (double)nseg * len * 8.0, (double)nseg * len, [&] {
-    return select_topk(e->st, in, stride, len, counts, nseg, k, out, a, b, need, &nl);
+    return select_topk(e->st, in, stride, len, counts, nseg, k, out, a, b, need, &nl, seg_off);
   });
 }
 
@@ -360,7 +360,7 @@ ts_status knn_ref_batch(ts_engine* e, const ts_params& p, const ts_set* db, cons
 
 // EXACT pipeline for queries [q0, q0+nq).
 ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, const ts_set* qs, int32_t q0,
-                          int32_t nq, int32_t cap, uint64_t* top, bool* overflowed) {
+                          int32_t nq, uint64_t* top) {
   const int32_t N = db->count;
   const int32_t k = p.k;
   const bool desc = desc_measure(p.measure);
@@ -368,10 +368,11 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   const int32_t nch = (N + 2047) / 2048;
   const SetView qv = qs->view(), dv = db->view();
   const MeasureArgs ma = margs(p);
-  uint64_t *bkeys = nullptr, *seeds = nullptr, *skeys = nullptr, *stop = nullptr, *rkeys = nullptr;
-  int32_t *seed_idx = nullptr, *region = nullptr, *counts = nullptr;
+  uint64_t *bkeys = nullptr, *seeds = nullptr, *skeys = nullptr, *stop = nullptr;
+  int32_t *seed_idx = nullptr, *counts = nullptr;
+  int64_t* d_off = nullptr;
   Item* items = nullptr;
-  float *vals = nullptr, *tau = nullptr, *rvals = nullptr;
+  float *vals = nullptr, *tau = nullptr;
   TS_TRY(ensure(e, B_KEYS, (int64_t)nq * nch * s, &bkeys));
   TS_TRY(ensure(e, B_CAND, (int64_t)nq * s, &seeds));
   TS_TRY(ensure(e, B_CANDIDX, (int64_t)nq * s, &seed_idx));
@@ -379,13 +380,11 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   TS_TRY(ensure(e, B_TOP, (int64_t)nq * k, &stop));
   TS_TRY(ensure(e, B_TAU, nq, &tau));
   TS_TRY(ensure(e, B_VALS, (int64_t)nq * s, &vals));
-  TS_TRY(ensure(e, B_ITEMS, std::max<int64_t>((int64_t)nq * s, (int64_t)nq * cap), &items));
-  TS_TRY(ensure(e, B_REGION, (int64_t)nq * cap, &region));
-  TS_TRY(ensure(e, B_RVALS, (int64_t)nq * cap, &rvals));
-  TS_TRY(ensure(e, B_COUNTS, 2 * nq + 4, &counts));
-  TS_TRY(ensure(e, B_SCORES, (int64_t)nq * cap * 2, (float**)&rkeys));
+  TS_TRY(ensure(e, B_ITEMS, (int64_t)nq * s, &items));
+  TS_TRY(ensure(e, B_COUNTS, nq + 1, &counts));
+  TS_TRY(ensure(e, B_REGION, 2 * (int64_t)(nq + 1), (int32_t**)&d_off));
   const double pairs = (double)nq * N;
-  // 1. seeds: s best bounds per query
+  // 1. seeds: good candidates by bound, refined exactly
   TS_TRY(run(e, "bound_seed", pairs * 36.0, pairs, [&] {
     return launch_bound_chunks(e->st, ma, k, qv, q0, nq, dv, s, 2048, bkeys);
   }));
@@ -398,33 +397,48 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   TS_TRY(run(e, "keys", 0, 0, [&] { return launch_keys_from_regions(e->st, vals, seed_idx, nullptr, nq, s, desc, skeys); }));
   TS_TRY(topk(e, skeys, s, s, nullptr, nq, k, stop));
   TS_TRY(run(e, "threshold", 0, 0, [&] { return launch_kth_threshold(e->st, stop, nq, k, k, desc, tau); }));
-  // 3. compaction of every pair that can still reach tau
-  int32_t* overflow = counts + nq;
-  int32_t* nitems = counts + nq + 2;  // [0] total, [1..nq] offsets
-  TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, counts, 0, nq + 2); }));
+  // 3. compaction of every pair that can still reach tau: count, size each
+  // query's region exactly, write.
+  TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, counts, 0, nq); }));
   TS_TRY(run(e, "bound_compact", pairs * 36.0, pairs, [&] {
-    return launch_bound_compact(e->st, ma, k, qv, q0, nq, dv, tau, cap, region, counts, overflow);
+    return launch_bound_compact(e->st, ma, k, qv, q0, nq, dv, tau, nullptr, counts, nullptr);
   }));
-  TS_TRY(run(e, "items", 0, 0, [&] { return launch_items_from_regions(e->st, region, counts, nq, cap, q0, items, nitems); }));
+  std::vector<int32_t> hc(nq);
+  std::vector<int64_t> hoff(nq + 1);
+  TS_HIP(hipMemcpyAsync(hc.data(), counts, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, e->st));
+  TS_HIP(hipStreamSynchronize(e->st));
+  hoff[0] = 0;
+  int32_t maxc = 0;
+  for (int32_t i = 0; i < nq; ++i) {
+    hoff[i + 1] = hoff[i] + hc[i];
+    maxc = std::max(maxc, hc[i]);
+  }
+  const int64_t total = hoff[nq];
+  if (total >= (1ll << 31)) return fail(TS_ERR_UNSUPPORTED, "more than 2^31 candidate pairs in one batch");
+  Item *citems = nullptr, *items2 = nullptr;
+  float* rvals = nullptr;
+  uint64_t* rkeys = nullptr;
+  TS_TRY(ensure(e, B_ITEMS2, total, &citems));
+  TS_TRY(ensure(e, B_RITEMS, total, &items2));
+  TS_TRY(ensure(e, B_RVALS, total, &rvals));
+  TS_TRY(ensure(e, B_SCORES, 2 * total, (float**)&rkeys));
+  TS_HIP(hipMemcpyAsync(d_off, hoff.data(), sizeof(int64_t) * (nq + 1), hipMemcpyHostToDevice, e->st));
+  TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, counts, 0, nq + 1); }));
+  TS_TRY(run(e, "bound_compact", pairs * 36.0, pairs, [&] {
+    return launch_bound_compact(e->st, ma, k, qv, q0, nq, dv, tau, d_off, counts, citems);
+  }));
   // 3b. tighter O(n+m) bound on the survivors
-  Item* items2 = nullptr;
-  TS_TRY(ensure(e, B_ITEMS2, (int64_t)nq * cap, &items2));
-  int32_t* nitems2 = counts + 2 * nq + 3;  // after the region offsets (counts[nq+3 .. 2nq+2])
-  TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, nitems2, 0, 1); }));
+  int32_t* nitems2 = counts + nq;
   TS_TRY(run(e, "tight_filter", 0, 0, [&] {
-    return launch_tight_filter(e->st, ma, k, qv, q0, dv, tau, items, nitems, (int64_t)nq * cap, items2, nitems2,
+    return launch_tight_filter(e->st, ma, k, qv, q0, dv, tau, citems, nullptr, total, items2, nitems2,
                                device_cu_count() * 16);
   }));
-  TS_TRY(fill_nan(e, rvals, (int64_t)nq * cap));
-  TS_TRY(refine(e, p, db, qs, items2, nitems2, (int64_t)nq * cap, rvals));
-  // 4. top-k over the refined candidates
-  TS_TRY(run(e, "keys", 0, 0, [&] { return launch_keys_from_regions(e->st, rvals, region, counts, nq, cap, desc, rkeys); }));
-  TS_TRY(topk(e, rkeys, cap, cap, counts, nq, k, top));
-  int32_t ov = 0;
-  TS_HIP(hipMemcpyAsync(&ov, overflow, sizeof(int32_t), hipMemcpyDeviceToHost, e->st));
-  TS_HIP(hipStreamSynchronize(e->st));
-  *overflowed = ov != 0;
-  return TS_OK;
+  TS_TRY(fill_nan(e, rvals, total));
+  TS_TRY(refine(e, p, db, qs, items2, nitems2, total, rvals));
+  // 4. top-k over each query's refined candidates (slot = position in citems)
+  TS_TRY(run(e, "keys", 0, 0, [&] { return launch_keys_from_items(e->st, rvals, citems, total, desc, rkeys); }));
+  TS_HIP(hipMemcpyAsync(counts, hc.data(), sizeof(int32_t) * nq, hipMemcpyHostToDevice, e->st));
+  return topk(e, rkeys, 0, std::max(maxc, 1), counts, nq, k, top, d_off);
 }
 
 ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_params* pp, int32_t* out_idx,
@@ -474,21 +488,11 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
     int64_t s = std::min<int64_t>(std::max<int64_t>(2 * k, 16), 1024);
     bq = (int32_t)std::max<int64_t>(1, std::min<int64_t>({(int64_t)Q, 4096, (int64_t)(512ll << 20) / (8 * nch * s)}));
   }
-  int32_t cap = 4096;
   for (int32_t q0 = 0; q0 < Q; q0 += bq) {
     const int32_t nq = std::min(bq, Q - q0);
     uint64_t* topq = top + (int64_t)q0 * k;
-    if (p.mode == TS_MODE_REF) {
-      TS_TRY(knn_ref_batch(e, p, db, qs, q0, nq, env, topq));
-    } else {
-      for (;;) {
-        bool ov = false;
-        TS_TRY(knn_exact_batch(e, p, db, qs, q0, nq, std::min(cap, N), topq, &ov));
-        if (!ov || cap >= N) break;
-        cap = (int32_t)std::min<int64_t>((int64_t)cap * 4, N);
-        e->overflow_retries++;
-      }
-    }
+    if (p.mode == TS_MODE_REF) TS_TRY(knn_ref_batch(e, p, db, qs, q0, nq, env, topq));
+    else TS_TRY(knn_exact_batch(e, p, db, qs, q0, nq, topq));
   }
   TS_TRY(run(e, "decode", 0, 0, [&] { return launch_decode_keys(e->st, top, (int64_t)Q * k, desc, d_idx, d_val); }));
   TS_HIP(hipEventRecord(e->call_b, e->st));

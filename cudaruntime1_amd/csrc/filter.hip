@@ -230,13 +230,15 @@ hipError_t launch_bound_chunks(hipStream_t st, MeasureArgs m, int32_t k, SetView
 // every query of the block. A pair survives when its bound can still reach
 // the query's k-th value tau: distances bound <= tau; LCSS ub >= tau and,
 // because zero-similarity ties are broken by index, (ub > 0 or j < k).
-// Survivors are appended with a wave ballot + prefix count, one atomic per
-// wave and query.
+// Two passes: COUNT adds each query's survivors to counts[q]; WRITE appends
+// them as Items at off[q] + cursor[q] (slot = their position). Both use a
+// wave ballot + prefix count, one atomic per wave and query.
 constexpr int kCompactQ = 8, kCompactJ = 2;
 
+template <bool WRITE>
 __global__ void __launch_bounds__(256) k_bound_compact(MeasureArgs ma, int32_t k, SetView q, int32_t q0, int32_t nq,
-                                                       SetView db, const float* tau, int32_t cap,
-                                                       int32_t* region_d, int32_t* counts, int32_t* overflow) {
+                                                       SetView db, const float* tau, const int64_t* off,
+                                                       int32_t* counts, Item* items) {
   __shared__ QSum sq[kCompactQ];
   __shared__ float st[kCompactQ];
   const int qb = blockIdx.y * kCompactQ;
@@ -267,22 +269,22 @@ __global__ void __launch_bounds__(256) k_bound_compact(MeasureArgs ma, int32_t k
       const int leader = __ffsll((unsigned long long)bal) - 1;
       int base = 0;
       if (lane == leader) base = atomicAdd(&counts[qb + u], __popcll(bal));
-      base = __shfl(base, leader);
-      if (pass) {
-        const int pos = base + __popcll(bal & ((1ull << lane) - 1ull));
-        if (pos < cap) region_d[(int64_t)(qb + u) * cap + pos] = j;
-        else atomicOr(overflow, 1);
+      if (WRITE) {
+        base = __shfl(base, leader);
+        if (pass) {
+          const int64_t pos = off[qb + u] + base + __popcll(bal & ((1ull << lane) - 1ull));
+          items[pos] = Item{q0 + qb + u, j, (int32_t)pos, 0};
+        }
       }
     }
   }
 }
 
 hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, int32_t nq,
-                                SetView db, const float* tau, int32_t cap, int32_t* region_d, int32_t* counts,
-                                int32_t* overflow) {
+                                SetView db, const float* tau, const int64_t* off, int32_t* counts, Item* items) {
   dim3 grid((db.count + 256 * kCompactJ - 1) / (256 * kCompactJ), (nq + kCompactQ - 1) / kCompactQ);
-  hipLaunchKernelGGL(k_bound_compact, grid, dim3(256), 0, st, m, k, q, q0, nq, db, tau, cap, region_d, counts,
-                     overflow);
+  if (items) hipLaunchKernelGGL(k_bound_compact<true>, grid, dim3(256), 0, st, m, k, q, q0, nq, db, tau, off, counts, items);
+  else hipLaunchKernelGGL(k_bound_compact<false>, grid, dim3(256), 0, st, m, k, q, q0, nq, db, tau, off, counts, items);
   return hipGetLastError();
 }
 
@@ -349,7 +351,7 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
                                                       const float* tau, const Item* items, const int32_t* nitems,
                                                       int64_t max_items, Item* out, int32_t* nout) {
   const int lane = threadIdx.x & 63;
-  int64_t n_items = min<int64_t>(max_items, (int64_t)*nitems);
+  const int64_t n_items = nitems ? min<int64_t>(max_items, (int64_t)*nitems) : max_items;
   const int64_t nw = (int64_t)gridDim.x * 4;
   for (int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); w < n_items; w += nw) {
     const Item it = items[w];

@@ -52,14 +52,14 @@ hipError_t launch_keys_from_rows(hipStream_t st, const float* vals, int32_t rows
 
 __global__ void __launch_bounds__(256) k_topk_stage(const uint64_t* in, int64_t seg_stride, int32_t seg_len,
                                                     const int32_t* counts, int32_t k, int32_t ch,
-                                                    uint64_t* out) {
+                                                    uint64_t* out, const int64_t* seg_off) {
   __shared__ uint64_t s[kSelChunk];
   const int seg = blockIdx.y, c = blockIdx.x;
   int len = seg_len;
   if (counts) len = min(counts[seg], seg_len);
   const int begin = c * ch;
   const int nh = max(0, min(ch, len - begin));
-  const uint64_t* src = in + (int64_t)seg * seg_stride + begin;
+  const uint64_t* src = in + (seg_off ? seg_off[seg] : (int64_t)seg * seg_stride) + begin;
   for (int i = threadIdx.x; i < ch; i += 256) s[i] = i < nh ? src[i] : ~0ull;
   __syncthreads();
   for (int kk = 2; kk <= ch; kk <<= 1) {
@@ -93,7 +93,7 @@ int64_t select_scratch_elems(int32_t seg_len, int32_t nseg, int32_t k) {
 
 hipError_t select_topk(hipStream_t st, const uint64_t* in, int64_t seg_stride, int32_t seg_len,
                        const int32_t* counts, int32_t nseg, int32_t k, uint64_t* out, uint64_t* scratch_a,
-                       uint64_t* scratch_b, int64_t scratch_elems, int* launches) {
+                       uint64_t* scratch_b, int64_t scratch_elems, int* launches, const int64_t* seg_off) {
   if (nseg == 0) return hipSuccess;
   if (k > kSelChunk / 2) return hipErrorInvalidValue;
   const uint64_t* cur = in;
@@ -108,7 +108,7 @@ hipError_t select_topk(hipStream_t st, const uint64_t* in, int64_t seg_stride, i
     int ch = nch == 1 ? chunk_for(len) : kSelChunk;
     uint64_t* dst = nch == 1 ? out : bufs[which];
     if (nch > 1 && (int64_t)nseg * nch * k > scratch_elems) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_topk_stage, dim3(nch, nseg), dim3(256), 0, st, cur, stride, len, cnt, k, ch, dst);
+    hipLaunchKernelGGL(k_topk_stage, dim3(nch, nseg), dim3(256), 0, st, cur, stride, len, cnt, k, ch, dst, seg_off);
     if (launches) ++*launches;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || nch == 1) return e;
@@ -116,6 +116,7 @@ hipError_t select_topk(hipStream_t st, const uint64_t* in, int64_t seg_stride, i
     stride = (int64_t)nch * k;
     len = nch * k;
     cnt = nullptr;
+    seg_off = nullptr;
     which ^= 1;
   }
 }
@@ -197,6 +198,21 @@ hipError_t launch_keys_from_regions(hipStream_t st, const float* vals, const int
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_keys_from_regions, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, vals, region_d,
                      counts, nq, cap, descending, keys);
+  return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) k_keys_from_items(const float* vals, const Item* items, int64_t n,
+                                                         int32_t desc, uint64_t* keys) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = key_of(vals[i], items[i].d, desc != 0);
+}
+
+hipError_t launch_keys_from_items(hipStream_t st, const float* vals, const Item* items, int64_t n, int32_t descending,
+                                  uint64_t* keys) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_keys_from_items, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, vals, items, n,
+                     descending, keys);
   return hipGetLastError();
 }
 

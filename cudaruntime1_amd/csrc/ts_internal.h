@@ -103,13 +103,12 @@ hipError_t launch_filter_ref(hipStream_t st, int32_t measure, SetView q, int32_t
 // keys[(q * nchunks + chunk) * s + t].
 hipError_t launch_bound_chunks(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0,
                                int32_t nq, SetView db, int32_t s, int32_t chunk, uint64_t* keys);
-// Threshold compaction: every DB trajectory whose bound can still reach the
-// query's threshold tau[q] becomes an Item, appended per query region
-// [q*cap, q*cap+cap) with counts[q]; overflow raises *overflow.
-hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0,
-                                int32_t nq, SetView db, const float* tau, int32_t cap,
-                                int32_t* region_d, int32_t* counts, int32_t* overflow);
-
+// Threshold compaction of every DB trajectory whose bound can still reach
+// the query's threshold tau[q]: with items == null counts survivors into
+// counts[q]; otherwise appends them as Items at off[q] + counts[q] (counts
+// then serve as cursors and must start at zero).
+hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, int32_t nq,
+                                SetView db, const float* tau, const int64_t* off, int32_t* counts, Item* items);
 // Second, O(n+m) admissible bound over compacted pairs (one wave per pair);
 // survivors are appended to `out` with count *nout (zeroed by the caller).
 hipError_t launch_tight_filter(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, SetView db,
@@ -121,11 +120,17 @@ hipError_t launch_tight_filter(hipStream_t st, MeasureArgs m, int32_t k, SetView
 hipError_t launch_keys_from_rows(hipStream_t st, const float* vals, int32_t rows, int32_t cols,
                                  int32_t descending, uint64_t* keys);
 // Segment top-k: segments of seg_len keys (or counts[s] if counts != null,
-// clamped to seg_len) starting at in + s*seg_stride; writes the k smallest
-// keys of each segment (sorted, padded with ~0ull) to out + s*k.
+// clamped to seg_len) starting at in + s*seg_stride, or at in + seg_off[s]
+// when seg_off != null; writes the k smallest keys of each segment (sorted,
+// padded with ~0ull) to out + s*k.
 hipError_t select_topk(hipStream_t st, const uint64_t* in, int64_t seg_stride, int32_t seg_len,
                        const int32_t* counts, int32_t nseg, int32_t k, uint64_t* out,
-                       uint64_t* scratch_a, uint64_t* scratch_b, int64_t scratch_elems, int* launches);
+                       uint64_t* scratch_a, uint64_t* scratch_b, int64_t scratch_elems, int* launches,
+                       const int64_t* seg_off = nullptr);
+// keys[i] = key(vals[i], items[i].d) for the first n items (~0 for d < 0 or
+// a missing value).
+hipError_t launch_keys_from_items(hipStream_t st, const float* vals, const Item* items, int64_t n, int32_t descending,
+                                  uint64_t* keys);
 int64_t select_scratch_elems(int32_t seg_len, int32_t nseg, int32_t k);
 // Decode sorted keys into (idx, value) with optional index remap.
 hipError_t launch_decode_keys(hipStream_t st, const uint64_t* keys, int64_t n, int32_t descending,
