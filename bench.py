@@ -152,12 +152,106 @@ def run_c3(ts, args):
     print(json.dumps(line), flush=True)
 
 
+def run_c5(ts, args):
+    """Query serving: a 2,000,000-trajectory DB (log-uniform lengths [16, 512])
+    resident in HBM; Poisson arrivals at each stated QPS; query lengths
+    log-uniform [16, 1024]; measure mix DTW 40% / LCSS 20% / Frechet 20% /
+    Hausdorff 20%; k uniform in {1, 10, 100}; EXACT top-k. Latency is the
+    server's own clock from submit to result ready."""
+    import threading
+    t0 = time.time()
+    db_len = ts.gen_lengths(DB_SEED, args.n_db, 16, 512, True)
+    db_xy, db_off = ts.gen_walks(DB_SEED, db_len)
+    db = ts.TrajSet(db_xy, db_off)
+    del db_xy
+    log("c5 DB ready in %.1fs: %d trajectories, %.2f GB resident" % (time.time() - t0, db.count, db.nbytes() / 1e9))
+    rng = np.random.default_rng(0xA11CE)
+    nreq_max = int(max(args.qps) * args.duration) + 16
+    q_len = ts.gen_lengths(Q_SEED, nreq_max, 16, 1024, True)
+    q_xy, q_off = ts.gen_walks(Q_SEED, q_len)
+    mix = rng.choice(4, size=nreq_max, p=[0.4, 0.2, 0.2, 0.2])
+    ks = rng.choice([1, 10, 100], size=nreq_max)
+    names = ["dtw", "lcss", "frechet", "hausdorff"]
+    prm = {(m, k): ts.params(names[m], mode=ts.EXACT, k=int(k), epsilon=2e-3) for m in range(4) for k in (1, 10, 100)}
+    results = []
+    for qps in args.qps:
+        srv = ts.Server(db, streams=4, max_batch=256, batch_wait_us=500)
+        n = int(qps * args.duration)
+        arrivals = np.cumsum(rng.exponential(1.0 / qps, size=n))
+        tickets = []
+        refused = 0
+        lat = []
+        lock = threading.Lock()
+        done = threading.Event()
+
+        def waiter():
+            while True:
+                with lock:
+                    if not tickets:
+                        if done.is_set():
+                            return
+                        item = None
+                    else:
+                        item = tickets.pop(0)
+                if item is None:
+                    time.sleep(0.0005)
+                    continue
+                t, k = item
+                _, _, ms = srv.wait(t, k, timeout_ms=120000)
+                with lock:
+                    lat.append(ms)
+
+        ws = [threading.Thread(target=waiter) for _ in range(4)]
+        for w in ws:
+            w.start()
+        start = time.perf_counter()
+        for i in range(n):
+            while time.perf_counter() - start < arrivals[i]:
+                pass
+            p = prm[(int(mix[i]), int(ks[i]))]
+            try:
+                t = srv.submit(q_xy[q_off[i]:q_off[i + 1]], p)
+                with lock:
+                    tickets.append((t, p.k))
+            except ts.TrajsimError as e:
+                if e.status != 6:
+                    raise
+                refused += 1
+        submit_s = time.perf_counter() - start
+        done.set()
+        for w in ws:
+            w.join()
+        wall = time.perf_counter() - start
+        c = srv.counters()
+        srv.close()
+        lat = np.array(lat)
+        r = {"offered_qps": qps, "requests": n, "completed": int(len(lat)), "refused": refused,
+             "achieved_qps": round(len(lat) / wall, 1), "submit_s": round(submit_s, 3), "wall_s": round(wall, 3),
+             "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+             "mean_ms": round(float(lat.mean()), 3), "batches": c["batches"],
+             "mean_batch": round(len(lat) / max(1, c["batches"]), 2)}
+        log("c5", r)
+        results.append(r)
+    best = results[0]
+    line = {"metric": METRIC, "value": best["p99_ms"], "unit": "ms (p99 latency at %d QPS)" % best["offered_qps"],
+            "n_gpus": 1, "steps": len(results), "warmup": 0, "ms_per_step": None, "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "serving: %d-trajectory DB (log-uniform [16,512]) resident, Poisson arrivals, "
+                                   "query lengths log-uniform [16,1024], mix DTW 40/LCSS 20/Frechet 20/Hausdorff 20, "
+                                   "k in {1,10,100}, EXACT top-k, continuous batching over 4 streams" % db.count,
+                       "db_trajectories": db.count, "duration_s": args.duration},
+            "extra": {"sweep": results}}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=["c2", "c3"])
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c5"])
+    ap.add_argument("--qps", type=lambda v: [int(x) for x in v.split(",")], default=[2000, 5000, 10000, 20000])
+    ap.add_argument("--duration", type=float, default=2.0)
     ap.add_argument("--n-db", type=int, default=None)
     ap.add_argument("--n-q", type=int, default=None)
     ap.add_argument("--plain-sample", type=int, default=256)
@@ -178,6 +272,9 @@ def main():
         args.n_db = args.n_db or 1_000_000
         args.n_q = args.n_q or 4096
         return run_c3(ts, args)
+    if args.config == "c5":
+        args.n_db = args.n_db or 2_000_000
+        return run_c5(ts, args)
     args.n_db = args.n_db or 100_000
     args.n_q = args.n_q or 64
     if args.copy_calibration:
