@@ -152,6 +152,69 @@ def run_c3(ts, args):
     print(json.dumps(line), flush=True)
 
 
+def run_c4(ts, args):
+    """Long trajectories: 256 queries against 20,000 DB random walks, lengths
+    uniform in [2048, 8192]; DTW with a Sakoe-Chiba band of
+    w = ceil(0.1 * max(n, m)) per pair, and LCSS with eps = 2e-3 and an index
+    window delta = 64; EXACT top-10. The figure is in-band DP cell updates per
+    second (cells inside the band or window of every refined pair / time)."""
+    t0 = time.time()
+    db_len = ts.gen_lengths(DB_SEED, args.n_db, 2048, 8192, False)
+    db_xy, db_off = ts.gen_walks(DB_SEED, db_len)
+    q_len = ts.gen_lengths(Q_SEED, args.n_q, 2048, 8192, False)
+    q_xy, q_off = ts.gen_walks(Q_SEED, q_len)
+    db = ts.TrajSet(db_xy, db_off)
+    qs = ts.TrajSet(q_xy, q_off)
+    ns = min(args.plain_sample, args.n_q)
+    sample = ts.TrajSet(q_xy[:q_off[ns]], q_off[:ns + 1])
+    log("c4 workload ready in %.1fs: %d DB trajectories (%d points), %d queries" %
+        (time.time() - t0, db.count, len(db_xy), qs.count))
+    eng = ts.Engine()
+    Q, N = qs.count, db.count
+    out = {}
+    legs = (("dtw", ts.params("dtw", mode=ts.EXACT, k=10, band=-10)),
+            ("lcss", ts.params("lcss", mode=ts.EXACT, k=10, epsilon=2e-3, delta=64)))
+    for meas, p in legs:
+        eng.knn(db, sample, p)  # warm-up (workspace growth)
+        eng.set_timing(True)
+        eng.reset_stats()
+        t1 = time.perf_counter()
+        idx, val, st = eng.knn(db, qs, p)
+        tb = time.perf_counter() - t1
+        kst = eng.kernel_stats()
+        eng.set_timing(False)
+        wave = {k: v for k, v in kst.items() if "wavefront" in k}
+        dp_ms = sum(v["ms"] for v in wave.values())
+        t1 = time.perf_counter()
+        pi, pv, pst = eng.knn_plain(db, sample, p)
+        tp = time.perf_counter() - t1
+        same = bool(np.array_equal(pi, idx[:ns]) and np.array_equal(pv, val[:ns]))
+        r = {"batched_s": round(tb, 4), "pairs_per_s": Q * N / tb, "cells": st["cells"],
+             "cell_updates_per_s": st["cells"] / tb, "dp_kernel_ms": round(dp_ms, 3),
+             "dp_kernel_cell_updates_per_s": st["cells"] / (dp_ms * 1e-3) if dp_ms > 0 else None,
+             "refined_pairs_per_query": st["refined_pairs"] / Q,
+             "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3)} for k, v in kst.items()},
+             "plain_s_for_sample": round(tp, 4), "plain_sample": ns, "plain_topk_equals_batched": same}
+        if args.oracle_queries > 0:
+            import oracle
+            nq = min(args.oracle_queries, Q)
+            prm = oracle.params(p.measure, oracle.EXACT, k=10, band=p.band, delta=p.delta, epsilon=p.epsilon)
+            t1 = time.perf_counter()
+            oi, ov, _ = oracle.knn(prm, db_xy, db_off, q_xy[:q_off[nq]], q_off[:nq + 1])
+            r["oracle_check"] = {"queries": nq, "seconds": round(time.perf_counter() - t1, 2),
+                                 "topk_equal": bool(np.array_equal(oi, idx[:nq]) and np.array_equal(ov, val[:nq]))}
+        out[meas] = r
+        log("c4", meas, {k: v for k, v in r.items() if k != "kernels"})
+    line = {"metric": METRIC, "value": round(out["dtw"]["cell_updates_per_s"], 1), "unit": "cell-updates/s",
+            "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": round(out["dtw"]["batched_s"] * 1e3, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "long trajectories: %d queries vs %d DB random walks, lengths U[2048,8192], "
+                                   "DTW band ceil(0.1 max(n,m)) and LCSS eps 2e-3 delta 64, EXACT top-10" % (Q, N),
+                       "queries": Q, "db_trajectories": N, "k": 10},
+            "extra": out}
+    print(json.dumps(line), flush=True)
+
+
 def run_c5(ts, args):
     """Query serving: a 2,000,000-trajectory DB (log-uniform lengths [16, 512])
     resident in HBM; Poisson arrivals at each stated QPS; query lengths
@@ -249,7 +312,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c5"])
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5"])
     ap.add_argument("--qps", type=lambda v: [int(x) for x in v.split(",")], default=[2000, 5000, 10000, 20000])
     ap.add_argument("--duration", type=float, default=2.0)
     ap.add_argument("--n-db", type=int, default=None)
@@ -257,6 +320,8 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=256)
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--oracle-queries", type=int, default=0,
+                    help="c4: also check the top-10 of this many queries against the CPU oracle (about a minute each)")
     ap.add_argument("--quick", action="store_true", help="skip plain/latency/frechet legs (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (HBM byte-counter calibration for rocprof --pmc)")
@@ -272,6 +337,11 @@ def main():
         args.n_db = args.n_db or 1_000_000
         args.n_q = args.n_q or 4096
         return run_c3(ts, args)
+    if args.config == "c4":
+        args.n_db = args.n_db or 20_000
+        args.n_q = args.n_q or 256
+        args.plain_sample = min(args.plain_sample, 16)
+        return run_c4(ts, args)
     if args.config == "c5":
         args.n_db = args.n_db or 2_000_000
         return run_c5(ts, args)

@@ -93,10 +93,36 @@ __device__ __forceinline__ void store_vec(float* p, const float (&v)[U]) {
 // each of the column trajectory's x and y (double-buffered per pair).
 __host__ __device__ constexpr int dp_group_floats(int cmax, bool stage) { return stage ? 5 * cmax : cmax; }
 
+// Chunk range [clo, chi] (chunks of U columns) that holds every in-band cell
+// of strip s (rows s*W .. s*W+W-1). DTW band: |row*C - col*R| <= bw, so row
+// r spans columns ceil((rC-bw)/R) .. floor((rC+bw)/R); LCSS window: r-delta
+// .. r+delta; otherwise the whole row.
+template <bool BAND, bool DELTA, int W, int U>
+__device__ __forceinline__ void strip_range(int s, int R, int C, int bw, int delta, int lastch, int& clo, int& chi) {
+  if (!BAND && !DELTA) {
+    clo = 0;
+    chi = lastch;
+    return;
+  }
+  const int r0 = s * W, r1 = min(s * W + W - 1, R - 1);
+  int lo, hi;
+  if (BAND) {
+    const int64_t a = (int64_t)r0 * C - bw;
+    lo = a <= 0 ? 0 : (int)((a + R - 1) / R);
+    hi = (int)min<int64_t>(C - 1, ((int64_t)r1 * C + bw) / R);
+  } else {
+    lo = max(0, r0 - delta);
+    hi = min(C - 1, r1 + delta);
+  }
+  clo = lo / U;
+  chi = hi / U;
+}
+
 template <int MEAS, int W, int U, bool BAND, bool DELTA, bool STAGE>
 __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
   constexpr int G = 64 / W;
   constexpr float BND = (MEAS == TS_LCSS) ? 0.0f : kInf;
+  constexpr bool RANGED = BAND || DELTA;
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int nwaves = blockDim.x >> 6;
@@ -119,15 +145,20 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
   // before this lane does, i.e. on the step where rem == lg.
   int row = lg, ch = -lg, rem = lg;
   bool done = false, live = false;
-  int R = 1, C = 1, Cc = W, SW = W, slot = 0, lastch = 0;
+  int R = 1, C = 1, S = 1, slot = 0, lastch = 0;
+  // Current strip: chunks [cstart, cend); previous strip's range (lane 0
+  // reads its halo only inside it).
+  int strip = 0, cstart = 0, cend = W, pclo = 0, pcend = 0;
+  int res_row = 0, res_col = 0;
   int par = 0, gpar = 0;  // buffer of this lane's pair / of the group's next pair
   const float *Ax = a.q.x, *Ay = a.q.y, *Bx = a.q.x, *By = a.q.y;
   float ax = 0.0f, ay = 0.0f, nax = 0.0f, nay = 0.0f;
   int bd = 0, bw = 0;
+  const int delta = a.delta;
   float o[U], cx[U], cy[U], hv[U];
 #pragma unroll
   for (int j = 0; j < U; ++j) { o[j] = 0.0f; hv[j] = 0.0f; cx[j] = 0.0f; cy[j] = 0.0f; }
-  float up_last = 0.0f;
+  float up_last = 0.0f, hdiag = 0.0f;
 
   for (;;) {
     // LDS is shared by the lanes of this wave only; the barrier keeps the
@@ -180,18 +211,49 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
         }
         par = gpar ^ 1;
         lastch = (C - 1) / U;
-        Cc = max(lastch + 1, W);
-        SW = ((R + W - 1) / W) * W;
-        rem = (SW / W) * Cc;
-        row = lg;
-        ch = 0;
-        if (BAND) bw = a.band * R;
-        if (STAGE) {
-          load_vec<U>(sbx + par * cmax, cx);
-          load_vec<U>(sby + par * cmax, cy);
+        const int SW = ((R + W - 1) / W) * W;
+        // band < 0: a half-width of -band percent of the longer length.
+        if (BAND) bw = (a.band > 0 ? a.band : (-a.band * R + 99) / 100) * R;
+        // The result cell: the corner, or for an LCSS window whose band
+        // misses it, the band's last cell in the last column.
+        res_col = C - 1;
+        res_row = DELTA ? min(R - 1, C - 1 + delta) : R - 1;
+        // Strips of the pair (an LCSS window stops at the result's row), and
+        // its steps: every strip lasts at least W steps plus the shift of the
+        // next strip's range, so the halo chunk a lane 0 needs was written at
+        // least one step earlier.
+        S = DELTA ? res_row / W + 1 : SW / W;
+        int clo0, chi0;
+        strip_range<BAND, DELTA, W, U>(0, R, C, bw, delta, lastch, clo0, chi0);
+        if (RANGED) {
+          int total = 0, clo = clo0, chi = chi0;
+          for (int s2 = 0; s2 < S; ++s2) {
+            int nlo = clo, nhi = chi;
+            if (s2 + 1 < S) strip_range<BAND, DELTA, W, U>(s2 + 1, R, C, bw, delta, lastch, nlo, nhi);
+            total += max(chi - clo + 1, W + (nlo - clo));
+            clo = nlo;
+            chi = nhi;
+          }
+          rem = total;
         } else {
-          load_vec<U>(Bx, cx);
-          load_vec<U>(By, cy);
+          rem = S * max(lastch + 1, W);
+        }
+        strip = 0;
+        int nlo = clo0, nhi = chi0;
+        if (S > 1) strip_range<BAND, DELTA, W, U>(1, R, C, bw, delta, lastch, nlo, nhi);
+        cstart = clo0;
+        cend = clo0 + max(chi0 - clo0 + 1, W + (nlo - clo0));
+        pclo = 0;
+        pcend = 0;
+        row = lg;
+        ch = cstart;
+        const int lc0 = U * min(cstart, lastch);
+        if (STAGE) {
+          load_vec<U>(sbx + par * cmax + lc0, cx);
+          load_vec<U>(sby + par * cmax + lc0, cy);
+        } else {
+          load_vec<U>(Bx + lc0, cx);
+          load_vec<U>(By + lc0, cy);
         }
         nax = Ax[min(lg, R - 1)];
         nay = Ay[min(lg, R - 1)];
@@ -200,24 +262,44 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
     if (__all(done)) break;
 
     // ---- start of a row (its point was loaded one row ahead)
-    if (ch == 0 && !done) {
+    if (ch == cstart && !done) {
       ax = nax;
       ay = nay;
       const int nr = min(row + W, R - 1);
       nax = Ax[nr];
       nay = Ay[nr];
-      if (BAND) bd = row * C;
+      if (BAND) bd = row * C - cstart * U * R;
+      // Lane 0's diagonal for the strip's first cell: the halo row's value
+      // left of the range when the previous strip computed it.
+      if (RANGED) {
+        const int dc = cstart * U - 1;
+        hdiag = (dc >= 0 && dc >= pclo * U && dc < pcend * U && dc < C) ? bnd[dc] : BND;
+      }
     }
 
     // ---- U cells
     float up[U], on[U];
+    const bool first = ch == cstart;
 #pragma unroll
     for (int j = 0; j < U; ++j) {
       const float sh = shift_from_prev_lane<W>(o[j]);
-      up[j] = (lg == 0) ? ((row == 0) ? BND : hv[j]) : sh;
+      float hu = hv[j];
+      if (RANGED) {
+        // Above the previous strip's range: DTW cells there are out of band
+        // (infinite); an in-window LCSS cell there reads its diagonal instead.
+        if (ch >= pcend) hu = BND;
+      }
+      up[j] = (lg == 0) ? ((row == 0) ? BND : hu) : sh;
     }
-    float d = (ch == 0) ? ((row == 0) ? 0.0f : BND) : up_last;
-    float l = (ch == 0) ? BND : o[U - 1];
+    float d, l;
+    if (RANGED) {
+      d = first ? ((row == 0) ? (cstart == 0 ? 0.0f : BND) : (lg == 0 ? hdiag : BND)) : up_last;
+      if (MEAS == TS_LCSS && first && cstart == 0) d = 0.0f;
+      l = first ? BND : o[U - 1];
+    } else {
+      d = first ? ((row == 0) ? 0.0f : BND) : up_last;
+      l = first ? BND : o[U - 1];
+    }
 #pragma unroll
     for (int j = 0; j < U; ++j) {
       const float dx = ax - cx[j], dy = ay - cy[j];
@@ -232,11 +314,17 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
         v = fmaxf(dx * dx + dy * dy, fminf(fminf(up[j], l), d));
       } else {
         bool match = fabsf(dx) < a.eps && fabsf(dy) < a.eps;
+        float uu = up[j], ll = l;
         if (DELTA) {
+          // In-band cells read an out-of-band neighbour as their diagonal
+          // (L is constant along a column below the band and along a row
+          // right of it); the diagonal of an in-band cell is in band.
           const int dd = row - (ch * U + j);
-          match = match && (dd < 0 ? -dd : dd) <= a.delta;
+          match = match && (dd < 0 ? -dd : dd) <= delta;
+          if (dd - 1 > delta || 1 - dd > delta) uu = d;   // (row-1, col) outside
+          if (dd + 1 > delta || -1 - dd > delta) ll = d;   // (row, col-1) outside
         }
-        v = match ? d + 1.0f : fmaxf(up[j], l);
+        v = match ? d + 1.0f : fmaxf(uu, ll);
       }
       on[j] = v;
       d = up[j];
@@ -247,13 +335,13 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
     for (int j = 0; j < U; ++j) o[j] = on[j];
 
     // ---- halo row for the next strip, and the pair's result
-    if (lg == W - 1 && ch >= 0 && !done) store_vec<U>(bnd + min(ch, Cc - 1) * U, on);
-    if (!done && live && row == R - 1 && ch == lastch) {
-      const int rem_c = (C - 1) - ch * U;
+    if (lg == W - 1 && ch >= 0 && ch <= lastch && !done) store_vec<U>(bnd + ch * U, on);
+    if (!done && live && row == res_row && ch == res_col / U) {
+      const int rc = res_col - ch * U;
       float v = on[0];
 #pragma unroll
       for (int j = 1; j < U; ++j)
-        if (rem_c == j) v = on[j];
+        if (rc == j) v = on[j];
       if (MEAS == TS_FRECHET) v = sqrtf(v);
       if (MEAS == TS_LCSS) v = v / (float)C;
       a.out[slot] = v;
@@ -263,14 +351,24 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
     --rem;
     ++ch;
     if (BAND) bd -= U * R;
-    if (ch == Cc) {
-      ch = 0;
+    if (ch == cend) {
+      // next strip of this lane
+      ++strip;
       row += W;
+      pclo = cstart;
+      pcend = cend;
+      int nlo = 0, nhi = 0, nnlo = 0, nnhi = 0;
+      strip_range<BAND, DELTA, W, U>(strip, R, C, bw, delta, lastch, nlo, nhi);
+      nnlo = nlo;
+      if (strip + 1 < S) strip_range<BAND, DELTA, W, U>(strip + 1, R, C, bw, delta, lastch, nnlo, nnhi);
+      cstart = nlo;
+      cend = nlo + max(nhi - nlo + 1, W + (nnlo - nlo));
+      ch = cstart;
     }
     // ---- operands of the next step: columns of chunk ch (from the staged
     // buffer, or from global memory for the longest rows), and the halo
-    // chunk ch, written by lane W-1 at least one step ago (a row has at
-    // least W chunks) or earlier in this step, which precedes this read.
+    // chunk ch, written by lane W-1 at least one step ago or earlier in this
+    // step, which precedes this read in program order.
     const int lc = U * min(max(ch, 0), lastch);
     if (STAGE) {
       load_vec<U>(sbx + par * cmax + lc, cx);
@@ -279,7 +377,7 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
       load_vec<U>(Bx + lc, cx);
       load_vec<U>(By + lc, cy);
     }
-    load_vec<U>(bnd + min(max(ch, 0), Cc - 1) * U, hv);
+    load_vec<U>(bnd + lc, hv);
   }
 }
 
@@ -333,7 +431,7 @@ hipError_t launch_dp(hipStream_t st, MeasureArgs m, int bucket, SetView q, SetVi
   DpArgs a{q, db, items, nitems, max_items, next, out, cmax, m.band, m.delta, m.eps};
   switch (m.measure) {
     case TS_DTW:
-      return m.band > 0 ? launch_dp_b<TS_DTW, true, false>(st, a, bucket, grid_waves)
+      return m.band != 0 ? launch_dp_b<TS_DTW, true, false>(st, a, bucket, grid_waves)
                         : launch_dp_b<TS_DTW, false, false>(st, a, bucket, grid_waves);
     case TS_FRECHET: return launch_dp_b<TS_FRECHET, false, false>(st, a, bucket, grid_waves);
     case TS_LCSS:

@@ -29,24 +29,60 @@
 
 namespace ts {
 
+// DP cells a pair needs: all of them, or those inside a Sakoe-Chiba band
+// (|i*C - j*R| <= w*R, R the longer length) or an LCSS index window
+// (|i - j| <= delta), counted row by row.
+__device__ unsigned long long pair_cells(int64_t a, int64_t b, int32_t measure, int32_t band, int32_t delta) {
+  if (measure == TS_EUCLIDEAN) return (unsigned long long)a;
+  const int64_t R = max(a, b), C = min(a, b);
+  if (measure == TS_DTW && band != 0) {
+    const int64_t bw = (band > 0 ? band : (-band * R + 99) / 100) * R;
+    unsigned long long c = 0;
+    for (int64_t r = 0; r < R; ++r) {
+      const int64_t x = r * C - bw;
+      const int64_t lo = x <= 0 ? 0 : (x + R - 1) / R;
+      const int64_t hi = min(C - 1, (r * C + bw) / R);
+      if (hi >= lo) c += (unsigned long long)(hi - lo + 1);
+    }
+    return c;
+  }
+  if (measure == TS_LCSS && delta >= 0) {
+    unsigned long long c = 0;
+    for (int64_t r = 0; r < R && r - delta <= C - 1; ++r)
+      c += (unsigned long long)(min(C - 1, r + delta) - max<int64_t>(0, r - delta) + 1);
+    return c;
+  }
+  return (unsigned long long)(a * b);
+}
+
 __global__ void k_count_cells(const Item* items, const int32_t* nitems, int64_t max_items, SetView q, SetView db,
-                              int32_t measure, unsigned long long* acc) {
-  __shared__ unsigned long long part[4];
+                              int32_t measure, int32_t band, int32_t delta, unsigned long long* acc,
+                              unsigned long long* acc_pairs) {
+  __shared__ unsigned long long part[4], partp[4];
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   int64_t n = max_items;
   if (nitems) n = min<int64_t>(n, (int64_t)*nitems);
-  unsigned long long c = 0;
+  unsigned long long c = 0, np = 0;
   if (i < n) {
     const Item it = items[i];
     if (it.d >= 0) {
-      const int64_t a = q.len[it.q], b = db.len[it.d];
-      c = measure == TS_EUCLIDEAN ? (unsigned long long)a : (unsigned long long)(a * b);
+      c = pair_cells(q.len[it.q], db.len[it.d], measure, band, delta);
+      np = 1;
     }
   }
-  for (int sh = 32; sh > 0; sh >>= 1) c += __shfl_xor(c, sh);
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+  for (int sh = 32; sh > 0; sh >>= 1) {
+    c += __shfl_xor(c, sh);
+    np += __shfl_xor(np, sh);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    part[threadIdx.x >> 6] = c;
+    partp[threadIdx.x >> 6] = np;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) atomicAdd(acc, part[0] + part[1] + part[2] + part[3]);
+  if (threadIdx.x == 0) {
+    atomicAdd(acc, part[0] + part[1] + part[2] + part[3]);
+    atomicAdd(acc_pairs, partp[0] + partp[1] + partp[2] + partp[3]);
+  }
 }
 
 __global__ void k_fill_f32(float* p, float v, int64_t n) {
@@ -226,8 +262,8 @@ ts_status check_params(const ts_params* p, const ts_set* db, const ts_set* qs) {
   if (p->k < 1 || p->k > 1000) return fail(TS_ERR_INVALID, "k must be in [1, 1000]");
   if (p->mode == TS_MODE_REF && (p->kc < p->k || p->kc > 1000))
     return fail(TS_ERR_INVALID, "REF mode needs k <= kc <= 1000");
-  if (p->band < 0) return fail(TS_ERR_INVALID, "band must be >= 0");
-  if (p->band > 0 && p->measure != TS_DTW) return fail(TS_ERR_UNSUPPORTED, "band is a DTW option");
+  if (p->band < -100) return fail(TS_ERR_INVALID, "band must be >= -100 (a percentage when negative)");
+  if (p->band != 0 && p->measure != TS_DTW) return fail(TS_ERR_UNSUPPORTED, "band is a DTW option");
   if (p->delta >= 0 && p->measure != TS_LCSS) return fail(TS_ERR_UNSUPPORTED, "delta is an LCSS option");
   if (p->measure == TS_LCSS && !(p->epsilon > 0.0f)) return fail(TS_ERR_INVALID, "LCSS needs epsilon > 0");
   if (p->mode == TS_MODE_REF && p->r < 0) return fail(TS_ERR_INVALID, "r must be >= 0");
@@ -244,11 +280,12 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   const MeasureArgs ma = margs(p);
   const int cus = device_cu_count();
   unsigned long long* cells = nullptr;
-  TS_TRY(ensure(e, B_CELLS, kCellSlots, &cells));
+  TS_TRY(ensure(e, B_CELLS, 2 * kCellSlots, &cells));
   const int slot = e->cells_used < kCellSlots ? e->cells_used++ : kCellSlots - 1;
   TS_TRY(run(e, "count_cells", 0, 0, [&] {
     hipLaunchKernelGGL(k_count_cells, dim3((unsigned)((max_items + 255) / 256)), dim3(256), 0, e->st, items,
-                       nitems_dev, max_items, qv, dv, p.measure, cells + slot);
+                       nitems_dev, max_items, qv, dv, p.measure, p.band, p.delta, cells + slot,
+                       cells + kCellSlots + slot);
     return hipGetLastError();
   }));
   int32_t* ctr = nullptr;
@@ -267,7 +304,7 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   // launch per bucket that can occur for these two sets.
   const int32_t lo = std::min(qs->min_len, db->min_len), hi = std::min(qs->max_len, db->max_len);
   const int b_lo = dp_bucket_of(lo), b_hi = dp_bucket_of(hi);
-  if (p.measure == TS_DTW && p.band > 0 && (int64_t)std::max(qs->max_len, db->max_len) * std::max(qs->max_len, db->max_len) >= (1ll << 31))
+  if (p.measure == TS_DTW && p.band != 0 && (int64_t)std::max(qs->max_len, db->max_len) * std::max(qs->max_len, db->max_len) >= (1ll << 31))
     return fail(TS_ERR_UNSUPPORTED, "band needs trajectory lengths below 46341");
   static const char* kNames[2][3] = {{"dtw_wavefront", "frechet_wavefront", "lcss_wavefront"},
                                      {"dtw_wavefront_seed", "frechet_wavefront_seed", "lcss_wavefront_seed"}};
@@ -463,9 +500,9 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
   TS_TRY(ensure(e, B_OVAL, (int64_t)Q * k, &d_val));
   TS_TRY(ensure(e, B_HOSTIN, (int64_t)Q * k, &top));
   unsigned long long* cells = nullptr;
-  TS_TRY(ensure(e, B_CELLS, kCellSlots, &cells));
+  TS_TRY(ensure(e, B_CELLS, 2 * kCellSlots, &cells));
   TS_TRY(run(e, "fill", 0, 0, [&] {
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, e->st, cells, kCellSlots);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, e->st, cells, 2 * kCellSlots);
     return hipGetLastError();
   }));
   if (!e->call_a) {
@@ -500,15 +537,18 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
   TS_HIP(hipMemcpyAsync(out_val, d_val, sizeof(float) * Q * k, hipMemcpyDeviceToHost, e->st));
   TS_HIP(hipStreamSynchronize(e->st));
   if (stats) {
-    std::vector<unsigned long long> hc(kCellSlots);
-    TS_HIP(hipMemcpy(hc.data(), cells, sizeof(unsigned long long) * kCellSlots, hipMemcpyDeviceToHost));
+    std::vector<unsigned long long> hc(2 * kCellSlots);
+    TS_HIP(hipMemcpy(hc.data(), cells, sizeof(unsigned long long) * 2 * kCellSlots, hipMemcpyDeviceToHost));
     float dms = 0;
     TS_HIP(hipEventElapsedTime(&dms, e->call_a, e->call_b));
     stats->ms_device = dms;
     stats->pairs = (int64_t)Q * N;
     stats->cells = 0;
-    for (int i = 0; i < e->cells_used && i < kCellSlots; ++i) stats->cells += (int64_t)hc[i];
-    stats->refined_pairs = -1;
+    stats->refined_pairs = 0;
+    for (int i = 0; i < e->cells_used && i < kCellSlots; ++i) {
+      stats->cells += (int64_t)hc[i];
+      stats->refined_pairs += (int64_t)hc[kCellSlots + i];
+    }
     stats->launches = e->launches - l0;
     stats->overflow_retries = e->overflow_retries;
   }
@@ -629,7 +669,7 @@ ts_status ts_pair_values(ts_engine* e, const ts_set* db, const ts_set* qs, const
   TS_HIP(hipMemcpyAsync(items, h.data(), sizeof(Item) * npairs, hipMemcpyHostToDevice, e->st));
   e->cells_used = 0;
   unsigned long long* cells = nullptr;
-  TS_TRY(ensure(e, B_CELLS, kCellSlots, &cells));
+  TS_TRY(ensure(e, B_CELLS, 2 * kCellSlots, &cells));
   TS_TRY(refine(e, *p, db, qs, items, nullptr, npairs, vals));
   TS_HIP(hipMemcpyAsync(out, vals, sizeof(float) * npairs, hipMemcpyDeviceToHost, e->st));
   TS_HIP(hipStreamSynchronize(e->st));

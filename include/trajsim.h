@@ -67,7 +67,10 @@ typedef struct ts_params {
   int32_t k;       /* results per query (REF: k <= kc) */
   int32_t kc;      /* REF candidates per query (reference: 10; Euclidean 50) */
   int32_t r;       /* REF envelope half-width (reference: 5) */
-  int32_t band;    /* DTW Sakoe-Chiba half-width w >= 1, 0 = unconstrained */
+  int32_t band;    /* DTW Sakoe-Chiba half-width: w = band when > 0, w =
+                      ceil(-band * max(n,m) / 100) per pair when in
+                      [-100, -1] (a percentage), 0 = unconstrained. Cell
+                      (i,j) is inside iff |i*m - j*n| <= w * max(n,m). */
   int32_t delta;   /* LCSS index window, < 0 = none (the reference) */
   float epsilon;   /* LCSS per-axis match threshold */
 } ts_params;

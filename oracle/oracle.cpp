@@ -106,7 +106,11 @@ template <class T> inline T pdist(Pt a, Pt b) {
 // Sakoe-Chiba band predicate (build extension, no reference counterpart):
 // cell (i,j), 0-based, is inside iff |i*m - j*n| <= w*max(n,m). For n == m it
 // is |i-j| <= w; w >= max(n,m) admits every cell; w >= 1 keeps (0,0),
-// (n-1,m-1) and a monotone path between them inside.
+// (n-1,m-1) and a monotone path between them inside. band > 0 is w itself;
+// band < 0 asks for w = ceil(-band * max(n,m) / 100), a per-pair percentage.
+inline int band_width(int band, int n, int m) {
+  return band > 0 ? band : (-band * std::max(n, m) + 99) / 100;
+}
 inline bool in_band(int i, int j, int n, int m, int w) {
   if (w <= 0) return true;
   int64_t d = (int64_t)i * m - (int64_t)j * n;
@@ -118,12 +122,13 @@ inline bool in_band(int i, int j, int n, int m, int w) {
 // dtw.cpp:64-100. Cells outside the band are 1e20f.
 template <class T> T dtw(const Pt* q, int n, const Pt* c, int m, int band) {
   const T inf = (T)kInf;
+  const int w = band == 0 ? 0 : band_width(band, n, m);
   std::vector<T> prev(m + 1, inf), curr(m + 1, inf);
   prev[0] = 0;
   for (int u = 1; u <= n; ++u) {
     curr[0] = inf;
     for (int v = 1; v <= m; ++v) {
-      if (!in_band(u - 1, v - 1, n, m, band)) { curr[v] = inf; continue; }
+      if (!in_band(u - 1, v - 1, n, m, w)) { curr[v] = inf; continue; }
       T cost = pdist<T>(q[u - 1], c[v - 1]);
       curr[v] = cost + std::min({prev[v], curr[v - 1], prev[v - 1]});
     }

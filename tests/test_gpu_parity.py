@@ -60,7 +60,7 @@ def test_pair_values_bit_exact_all_shapes(eng, shapes, measure):
     check_pairs(eng, shapes, measure, epsilon=3e-2)
 
 
-@pytest.mark.parametrize("band", [1, 3, 10, 40])
+@pytest.mark.parametrize("band", [1, 3, 10, 40, -1, -10, -55])
 def test_dtw_band_bit_exact(eng, shapes, band):
     say("dtw band", band)
     check_pairs(eng, shapes, "dtw", band=band)
@@ -92,7 +92,9 @@ def test_long_pairs_bit_exact(eng):
     q_xy, q_off = walks(25, [2500, 900])
     db, qs = ts.TrajSet(db_xy, db_off), ts.TrajSet(q_xy, q_off)
     qi, di = np.array([0, 0, 1, 1, 0]), np.array([0, 1, 0, 2, 2])
-    for measure, kw in [("dtw", {}), ("dtw", {"band": 205}), ("frechet", {}), ("lcss", {"epsilon": 2e-3, "delta": 64})]:
+    for measure, kw in [("dtw", {}), ("dtw", {"band": 205}), ("dtw", {"band": -10}), ("frechet", {}),
+                        ("lcss", {"epsilon": 2e-3, "delta": 64}), ("lcss", {"epsilon": 2e-3, "delta": 0}),
+                        ("lcss", {"epsilon": 2e-3, "delta": 1000})]:
         p = ts.params(measure, **kw)
         prm = oracle.params(p.measure, band=p.band, delta=p.delta, epsilon=p.epsilon)
         got = eng.pair_values(db, qs, p, qi, di)
@@ -166,6 +168,24 @@ def test_knn_batched_and_plain_match_oracle(eng, measure, mode):
         gi, gv, st = fn(db, qs, p)
         assert np.array_equal(gi, wi), (fn.__name__, gi[:2], wi[:2])
         assert np.array_equal(gv.view(np.int32), wv.view(np.int32)), (fn.__name__, gv[:2], wv[:2])
+
+
+@pytest.mark.parametrize("measure,kw", [("dtw", {"band": -10}), ("lcss", {"epsilon": 2e-3, "delta": 64})])
+def test_knn_long_banded_match_oracle(eng, measure, kw):
+    """Long-trajectory shape (lengths U[2048,4096]) with a per-pair 10% band
+    or an LCSS window: EXACT top-5 through the banded multi-strip DP."""
+    say("knn long", measure, kw)
+    L = uniform_lengths(31, 60, 2048, 4096)
+    db_xy, db_off = walks(31, L, 2e-3)
+    q_xy, q_off = planted(db_xy, db_off, np.array([3, 40]), L[[3, 40]] // 2 + 17, 5e-4, 32)
+    db, qs = ts.TrajSet(db_xy, db_off), ts.TrajSet(q_xy, q_off)
+    p = ts.params(measure, mode=ts.EXACT, k=5, **kw)
+    prm = oracle.params(p.measure, p.mode, k=p.k, band=p.band, delta=p.delta, epsilon=p.epsilon)
+    wi, wv, _ = oracle.knn(prm, db_xy, db_off, q_xy, q_off)
+    for fn in (eng.knn, eng.knn_plain):
+        gi, gv, st = fn(db, qs, p)
+        assert np.array_equal(gi, wi), (fn.__name__, gi, wi)
+        assert np.array_equal(gv.view(np.int32), wv.view(np.int32)), (fn.__name__, gv, wv)
 
 
 def test_compat_reproduces_reference_fixtures(eng):

@@ -109,6 +109,11 @@ def test_dtw_band_brute_force():
     # a band at least as wide as the longer side is the unconstrained DTW
     q, c = rand_traj(rng, 30), rand_traj(rng, 21)
     assert oracle.dtw(q, c, band=30) == oracle.dtw(q, c)
+    # a negative band is a percentage of the longer length, rounded up
+    assert oracle.dtw(q, c, band=-100) == oracle.dtw(q, c)
+    assert oracle.dtw(q, c, band=-10) == oracle.dtw(q, c, band=3)  # ceil(0.1 * 30)
+    assert oracle.dtw(c, q, band=-34) == oracle.dtw(c, q, band=11)  # ceil(0.34 * 30)
+    assert oracle.dtw(q, c, band=-1) == oracle.dtw(q, c, band=1)
 
 
 def test_dtw_closed_forms_and_symmetry():
