@@ -50,7 +50,8 @@ struct DpArgs {
   int64_t max_items;
   int32_t* next;
   float* out;
-  int32_t cmax;
+  int32_t cmax;   // LDS halo floats per group (and staged column capacity)
+  int32_t hmask;  // halo index mask: -1, or ring length - 1
   int32_t band;
   int32_t delta;
   float eps;
@@ -128,6 +129,10 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
   const int nwaves = blockDim.x >> 6;
   const int g = lane / W, lg = lane & (W - 1);
   const int cmax = a.cmax;
+  // Halo slot of a column: with a band the halo is a ring that holds one
+  // strip's range (its length is at least the longest range plus a chunk,
+  // so a slot is rewritten only after lane 0 of the next strip read it).
+  const int hmask = RANGED ? a.hmask : -1;
   extern __shared__ float4 lds_dyn4[];
   int* slot_item = (int*)lds_dyn4 + wv * G + g;
   // Group areas start on 16-byte boundaries (cmax is a multiple of 4).
@@ -273,7 +278,7 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
       // left of the range when the previous strip computed it.
       if (RANGED) {
         const int dc = cstart * U - 1;
-        hdiag = (dc >= 0 && dc >= pclo * U && dc < pcend * U && dc < C) ? bnd[dc] : BND;
+        hdiag = (dc >= 0 && dc >= pclo * U && dc < pcend * U && dc < C) ? bnd[dc & hmask] : BND;
       }
     }
 
@@ -335,7 +340,7 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
     for (int j = 0; j < U; ++j) o[j] = on[j];
 
     // ---- halo row for the next strip, and the pair's result
-    if (lg == W - 1 && ch >= 0 && ch <= lastch && !done) store_vec<U>(bnd + ch * U, on);
+    if (lg == W - 1 && ch >= 0 && ch <= lastch && !done) store_vec<U>(bnd + ((ch * U) & hmask), on);
     if (!done && live && row == res_row && ch == res_col / U) {
       const int rc = res_col - ch * U;
       float v = on[0];
@@ -377,7 +382,7 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
       load_vec<U>(Bx + lc, cx);
       load_vec<U>(By + lc, cy);
     }
-    load_vec<U>(bnd + lc, hv);
+    load_vec<U>(bnd + (lc & hmask), hv);
   }
 }
 
@@ -389,12 +394,26 @@ struct Bucket {
 constexpr Bucket kBuckets[kDpBuckets] = {{16, 1, 31},  {16, 2, 63},   {16, 4, 127},
                                           {32, 4, 255}, {64, 4, 1023}, {64, 4, 1 << 24}};
 
-// Halo length (floats) a bucket needs when the longest C in it is cmax_c.
-int32_t dp_halo_len(int b, int32_t cmax_c) {
+// Halo of a bucket. Without a band: a whole row of the longest C in the
+// bucket. With a band: a row of the longest C when that is short, else a
+// ring of 2^k floats covering a strip's range, which spans at most
+// W-1+2w+2U columns (band half-width w, 2w+1 <= width) or W+1 chunks plus
+// the range's shift, plus the chunk read before the strip starts.
+DpHalo dp_halo(int b, const MeasureArgs& m, int32_t cmax_c, int32_t wmax_c) {
   const Bucket& k = kBuckets[b];
-  const int32_t c = std::min<int32_t>(cmax_c, k.cmax_cap);
-  const int32_t chunks = std::max((c + k.U - 1) / k.U, k.W);
-  return ((chunks * k.U + 3) / 4) * 4;
+  if (!dp_ranged(m)) {
+    const int32_t c = std::min<int32_t>(cmax_c, k.cmax_cap);
+    const int32_t chunks = std::max((c + k.U - 1) / k.U, k.W);
+    return DpHalo{((chunks * k.U + 3) / 4) * 4, -1};
+  }
+  const int32_t chunks = std::max((cmax_c + k.U - 1) / k.U, k.W);
+  const int32_t full = ((chunks * k.U + 3) / 4) * 4;
+  const int32_t wd = std::min<int32_t>(wmax_c, k.cmax_cap);
+  const int32_t need = std::max(k.W + wd + 4 * k.U, k.W * k.U + k.W + 4 * k.U);
+  int32_t h = 4;
+  while (h < need) h *= 2;
+  if (h >= full) return DpHalo{full, -1};
+  return DpHalo{h, h - 1};
 }
 
 template <int MEAS, int W, int U, bool BAND, bool DELTA, bool STAGE>
@@ -415,6 +434,17 @@ static hipError_t launch_dp_t(hipStream_t st, const DpArgs& a, int grid_waves) {
 
 template <int MEAS, bool BAND, bool DELTA>
 static hipError_t launch_dp_b(hipStream_t st, const DpArgs& a, int bucket, int grid_waves) {
+  if (BAND || DELTA) {
+    // Banded rows read their columns from global memory: a pair's column
+    // trajectory can be far longer than the band the LDS holds.
+    switch (bucket) {
+      case 0: return launch_dp_t<MEAS, 16, 1, BAND, DELTA, false>(st, a, grid_waves);
+      case 1: return launch_dp_t<MEAS, 16, 2, BAND, DELTA, false>(st, a, grid_waves);
+      case 2: return launch_dp_t<MEAS, 16, 4, BAND, DELTA, false>(st, a, grid_waves);
+      case 3: return launch_dp_t<MEAS, 32, 4, BAND, DELTA, false>(st, a, grid_waves);
+      default: return launch_dp_t<MEAS, 64, 4, BAND, DELTA, false>(st, a, grid_waves);
+    }
+  }
   switch (bucket) {
     case 0: return launch_dp_t<MEAS, 16, 1, BAND, DELTA, true>(st, a, grid_waves);
     case 1: return launch_dp_t<MEAS, 16, 2, BAND, DELTA, true>(st, a, grid_waves);
@@ -426,9 +456,9 @@ static hipError_t launch_dp_b(hipStream_t st, const DpArgs& a, int bucket, int g
 }
 
 hipError_t launch_dp(hipStream_t st, MeasureArgs m, int bucket, SetView q, SetView db, const Item* items,
-                     const int32_t* nitems, int64_t max_items, int32_t* next, float* out, int32_t cmax,
+                     const int32_t* nitems, int64_t max_items, int32_t* next, float* out, DpHalo halo,
                      int grid_waves) {
-  DpArgs a{q, db, items, nitems, max_items, next, out, cmax, m.band, m.delta, m.eps};
+  DpArgs a{q, db, items, nitems, max_items, next, out, halo.len, halo.mask, m.band, m.delta, m.eps};
   switch (m.measure) {
     case TS_DTW:
       return m.band != 0 ? launch_dp_b<TS_DTW, true, false>(st, a, bucket, grid_waves)

@@ -300,10 +300,12 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
       return launch_euclidean(e->st, qv, dv, items, nitems_dev, max_items, vals);
     }, slot);
   }
-  // Wavefront DP: split by bucket of the shorter length, one persistent
-  // launch per bucket that can occur for these two sets.
+  // Wavefront DP: split by bucket of the row width the DP visits, one
+  // persistent launch per bucket that can occur for these two sets (the
+  // width grows with both lengths).
   const int32_t lo = std::min(qs->min_len, db->min_len), hi = std::min(qs->max_len, db->max_len);
-  const int b_lo = dp_bucket_of(lo), b_hi = dp_bucket_of(hi);
+  const int32_t wlo = dp_width_of(qs->min_len, db->min_len, ma), whi = dp_width_of(qs->max_len, db->max_len, ma);
+  const int b_lo = dp_bucket_of(wlo), b_hi = dp_bucket_of(whi);
   if (p.measure == TS_DTW && p.band != 0 && (int64_t)std::max(qs->max_len, db->max_len) * std::max(qs->max_len, db->max_len) >= (1ll << 31))
     return fail(TS_ERR_UNSUPPORTED, "band needs trajectory lengths below 46341");
   static const char* kNames[2][3] = {{"dtw_wavefront", "frechet_wavefront", "lcss_wavefront"},
@@ -312,26 +314,27 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   const int waves = cus * dp_waves_per_cu(e, p.measure);
   if (b_lo == b_hi) {
     TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, ctr, 0, 1); }));
-    const int32_t cmax = dp_halo_len(b_lo, hi);
+    const DpHalo halo = dp_halo(b_lo, ma, hi, whi);
     return run(e, nm, 0, 0, [&] {
-      return launch_dp(e->st, ma, b_lo, qv, dv, items, nitems_dev, max_items, ctr, vals, cmax, waves);
+      return launch_dp(e->st, ma, b_lo, qv, dv, items, nitems_dev, max_items, ctr, vals, halo, waves);
     }, slot);
   }
   Item* itemsb = nullptr;
   TS_TRY(ensure(e, B_ITEMS3, (int64_t)kDpBuckets * max_items, &itemsb));
   TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, ctr, 0, 2 * kDpBuckets); }));
   TS_TRY(run(e, "items_by_bucket", 0, 0, [&] {
-    return launch_items_by_bucket(e->st, items, nitems_dev, max_items, qv, dv, itemsb, max_items, ctr + kDpBuckets);
+    return launch_items_by_bucket(e->st, items, nitems_dev, max_items, qv, dv, ma, itemsb, max_items,
+                                  ctr + kDpBuckets);
   }));
   // Buckets run concurrently on their own streams (fork/join by events), so
   // the latency-bound tail of one overlaps the others.
   TS_HIP(hipEventRecord(e->fork, e->st));
   for (int b = b_lo; b <= b_hi; ++b) {
-    const int32_t cmax = dp_halo_len(b, hi);
+    const DpHalo halo = dp_halo(b, ma, hi, whi);
     TS_HIP(hipStreamWaitEvent(e->aux[b], e->fork, 0));
     TS_TRY(run_on(e, e->aux[b], nm, 0, 0, [&] {
       return launch_dp(e->aux[b], ma, b, qv, dv, itemsb + (int64_t)b * max_items, ctr + kDpBuckets + b, max_items,
-                       ctr + b, vals, cmax, waves);
+                       ctr + b, vals, halo, waves);
     }, slot));
     TS_HIP(hipEventRecord(e->join[b], e->aux[b]));
     TS_HIP(hipStreamWaitEvent(e->st, e->join[b], 0));

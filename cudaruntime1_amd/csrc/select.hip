@@ -228,14 +228,15 @@ hipError_t launch_fill_i32(hipStream_t st, int32_t* p, int32_t v, int64_t n) {
 }
 
 __global__ void __launch_bounds__(256) k_items_by_bucket(const Item* items, const int32_t* nitems,
-                                                         int64_t max_items, SetView q, SetView db, Item* outb,
-                                                         int64_t cap_each, int32_t* countsb) {
+                                                         int64_t max_items, SetView q, SetView db,
+                                                         MeasureArgs m, Item* outb, int64_t cap_each,
+                                                         int32_t* countsb) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   int64_t n = nitems ? (int64_t)*nitems : max_items;
   if (n > max_items) n = max_items;
   Item it{0, -1, 0, 0};
   if (i < n) it = items[i];
-  const int b = it.d >= 0 ? dp_bucket_of(min(q.len[it.q], db.len[it.d])) : -1;
+  const int b = it.d >= 0 ? dp_bucket_of(dp_width_of(q.len[it.q], db.len[it.d], m)) : -1;
   const int lane = threadIdx.x & 63;
   // One atomic per wave and bucket (wave ballot + prefix count).
   for (int bb = 0; bb < kDpBuckets; ++bb) {
@@ -253,10 +254,11 @@ __global__ void __launch_bounds__(256) k_items_by_bucket(const Item* items, cons
 }
 
 hipError_t launch_items_by_bucket(hipStream_t st, const Item* items, const int32_t* nitems, int64_t max_items,
-                                  SetView q, SetView db, Item* outb, int64_t cap_each, int32_t* countsb) {
+                                  SetView q, SetView db, MeasureArgs m, Item* outb, int64_t cap_each,
+                                  int32_t* countsb) {
   if (max_items == 0) return hipSuccess;
   hipLaunchKernelGGL(k_items_by_bucket, dim3((unsigned)((max_items + 255) / 256)), dim3(256), 0, st, items, nitems,
-                     max_items, q, db, outb, cap_each, countsb);
+                     max_items, q, db, m, outb, cap_each, countsb);
   return hipGetLastError();
 }
 

@@ -58,6 +58,27 @@ struct MeasureArgs {
   float eps;
 };
 
+// True when the DP visits only a band of each row: a DTW Sakoe-Chiba band
+// or an LCSS index window.
+__host__ __device__ inline bool dp_ranged(const MeasureArgs& m) {
+  return (m.measure == TS_DTW && m.band != 0) || (m.measure == TS_LCSS && m.delta >= 0);
+}
+
+// Columns of a row the DP visits for an n x m pair: the shorter length C,
+// or for a band of half-width w (w = band, or ceil(-band * max(n,m) / 100)
+// for a negative band) or a window delta, at most 2w+1 or 2*delta+1.
+__host__ __device__ inline int32_t dp_width_of(int32_t n, int32_t m, const MeasureArgs& a) {
+  const int32_t c = n < m ? n : m, r = n < m ? m : n;
+  int64_t e = c;
+  if (a.measure == TS_DTW && a.band != 0) {
+    const int64_t w = a.band > 0 ? a.band : ((int64_t)-a.band * r + 99) / 100;
+    e = 2 * w + 1;
+  } else if (a.measure == TS_LCSS && a.delta >= 0) {
+    e = 2 * (int64_t)a.delta + 1;
+  }
+  return (int32_t)(e < c ? e : c);
+}
+
 }  // namespace ts
 
 // Device-resident set (the opaque ts_set of the C ABI).
@@ -148,24 +169,31 @@ hipError_t launch_keys_from_regions(hipStream_t st, const float* vals, const int
                                     const int32_t* counts, int32_t nq, int32_t cap, int32_t descending,
                                     uint64_t* keys);
 hipError_t launch_fill_i32(hipStream_t st, int32_t* p, int32_t v, int64_t n);
-// Split items by DP bucket (dp_bucket_of the shorter length) into
-// kDpBuckets lists of cap_each entries.
+// Split items by DP bucket (dp_bucket_of the visited row width,
+// dp_width_of) into kDpBuckets lists of cap_each entries.
 hipError_t launch_items_by_bucket(hipStream_t st, const Item* items, const int32_t* nitems, int64_t max_items,
-                                  SetView q, SetView db, Item* outb, int64_t cap_each, int32_t* countsb);
+                                  SetView q, SetView db, MeasureArgs m, Item* outb, int64_t cap_each,
+                                  int32_t* countsb);
 hipError_t launch_items_from_regions(hipStream_t st, const int32_t* region_d, const int32_t* counts,
                                      int32_t nq, int32_t cap, int32_t q0, Item* items, int32_t* nitems);
 
 // dp.hip -- persistent wavefront DP over a device item list, one launch per
-// bucket of the pair's shorter length C: < 32, < 64, < 128, < 256, < 1024,
-// >= 1024.
+// bucket of the row width the DP visits (the shorter length C, or the band
+// width): < 32, < 64, < 128, < 256, < 1024, >= 1024.
 constexpr int kDpBuckets = 6;
 __host__ __device__ inline int dp_bucket_of(int32_t c) {
   return c <= 31 ? 0 : (c <= 63 ? 1 : (c <= 127 ? 2 : (c <= 255 ? 3 : (c <= 1023 ? 4 : 5))));
 }
-// LDS halo length (floats) for bucket b when its longest C is at most cmax_c.
-int32_t dp_halo_len(int b, int32_t cmax_c);
+// LDS halo of bucket b: its length in floats, and the mask applied to
+// column indexes (-1: the halo holds a whole row; otherwise a ring of
+// len = mask + 1 floats that holds the band of one strip). cmax_c bounds
+// the pairs' shorter length, wmax_c their dp_width_of.
+struct DpHalo {
+  int32_t len, mask;
+};
+DpHalo dp_halo(int b, const MeasureArgs& m, int32_t cmax_c, int32_t wmax_c);
 hipError_t launch_dp(hipStream_t st, MeasureArgs m, int bucket, SetView q, SetView db, const Item* items,
-                     const int32_t* nitems, int64_t max_items, int32_t* next, float* out, int32_t cmax,
+                     const int32_t* nitems, int64_t max_items, int32_t* next, float* out, DpHalo halo,
                      int grid_waves);
 
 // reduce.hip -- Hausdorff (wave per pair) and lock-step Euclidean (lane per pair).
