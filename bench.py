@@ -106,6 +106,90 @@ def copy_calibration(nbytes=256 << 20, reps=3):
     log("copy calibration: %d x %d MiB device copies" % (reps, nbytes >> 20))
 
 
+def c2_other_measures(ts, eng, db, qs, args):
+    """The other measures of the 64 x 100k workload, batched EXACT top-10:
+    LCSS (eps 2e-3, window 16), Hausdorff, and lock-step Euclidean on a
+    separately generated DB of 100,000 walks of 128 points (Euclidean needs
+    equal lengths) with 64 queries of 128 points."""
+    Q, N = qs.count, db.count
+    out = {}
+    legs = [("lcss", ts.params("lcss", mode=ts.EXACT, k=10, epsilon=2e-3, delta=16), db, qs),
+            ("hausdorff", ts.params("hausdorff", mode=ts.EXACT, k=10), db, qs)]
+    e_db_xy, e_db_off = ts.gen_walks(DB_SEED + 1, np.full(N, 128, np.int32))
+    e_q_xy, e_q_off = ts.gen_walks(Q_SEED + 1, np.full(Q, 128, np.int32))
+    e_db, e_qs = ts.TrajSet(e_db_xy, e_db_off), ts.TrajSet(e_q_xy, e_q_off)
+    legs.append(("euclidean", ts.params("euclidean", mode=ts.EXACT, k=10), e_db, e_qs))
+    for name, p, d, q in legs:
+        ms, (_, _, st) = timed(lambda: eng.knn(d, q, p), max(1, args.steps // 2), 1)
+        out[name] = {"pairs_per_s": q.count * d.count / (ms * 1e-3), "ms_per_step": round(ms, 4),
+                     "cells_per_step": st["cells"], "cell_updates_per_s": st["cells"] / (ms * 1e-3),
+                     "refined_pairs_per_step": st["refined_pairs"]}
+        log("c2", name, out[name])
+    return out
+
+
+def cpu_extra(db_xy, db_off, q_xy, q_off, n_one=4):
+    """More CPU figures on the same workload: the oracle with one thread
+    (the reference's CPU build ran single-threaded) on n_one queries, and the
+    REF pipeline (envelope filter, top-10 candidates, exact refine) over all
+    queries on every core."""
+    import oracle
+    n_db = len(db_off) - 1
+    cores = oracle.num_threads()
+    prm = oracle.params(oracle.DTW, oracle.EXACT, k=10)
+    oracle.set_threads(1)
+    try:
+        t0 = time.perf_counter()
+        oracle.knn(prm, db_xy, db_off, q_xy[:q_off[n_one]], q_off[:n_one + 1])
+        dt1 = time.perf_counter() - t0
+    finally:
+        oracle.set_threads(cores)
+    ref = oracle.params(oracle.DTW, oracle.REF, k=10)
+    t0 = time.perf_counter()
+    oracle.knn(ref, db_xy, db_off, q_xy, q_off)
+    dtr = time.perf_counter() - t0
+    nq = len(q_off) - 1
+    return {"exact_1_thread_pairs_per_s": round(n_one * n_db / dt1, 1), "exact_1_thread_sample": n_one,
+            "ref_pairs_per_s": round(nq * n_db / dtr, 1), "ref_cores": cores, "ref_seconds": round(dtr, 3)}
+
+
+def run_c1(ts, args):
+    """The reference's CPU-sized case: 1 query against 2,000 DB random walks,
+    lengths uniform [32, 128]; DTW REF (r=5, 10 candidates) and EXACT top-10;
+    the oracle on the host cores beside it."""
+    import oracle
+    db_xy, db_off, q_xy, q_off = build_workload(ts, 2000, 1, 32, 128)
+    db, qs = ts.TrajSet(db_xy, db_off), ts.TrajSet(q_xy, q_off)
+    eng = ts.Engine()
+    N = db.count
+    out = {}
+    for mode, name in ((ts.REF, "ref"), (ts.EXACT, "exact")):
+        p = ts.params("dtw", mode=mode, k=10)
+        ms, (gi, gv, st) = timed(lambda: eng.knn(db, qs, p), max(args.steps, 50), args.warmup)
+        prm = oracle.params(oracle.DTW, mode, k=10)
+        reps = 20
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            oi, ov, _ = oracle.knn(prm, db_xy, db_off, q_xy, q_off)
+        cms = (time.perf_counter() - t0) * 1e3 / reps
+        out[name] = {"gpu_ms_per_query": round(ms, 4), "gpu_pairs_per_s": N / (ms * 1e-3),
+                     "gpu_cell_updates_per_s": st["cells"] / (ms * 1e-3), "cells": st["cells"],
+                     "cpu_ms_per_query": round(cms, 4), "cpu_pairs_per_s": N / (cms * 1e-3),
+                     "topk_equal_oracle": bool(np.array_equal(gi, oi) and np.array_equal(gv, ov))}
+        log("c1", name, out[name])
+    line = {"metric": METRIC, "value": round(out["exact"]["gpu_pairs_per_s"], 1), "unit": "pairs/s", "n_gpus": 1,
+            "steps": max(args.steps, 50), "warmup": args.warmup, "ms_per_step": out["exact"]["gpu_ms_per_query"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "1 query vs 2,000 DB random walks, lengths U[32,128], DTW top-10 (REF and EXACT)",
+                       "queries": 1, "db_trajectories": N, "k": 10},
+            "cpu_baseline": {"value": round(out["exact"]["cpu_pairs_per_s"], 1), "unit": "pairs/s",
+                             "cores": 1, "kind": "port",
+                             "sample": "the whole workload (one query: the oracle's loop over queries runs on one "
+                                       "thread), EXACT DTW top-10, mean of 20 runs"},
+            "extra": out}
+    print(json.dumps(line), flush=True)
+
+
 def run_c3(ts, args):
     """Uneven-length batching workload: 4,096 queries (log-uniform lengths
     [16, 2048]) against 1,000,000 DB random walks (log-uniform [16, 1024]),
@@ -142,13 +226,25 @@ def run_c3(ts, args):
                      "batched_vs_plain": round((Q * N / tb) / (ns * N / tp), 2), "plain_topk_equals_batched": same,
                      "overflow_retries": st["overflow_retries"]}
         log(meas, out[meas])
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        nc = min(16, Q)
+        prm = oracle.params(oracle.DTW, oracle.EXACT, k=10)
+        t1 = time.perf_counter()
+        oracle.knn(prm, db_xy, db_off, q_xy[:q_off[nc]], q_off[:nc + 1])
+        dt = time.perf_counter() - t1
+        cpu = {"value": round(nc * N / dt, 1), "unit": "pairs/s", "cores": min(nc, oracle.num_threads()),
+               "kind": "port", "sample": "%d of the %d queries against the full DB, EXACT DTW top-10 (oracle "
+                                         "best-first search, one thread per query), %.2f s" % (nc, Q, dt)}
+        log("c3 cpu", cpu)
     line = {"metric": METRIC, "value": round(out["dtw"]["batched_pairs_per_s"], 1), "unit": "pairs/s", "n_gpus": 1,
             "steps": 1, "warmup": 1, "ms_per_step": round(out["dtw"]["batched_s"] * 1e3, 3), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": "uneven-length batching: %d queries log-uniform [16,2048] vs %d DB random walks "
                                    "log-uniform [16,1024], DTW/Frechet top-10, EXACT bounding-box filter" % (Q, N),
                        "queries": Q, "db_trajectories": N, "k": 10},
-            "extra": out}
+            "cpu_baseline": cpu, "extra": out}
     print(json.dumps(line), flush=True)
 
 
@@ -205,14 +301,56 @@ def run_c4(ts, args):
                                  "topk_equal": bool(np.array_equal(oi, idx[:nq]) and np.array_equal(ov, val[:nq]))}
         out[meas] = r
         log("c4", meas, {k: v for k, v in r.items() if k != "kernels"})
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        rng = np.random.default_rng(0xC4)
+        npairs = 192
+        qi = rng.integers(0, Q, npairs).astype(np.int32)
+        di = rng.integers(0, N, npairs).astype(np.int32)
+        cpu = {}
+        for meas, p in legs:
+            prm = oracle.params(p.measure, band=p.band, delta=p.delta, epsilon=p.epsilon)
+            t1 = time.perf_counter()
+            oracle.pair_values(prm, db_xy, db_off, q_xy, q_off, qi, di)
+            dt = time.perf_counter() - t1
+            cells = band_cells(p, q_len[qi], db_len[di])
+            cpu[meas] = {"value": round(cells / dt, 1), "unit": "cell-updates/s", "seconds": round(dt, 3)}
+        cpu = {"value": cpu["dtw"]["value"], "unit": "cell-updates/s", "cores": oracle.num_threads(),
+               "kind": "port", "sample": "%d random (query, DB) pairs of the workload, exact values on every core "
+                                         "(the banded DTW visits only in-band cells; the LCSS oracle fills the "
+                                         "whole table)" % npairs, "lcss": cpu["lcss"]}
+        log("c4 cpu", cpu)
     line = {"metric": METRIC, "value": round(out["dtw"]["cell_updates_per_s"], 1), "unit": "cell-updates/s",
             "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": round(out["dtw"]["batched_s"] * 1e3, 3),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": "long trajectories: %d queries vs %d DB random walks, lengths U[2048,8192], "
                                    "DTW band ceil(0.1 max(n,m)) and LCSS eps 2e-3 delta 64, EXACT top-10" % (Q, N),
                        "queries": Q, "db_trajectories": N, "k": 10},
-            "extra": out}
+            "cpu_baseline": cpu, "extra": out}
     print(json.dumps(line), flush=True)
+
+
+def band_cells(p, n, m):
+    """In-band DP cells of pairs of lengths n, m (arrays) for a DTW band or an
+    LCSS window, as the engine counts them."""
+    tot = 0
+    for a, b in zip(np.asarray(n, np.int64), np.asarray(m, np.int64)):
+        R, C = max(a, b), min(a, b)
+        r = np.arange(R, dtype=np.int64)
+        if p.measure == 0 and p.band != 0:
+            w = p.band if p.band > 0 else (-p.band * R + 99) // 100
+            bw = w * R
+            x = r * C - bw
+            lo = np.where(x <= 0, 0, (x + R - 1) // R)
+            hi = np.minimum(C - 1, (r * C + bw) // R)
+            tot += int(np.maximum(hi - lo + 1, 0).sum())
+        elif p.measure == 1 and p.delta >= 0:
+            r = r[r - p.delta <= C - 1]
+            tot += int((np.minimum(C - 1, r + p.delta) - np.maximum(0, r - p.delta) + 1).sum())
+        else:
+            tot += int(R * C)
+    return tot
 
 
 def run_c5(ts, args):
@@ -226,7 +364,6 @@ def run_c5(ts, args):
     db_len = ts.gen_lengths(DB_SEED, args.n_db, 16, 512, True)
     db_xy, db_off = ts.gen_walks(DB_SEED, db_len)
     db = ts.TrajSet(db_xy, db_off)
-    del db_xy
     log("c5 DB ready in %.1fs: %d trajectories, %.2f GB resident" % (time.time() - t0, db.count, db.nbytes() / 1e9))
     rng = np.random.default_rng(0xA11CE)
     nreq_max = int(max(args.qps) * args.duration) + 16
@@ -295,6 +432,27 @@ def run_c5(ts, args):
              "mean_batch": round(len(lat) / max(1, c["batches"]), 2)}
         log("c5", r)
         results.append(r)
+    cpu = None
+    if not args.no_cpu:
+        # CPU QPS: the oracle answers the first 200 requests of the stream,
+        # grouped by (measure, k), each group parallel over its queries.
+        import oracle
+        ncpu = min(200, nreq_max)
+        t1 = time.perf_counter()
+        for (m, k), _ in prm.items():
+            sel = [i for i in range(ncpu) if mix[i] == m and ks[i] == k]
+            if not sel:
+                continue
+            sub = [q_xy[q_off[i]:q_off[i + 1]] for i in sel]
+            so = ts.offsets_of([len(t) for t in sub])
+            meas = {0: oracle.DTW, 1: oracle.LCSS, 2: oracle.FRECHET, 3: oracle.HAUSDORFF}[m]
+            op = oracle.params(meas, oracle.EXACT, k=int(k), epsilon=2e-3)
+            oracle.knn(op, db_xy, db_off, np.concatenate(sub), so)
+        dt = time.perf_counter() - t1
+        cpu = {"value": round(ncpu / dt, 2), "unit": "queries/s", "cores": oracle.num_threads(), "kind": "port",
+               "sample": "the first %d requests of the stream (same mix and k), oracle EXACT k-NN grouped by "
+                         "(measure, k), %.2f s" % (ncpu, dt)}
+        log("c5 cpu", cpu)
     best = results[0]
     line = {"metric": METRIC, "value": best["p99_ms"], "unit": "ms (p99 latency at %d QPS)" % best["offered_qps"],
             "n_gpus": 1, "steps": len(results), "warmup": 0, "ms_per_step": None, "higher_is_better": False,
@@ -303,7 +461,7 @@ def run_c5(ts, args):
                                    "query lengths log-uniform [16,1024], mix DTW 40/LCSS 20/Frechet 20/Hausdorff 20, "
                                    "k in {1,10,100}, EXACT top-k, continuous batching over 4 streams" % db.count,
                        "db_trajectories": db.count, "duration_s": args.duration},
-            "extra": {"sweep": results}}
+            "cpu_baseline": cpu, "extra": {"sweep": results}}
     print(json.dumps(line), flush=True)
 
 
@@ -312,7 +470,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--qps", type=lambda v: [int(x) for x in v.split(",")], default=[2000, 5000, 10000, 20000])
     ap.add_argument("--duration", type=float, default=2.0)
     ap.add_argument("--n-db", type=int, default=None)
@@ -333,6 +491,8 @@ def main():
 
     import cudaruntime1_amd as ts
     ts.init(0)
+    if args.config == "c1":
+        return run_c1(ts, args)
     if args.config == "c3":
         args.n_db = args.n_db or 1_000_000
         args.n_q = args.n_q or 4096
@@ -410,6 +570,7 @@ def main():
                       "query_latency_ms_p99": round(float(np.percentile(lat, 99)), 4),
                       "latency_note": "single-query DTW top-10 calls, serial, DB resident"})
         log("frechet %.3f ms, plain %.3f ms, p50 %.3f ms" % (ms_fr, ms_plain, extra["query_latency_ms_p50"]))
+        extra["measures"] = c2_other_measures(ts, eng, db, qs, args)
 
     cpu = None
     if not args.no_cpu:
@@ -417,6 +578,8 @@ def main():
         extra["cpu_matches_gpu"] = bool(np.array_equal(ci, idx[:args.cpu_sample]) and
                                         np.array_equal(cv, val[:args.cpu_sample]))
         log("cpu baseline", cpu)
+        extra["cpu_other"] = cpu_extra(db_xy, db_off, q_xy, q_off)
+        log("cpu other", extra["cpu_other"])
 
     line = {"metric": METRIC, "value": round(value, 1), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",

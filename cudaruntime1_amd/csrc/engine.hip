@@ -398,6 +398,31 @@ ts_status knn_ref_batch(ts_engine* e, const ts_params& p, const ts_set* db, cons
   return topk(e, keys2, kc, kc, nullptr, nq, p.k, top);
 }
 
+// EXACT for small problems: when every pair of the batch fits in the DP
+// kernels' concurrent pair slots at once, refining all of them takes one DP
+// round, where seeds + filter + refine take two. The result is the exact
+// top-k by definition (ties by index, as everywhere).
+constexpr int64_t kAllPairsMax = 2048;
+
+ts_status knn_all_batch(ts_engine* e, const ts_params& p, const ts_set* db, const ts_set* qs, int32_t q0,
+                        int32_t nq, uint64_t* top) {
+  const int32_t N = db->count;
+  const int64_t n = (int64_t)nq * N;
+  Item* items = nullptr;
+  float* vals = nullptr;
+  uint64_t* keys = nullptr;
+  TS_TRY(ensure(e, B_ITEMS, n, &items));
+  TS_TRY(ensure(e, B_VALS, n, &vals));
+  TS_TRY(ensure(e, B_KEYS, n, &keys));
+  TS_TRY(run(e, "items", 0, 0, [&] { return launch_items_all(e->st, q0, nq, N, items); }));
+  TS_TRY(fill_nan(e, vals, n));
+  TS_TRY(refine(e, p, db, qs, items, nullptr, n, vals));
+  TS_TRY(run(e, "keys", 0, 0, [&] {
+    return launch_keys_from_items(e->st, vals, items, n, desc_measure(p.measure), keys);
+  }));
+  return topk(e, keys, N, N, nullptr, nq, p.k, top);
+}
+
 // EXACT pipeline for queries [q0, q0+nq).
 ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, const ts_set* qs, int32_t q0,
                           int32_t nq, uint64_t* top) {
@@ -528,10 +553,14 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
     int64_t s = std::min<int64_t>(std::max<int64_t>(2 * k, 16), 1024);
     bq = (int32_t)std::max<int64_t>(1, std::min<int64_t>({(int64_t)Q, 4096, (int64_t)(512ll << 20) / (8 * nch * s)}));
   }
+  // Small EXACT problems refine every pair (Euclidean keeps its filter,
+  // which also drops pairs of unequal length).
+  const bool all_pairs = p.mode == TS_MODE_EXACT && p.measure != TS_EUCLIDEAN && (int64_t)bq * N <= kAllPairsMax;
   for (int32_t q0 = 0; q0 < Q; q0 += bq) {
     const int32_t nq = std::min(bq, Q - q0);
     uint64_t* topq = top + (int64_t)q0 * k;
     if (p.mode == TS_MODE_REF) TS_TRY(knn_ref_batch(e, p, db, qs, q0, nq, env, topq));
+    else if (all_pairs) TS_TRY(knn_all_batch(e, p, db, qs, q0, nq, topq));
     else TS_TRY(knn_exact_batch(e, p, db, qs, q0, nq, topq));
   }
   TS_TRY(run(e, "decode", 0, 0, [&] { return launch_decode_keys(e->st, top, (int64_t)Q * k, desc, d_idx, d_val); }));

@@ -152,9 +152,12 @@ __device__ __forceinline__ float pair_bound(const MeasureArgs& ma, const QSum& q
       return (float)(sqrt((double)n * (double)g2) * (1.0 - 1e-6));
     case TS_LCSS: {
       // Points of q that can match any point of c lie inside c's
-      // eps-inflated MBR (f32 subtraction is monotone).
+      // eps-inflated MBR (f32 subtraction is monotone). When q's MBR lies
+      // eps or more outside it on one side, no point of q does: the count
+      // is 0 without visiting them.
       int cnt = 0;
       const float eps = ma.eps;
+      if (a.x - cm.y >= eps || cm.x - a.y >= eps || a.z - cm.w >= eps || cm.z - a.w >= eps) return 0.0f;
       for (int i = 0; i < n; ++i) {
         float px = q.x[qs.off + i], py = q.y[qs.off + i];
         cnt += (px - cm.y < eps && cm.x - px < eps && py - cm.w < eps && cm.z - py < eps) ? 1 : 0;

@@ -181,6 +181,21 @@ hipError_t launch_items_from_keys(hipStream_t st, const uint64_t* keys, int32_t 
   return hipGetLastError();
 }
 
+// Every (query, DB trajectory) pair of queries [q0, q0+nq): item i is
+// (q0 + i / N, i % N) and writes slot i.
+__global__ void __launch_bounds__(256) k_items_all(int32_t q0, int32_t nq, int32_t N, Item* items) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (int64_t)nq * N) return;
+  items[i] = Item{q0 + (int32_t)(i / N), (int32_t)(i % N), (int32_t)i, 0};
+}
+
+hipError_t launch_items_all(hipStream_t st, int32_t q0, int32_t nq, int32_t N, Item* items) {
+  const int64_t n = (int64_t)nq * N;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_items_all, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, q0, nq, N, items);
+  return hipGetLastError();
+}
+
 __global__ void __launch_bounds__(256) k_keys_from_regions(const float* vals, const int32_t* region_d,
                                                            const int32_t* counts, int32_t nq, int32_t cap,
                                                            int32_t desc, uint64_t* keys) {

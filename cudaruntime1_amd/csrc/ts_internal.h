@@ -174,6 +174,8 @@ hipError_t launch_fill_i32(hipStream_t st, int32_t* p, int32_t v, int64_t n);
 hipError_t launch_items_by_bucket(hipStream_t st, const Item* items, const int32_t* nitems, int64_t max_items,
                                   SetView q, SetView db, MeasureArgs m, Item* outb, int64_t cap_each,
                                   int32_t* countsb);
+// Every pair of queries [q0, q0+nq) with all N DB trajectories, slot = index.
+hipError_t launch_items_all(hipStream_t st, int32_t q0, int32_t nq, int32_t N, Item* items);
 hipError_t launch_items_from_regions(hipStream_t st, const int32_t* region_d, const int32_t* counts,
                                      int32_t nq, int32_t cap, int32_t q0, Item* items, int32_t* nitems);
 

@@ -45,6 +45,7 @@ for name, res, args in [
     ("orc_pair_values_f64", None, [_P, _P, _P, _P, _P, _P, _P, _i, _P]),
     ("orc_pair_values", None, [_P, _P, _P, _P, _P, _P, _P, _i, _P]),
     ("orc_num_threads", _i, []),
+    ("orc_set_threads", None, [_i]),
 ]:
     fn = getattr(_o, name)
     fn.restype = res
@@ -174,6 +175,11 @@ def pair_values(prm, db_xy, db_off, q_xy, q_off, qidx, didx, f64=False):
 
 def num_threads():
     return _o.orc_num_threads()
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's parallel loops (over queries or pairs)."""
+    _o.orc_set_threads(int(n))
 
 
 # ------------------------------------------------- the reference's own build

@@ -125,14 +125,30 @@ template <class T> T dtw(const Pt* q, int n, const Pt* c, int m, int band) {
   const int w = band == 0 ? 0 : band_width(band, n, m);
   std::vector<T> prev(m + 1, inf), curr(m + 1, inf);
   prev[0] = 0;
+  // Row u (1-based) visits columns [vlo, vhi]: the cells in_band admits, a
+  // contiguous run whose ends never move left from one row to the next.
+  // Every other entry of a row's buffer is inf: before a buffer takes row u
+  // its entries from row u-2 left of vlo are reset (those right of vhi are
+  // inf already, as vhi never decreases).
+  int lo_curr = 1, lo_prev = 1;  // first visited column of the rows the buffers hold
+  int vlo = 1, vhi = 0;
   for (int u = 1; u <= n; ++u) {
+    if (w > 0) {
+      while (vlo <= m && !in_band(u - 1, vlo - 1, n, m, w)) ++vlo;
+      if (vhi < vlo - 1) vhi = vlo - 1;
+      while (vhi < m && in_band(u - 1, vhi, n, m, w)) ++vhi;
+      for (int v = lo_curr; v < vlo && v <= m; ++v) curr[v] = inf;
+    } else {
+      vhi = m;
+    }
     curr[0] = inf;
-    for (int v = 1; v <= m; ++v) {
-      if (!in_band(u - 1, v - 1, n, m, w)) { curr[v] = inf; continue; }
+    for (int v = vlo; v <= vhi; ++v) {
       T cost = pdist<T>(q[u - 1], c[v - 1]);
       curr[v] = cost + std::min({prev[v], curr[v - 1], prev[v - 1]});
     }
+    lo_curr = vlo;
     std::swap(prev, curr);
+    std::swap(lo_prev, lo_curr);
   }
   return prev[m];
 }
@@ -480,5 +496,6 @@ void orc_pair_values(const int32_t* params, const float* db_xy, const int64_t* d
 }
 
 int orc_num_threads(void) { return omp_get_max_threads(); }
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 
 }  // extern "C"

@@ -170,6 +170,25 @@ def test_knn_batched_and_plain_match_oracle(eng, measure, mode):
         assert np.array_equal(gv.view(np.int32), wv.view(np.int32)), (fn.__name__, gv[:2], wv[:2])
 
 
+@pytest.mark.parametrize("measure", ["dtw", "frechet", "lcss", "hausdorff"])
+@pytest.mark.parametrize("n_db,n_q,k", [(300, 3, 10), (6, 2, 10), (1, 1, 3)])
+def test_knn_small_all_pairs_match_oracle(eng, measure, n_db, n_q, k):
+    """Small EXACT problems (every pair refined in one DP round), including
+    k larger than the DB."""
+    say("knn small", measure, n_db, n_q, k)
+    L = uniform_lengths(33, n_db, 8, 200)
+    db_xy, db_off = walks(33, L, 5e-3)
+    q_xy, q_off = planted(db_xy, db_off, np.arange(n_q) % n_db, (L[np.arange(n_q) % n_db] * 3) // 4 + 1, 1e-3, 34)
+    db, qs = ts.TrajSet(db_xy, db_off), ts.TrajSet(q_xy, q_off)
+    p = ts.params(measure, mode=ts.EXACT, k=k, epsilon=2e-3)
+    prm = oracle.params(p.measure, p.mode, k=p.k, epsilon=p.epsilon)
+    wi, wv, _ = oracle.knn(prm, db_xy, db_off, q_xy, q_off)
+    for fn in (eng.knn, eng.knn_plain):
+        gi, gv, st = fn(db, qs, p)
+        assert np.array_equal(gi, wi), (fn.__name__, gi, wi)
+        assert np.array_equal(gv.view(np.int32), wv.view(np.int32)), (fn.__name__, gv, wv)
+
+
 @pytest.mark.parametrize("measure,kw", [("dtw", {"band": -10}), ("lcss", {"epsilon": 2e-3, "delta": 64})])
 def test_knn_long_banded_match_oracle(eng, measure, kw):
     """Long-trajectory shape (lengths U[2048,4096]) with a per-pair 10% band

@@ -116,6 +116,28 @@ def test_dtw_band_brute_force():
     assert oracle.dtw(q, c, band=-1) == oracle.dtw(q, c, band=1)
 
 
+def test_dtw_band_matches_masked_full_dp():
+    """The banded oracle visits only in-band columns; it must equal the full
+    f32 recurrence with out-of-band cells set to 1e20f, bit for bit."""
+    rng = np.random.default_rng(41)
+    F = np.float32
+    for _ in range(60):
+        n, m = int(rng.integers(1, 28)), int(rng.integers(1, 28))
+        band = int(rng.choice([1, 2, 3, 7, 40, -1, -10, -34, -100]))
+        q, c = rand_traj(rng, n), rand_traj(rng, m)
+        w = band if band > 0 else (-band * max(n, m) + 99) // 100
+        D = np.full((n + 1, m + 1), F(1e20), np.float32)
+        D[0, 0] = 0
+        for i in range(1, n + 1):
+            for j in range(1, m + 1):
+                if abs((i - 1) * m - (j - 1) * n) > w * max(n, m):
+                    continue
+                dx, dy = F(q[i - 1, 0] - c[j - 1, 0]), F(q[i - 1, 1] - c[j - 1, 1])
+                d = np.sqrt(F(F(dx * dx) + F(dy * dy)))
+                D[i, j] = F(d + min(D[i - 1, j], D[i, j - 1], D[i - 1, j - 1]))
+        assert oracle.dtw(q, c, band=band) == D[n, m], (n, m, band)
+
+
 def test_dtw_closed_forms_and_symmetry():
     rng = np.random.default_rng(5)
     a = rand_traj(rng, 37)
