@@ -120,7 +120,7 @@ def c2_other_measures(ts, eng, db, qs, args):
     e_db, e_qs = ts.TrajSet(e_db_xy, e_db_off), ts.TrajSet(e_q_xy, e_q_off)
     legs.append(("euclidean", ts.params("euclidean", mode=ts.EXACT, k=10), e_db, e_qs))
     for name, p, d, q in legs:
-        ms, (_, _, st) = timed(lambda: eng.knn(d, q, p), max(1, args.steps // 2), 1)
+        ms, (_, _, st) = timed(lambda: eng.knn(d, q, p), max(1, args.steps // 2), 2)
         out[name] = {"pairs_per_s": q.count * d.count / (ms * 1e-3), "ms_per_step": round(ms, 4),
                      "cells_per_step": st["cells"], "cell_updates_per_s": st["cells"] / (ms * 1e-3),
                      "refined_pairs_per_step": st["refined_pairs"]}
@@ -160,7 +160,7 @@ def run_c1(ts, args):
     import oracle
     db_xy, db_off, q_xy, q_off = build_workload(ts, 2000, 1, 32, 128)
     db, qs = ts.TrajSet(db_xy, db_off), ts.TrajSet(q_xy, q_off)
-    eng = ts.Engine()
+    eng = ts.Engine(graphs=not args.no_graphs)
     N = db.count
     out = {}
     for mode, name in ((ts.REF, "ref"), (ts.EXACT, "exact")):
@@ -207,12 +207,12 @@ def run_c3(ts, args):
     sample = ts.TrajSet(q_xy[:q_off[ns]], q_off[:ns + 1])
     log("c3 workload ready in %.1fs: %d DB trajectories (%d points), %d queries" %
         (time.time() - t0, db.count, len(db_xy), qs.count))
-    eng = ts.Engine()
+    eng = ts.Engine(graphs=not args.no_graphs)
     Q, N = qs.count, db.count
     out = {}
     for meas in ("dtw", "frechet"):
         p = ts.params(meas, mode=ts.EXACT, k=10)
-        eng.knn(db, sample, p)  # warm-up (workspace growth)
+        eng.knn(db, sample, p)  # warm-up (workspace growth); batches this large launch eagerly
         t1 = time.perf_counter()
         idx, val, st = eng.knn(db, qs, p)
         tb = time.perf_counter() - t1
@@ -220,7 +220,7 @@ def run_c3(ts, args):
         pi, pv, pst = eng.knn_plain(db, sample, p)
         tp = time.perf_counter() - t1
         same = bool(np.array_equal(pi, idx[:ns]) and np.array_equal(pv, val[:ns]))
-        out[meas] = {"batched_pairs_per_s": Q * N / tb, "batched_s": round(tb, 4),
+        out[meas] = {"batched_pairs_per_s": Q * N / tb, "batched_s": round(tb, 4), "graphs": eng.graph_info(),
                      "batched_cell_updates_per_s": st["cells"] / tb, "cells": st["cells"],
                      "plain_pairs_per_s": ns * N / tp, "plain_s_for_sample": round(tp, 4), "plain_sample": ns,
                      "batched_vs_plain": round((Q * N / tb) / (ns * N / tp), 2), "plain_topk_equals_batched": same,
@@ -478,6 +478,7 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=256)
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true", help="c2: launch eagerly instead of replaying HIP graphs")
     ap.add_argument("--oracle-queries", type=int, default=0,
                     help="c4: also check the top-10 of this many queries against the CPU oracle (about a minute each)")
     ap.add_argument("--quick", action="store_true", help="skip plain/latency/frechet legs (profiling runs)")
@@ -518,24 +519,34 @@ def main():
     log("workload ready in %.1fs: %d DB trajectories (%d points), %d queries" %
         (time.time() - t0, db.count, len(db_xy), qs.count))
     Q, N = qs.count, db.count
-    eng = ts.Engine()
+    eng = ts.Engine(graphs=not args.no_graphs)
     p_dtw = ts.params("dtw", mode=ts.EXACT, k=10)
     p_fr = ts.params("frechet", mode=ts.EXACT, k=10)
 
-    # Timed region: K batched DTW steps with per-kernel HIP events.
-    for _ in range(args.warmup):
+    # Timed region: K batched DTW steps, replayed from a HIP graph (the
+    # warm-up records it; --no-graphs launches eagerly).
+    for _ in range(max(args.warmup, 2)):
         eng.knn(db, qs, p_dtw)
-    eng.set_timing(True)
-    eng.reset_stats()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         idx, val, st = eng.knn(db, qs, p_dtw)
     ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
-    kstats = eng.kernel_stats()
-    eng.set_timing(False)
     value = Q * N / (ms_step * 1e-3)
     cells = st["cells"]
-    log("DTW batched: %.3f ms/step, %.3e pairs/s, %.3e cells/step" % (ms_step, value, cells))
+    ginfo = eng.graph_info()
+    log("DTW batched: %.3f ms/step, %.3e pairs/s, %.3e cells/step, graphs %s" % (ms_step, value, cells, ginfo))
+    # Per-kernel durations (HIP events around every launch, so eager): the
+    # same K steps again, for the roofline and the kernel table.
+    eng.set_timing(True)
+    eng.reset_stats()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ti, tv, _ = eng.knn(db, qs, p_dtw)
+    ms_eager = (time.perf_counter() - t0) * 1e3 / args.steps
+    kstats = eng.kernel_stats()
+    eng.set_timing(False)
+    assert np.array_equal(ti, idx) and np.array_equal(tv, val)
+    log("DTW batched eager with kernel events: %.3f ms/step" % ms_eager)
 
     pmc = None
     pmc_path = os.path.join(ROOT, "profiles", "pmc_traffic_c2.json")
@@ -543,13 +554,14 @@ def main():
         pmc = json.load(open(pmc_path))
     roof = roofline_of(kstats, pmc)
     extra = {"device": dev, "dtw_ms_per_step": round(ms_step, 4), "dtw_cells_per_step": cells,
+             "graphs": ginfo, "dtw_ms_per_step_eager_timed": round(ms_eager, 4),
              "dtw_cell_updates_per_s": cells / (ms_step * 1e-3),
              "dtw_device_ms": round(st["ms_device"], 4), "launches_per_step": st["launches"],
              "kernels": {k: {"launches": v["launches"], "ms_mean": round(v["ms"] / max(v["launches"], 1), 5)}
                          for k, v in kstats.items()}}
 
     if not args.quick:
-        ms_fr, (fi, fv, fst) = timed(lambda: eng.knn(db, qs, p_fr), args.steps, 1)
+        ms_fr, (fi, fv, fst) = timed(lambda: eng.knn(db, qs, p_fr), args.steps, 2)
         extra.update({"frechet_pairs_per_s": Q * N / (ms_fr * 1e-3), "frechet_ms_per_step": round(ms_fr, 4),
                       "frechet_cell_updates_per_s": fst["cells"] / (ms_fr * 1e-3)})
         ms_plain, (pi, pv, _) = timed(lambda: eng.knn_plain(db, qs, p_dtw), max(1, args.steps // 3), 1)

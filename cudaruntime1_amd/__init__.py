@@ -60,12 +60,15 @@ _sig = {
     "ts_init": (ctypes.c_int, [ctypes.c_int32]),
     "ts_device_name": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32]),
     "ts_set_create": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.POINTER(_P)]),
+    "ts_set_reload": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32]),
     "ts_set_destroy": (ctypes.c_int, [_P]),
     "ts_set_info": (ctypes.c_int, [_P, _i32p, ctypes.POINTER(ctypes.c_int64), _i32p, _i32p]),
     "ts_set_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "ts_engine_create": (ctypes.c_int, [ctypes.POINTER(_P)]),
     "ts_engine_destroy": (ctypes.c_int, [_P]),
     "ts_engine_set_timing": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "ts_engine_set_graphs": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "ts_engine_graph_info": (ctypes.c_int, [_P, _P, _P, _P]),
     "ts_engine_kernel_stats": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32,
                                               ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
                                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
@@ -170,6 +173,15 @@ class TrajSet:
         self._h = ctypes.c_void_p()
         _check(_lib.ts_set_create(_ptr(xy), _ptr(offsets), self.count, ctypes.byref(self._h)), "ts_set_create")
 
+    def reload(self, xy, offsets):
+        """Replace the trajectories in place (same handle, buffers reused)."""
+        xy = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)
+        offsets = np.ascontiguousarray(offsets, np.int64)
+        _check(_lib.ts_set_reload(self._h, _ptr(xy), _ptr(offsets), len(offsets) - 1), "ts_set_reload")
+        self.count = len(offsets) - 1
+        self.offsets = offsets
+        self.lengths = np.diff(offsets).astype(np.int32)
+
     @property
     def handle(self):
         return self._h
@@ -201,14 +213,26 @@ class TrajSet:
 class Engine:
     """One stream, its workspace and optional per-kernel timing."""
 
-    def __init__(self, timing=False):
+    def __init__(self, timing=False, graphs=False):
         self._h = ctypes.c_void_p()
         _check(_lib.ts_engine_create(ctypes.byref(self._h)), "ts_engine_create")
         if timing:
             self.set_timing(True)
+        if graphs:
+            self.set_graphs(True)
 
     def set_timing(self, on):
         _check(_lib.ts_engine_set_timing(self._h, int(on)), "ts_engine_set_timing")
+
+    def set_graphs(self, on):
+        """Replay batched k-NN calls of a repeated shape from HIP graphs."""
+        _check(_lib.ts_engine_set_graphs(self._h, int(on)), "ts_engine_set_graphs")
+
+    def graph_info(self):
+        c, r, n = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _check(_lib.ts_engine_graph_info(self._h, ctypes.byref(c), ctypes.byref(r), ctypes.byref(n)),
+               "ts_engine_graph_info")
+        return {"captures": c.value, "replays": r.value, "cached": n.value}
 
     def reset_stats(self):
         _check(_lib.ts_engine_reset_stats(self._h), "ts_engine_reset_stats")

@@ -1,7 +1,7 @@
 """Build libtrajsim.so (HIP, gfx950) in-tree, and the CPU oracle.
 
-    python -m cudaruntime1_amd.build            # library only
-    python -m cudaruntime1_amd.build --oracle   # library + oracle/
+    python cudaruntime1_amd/build.py            # library only
+    python cudaruntime1_amd/build.py --oracle   # library + oracle/
 
 Flags: -ffp-contract=off keeps f32 rounding identical to the reference's
 non-contracting CPU build (no FMA fusion), and packed-fp32 instructions are

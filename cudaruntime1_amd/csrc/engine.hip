@@ -97,7 +97,8 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 
 enum BufId {
   B_ENV, B_SCORES, B_KEYS, B_KEYS2, B_SCA, B_SCB, B_CAND, B_CANDIDX, B_ITEMS, B_ITEMS3, B_VALS, B_REGION,
-  B_RVALS, B_COUNTS, B_CTR, B_TAU, B_TOP, B_OIDX, B_OVAL, B_CELLS, B_HOSTIN, B_ITEMS2, B_RITEMS, B_NBUF
+  B_RVALS, B_COUNTS, B_CTR, B_TAU, B_TOP, B_OIDX, B_OVAL, B_CELLS, B_HOSTIN, B_ITEMS2, B_RITEMS, B_SEGC, B_NEED,
+  B_NBUF
 };
 
 struct Buf {
@@ -116,6 +117,18 @@ struct Pending {
   hipEvent_t a, b;
   double bytes, units;
   int cells_slot;  // >= 0: units are read from the cell counter slot
+};
+
+// A captured batch: the key lists every host-side input its launches were
+// built from (set handles and shapes, parameters, batch range, candidate
+// capacity, workspace generation), so a replay is valid exactly when the key
+// matches; the data behind the device pointers may change between replays.
+struct GraphEntry {
+  std::vector<int64_t> key;
+  hipGraphExec_t exec = nullptr;
+  int64_t launches = 0;
+  int cells_to = 0;
+  uint64_t last_use = 0;
 };
 
 }  // namespace ts
@@ -140,6 +153,22 @@ struct ts_engine {
   std::vector<unsigned long long> cells_host;
   int cells_used = 0;
   int64_t overflow_retries = 0;
+  // hipGraph replay of batches (off by default; never with timing or for
+  // the plain method).
+  bool graphs = false;
+  bool capturing = false;       // launches are being recorded, not run
+  bool capture_broken = false;  // the workspace had to grow while recording
+  uint64_t ws_gen = 0;          // bumped whenever a workspace buffer moves
+  uint64_t use_clock = 0;
+  int64_t graph_captures = 0, graph_replays = 0;
+  std::vector<GraphEntry> gcache;
+  std::vector<std::vector<int64_t>> gseen;  // keys run once eagerly
+  int64_t exact_cap = 0;                    // candidate capacity of device-sized batches
+  unsigned long long* need_dev = nullptr;   // largest candidate total of the call
+  // host staging of the eager candidate sizing; engine-owned so that async
+  // copies from them stay valid
+  std::vector<int32_t> h_counts;
+  std::vector<int64_t> h_off;
 };
 
 namespace ts {
@@ -152,6 +181,11 @@ ts_status ensure(ts_engine* e, BufId id, int64_t n, T** out) {
   Buf& b = e->bufs[id];
   size_t bytes = (size_t)std::max<int64_t>(n, 1) * sizeof(T);
   if (bytes > b.cap) {
+    if (e->capturing) {
+      e->capture_broken = true;
+      return fail(TS_ERR_BUSY, "workspace growth while recording a graph");
+    }
+    ++e->ws_gen;
     TS_HIP(hipStreamSynchronize(e->st));
     if (b.p) TS_HIP(hipFree(b.p));
     b.p = nullptr;
@@ -463,51 +497,155 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   TS_TRY(topk(e, skeys, s, s, nullptr, nq, k, stop));
   TS_TRY(run(e, "threshold", 0, 0, [&] { return launch_kth_threshold(e->st, stop, nq, k, k, desc, tau); }));
   // 3. compaction of every pair that can still reach tau: count, size each
-  // query's region exactly, write.
+  // query's region, write. Eager calls size the regions exactly from the
+  // counts read back to the host; graph-captured calls cannot read back, so
+  // the device sizes them within the engine's candidate capacity and the
+  // call is repeated with a larger capacity if the total exceeded it.
   TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, counts, 0, nq); }));
   TS_TRY(run(e, "bound_compact", pairs * 36.0, pairs, [&] {
-    return launch_bound_compact(e->st, ma, k, qv, q0, nq, dv, tau, nullptr, counts, nullptr);
+    return launch_bound_compact(e->st, ma, k, qv, q0, nq, dv, tau, nullptr, counts, nullptr, 0);
   }));
-  std::vector<int32_t> hc(nq);
-  std::vector<int64_t> hoff(nq + 1);
-  TS_HIP(hipMemcpyAsync(hc.data(), counts, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, e->st));
-  TS_HIP(hipStreamSynchronize(e->st));
-  hoff[0] = 0;
-  int32_t maxc = 0;
-  for (int32_t i = 0; i < nq; ++i) {
-    hoff[i + 1] = hoff[i] + hc[i];
-    maxc = std::max(maxc, hc[i]);
+  const bool dev_sized = e->capturing;
+  int64_t total = 0;  // candidates written (eager) or the capacity (device-sized)
+  int64_t cap = 0;    // length of the candidate buffers
+  int32_t seg_len = 1;
+  int32_t *segc = nullptr, *ntot = nullptr;
+  if (dev_sized) {
+    total = cap = e->exact_cap;
+    seg_len = N;
+    TS_TRY(ensure(e, B_SEGC, nq + 2, &segc));
+    ntot = segc + nq;
+    TS_TRY(run(e, "scan", 0, 0, [&] {
+      return launch_scan_offsets(e->st, counts, nq, cap, d_off, ntot, segc, e->need_dev);
+    }));
+  } else {
+    std::vector<int32_t>& hc = e->h_counts;
+    std::vector<int64_t>& hoff = e->h_off;
+    hc.resize(nq);
+    hoff.resize(nq + 1);
+    TS_HIP(hipMemcpyAsync(hc.data(), counts, sizeof(int32_t) * nq, hipMemcpyDeviceToHost, e->st));
+    TS_HIP(hipStreamSynchronize(e->st));
+    hoff[0] = 0;
+    for (int32_t i = 0; i < nq; ++i) {
+      hoff[i + 1] = hoff[i] + hc[i];
+      seg_len = std::max(seg_len, hc[i]);
+    }
+    total = hoff[nq];
+    if (total >= (1ll << 31)) return fail(TS_ERR_UNSUPPORTED, "more than 2^31 candidate pairs in one batch");
+    cap = total;
+    if (e->graphs) {
+      // Size everything as a recorded replay of this batch will need it, so
+      // that recording allocates nothing.
+      e->exact_cap = std::max<int64_t>(e->exact_cap, std::min<int64_t>(total + total / 4 + 4096, (1ll << 31) - 1));
+      cap = e->exact_cap;
+      TS_TRY(ensure(e, B_SEGC, nq + 2, &segc));
+      const int64_t need = select_scratch_elems(N, nq, k);
+      uint64_t* sc = nullptr;
+      TS_TRY(ensure(e, B_SCA, need, &sc));
+      TS_TRY(ensure(e, B_SCB, need, &sc));
+    }
+    TS_HIP(hipMemcpyAsync(d_off, hoff.data(), sizeof(int64_t) * (nq + 1), hipMemcpyHostToDevice, e->st));
   }
-  const int64_t total = hoff[nq];
-  if (total >= (1ll << 31)) return fail(TS_ERR_UNSUPPORTED, "more than 2^31 candidate pairs in one batch");
   Item *citems = nullptr, *items2 = nullptr;
   float* rvals = nullptr;
   uint64_t* rkeys = nullptr;
-  TS_TRY(ensure(e, B_ITEMS2, total, &citems));
-  TS_TRY(ensure(e, B_RITEMS, total, &items2));
-  TS_TRY(ensure(e, B_RVALS, total, &rvals));
-  TS_TRY(ensure(e, B_SCORES, 2 * total, (float**)&rkeys));
-  TS_HIP(hipMemcpyAsync(d_off, hoff.data(), sizeof(int64_t) * (nq + 1), hipMemcpyHostToDevice, e->st));
+  TS_TRY(ensure(e, B_ITEMS2, cap, &citems));
+  TS_TRY(ensure(e, B_RITEMS, cap, &items2));
+  TS_TRY(ensure(e, B_RVALS, cap, &rvals));
+  TS_TRY(ensure(e, B_SCORES, 2 * cap, (float**)&rkeys));
   TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, counts, 0, nq + 1); }));
   TS_TRY(run(e, "bound_compact", pairs * 36.0, pairs, [&] {
-    return launch_bound_compact(e->st, ma, k, qv, q0, nq, dv, tau, d_off, counts, citems);
+    return launch_bound_compact(e->st, ma, k, qv, q0, nq, dv, tau, d_off, counts, citems, cap);
   }));
   // 3b. tighter O(n+m) bound on the survivors
   int32_t* nitems2 = counts + nq;
   TS_TRY(run(e, "tight_filter", 0, 0, [&] {
-    return launch_tight_filter(e->st, ma, k, qv, q0, dv, tau, citems, nullptr, total, items2, nitems2,
+    return launch_tight_filter(e->st, ma, k, qv, q0, dv, tau, citems, ntot, total, items2, nitems2,
                                device_cu_count() * 16);
   }));
-  TS_TRY(fill_nan(e, rvals, total));
-  TS_TRY(refine(e, p, db, qs, items2, nitems2, total, rvals));
+  TS_TRY(fill_nan(e, rvals, cap));
+  TS_TRY(refine(e, p, db, qs, items2, nitems2, cap, rvals));
   // 4. top-k over each query's refined candidates (slot = position in citems)
-  TS_TRY(run(e, "keys", 0, 0, [&] { return launch_keys_from_items(e->st, rvals, citems, total, desc, rkeys); }));
-  TS_HIP(hipMemcpyAsync(counts, hc.data(), sizeof(int32_t) * nq, hipMemcpyHostToDevice, e->st));
-  return topk(e, rkeys, 0, std::max(maxc, 1), counts, nq, k, top, d_off);
+  TS_TRY(run(e, "keys", 0, 0, [&] { return launch_keys_from_items(e->st, rvals, citems, cap, desc, rkeys); }));
+  // Region lengths: clamped to the capacity (device-sized), or the write
+  // cursors, which end at the counted lengths (both passes test the same
+  // bounds).
+  return topk(e, rkeys, 0, seg_len, dev_sized ? segc : counts, nq, k, top, d_off);
+}
+
+// Run one batch through the graph cache: replay a recorded graph whose key
+// matches; run a key's first occurrence eagerly (it sizes the workspace);
+// record on the second and replay from then on. A recording that fails
+// (workspace growth, capture error) runs the batch eagerly instead.
+constexpr size_t kGraphCacheMax = 16;
+constexpr int64_t kGraphMaxPairs = 1ll << 26;
+
+template <class F>
+ts_status run_graph(ts_engine* e, const std::vector<int64_t>& base, F body) {
+  const int c_entry = e->cells_used;
+  auto full_key = [&] {
+    std::vector<int64_t> k = base;
+    k.push_back((int64_t)e->ws_gen);
+    k.push_back(e->exact_cap);
+    k.push_back(c_entry);
+    return k;
+  };
+  const std::vector<int64_t> key = full_key();
+  for (GraphEntry& g : e->gcache) {
+    if (g.key != key) continue;
+    TS_HIP(hipGraphLaunch(g.exec, e->st));
+    ++e->graph_replays;
+    e->launches += g.launches;
+    e->cells_used = g.cells_to;
+    g.last_use = ++e->use_clock;
+    return TS_OK;
+  }
+  if (std::find(e->gseen.begin(), e->gseen.end(), key) == e->gseen.end()) {
+    // First occurrence: run eagerly, and remember the key as it stands
+    // afterwards (the run may have grown the workspace or the capacity).
+    TS_TRY(body());
+    if (e->gseen.size() >= 4 * kGraphCacheMax) e->gseen.erase(e->gseen.begin());
+    e->gseen.push_back(full_key());
+    return TS_OK;
+  }
+  const int c0 = e->cells_used;
+  const int64_t l0 = e->launches;
+  TS_HIP(hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal));
+  e->capturing = true;
+  e->capture_broken = false;
+  const ts_status st = body();
+  e->capturing = false;
+  hipGraph_t g = nullptr;
+  const hipError_t err = hipStreamEndCapture(e->st, &g);
+  hipGraphExec_t ex = nullptr;
+  bool ok = st == TS_OK && err == hipSuccess && !e->capture_broken && g != nullptr;
+  if (ok) ok = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess;
+  if (g) (void)hipGraphDestroy(g);
+  if (!ok) {
+    (void)hipGetLastError();
+    e->cells_used = c0;
+    e->launches = l0;
+    return body();
+  }
+  if (e->gcache.size() >= kGraphCacheMax) {
+    auto lru = std::min_element(e->gcache.begin(), e->gcache.end(),
+                                [](const GraphEntry& a, const GraphEntry& b) { return a.last_use < b.last_use; });
+    (void)hipGraphExecDestroy(lru->exec);
+    e->gcache.erase(lru);
+  }
+  e->gcache.push_back(GraphEntry{key, ex, e->launches - l0, e->cells_used, ++e->use_clock});
+  ++e->graph_captures;
+  TS_HIP(hipGraphLaunch(ex, e->st));
+  return TS_OK;
+}
+
+std::vector<int64_t> set_key(const ts_set* s) {
+  return {(int64_t)(intptr_t)s, s->count, s->min_len, s->max_len, s->dev_points, (int64_t)(intptr_t)s->x,
+          (int64_t)(intptr_t)s->off};
 }
 
 ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_params* pp, int32_t* out_idx,
-                   float* out_val, ts_stats* stats, bool plain) {
+                   float* out_val, ts_stats* stats, bool plain, int retry = 0) {
   auto t0 = std::chrono::steady_clock::now();
   if (!e) return fail(TS_ERR_INVALID, "null engine");
   TS_TRY(check_params(pp, db, qs));
@@ -528,9 +666,10 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
   TS_TRY(ensure(e, B_OVAL, (int64_t)Q * k, &d_val));
   TS_TRY(ensure(e, B_HOSTIN, (int64_t)Q * k, &top));
   unsigned long long* cells = nullptr;
-  TS_TRY(ensure(e, B_CELLS, 2 * kCellSlots, &cells));
+  TS_TRY(ensure(e, B_CELLS, 2 * kCellSlots + 1, &cells));
+  e->need_dev = cells + 2 * kCellSlots;
   TS_TRY(run(e, "fill", 0, 0, [&] {
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, e->st, cells, 2 * kCellSlots);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, e->st, cells, 2 * kCellSlots + 1);
     return hipGetLastError();
   }));
   if (!e->call_a) {
@@ -556,18 +695,49 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
   // Small EXACT problems refine every pair (Euclidean keeps its filter,
   // which also drops pairs of unequal length).
   const bool all_pairs = p.mode == TS_MODE_EXACT && p.measure != TS_EUCLIDEAN && (int64_t)bq * N <= kAllPairsMax;
+  // Graph replay pays where launch overhead is a visible share of a batch:
+  // at most kGraphMaxPairs pairs. Above that (measured on 4096 x 1M DTW:
+  // replays 1.28-1.48 s against 1.25 s eager, with the DP buckets' parallel
+  // branches) batches launch eagerly.
+  const bool use_graphs = e->graphs && !e->timing && !plain && (int64_t)bq * N <= kGraphMaxPairs;
   for (int32_t q0 = 0; q0 < Q; q0 += bq) {
     const int32_t nq = std::min(bq, Q - q0);
     uint64_t* topq = top + (int64_t)q0 * k;
-    if (p.mode == TS_MODE_REF) TS_TRY(knn_ref_batch(e, p, db, qs, q0, nq, env, topq));
-    else if (all_pairs) TS_TRY(knn_all_batch(e, p, db, qs, q0, nq, topq));
-    else TS_TRY(knn_exact_batch(e, p, db, qs, q0, nq, topq));
+    auto body = [&]() -> ts_status {
+      if (p.mode == TS_MODE_REF) return knn_ref_batch(e, p, db, qs, q0, nq, env, topq);
+      if (all_pairs) return knn_all_batch(e, p, db, qs, q0, nq, topq);
+      return knn_exact_batch(e, p, db, qs, q0, nq, topq);
+    };
+    if (use_graphs) {
+      std::vector<int64_t> key = set_key(db);
+      const std::vector<int64_t> qk = set_key(qs);
+      key.insert(key.end(), qk.begin(), qk.end());
+      int32_t eps_bits;
+      std::memcpy(&eps_bits, &p.epsilon, 4);
+      key.insert(key.end(), {p.measure, p.mode, p.k, p.kc, p.r, p.band, p.delta, eps_bits, q0, nq,
+                             (int64_t)(intptr_t)env, (int64_t)(intptr_t)topq});
+      TS_TRY(run_graph(e, key, body));
+    } else {
+      TS_TRY(body());
+    }
   }
   TS_TRY(run(e, "decode", 0, 0, [&] { return launch_decode_keys(e->st, top, (int64_t)Q * k, desc, d_idx, d_val); }));
   TS_HIP(hipEventRecord(e->call_b, e->st));
   TS_HIP(hipMemcpyAsync(out_idx, d_idx, sizeof(int32_t) * Q * k, hipMemcpyDeviceToHost, e->st));
   TS_HIP(hipMemcpyAsync(out_val, d_val, sizeof(float) * Q * k, hipMemcpyDeviceToHost, e->st));
   TS_HIP(hipStreamSynchronize(e->st));
+  if (use_graphs && p.mode == TS_MODE_EXACT && !all_pairs) {
+    // A replayed batch sized its candidates on the device: if any total
+    // exceeded the capacity its results are incomplete; grow and repeat.
+    unsigned long long need = 0;
+    TS_HIP(hipMemcpy(&need, e->need_dev, sizeof(need), hipMemcpyDeviceToHost));
+    if ((int64_t)need > e->exact_cap) {
+      if (retry > 0) return fail(TS_ERR_HIP, "candidate capacity still exceeded after growth");
+      e->exact_cap = std::min<int64_t>((int64_t)need + (int64_t)need / 4 + 4096, (1ll << 31) - 1);
+      ++e->overflow_retries;
+      return knn_impl(e, db, qs, pp, out_idx, out_val, stats, plain, retry + 1);
+    }
+  }
   if (stats) {
     std::vector<unsigned long long> hc(2 * kCellSlots);
     TS_HIP(hipMemcpy(hc.data(), cells, sizeof(unsigned long long) * 2 * kCellSlots, hipMemcpyDeviceToHost));
@@ -617,6 +787,7 @@ ts_status ts_engine_create(ts_engine** out) {
 ts_status ts_engine_destroy(ts_engine* e) {
   if (!e) return TS_OK;
   if (e->st) (void)hipStreamSynchronize(e->st);
+  for (GraphEntry& g : e->gcache) (void)hipGraphExecDestroy(g.exec);
   for (auto& b : e->bufs)
     if (b.p) (void)hipFree(b.p);
   for (auto ev : e->evpool) (void)hipEventDestroy(ev);
@@ -635,6 +806,20 @@ ts_status ts_engine_destroy(ts_engine* e) {
 ts_status ts_engine_set_timing(ts_engine* e, int32_t on) {
   if (!e) return fail(TS_ERR_INVALID, "null engine");
   e->timing = on != 0;
+  return TS_OK;
+}
+
+ts_status ts_engine_set_graphs(ts_engine* e, int32_t on) {
+  if (!e) return fail(TS_ERR_INVALID, "null engine");
+  e->graphs = on != 0;
+  return TS_OK;
+}
+
+ts_status ts_engine_graph_info(ts_engine* e, int64_t* captures, int64_t* replays, int64_t* cached) {
+  if (!e) return fail(TS_ERR_INVALID, "null engine");
+  if (captures) *captures = e->graph_captures;
+  if (replays) *replays = e->graph_replays;
+  if (cached) *cached = (int64_t)e->gcache.size();
   return TS_OK;
 }
 

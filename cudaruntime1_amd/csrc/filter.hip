@@ -241,7 +241,7 @@ constexpr int kCompactQ = 8, kCompactJ = 2;
 template <bool WRITE>
 __global__ void __launch_bounds__(256) k_bound_compact(MeasureArgs ma, int32_t k, SetView q, int32_t q0, int32_t nq,
                                                        SetView db, const float* tau, const int64_t* off,
-                                                       int32_t* counts, Item* items) {
+                                                       int32_t* counts, Item* items, int64_t cap) {
   __shared__ QSum sq[kCompactQ];
   __shared__ float st[kCompactQ];
   const int qb = blockIdx.y * kCompactQ;
@@ -276,7 +276,7 @@ __global__ void __launch_bounds__(256) k_bound_compact(MeasureArgs ma, int32_t k
         base = __shfl(base, leader);
         if (pass) {
           const int64_t pos = off[qb + u] + base + __popcll(bal & ((1ull << lane) - 1ull));
-          items[pos] = Item{q0 + qb + u, j, (int32_t)pos, 0};
+          if (pos < cap) items[pos] = Item{q0 + qb + u, j, (int32_t)pos, 0};
         }
       }
     }
@@ -284,10 +284,13 @@ __global__ void __launch_bounds__(256) k_bound_compact(MeasureArgs ma, int32_t k
 }
 
 hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, int32_t nq,
-                                SetView db, const float* tau, const int64_t* off, int32_t* counts, Item* items) {
+                                SetView db, const float* tau, const int64_t* off, int32_t* counts, Item* items,
+                                int64_t cap) {
   dim3 grid((db.count + 256 * kCompactJ - 1) / (256 * kCompactJ), (nq + kCompactQ - 1) / kCompactQ);
-  if (items) hipLaunchKernelGGL(k_bound_compact<true>, grid, dim3(256), 0, st, m, k, q, q0, nq, db, tau, off, counts, items);
-  else hipLaunchKernelGGL(k_bound_compact<false>, grid, dim3(256), 0, st, m, k, q, q0, nq, db, tau, off, counts, items);
+  if (items)
+    hipLaunchKernelGGL(k_bound_compact<true>, grid, dim3(256), 0, st, m, k, q, q0, nq, db, tau, off, counts, items, cap);
+  else
+    hipLaunchKernelGGL(k_bound_compact<false>, grid, dim3(256), 0, st, m, k, q, q0, nq, db, tau, off, counts, items, cap);
   return hipGetLastError();
 }
 

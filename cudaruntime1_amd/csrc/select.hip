@@ -50,33 +50,45 @@ hipError_t launch_keys_from_rows(hipStream_t st, const float* vals, int32_t rows
   return hipGetLastError();
 }
 
+// One selection stage: segment seg's keys [0, len) in chunks of ch; chunk c
+// writes its best k to out[(seg * nch + c) * k]. Blocks stride over the
+// chunks (the grid's x is capped), so an upper-bound seg_len costs little:
+// chunks past a segment's end only write empty keys.
 __global__ void __launch_bounds__(256) k_topk_stage(const uint64_t* in, int64_t seg_stride, int32_t seg_len,
-                                                    const int32_t* counts, int32_t k, int32_t ch,
+                                                    const int32_t* counts, int32_t k, int32_t ch, int32_t nch,
                                                     uint64_t* out, const int64_t* seg_off) {
   __shared__ uint64_t s[kSelChunk];
-  const int seg = blockIdx.y, c = blockIdx.x;
+  const int seg = blockIdx.y;
   int len = seg_len;
   if (counts) len = min(counts[seg], seg_len);
-  const int begin = c * ch;
-  const int nh = max(0, min(ch, len - begin));
-  const uint64_t* src = in + (seg_off ? seg_off[seg] : (int64_t)seg * seg_stride) + begin;
-  for (int i = threadIdx.x; i < ch; i += 256) s[i] = i < nh ? src[i] : ~0ull;
-  __syncthreads();
-  for (int kk = 2; kk <= ch; kk <<= 1) {
-    for (int j = kk >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < ch; i += 256) {
-        int ixj = i ^ j;
-        if (ixj > i) {
-          uint64_t a = s[i], b = s[ixj];
-          bool asc = (i & kk) == 0;
-          if ((a > b) == asc) { s[i] = b; s[ixj] = a; }
-        }
-      }
-      __syncthreads();
+  const uint64_t* base = in + (seg_off ? seg_off[seg] : (int64_t)seg * seg_stride);
+  for (int c = blockIdx.x; c < nch; c += gridDim.x) {
+    const int begin = c * ch;
+    const int nh = max(0, min(ch, len - begin));
+    uint64_t* dst = out + ((int64_t)seg * nch + c) * k;
+    if (nh == 0) {  // block-uniform
+      for (int t = threadIdx.x; t < k; t += 256) dst[t] = ~0ull;
+      continue;
     }
+    __syncthreads();  // the previous chunk's results are out of s
+    const uint64_t* src = base + begin;
+    for (int i = threadIdx.x; i < ch; i += 256) s[i] = i < nh ? src[i] : ~0ull;
+    __syncthreads();
+    for (int kk = 2; kk <= ch; kk <<= 1) {
+      for (int j = kk >> 1; j > 0; j >>= 1) {
+        for (int i = threadIdx.x; i < ch; i += 256) {
+          int ixj = i ^ j;
+          if (ixj > i) {
+            uint64_t a = s[i], b = s[ixj];
+            bool asc = (i & kk) == 0;
+            if ((a > b) == asc) { s[i] = b; s[ixj] = a; }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int t = threadIdx.x; t < k; t += 256) dst[t] = t < ch ? s[t] : ~0ull;
   }
-  uint64_t* dst = out + ((int64_t)seg * gridDim.x + c) * k;
-  for (int t = threadIdx.x; t < k; t += 256) dst[t] = t < ch ? s[t] : ~0ull;
 }
 
 static int chunk_for(int32_t len) {
@@ -108,7 +120,9 @@ hipError_t select_topk(hipStream_t st, const uint64_t* in, int64_t seg_stride, i
     int ch = nch == 1 ? chunk_for(len) : kSelChunk;
     uint64_t* dst = nch == 1 ? out : bufs[which];
     if (nch > 1 && (int64_t)nseg * nch * k > scratch_elems) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_topk_stage, dim3(nch, nseg), dim3(256), 0, st, cur, stride, len, cnt, k, ch, dst, seg_off);
+    const int gx = nch < 64 ? nch : 64;
+    hipLaunchKernelGGL(k_topk_stage, dim3(gx, nseg), dim3(256), 0, st, cur, stride, len, cnt, k, ch, nch, dst,
+                       seg_off);
     if (launches) ++*launches;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || nch == 1) return e;
@@ -178,6 +192,46 @@ hipError_t launch_items_from_keys(hipStream_t st, const uint64_t* keys, int32_t 
   hipLaunchKernelGGL(k_items_from_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys, rows, s, q0,
                      items);
   (void)nitems;
+  return hipGetLastError();
+}
+
+// Candidate regions sized on the device: exclusive prefix of counts[nq]
+// into off[nq+1] (int64), the regions' total clamped to cap into *ntot, each
+// region's length clamped to the capacity into segc[nq], and the unclamped
+// total folded into *need (atomic max over the batches of one call). One
+// block of 1024 threads.
+__global__ void __launch_bounds__(1024) k_scan_offsets(const int32_t* counts, int32_t nq, int64_t cap, int64_t* off,
+                                                       int32_t* ntot, int32_t* segc, unsigned long long* need) {
+  __shared__ int64_t part[1024];
+  const int per = (nq + 1023) / 1024;
+  const int b = threadIdx.x * per;
+  int64_t s = 0;
+  for (int i = b; i < min(nq, b + per); ++i) s += counts[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const int64_t v = threadIdx.x >= d ? part[threadIdx.x - d] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int64_t run = part[threadIdx.x] - s;
+  for (int i = b; i < min(nq, b + per); ++i) {
+    off[i] = run;
+    segc[i] = (int32_t)max<int64_t>(0, min<int64_t>(counts[i], cap - run));
+    run += counts[i];
+  }
+  if (threadIdx.x == 1023) {
+    const int64_t total = part[1023];
+    off[nq] = total;
+    *ntot = (int32_t)min<int64_t>(total, cap);
+    atomicMax(need, (unsigned long long)total);
+  }
+}
+
+hipError_t launch_scan_offsets(hipStream_t st, const int32_t* counts, int32_t nq, int64_t cap, int64_t* off,
+                               int32_t* ntot, int32_t* segc, unsigned long long* need) {
+  hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, st, counts, nq, cap, off, ntot, segc, need);
   return hipGetLastError();
 }
 

@@ -229,6 +229,14 @@ ts_status ts_set_create(const ts_point* xy, const int64_t* offsets, int32_t coun
   return TS_OK;
 }
 
+ts_status ts_set_reload(ts_set* s, const ts_point* xy, const int64_t* offsets, int32_t count) {
+  if (!s) return fail(TS_ERR_INVALID, "ts_set_reload: null set");
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (dev != s->device) return fail(TS_ERR_INVALID, "ts_set_reload: set lives on another device");
+  return set_load(s, xy, offsets, count, nullptr);
+}
+
 ts_status ts_set_destroy(ts_set* s) {
   if (!s) return TS_OK;
   (void)hipFree(s->x); (void)hipFree(s->y); (void)hipFree(s->off); (void)hipFree(s->len);

@@ -127,9 +127,11 @@ hipError_t launch_bound_chunks(hipStream_t st, MeasureArgs m, int32_t k, SetView
 // Threshold compaction of every DB trajectory whose bound can still reach
 // the query's threshold tau[q]: with items == null counts survivors into
 // counts[q]; otherwise appends them as Items at off[q] + counts[q] (counts
-// then serve as cursors and must start at zero).
+// then serve as cursors and must start at zero); positions >= cap are
+// dropped.
 hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, int32_t nq,
-                                SetView db, const float* tau, const int64_t* off, int32_t* counts, Item* items);
+                                SetView db, const float* tau, const int64_t* off, int32_t* counts, Item* items,
+                                int64_t cap);
 // Second, O(n+m) admissible bound over compacted pairs (one wave per pair);
 // survivors are appended to `out` with count *nout (zeroed by the caller).
 hipError_t launch_tight_filter(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, SetView db,
@@ -174,6 +176,11 @@ hipError_t launch_fill_i32(hipStream_t st, int32_t* p, int32_t v, int64_t n);
 hipError_t launch_items_by_bucket(hipStream_t st, const Item* items, const int32_t* nitems, int64_t max_items,
                                   SetView q, SetView db, MeasureArgs m, Item* outb, int64_t cap_each,
                                   int32_t* countsb);
+// Exclusive prefix of counts into off[nq+1] on the device; *ntot = total
+// clamped to cap, segc[q] = region length clamped to cap, *need = max(*need,
+// total).
+hipError_t launch_scan_offsets(hipStream_t st, const int32_t* counts, int32_t nq, int64_t cap, int64_t* off,
+                               int32_t* ntot, int32_t* segc, unsigned long long* need);
 // Every pair of queries [q0, q0+nq) with all N DB trajectories, slot = index.
 hipError_t launch_items_all(hipStream_t st, int32_t q0, int32_t nq, int32_t N, Item* items);
 hipError_t launch_items_from_regions(hipStream_t st, const int32_t* region_d, const int32_t* counts,

@@ -104,6 +104,10 @@ ts_status ts_device_name(char* buf, int32_t buflen);
 typedef struct ts_set ts_set;
 ts_status ts_set_create(const ts_point* xy, const int64_t* offsets, int32_t count, ts_set** out);
 ts_status ts_set_destroy(ts_set* s);
+/* Replace a set's trajectories in place, keeping its device buffers when the
+ * new data fits (they grow otherwise). No call that reads the set may be in
+ * flight. */
+ts_status ts_set_reload(ts_set* s, const ts_point* xy, const int64_t* offsets, int32_t count);
 ts_status ts_set_info(const ts_set* s, int32_t* count, int64_t* points, int32_t* min_len, int32_t* max_len);
 /* Device memory held by the set, in bytes. */
 ts_status ts_set_bytes(const ts_set* s, int64_t* bytes);
@@ -115,6 +119,19 @@ ts_status ts_engine_destroy(ts_engine* e);
 /* Record a HIP event pair around every kernel launch (per-kernel times for
  * ts_engine_kernel_stats); off by default. */
 ts_status ts_engine_set_timing(ts_engine* e, int32_t on);
+/* Replay batches of ts_knn from HIP graphs (off by default; not used while
+ * timing is on, nor by ts_knn_plain). A batch's launch sequence is recorded
+ * the second time the same shape is seen (same set handles, lengths range,
+ * parameters, batch range and workspace) and replayed from then on; the data
+ * behind the handles may change between calls (ts_set reload in place).
+ * Recorded EXACT batches size their candidate lists on the device within a
+ * capacity; a call whose candidates exceed it grows the capacity and runs
+ * again (ts_stats.overflow_retries). Results are identical either way.
+ * Batches of more than 2^26 (query, DB) pairs always launch eagerly: there
+ * launch overhead is negligible and replays measured slower. */
+ts_status ts_engine_set_graphs(ts_engine* e, int32_t on);
+/* Graphs recorded and replayed so far, and graphs currently cached. */
+ts_status ts_engine_graph_info(ts_engine* e, int64_t* captures, int64_t* replays, int64_t* cached);
 /* Per-kernel totals since the last reset: launches, summed kernel ms and
  * summed algorithmic bytes / work units that the launcher attributes to it.
  * idx enumerates kernels; returns TS_ERR_INVALID past the last one. */

@@ -32,3 +32,8 @@ def planted(db_xy, db_off, picks, lengths, noise, seed):
 
 def traj(xy, off, i):
     return xy[off[i]:off[i + 1]]
+
+
+def say(*a):
+    """Name the case before a GPU step (flushed, so a crash still shows it)."""
+    print("[case]", *a, flush=True)
