@@ -68,6 +68,12 @@ _sig = {
     "ts_engine_destroy": (ctypes.c_int, [_P]),
     "ts_engine_set_timing": (ctypes.c_int, [_P, ctypes.c_int32]),
     "ts_engine_set_graphs": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "ts_dp_shape_count": (ctypes.c_int32, []),
+    "ts_dp_shape": (ctypes.c_int, [ctypes.c_int32, _P, _P]),
+    "ts_engine_load_tuning": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "ts_engine_dp_fallbacks": (ctypes.c_int, [_P, _P]),
+    "ts_engine_set_dp_shape": (ctypes.c_int, [_P] + [ctypes.c_int32] * 5),
+    "ts_engine_get_dp_shape": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P]),
     "ts_engine_graph_info": (ctypes.c_int, [_P, _P, _P, _P]),
     "ts_engine_kernel_stats": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32,
                                               ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
@@ -94,6 +100,22 @@ for _name, (_res, _args) in _sig.items():
     _f = getattr(_lib, _name)
     _f.restype = _res
     _f.argtypes = _args
+
+
+# DP variants (rows of the tuning table) and length buckets (largest row
+# width the DP visits in each).
+DP_VARIANTS = ["dtw", "dtw_band", "frechet", "lcss", "lcss_window"]
+DP_BUCKET_CAPS = [31, 63, 127, 255, 1023, 1 << 24]
+
+
+def dp_shapes():
+    """[(lanes per pair, cells per lane step)] the DP kernels are built for."""
+    out = []
+    for i in range(_lib.ts_dp_shape_count()):
+        w, u = ctypes.c_int32(), ctypes.c_int32()
+        _lib.ts_dp_shape(i, ctypes.byref(w), ctypes.byref(u))
+        out.append((w.value, u.value))
+    return out
 
 
 class TrajsimError(RuntimeError):
@@ -227,6 +249,25 @@ class Engine:
     def set_graphs(self, on):
         """Replay batched k-NN calls of a repeated shape from HIP graphs."""
         _check(_lib.ts_engine_set_graphs(self._h, int(on)), "ts_engine_set_graphs")
+
+    def load_tuning(self, path):
+        _check(_lib.ts_engine_load_tuning(self._h, str(path).encode()), "ts_engine_load_tuning")
+
+    def set_dp_shape(self, variant, bucket, lanes, cells, waves_per_cu):
+        """Force the DP launch shape of one (variant, bucket); see DP_VARIANTS."""
+        _check(_lib.ts_engine_set_dp_shape(self._h, variant, bucket, lanes, cells, waves_per_cu),
+               "ts_engine_set_dp_shape")
+
+    def get_dp_shape(self, variant, bucket):
+        w, u, g = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(_lib.ts_engine_get_dp_shape(self._h, variant, bucket, ctypes.byref(w), ctypes.byref(u),
+                                           ctypes.byref(g)), "ts_engine_get_dp_shape")
+        return w.value, u.value, g.value
+
+    def dp_fallbacks(self):
+        n = ctypes.c_int64()
+        _check(_lib.ts_engine_dp_fallbacks(self._h, ctypes.byref(n)), "ts_engine_dp_fallbacks")
+        return n.value
 
     def graph_info(self):
         c, r, n = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()

@@ -68,7 +68,11 @@ __device__ __forceinline__ float shift_from_prev_lane(float v) {
 
 template <int U>
 __device__ __forceinline__ void load_vec(const float* p, float (&v)[U]) {
-  if constexpr (U == 4) {
+  if constexpr (U == 8) {
+    const float4 t = *(const float4*)p, u = *(const float4*)(p + 4);
+    v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+    v[4] = u.x; v[5] = u.y; v[6] = u.z; v[7] = u.w;
+  } else if constexpr (U == 4) {
     const float4 t = *(const float4*)p;
     v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
   } else if constexpr (U == 2) {
@@ -81,7 +85,10 @@ __device__ __forceinline__ void load_vec(const float* p, float (&v)[U]) {
 
 template <int U>
 __device__ __forceinline__ void store_vec(float* p, const float (&v)[U]) {
-  if constexpr (U == 4) {
+  if constexpr (U == 8) {
+    *(float4*)p = make_float4(v[0], v[1], v[2], v[3]);
+    *(float4*)(p + 4) = make_float4(v[4], v[5], v[6], v[7]);
+  } else if constexpr (U == 4) {
     *(float4*)p = make_float4(v[0], v[1], v[2], v[3]);
   } else if constexpr (U == 2) {
     *(float2*)p = make_float2(v[0], v[1]);
@@ -386,29 +393,22 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
   }
 }
 
-// Buckets by the pair's shorter length C (dp_bucket_of): lanes per pair W,
-// cells per lane step U, largest C. A row has max(ceil(C/U), W) chunks.
-struct Bucket {
-  int W, U, cmax_cap;
-};
-constexpr Bucket kBuckets[kDpBuckets] = {{16, 1, 31},  {16, 2, 63},   {16, 4, 127},
-                                          {32, 4, 255}, {64, 4, 1023}, {64, 4, 1 << 24}};
-
-// Halo of a bucket. Without a band: a whole row of the longest C in the
-// bucket. With a band: a row of the longest C when that is short, else a
-// ring of 2^k floats covering a strip's range, which spans at most
-// W-1+2w+2U columns (band half-width w, 2w+1 <= width) or W+1 chunks plus
-// the range's shift, plus the chunk read before the strip starts.
-DpHalo dp_halo(int b, const MeasureArgs& m, int32_t cmax_c, int32_t wmax_c) {
-  const Bucket& k = kBuckets[b];
+// Halo of a launch of shape `shape` over bucket b. Without a band: a whole
+// row of the longest C in the bucket (at least W chunks). With a band: a row
+// of the longest C when that is short, else a ring of 2^k floats covering a
+// strip's range, which spans at most W-1+2w+2U columns (band half-width w,
+// 2w+1 <= width) or W+1 chunks plus the range's shift, plus the chunk read
+// before the strip starts.
+DpHalo dp_halo(int shape, int b, const MeasureArgs& m, int32_t cmax_c, int32_t wmax_c) {
+  const DpShape& k = kDpShapeList[shape];
   if (!dp_ranged(m)) {
-    const int32_t c = std::min<int32_t>(cmax_c, k.cmax_cap);
+    const int32_t c = std::min<int32_t>(cmax_c, kDpBucketCap[b]);
     const int32_t chunks = std::max((c + k.U - 1) / k.U, k.W);
     return DpHalo{((chunks * k.U + 3) / 4) * 4, -1};
   }
   const int32_t chunks = std::max((cmax_c + k.U - 1) / k.U, k.W);
   const int32_t full = ((chunks * k.U + 3) / 4) * 4;
-  const int32_t wd = std::min<int32_t>(wmax_c, k.cmax_cap);
+  const int32_t wd = std::min<int32_t>(wmax_c, kDpBucketCap[b]);
   const int32_t need = std::max(k.W + wd + 4 * k.U, k.W * k.U + k.W + 4 * k.U);
   int32_t h = 4;
   while (h < need) h *= 2;
@@ -432,41 +432,53 @@ static hipError_t launch_dp_t(hipStream_t st, const DpArgs& a, int grid_waves) {
   return hipGetLastError();
 }
 
+// Whether a launch of `shape` with this halo fits the 64 KiB of LDS a block
+// may use (one wave per block at least; staging is only chosen when it fits
+// in 20 KiB per wave).
+bool dp_fits(int shape, DpHalo halo) {
+  const int G = 64 / kDpShapeList[shape].W;
+  return (size_t)G * dp_group_floats(halo.len, false) * sizeof(float) + G * sizeof(int) + 64 <= 65536;
+}
+
+// Staging the column trajectory in LDS pays while it leaves room for 8
+// waves per CU (20 KiB per wave); banded rows never stage (a pair's column
+// trajectory can be far longer than the band the LDS holds).
+template <int MEAS, int W, int U, bool BAND, bool DELTA>
+static hipError_t launch_dp_s(hipStream_t st, const DpArgs& a, int grid_waves) {
+  constexpr int G = 64 / W;
+  const bool stage = !(BAND || DELTA) && (size_t)G * dp_group_floats(a.cmax, true) * sizeof(float) <= 20480;
+  if (BAND || DELTA || !stage) return launch_dp_t<MEAS, W, U, BAND, DELTA, false>(st, a, grid_waves);
+  return launch_dp_t<MEAS, W, U, BAND, DELTA, true>(st, a, grid_waves);
+}
+
 template <int MEAS, bool BAND, bool DELTA>
-static hipError_t launch_dp_b(hipStream_t st, const DpArgs& a, int bucket, int grid_waves) {
-  if (BAND || DELTA) {
-    // Banded rows read their columns from global memory: a pair's column
-    // trajectory can be far longer than the band the LDS holds.
-    switch (bucket) {
-      case 0: return launch_dp_t<MEAS, 16, 1, BAND, DELTA, false>(st, a, grid_waves);
-      case 1: return launch_dp_t<MEAS, 16, 2, BAND, DELTA, false>(st, a, grid_waves);
-      case 2: return launch_dp_t<MEAS, 16, 4, BAND, DELTA, false>(st, a, grid_waves);
-      case 3: return launch_dp_t<MEAS, 32, 4, BAND, DELTA, false>(st, a, grid_waves);
-      default: return launch_dp_t<MEAS, 64, 4, BAND, DELTA, false>(st, a, grid_waves);
-    }
-  }
-  switch (bucket) {
-    case 0: return launch_dp_t<MEAS, 16, 1, BAND, DELTA, true>(st, a, grid_waves);
-    case 1: return launch_dp_t<MEAS, 16, 2, BAND, DELTA, true>(st, a, grid_waves);
-    case 2: return launch_dp_t<MEAS, 16, 4, BAND, DELTA, true>(st, a, grid_waves);
-    case 3: return launch_dp_t<MEAS, 32, 4, BAND, DELTA, true>(st, a, grid_waves);
-    case 4: return launch_dp_t<MEAS, 64, 4, BAND, DELTA, true>(st, a, grid_waves);
-    default: return launch_dp_t<MEAS, 64, 4, BAND, DELTA, false>(st, a, grid_waves);
+static hipError_t launch_dp_b(hipStream_t st, const DpArgs& a, int shape, int grid_waves) {
+  switch (shape) {
+    case 0: return launch_dp_s<MEAS, 16, 1, BAND, DELTA>(st, a, grid_waves);
+    case 1: return launch_dp_s<MEAS, 16, 2, BAND, DELTA>(st, a, grid_waves);
+    case 2: return launch_dp_s<MEAS, 16, 4, BAND, DELTA>(st, a, grid_waves);
+    case 3: return launch_dp_s<MEAS, 32, 2, BAND, DELTA>(st, a, grid_waves);
+    case 4: return launch_dp_s<MEAS, 32, 4, BAND, DELTA>(st, a, grid_waves);
+    case 5: return launch_dp_s<MEAS, 64, 2, BAND, DELTA>(st, a, grid_waves);
+    case 6: return launch_dp_s<MEAS, 64, 4, BAND, DELTA>(st, a, grid_waves);
+    case 7: return launch_dp_s<MEAS, 32, 8, BAND, DELTA>(st, a, grid_waves);
+    case 8: return launch_dp_s<MEAS, 64, 8, BAND, DELTA>(st, a, grid_waves);
+    default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t launch_dp(hipStream_t st, MeasureArgs m, int bucket, SetView q, SetView db, const Item* items,
+hipError_t launch_dp(hipStream_t st, MeasureArgs m, int shape, SetView q, SetView db, const Item* items,
                      const int32_t* nitems, int64_t max_items, int32_t* next, float* out, DpHalo halo,
                      int grid_waves) {
   DpArgs a{q, db, items, nitems, max_items, next, out, halo.len, halo.mask, m.band, m.delta, m.eps};
   switch (m.measure) {
     case TS_DTW:
-      return m.band != 0 ? launch_dp_b<TS_DTW, true, false>(st, a, bucket, grid_waves)
-                        : launch_dp_b<TS_DTW, false, false>(st, a, bucket, grid_waves);
-    case TS_FRECHET: return launch_dp_b<TS_FRECHET, false, false>(st, a, bucket, grid_waves);
+      return m.band != 0 ? launch_dp_b<TS_DTW, true, false>(st, a, shape, grid_waves)
+                        : launch_dp_b<TS_DTW, false, false>(st, a, shape, grid_waves);
+    case TS_FRECHET: return launch_dp_b<TS_FRECHET, false, false>(st, a, shape, grid_waves);
     case TS_LCSS:
-      return m.delta >= 0 ? launch_dp_b<TS_LCSS, false, true>(st, a, bucket, grid_waves)
-                          : launch_dp_b<TS_LCSS, false, false>(st, a, bucket, grid_waves);
+      return m.delta >= 0 ? launch_dp_b<TS_LCSS, false, true>(st, a, shape, grid_waves)
+                          : launch_dp_b<TS_LCSS, false, false>(st, a, shape, grid_waves);
     default: return hipErrorInvalidValue;
   }
 }

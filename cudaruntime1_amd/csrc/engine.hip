@@ -15,8 +15,11 @@
 // Batched (ts_knn) runs each stage once for a whole batch of queries, with
 // persistent refine kernels draining one work list. Plain (ts_knn_plain)
 // runs the same stages once per query.
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -55,33 +58,34 @@ __device__ unsigned long long pair_cells(int64_t a, int64_t b, int32_t measure, 
   return (unsigned long long)(a * b);
 }
 
+// Cells and pairs of an item list, per DP bucket (acc[b], acc_pairs[b]), so
+// that each bucket's launch is credited with its own work. Measures without
+// DP count into bucket 0.
 __global__ void k_count_cells(const Item* items, const int32_t* nitems, int64_t max_items, SetView q, SetView db,
-                              int32_t measure, int32_t band, int32_t delta, unsigned long long* acc,
+                              MeasureArgs ma, int32_t by_bucket, unsigned long long* acc,
                               unsigned long long* acc_pairs) {
-  __shared__ unsigned long long part[4], partp[4];
+  __shared__ unsigned long long part[kDpBuckets], partp[kDpBuckets];
+  if (threadIdx.x < kDpBuckets) {
+    part[threadIdx.x] = 0;
+    partp[threadIdx.x] = 0;
+  }
+  __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   int64_t n = max_items;
   if (nitems) n = min<int64_t>(n, (int64_t)*nitems);
-  unsigned long long c = 0, np = 0;
   if (i < n) {
     const Item it = items[i];
     if (it.d >= 0) {
-      c = pair_cells(q.len[it.q], db.len[it.d], measure, band, delta);
-      np = 1;
+      const int a = q.len[it.q], b = db.len[it.d];
+      const int bk = by_bucket ? dp_bucket_of(dp_width_of(a, b, ma)) : 0;
+      atomicAdd(&part[bk], pair_cells(a, b, ma.measure, ma.band, ma.delta));
+      atomicAdd(&partp[bk], 1ull);
     }
   }
-  for (int sh = 32; sh > 0; sh >>= 1) {
-    c += __shfl_xor(c, sh);
-    np += __shfl_xor(np, sh);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    part[threadIdx.x >> 6] = c;
-    partp[threadIdx.x >> 6] = np;
-  }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    atomicAdd(acc, part[0] + part[1] + part[2] + part[3]);
-    atomicAdd(acc_pairs, partp[0] + partp[1] + partp[2] + partp[3]);
+  if (threadIdx.x < kDpBuckets && partp[threadIdx.x] != 0) {
+    atomicAdd(acc + threadIdx.x, part[threadIdx.x]);
+    atomicAdd(acc_pairs + threadIdx.x, partp[threadIdx.x]);
   }
 }
 
@@ -111,6 +115,13 @@ struct KStat {
   int64_t launches = 0;
   double ms = 0, bytes = 0, units = 0;
 };
+
+// DP launch shape and persistent grid (waves per CU) of one bucket.
+struct DpTune {
+  int shape;
+  int waves_per_cu;
+};
+constexpr int kDpVariants = 5;  // DTW, DTW banded, Frechet, LCSS, LCSS windowed
 
 struct Pending {
   int kid;
@@ -153,6 +164,10 @@ struct ts_engine {
   std::vector<unsigned long long> cells_host;
   int cells_used = 0;
   int64_t overflow_retries = 0;
+  // DP shape per (variant, bucket): defaults, then the tuning table.
+  DpTune tune[kDpVariants][kDpBuckets];
+  std::string tuning_source = "defaults";
+  int64_t dp_fallbacks = 0;  // launches that fell back to the default shape
   // hipGraph replay of batches (off by default; never with timing or for
   // the plain method).
   bool graphs = false;
@@ -174,7 +189,15 @@ struct ts_engine {
 namespace ts {
 namespace {
 
-constexpr int kCellSlots = 64;
+// Cell / pair counters: kCellSlots of each, then the candidate total of
+// device-sized EXACT batches.
+constexpr int kCellSlots = 1024;
+constexpr int kCellBuf = 2 * kCellSlots + 1;
+
+hipError_t zero_cells(hipStream_t st, unsigned long long* cells) {
+  hipLaunchKernelGGL(k_zero_u64, dim3((kCellBuf + 255) / 256), dim3(256), 0, st, cells, kCellBuf);
+  return hipGetLastError();
+}
 
 template <class T>
 ts_status ensure(ts_engine* e, BufId id, int64_t n, T** out) {
@@ -272,17 +295,25 @@ ts_status resolve_timing(ts_engine* e) {
   return TS_OK;
 }
 
-// Persistent DP grid: waves per CU (tuned per measure; TS_DP_WAVES_PER_CU
-// overrides for sweeps).
-int dp_waves_per_cu(ts_engine* e, int measure) {
-  static int env = [] {
-    const char* v = std::getenv("TS_DP_WAVES_PER_CU");
-    return v ? std::atoi(v) : 0;
-  }();
-  if (env > 0) return env;
-  (void)e;
-  (void)measure;
-  return 8;  // best of a 2..32 sweep on the c2 workload (DTW, 64 x 100k)
+// Halo of the tuned shape; a shape whose halo does not fit a block's LDS
+// (a long band on a narrow group) falls back to the bucket's default shape.
+DpHalo shape_halo(ts_engine* e, int& sh, int b, const MeasureArgs& ma, int32_t hi, int32_t whi) {
+  DpHalo h = dp_halo(sh, b, ma, hi, whi);
+  if (!dp_fits(sh, h) && sh != kDpDefaultShape[b]) {
+    sh = kDpDefaultShape[b];
+    h = dp_halo(sh, b, ma, hi, whi);
+    ++e->dp_fallbacks;
+  }
+  return h;
+}
+
+// DP variant of a parameter set: its row in the tuning table.
+int dp_variant(const ts_params& p) {
+  switch (p.measure) {
+    case TS_DTW: return p.band != 0 ? 1 : 0;
+    case TS_FRECHET: return 2;
+    default: return p.delta >= 0 ? 4 : 3;
+  }
 }
 
 MeasureArgs margs(const ts_params& p) { return MeasureArgs{p.measure, p.band, p.delta, p.epsilon}; }
@@ -314,12 +345,16 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   const MeasureArgs ma = margs(p);
   const int cus = device_cu_count();
   unsigned long long* cells = nullptr;
-  TS_TRY(ensure(e, B_CELLS, 2 * kCellSlots, &cells));
-  const int slot = e->cells_used < kCellSlots ? e->cells_used++ : kCellSlots - 1;
+  TS_TRY(ensure(e, B_CELLS, kCellBuf, &cells));
+  // kDpBuckets counter slots for this refine (the last group is shared once
+  // the slots run out; totals stay right, per-launch credit blurs)
+  int slot = e->cells_used;
+  if (slot + kDpBuckets <= kCellSlots) e->cells_used += kDpBuckets;
+  else slot = kCellSlots - kDpBuckets;
+  const bool dp = p.measure == TS_DTW || p.measure == TS_FRECHET || p.measure == TS_LCSS;
   TS_TRY(run(e, "count_cells", 0, 0, [&] {
     hipLaunchKernelGGL(k_count_cells, dim3((unsigned)((max_items + 255) / 256)), dim3(256), 0, e->st, items,
-                       nitems_dev, max_items, qv, dv, p.measure, p.band, p.delta, cells + slot,
-                       cells + kCellSlots + slot);
+                       nitems_dev, max_items, qv, dv, ma, dp ? 1 : 0, cells + slot, cells + kCellSlots + slot);
     return hipGetLastError();
   }));
   int32_t* ctr = nullptr;
@@ -345,13 +380,15 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   static const char* kNames[2][3] = {{"dtw_wavefront", "frechet_wavefront", "lcss_wavefront"},
                                      {"dtw_wavefront_seed", "frechet_wavefront_seed", "lcss_wavefront_seed"}};
   const char* nm = kNames[seed ? 1 : 0][p.measure == TS_DTW ? 0 : (p.measure == TS_FRECHET ? 1 : 2)];
-  const int waves = cus * dp_waves_per_cu(e, p.measure);
+  const DpTune* tune = e->tune[dp_variant(p)];
   if (b_lo == b_hi) {
     TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, ctr, 0, 1); }));
-    const DpHalo halo = dp_halo(b_lo, ma, hi, whi);
+    int sh = tune[b_lo].shape;
+    const DpHalo halo = shape_halo(e, sh, b_lo, ma, hi, whi);
     return run(e, nm, 0, 0, [&] {
-      return launch_dp(e->st, ma, b_lo, qv, dv, items, nitems_dev, max_items, ctr, vals, halo, waves);
-    }, slot);
+      return launch_dp(e->st, ma, sh, qv, dv, items, nitems_dev, max_items, ctr, vals, halo,
+                       cus * tune[b_lo].waves_per_cu);
+    }, slot + b_lo);
   }
   Item* itemsb = nullptr;
   TS_TRY(ensure(e, B_ITEMS3, (int64_t)kDpBuckets * max_items, &itemsb));
@@ -364,12 +401,13 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   // the latency-bound tail of one overlaps the others.
   TS_HIP(hipEventRecord(e->fork, e->st));
   for (int b = b_lo; b <= b_hi; ++b) {
-    const DpHalo halo = dp_halo(b, ma, hi, whi);
+    int sh = tune[b].shape;
+    const DpHalo halo = shape_halo(e, sh, b, ma, hi, whi);
     TS_HIP(hipStreamWaitEvent(e->aux[b], e->fork, 0));
     TS_TRY(run_on(e, e->aux[b], nm, 0, 0, [&] {
-      return launch_dp(e->aux[b], ma, b, qv, dv, itemsb + (int64_t)b * max_items, ctr + kDpBuckets + b, max_items,
-                       ctr + b, vals, halo, waves);
-    }, slot));
+      return launch_dp(e->aux[b], ma, sh, qv, dv, itemsb + (int64_t)b * max_items, ctr + kDpBuckets + b, max_items,
+                       ctr + b, vals, halo, cus * tune[b].waves_per_cu);
+    }, slot + b));
     TS_HIP(hipEventRecord(e->join[b], e->aux[b]));
     TS_HIP(hipStreamWaitEvent(e->st, e->join[b], 0));
   }
@@ -666,12 +704,9 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
   TS_TRY(ensure(e, B_OVAL, (int64_t)Q * k, &d_val));
   TS_TRY(ensure(e, B_HOSTIN, (int64_t)Q * k, &top));
   unsigned long long* cells = nullptr;
-  TS_TRY(ensure(e, B_CELLS, 2 * kCellSlots + 1, &cells));
+  TS_TRY(ensure(e, B_CELLS, kCellBuf, &cells));
   e->need_dev = cells + 2 * kCellSlots;
-  TS_TRY(run(e, "fill", 0, 0, [&] {
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, e->st, cells, 2 * kCellSlots + 1);
-    return hipGetLastError();
-  }));
+  TS_TRY(run(e, "fill", 0, 0, [&] { return zero_cells(e->st, cells); }));
   if (!e->call_a) {
     TS_HIP(hipEventCreate(&e->call_a));
     TS_HIP(hipEventCreate(&e->call_b));
@@ -739,8 +774,13 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
     }
   }
   if (stats) {
-    std::vector<unsigned long long> hc(2 * kCellSlots);
-    TS_HIP(hipMemcpy(hc.data(), cells, sizeof(unsigned long long) * 2 * kCellSlots, hipMemcpyDeviceToHost));
+    const int used = std::min(e->cells_used, kCellSlots);
+    std::vector<unsigned long long> hc(2 * kCellSlots, 0ull);
+    if (used > 0) {
+      TS_HIP(hipMemcpy(hc.data(), cells, sizeof(unsigned long long) * used, hipMemcpyDeviceToHost));
+      TS_HIP(hipMemcpy(hc.data() + kCellSlots, cells + kCellSlots, sizeof(unsigned long long) * used,
+                       hipMemcpyDeviceToHost));
+    }
     float dms = 0;
     TS_HIP(hipEventElapsedTime(&dms, e->call_a, e->call_b));
     stats->ms_device = dms;
@@ -764,12 +804,91 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
 
 extern "C" {
 
+namespace ts {
+namespace {
+
+const char* const kVariantNames[kDpVariants] = {"dtw", "dtw_band", "frechet", "lcss", "lcss_window"};
+
+void tune_defaults(ts_engine* e) {
+  const char* v = std::getenv("TS_DP_WAVES_PER_CU");
+  int w = v ? std::atoi(v) : 0;
+  if (w <= 0) w = 8;  // best of a 2..32 sweep on 64 x 100k DTW
+  for (int var = 0; var < kDpVariants; ++var)
+    for (int b = 0; b < kDpBuckets; ++b) e->tune[var][b] = DpTune{kDpDefaultShape[b], w};
+}
+
+int shape_of(int W, int U) {
+  for (int i = 0; i < kDpShapes; ++i)
+    if (kDpShapeList[i].W == W && kDpShapeList[i].U == U) return i;
+  return -1;
+}
+
+// Tuning table: text, one line per (variant, bucket),
+//   <variant> <bucket> <W> <U> <waves_per_cu>
+// variant one of dtw, dtw_band, frechet, lcss, lcss_window; '#' comments.
+// Lines not listed keep their current values.
+ts_status load_tuning(ts_engine* e, const char* path) {
+  FILE* f = std::fopen(path, "r");
+  if (!f) return fail(TS_ERR_INVALID, std::string("cannot open tuning table ") + path);
+  char line[256];
+  int lineno = 0;
+  ts_status st = TS_OK;
+  while (std::fgets(line, sizeof line, f)) {
+    ++lineno;
+    char* hash = std::strchr(line, '#');
+    if (hash) *hash = 0;
+    char name[32];
+    int b, W, U, waves;
+    const int n = std::sscanf(line, "%31s %d %d %d %d", name, &b, &W, &U, &waves);
+    if (n <= 0) continue;
+    int var = -1;
+    for (int i = 0; i < kDpVariants; ++i)
+      if (n == 5 && std::strcmp(name, kVariantNames[i]) == 0) var = i;
+    const int sh = shape_of(W, U);
+    if (n != 5 || var < 0 || b < 0 || b >= kDpBuckets || sh < 0 || waves < 1 || waves > 64) {
+      st = fail(TS_ERR_INVALID, std::string(path) + ":" + std::to_string(lineno) + ": bad tuning line");
+      break;
+    }
+    e->tune[var][b] = DpTune{sh, waves};
+  }
+  std::fclose(f);
+  if (st == TS_OK) e->tuning_source = path;
+  return st;
+}
+
+// The table beside the library (dp_tuning.txt), or TS_DP_TUNING; absent is
+// fine (defaults), unreadable is an error.
+ts_status load_default_tuning(ts_engine* e) {
+  const char* env = std::getenv("TS_DP_TUNING");
+  if (env && *env) return load_tuning(e, env);
+  Dl_info info;
+  if (dladdr((void*)&tune_defaults, &info) && info.dli_fname) {
+    std::string p = info.dli_fname;
+    const size_t slash = p.rfind('/');
+    p = (slash == std::string::npos ? std::string(".") : p.substr(0, slash)) + "/dp_tuning.txt";
+    FILE* f = std::fopen(p.c_str(), "r");
+    if (f) {
+      std::fclose(f);
+      return load_tuning(e, p.c_str());
+    }
+  }
+  return TS_OK;
+}
+
+}  // namespace
+}  // namespace ts
+
 ts_status ts_engine_create(ts_engine** out) {
   if (!out) return fail(TS_ERR_INVALID, "null out");
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(TS_ERR_NO_DEVICE, "no HIP device available");
   auto e = new ts_engine();
   (void)hipGetDevice(&e->device);
+  tune_defaults(e);
+  if (load_default_tuning(e) != TS_OK) {
+    delete e;
+    return TS_ERR_INVALID;
+  }
   hipError_t err = hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking);
   for (int b = 0; b < kDpBuckets && err == hipSuccess; ++b) {
     err = hipStreamCreateWithFlags(&e->aux[b], hipStreamNonBlocking);
@@ -806,6 +925,48 @@ ts_status ts_engine_destroy(ts_engine* e) {
 ts_status ts_engine_set_timing(ts_engine* e, int32_t on) {
   if (!e) return fail(TS_ERR_INVALID, "null engine");
   e->timing = on != 0;
+  return TS_OK;
+}
+
+ts_status ts_engine_load_tuning(ts_engine* e, const char* path) {
+  if (!e || !path) return fail(TS_ERR_INVALID, "null argument");
+  return load_tuning(e, path);
+}
+
+ts_status ts_engine_set_dp_shape(ts_engine* e, int32_t variant, int32_t bucket, int32_t lanes, int32_t cells,
+                                 int32_t waves_per_cu) {
+  if (!e) return fail(TS_ERR_INVALID, "null engine");
+  const int sh = shape_of(lanes, cells);
+  if (variant < 0 || variant >= kDpVariants || bucket < 0 || bucket >= kDpBuckets || sh < 0 || waves_per_cu < 1 ||
+      waves_per_cu > 64)
+    return fail(TS_ERR_INVALID, "ts_engine_set_dp_shape arguments");
+  e->tune[variant][bucket] = DpTune{sh, waves_per_cu};
+  return TS_OK;
+}
+
+ts_status ts_engine_get_dp_shape(ts_engine* e, int32_t variant, int32_t bucket, int32_t* lanes, int32_t* cells,
+                                 int32_t* waves_per_cu) {
+  if (!e || variant < 0 || variant >= kDpVariants || bucket < 0 || bucket >= kDpBuckets)
+    return fail(TS_ERR_INVALID, "ts_engine_get_dp_shape arguments");
+  const DpTune& t = e->tune[variant][bucket];
+  if (lanes) *lanes = kDpShapeList[t.shape].W;
+  if (cells) *cells = kDpShapeList[t.shape].U;
+  if (waves_per_cu) *waves_per_cu = t.waves_per_cu;
+  return TS_OK;
+}
+
+int32_t ts_dp_shape_count(void) { return kDpShapes; }
+
+ts_status ts_engine_dp_fallbacks(ts_engine* e, int64_t* n) {
+  if (!e || !n) return fail(TS_ERR_INVALID, "null argument");
+  *n = e->dp_fallbacks;
+  return TS_OK;
+}
+
+ts_status ts_dp_shape(int32_t i, int32_t* lanes, int32_t* cells) {
+  if (i < 0 || i >= kDpShapes) return TS_ERR_INVALID;
+  if (lanes) *lanes = kDpShapeList[i].W;
+  if (cells) *cells = kDpShapeList[i].U;
   return TS_OK;
 }
 
@@ -886,7 +1047,8 @@ ts_status ts_pair_values(ts_engine* e, const ts_set* db, const ts_set* qs, const
   TS_HIP(hipMemcpyAsync(items, h.data(), sizeof(Item) * npairs, hipMemcpyHostToDevice, e->st));
   e->cells_used = 0;
   unsigned long long* cells = nullptr;
-  TS_TRY(ensure(e, B_CELLS, 2 * kCellSlots, &cells));
+  TS_TRY(ensure(e, B_CELLS, kCellBuf, &cells));
+  TS_TRY(run(e, "fill", 0, 0, [&] { return zero_cells(e->st, cells); }));
   TS_TRY(refine(e, *p, db, qs, items, nullptr, npairs, vals));
   TS_HIP(hipMemcpyAsync(out, vals, sizeof(float) * npairs, hipMemcpyDeviceToHost, e->st));
   TS_HIP(hipStreamSynchronize(e->st));

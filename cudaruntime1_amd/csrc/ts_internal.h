@@ -30,10 +30,11 @@ ts_status hip_fail(hipError_t e, const char* what, const char* file, int line);
     if (_s != TS_OK) return _s;         \
   } while (0)
 
-// Trajectories start on multiples of kPad points (16 B of x and of y), the
+// Trajectories start on multiples of kPad points (32 B of x and of y), the
 // tail of each padded with copies of its last point, so DP kernels can load
-// kPad consecutive points with one aligned vector load.
-constexpr int kPad = 4;
+// up to kPad consecutive points (a chunk of U <= 8 columns) with aligned
+// vector loads that never leave the padded trajectory.
+constexpr int kPad = 8;
 
 // Read-only device view of a trajectory set, passed to kernels by value.
 struct SetView {
@@ -188,8 +189,20 @@ hipError_t launch_items_from_regions(hipStream_t st, const int32_t* region_d, co
 
 // dp.hip -- persistent wavefront DP over a device item list, one launch per
 // bucket of the row width the DP visits (the shorter length C, or the band
-// width): < 32, < 64, < 128, < 256, < 1024, >= 1024.
+// width): < 32, < 64, < 128, < 256, < 1024, >= 1024. Each launch runs one
+// lane-group shape (lanes per pair W, cells per lane step U) out of
+// kDpShapes; a tuning table picks the shape and grid per bucket.
 constexpr int kDpBuckets = 6;
+constexpr int kDpShapes = 9;
+struct DpShape {
+  int W, U;
+};
+constexpr DpShape kDpShapeList[kDpShapes] = {{16, 1}, {16, 2}, {16, 4}, {32, 2}, {32, 4},
+                                            {64, 2}, {64, 4}, {32, 8}, {64, 8}};
+// Largest row width of each bucket.
+constexpr int32_t kDpBucketCap[kDpBuckets] = {31, 63, 127, 255, 1023, 1 << 24};
+// Shape used when no tuning table says otherwise.
+constexpr int kDpDefaultShape[kDpBuckets] = {0, 1, 2, 4, 6, 6};
 __host__ __device__ inline int dp_bucket_of(int32_t c) {
   return c <= 31 ? 0 : (c <= 63 ? 1 : (c <= 127 ? 2 : (c <= 255 ? 3 : (c <= 1023 ? 4 : 5))));
 }
@@ -200,8 +213,9 @@ __host__ __device__ inline int dp_bucket_of(int32_t c) {
 struct DpHalo {
   int32_t len, mask;
 };
-DpHalo dp_halo(int b, const MeasureArgs& m, int32_t cmax_c, int32_t wmax_c);
-hipError_t launch_dp(hipStream_t st, MeasureArgs m, int bucket, SetView q, SetView db, const Item* items,
+DpHalo dp_halo(int shape, int b, const MeasureArgs& m, int32_t cmax_c, int32_t wmax_c);
+bool dp_fits(int shape, DpHalo halo);
+hipError_t launch_dp(hipStream_t st, MeasureArgs m, int shape, SetView q, SetView db, const Item* items,
                      const int32_t* nitems, int64_t max_items, int32_t* next, float* out, DpHalo halo,
                      int grid_waves);
 

@@ -132,6 +132,29 @@ ts_status ts_engine_set_timing(ts_engine* e, int32_t on);
 ts_status ts_engine_set_graphs(ts_engine* e, int32_t on);
 /* Graphs recorded and replayed so far, and graphs currently cached. */
 ts_status ts_engine_graph_info(ts_engine* e, int64_t* captures, int64_t* replays, int64_t* cached);
+
+/* DP launch shapes. The wavefront DP runs a (query, DB) pair on a group of
+ * `lanes` lanes (16, 32 or 64: a quarter, half or whole wave) that updates
+ * `cells` DP cells per lane step; pairs are split into 6 buckets by the row
+ * width the DP visits (shorter length, or band width): <= 31, 63, 127, 255,
+ * 1023, larger. Variants: 0 DTW, 1 DTW with a band, 2 Frechet, 3 LCSS,
+ * 4 LCSS with a window. Each (variant, bucket) has a shape and a persistent
+ * grid of waves_per_cu waves per CU. Engines start from built-in defaults,
+ * then read dp_tuning.txt beside the library (or the file TS_DP_TUNING
+ * names), written by the autotuner (cudaruntime1_amd/tune.py): lines
+ * "<variant> <bucket> <lanes> <cells> <waves_per_cu>" with variant one of
+ * dtw, dtw_band, frechet, lcss, lcss_window. Results never depend on the
+ * shape. */
+int32_t ts_dp_shape_count(void);
+ts_status ts_dp_shape(int32_t i, int32_t* lanes, int32_t* cells);
+ts_status ts_engine_load_tuning(ts_engine* e, const char* path);
+ts_status ts_engine_set_dp_shape(ts_engine* e, int32_t variant, int32_t bucket, int32_t lanes, int32_t cells,
+                                 int32_t waves_per_cu);
+ts_status ts_engine_get_dp_shape(ts_engine* e, int32_t variant, int32_t bucket, int32_t* lanes, int32_t* cells,
+                                 int32_t* waves_per_cu);
+/* DP launches so far whose shape's LDS halo did not fit a block (a long band
+ * on a 16- or 32-lane group) and that ran the bucket's default shape. */
+ts_status ts_engine_dp_fallbacks(ts_engine* e, int64_t* n);
 /* Per-kernel totals since the last reset: launches, summed kernel ms and
  * summed algorithmic bytes / work units that the launcher attributes to it.
  * idx enumerates kernels; returns TS_ERR_INVALID past the last one. */

@@ -72,6 +72,20 @@ def test_lcss_window_bit_exact(eng, shapes, delta):
     check_pairs(eng, shapes, "lcss", delta=delta, epsilon=3e-2)
 
 
+@pytest.mark.parametrize("variant,measure,kw", [(0, "dtw", {}), (1, "dtw", {"band": 3}), (1, "dtw", {"band": -20}),
+                                                (2, "frechet", {}), (3, "lcss", {"epsilon": 3e-2}),
+                                                (4, "lcss", {"epsilon": 3e-2, "delta": 5})])
+def test_every_dp_shape_bit_exact(shapes, variant, measure, kw):
+    """Each lane-group shape the autotuner can choose gives the oracle's
+    values bit for bit, in every bucket."""
+    e = ts.Engine()
+    for (W, U) in ts.dp_shapes():
+        say("dp shape", measure, kw, W, U)
+        for b in range(len(ts.DP_BUCKET_CAPS)):
+            e.set_dp_shape(variant, b, W, U, 4)
+        check_pairs(e, shapes, measure, **kw)
+
+
 def test_euclidean_pairs_bit_exact(eng):
     say("euclidean pairs")
     xy, off = walks(23, [128] * 40 + [7] * 10)
