@@ -307,6 +307,13 @@ DpHalo shape_halo(ts_engine* e, int& sh, int b, const MeasureArgs& ma, int32_t h
   return h;
 }
 
+// Kernel-statistics label of a DP launch: measure, bucket and shape, e.g.
+// "dtw_wavefront/b3/w32u4" (the k_dp<measure, 32, 4, ...> instantiation).
+std::string dp_label(const char* base, int b, int sh) {
+  return std::string(base) + "/b" + std::to_string(b) + "/w" + std::to_string(kDpShapeList[sh].W) + "u" +
+         std::to_string(kDpShapeList[sh].U);
+}
+
 // DP variant of a parameter set: its row in the tuning table.
 int dp_variant(const ts_params& p) {
   switch (p.measure) {
@@ -385,7 +392,8 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
     TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, ctr, 0, 1); }));
     int sh = tune[b_lo].shape;
     const DpHalo halo = shape_halo(e, sh, b_lo, ma, hi, whi);
-    return run(e, nm, 0, 0, [&] {
+    const std::string label = dp_label(nm, b_lo, sh);
+    return run(e, label.c_str(), 0, 0, [&] {
       return launch_dp(e->st, ma, sh, qv, dv, items, nitems_dev, max_items, ctr, vals, halo,
                        cus * tune[b_lo].waves_per_cu);
     }, slot + b_lo);
@@ -403,8 +411,9 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   for (int b = b_lo; b <= b_hi; ++b) {
     int sh = tune[b].shape;
     const DpHalo halo = shape_halo(e, sh, b, ma, hi, whi);
+    const std::string label = dp_label(nm, b, sh);
     TS_HIP(hipStreamWaitEvent(e->aux[b], e->fork, 0));
-    TS_TRY(run_on(e, e->aux[b], nm, 0, 0, [&] {
+    TS_TRY(run_on(e, e->aux[b], label.c_str(), 0, 0, [&] {
       return launch_dp(e->aux[b], ma, sh, qv, dv, itemsb + (int64_t)b * max_items, ctr + kDpBuckets + b, max_items,
                        ctr + b, vals, halo, cus * tune[b].waves_per_cu);
     }, slot + b));
