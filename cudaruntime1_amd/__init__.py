@@ -23,7 +23,7 @@ MEASURES = {"dtw": DTW, "lcss": LCSS, "frechet": FRECHET, "hausdorff": HAUSDORFF
 REF, EXACT = 0, 1
 
 if not os.path.exists(LIB_PATH):
-    raise ImportError("libtrajsim.so is not built (run `python -m cudaruntime1_amd.build` or "
+    raise ImportError("libtrajsim.so is not built (run `python cudaruntime1_amd/build.py` or "
                       "__graft_entry__.build()); there is no fallback path")
 
 _lib = ctypes.CDLL(LIB_PATH)
