@@ -72,8 +72,8 @@ _sig = {
     "ts_dp_shape": (ctypes.c_int, [ctypes.c_int32, _P, _P]),
     "ts_engine_load_tuning": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "ts_engine_dp_fallbacks": (ctypes.c_int, [_P, _P]),
-    "ts_engine_set_dp_shape": (ctypes.c_int, [_P] + [ctypes.c_int32] * 5),
-    "ts_engine_get_dp_shape": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P]),
+    "ts_engine_set_dp_shape": (ctypes.c_int, [_P] + [ctypes.c_int32] * 6),
+    "ts_engine_get_dp_shape": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P, _P, _P]),
     "ts_engine_graph_info": (ctypes.c_int, [_P, _P, _P, _P]),
     "ts_engine_kernel_stats": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32,
                                               ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
@@ -253,14 +253,15 @@ class Engine:
     def load_tuning(self, path):
         _check(_lib.ts_engine_load_tuning(self._h, str(path).encode()), "ts_engine_load_tuning")
 
-    def set_dp_shape(self, variant, bucket, lanes, cells, waves_per_cu):
-        """Force the DP launch shape of one (variant, bucket); see DP_VARIANTS."""
-        _check(_lib.ts_engine_set_dp_shape(self._h, variant, bucket, lanes, cells, waves_per_cu),
+    def set_dp_shape(self, variant, bucket, lanes, cells, waves_per_cu, table=2):
+        """Force the DP launch shape of one (variant, bucket); see DP_VARIANTS.
+        table 0: throughput, 1: small refines, 2: both."""
+        _check(_lib.ts_engine_set_dp_shape(self._h, table, variant, bucket, lanes, cells, waves_per_cu),
                "ts_engine_set_dp_shape")
 
-    def get_dp_shape(self, variant, bucket):
+    def get_dp_shape(self, variant, bucket, table=0):
         w, u, g = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
-        _check(_lib.ts_engine_get_dp_shape(self._h, variant, bucket, ctypes.byref(w), ctypes.byref(u),
+        _check(_lib.ts_engine_get_dp_shape(self._h, table, variant, bucket, ctypes.byref(w), ctypes.byref(u),
                                            ctypes.byref(g)), "ts_engine_get_dp_shape")
         return w.value, u.value, g.value
 

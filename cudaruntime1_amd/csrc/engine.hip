@@ -122,6 +122,8 @@ struct DpTune {
   int waves_per_cu;
 };
 constexpr int kDpVariants = 5;  // DTW, DTW banded, Frechet, LCSS, LCSS windowed
+// Refines of at most this many pairs use the small-refine shape table.
+constexpr int64_t kLatencyPairs = 4096;
 
 struct Pending {
   int kid;
@@ -165,7 +167,9 @@ struct ts_engine {
   int cells_used = 0;
   int64_t overflow_retries = 0;
   // DP shape per (variant, bucket): defaults, then the tuning table.
-  DpTune tune[kDpVariants][kDpBuckets];
+  // [0]: throughput table; [1]: small refines (at most kLatencyPairs pairs,
+  // one round of the grid or less: per-pair latency decides)
+  DpTune tune[2][kDpVariants][kDpBuckets];
   std::string tuning_source = "defaults";
   int64_t dp_fallbacks = 0;  // launches that fell back to the default shape
   // hipGraph replay of batches (off by default; never with timing or for
@@ -387,7 +391,7 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   static const char* kNames[2][3] = {{"dtw_wavefront", "frechet_wavefront", "lcss_wavefront"},
                                      {"dtw_wavefront_seed", "frechet_wavefront_seed", "lcss_wavefront_seed"}};
   const char* nm = kNames[seed ? 1 : 0][p.measure == TS_DTW ? 0 : (p.measure == TS_FRECHET ? 1 : 2)];
-  const DpTune* tune = e->tune[dp_variant(p)];
+  const DpTune* tune = e->tune[max_items <= kLatencyPairs ? 1 : 0][dp_variant(p)];
   if (b_lo == b_hi) {
     TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, ctr, 0, 1); }));
     int sh = tune[b_lo].shape;
@@ -822,8 +826,9 @@ void tune_defaults(ts_engine* e) {
   const char* v = std::getenv("TS_DP_WAVES_PER_CU");
   int w = v ? std::atoi(v) : 0;
   if (w <= 0) w = 8;  // best of a 2..32 sweep on 64 x 100k DTW
-  for (int var = 0; var < kDpVariants; ++var)
-    for (int b = 0; b < kDpBuckets; ++b) e->tune[var][b] = DpTune{kDpDefaultShape[b], w};
+  for (int m = 0; m < 2; ++m)
+    for (int var = 0; var < kDpVariants; ++var)
+      for (int b = 0; b < kDpBuckets; ++b) e->tune[m][var][b] = DpTune{kDpDefaultShape[b], w};
 }
 
 int shape_of(int W, int U) {
@@ -832,33 +837,43 @@ int shape_of(int W, int U) {
   return -1;
 }
 
-// Tuning table: text, one line per (variant, bucket),
-//   <variant> <bucket> <W> <U> <waves_per_cu>
-// variant one of dtw, dtw_band, frechet, lcss, lcss_window; '#' comments.
-// Lines not listed keep their current values.
+// Tuning table: text, one line per (variant, bucket[, table]),
+//   <variant> <bucket> <W> <U> <waves_per_cu> [lat]
+// variant one of dtw, dtw_band, frechet, lcss, lcss_window; a trailing
+// "lat" fills the small-refine table (else the throughput table; small
+// refines then use the same line); '#' comments. Lines not listed keep their
+// current values.
 ts_status load_tuning(ts_engine* e, const char* path) {
   FILE* f = std::fopen(path, "r");
   if (!f) return fail(TS_ERR_INVALID, std::string("cannot open tuning table ") + path);
   char line[256];
   int lineno = 0;
   ts_status st = TS_OK;
+  bool lat_set[kDpVariants][kDpBuckets] = {};
   while (std::fgets(line, sizeof line, f)) {
     ++lineno;
     char* hash = std::strchr(line, '#');
     if (hash) *hash = 0;
-    char name[32];
+    char name[32], mode[8] = "";
     int b, W, U, waves;
-    const int n = std::sscanf(line, "%31s %d %d %d %d", name, &b, &W, &U, &waves);
+    const int n = std::sscanf(line, "%31s %d %d %d %d %7s", name, &b, &W, &U, &waves, mode);
     if (n <= 0) continue;
     int var = -1;
     for (int i = 0; i < kDpVariants; ++i)
-      if (n == 5 && std::strcmp(name, kVariantNames[i]) == 0) var = i;
+      if (n >= 5 && std::strcmp(name, kVariantNames[i]) == 0) var = i;
     const int sh = shape_of(W, U);
-    if (n != 5 || var < 0 || b < 0 || b >= kDpBuckets || sh < 0 || waves < 1 || waves > 64) {
+    const bool lat = n == 6 && std::strcmp(mode, "lat") == 0;
+    if (n < 5 || (n == 6 && !lat) || var < 0 || b < 0 || b >= kDpBuckets || sh < 0 || waves < 1 || waves > 64) {
       st = fail(TS_ERR_INVALID, std::string(path) + ":" + std::to_string(lineno) + ": bad tuning line");
       break;
     }
-    e->tune[var][b] = DpTune{sh, waves};
+    if (lat) {
+      e->tune[1][var][b] = DpTune{sh, waves};
+      lat_set[var][b] = true;
+    } else {
+      e->tune[0][var][b] = DpTune{sh, waves};
+      if (!lat_set[var][b]) e->tune[1][var][b] = DpTune{sh, waves};
+    }
   }
   std::fclose(f);
   if (st == TS_OK) e->tuning_source = path;
@@ -942,22 +957,23 @@ ts_status ts_engine_load_tuning(ts_engine* e, const char* path) {
   return load_tuning(e, path);
 }
 
-ts_status ts_engine_set_dp_shape(ts_engine* e, int32_t variant, int32_t bucket, int32_t lanes, int32_t cells,
-                                 int32_t waves_per_cu) {
+ts_status ts_engine_set_dp_shape(ts_engine* e, int32_t table, int32_t variant, int32_t bucket, int32_t lanes,
+                                 int32_t cells, int32_t waves_per_cu) {
   if (!e) return fail(TS_ERR_INVALID, "null engine");
   const int sh = shape_of(lanes, cells);
-  if (variant < 0 || variant >= kDpVariants || bucket < 0 || bucket >= kDpBuckets || sh < 0 || waves_per_cu < 1 ||
-      waves_per_cu > 64)
+  if (table < 0 || table > 2 || variant < 0 || variant >= kDpVariants || bucket < 0 || bucket >= kDpBuckets ||
+      sh < 0 || waves_per_cu < 1 || waves_per_cu > 64)
     return fail(TS_ERR_INVALID, "ts_engine_set_dp_shape arguments");
-  e->tune[variant][bucket] = DpTune{sh, waves_per_cu};
+  for (int m = 0; m < 2; ++m)
+    if (table == 2 || table == m) e->tune[m][variant][bucket] = DpTune{sh, waves_per_cu};
   return TS_OK;
 }
 
-ts_status ts_engine_get_dp_shape(ts_engine* e, int32_t variant, int32_t bucket, int32_t* lanes, int32_t* cells,
-                                 int32_t* waves_per_cu) {
-  if (!e || variant < 0 || variant >= kDpVariants || bucket < 0 || bucket >= kDpBuckets)
+ts_status ts_engine_get_dp_shape(ts_engine* e, int32_t table, int32_t variant, int32_t bucket, int32_t* lanes,
+                                 int32_t* cells, int32_t* waves_per_cu) {
+  if (!e || table < 0 || table > 1 || variant < 0 || variant >= kDpVariants || bucket < 0 || bucket >= kDpBuckets)
     return fail(TS_ERR_INVALID, "ts_engine_get_dp_shape arguments");
-  const DpTune& t = e->tune[variant][bucket];
+  const DpTune& t = e->tune[table][variant][bucket];
   if (lanes) *lanes = kDpShapeList[t.shape].W;
   if (cells) *cells = kDpShapeList[t.shape].U;
   if (waves_per_cu) *waves_per_cu = t.waves_per_cu;

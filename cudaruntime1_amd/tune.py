@@ -1,6 +1,7 @@
 """Autotuner for the wavefront DP: per DP variant and length bucket, the
 lane-group shape (lanes per pair, cells per lane step) and the persistent
-grid (waves per CU) with the highest measured cell rate.
+grid (waves per CU) with the highest measured cell rate, and for small
+refines (one round of pairs, up to 4096) the one with the lowest time.
 
     python -m cudaruntime1_amd.tune [--out PATH] [--quick]
 
@@ -29,6 +30,8 @@ TARGET_CELLS = 6e8
 # 64-lane group), so tails do not decide
 MIN_PAIRS = 16384
 WAVES = [4, 6, 8, 12, 16]
+# pairs of the small-refine measurement (a seed round; at most kLatencyPairs)
+NLAT = 1024
 
 
 def variant_params(v, width):
@@ -95,35 +98,50 @@ def tune(quick=False, log=print):
             widths = np.array([dp_width(v, p, int(x), int(y)) for x, y in zip(na[:256], nb[:256])])
             assert widths.min() > (BUCKET_WIDTHS[b - 1][1] if b else 0) and widths.max() <= ts.DP_BUCKET_CAPS[b], \
                 (v, b, widths.min(), widths.max())
-            default = eng.get_dp_shape(vi, b)
+            default = eng.get_dp_shape(vi, b, 0)
+            default_lat = eng.get_dp_shape(vi, b, 1)
             ref = eng.pair_values(db, qs, p, idx, idx)
-            best = None
+            lat_idx = idx[:NLAT]
+            best, best_lat = None, None
+
+            def dp_time(sel):
+                eng.reset_stats()
+                got = eng.pair_values(db, qs, p, sel, sel)
+                ks = eng.kernel_stats()
+                ms = sum(s["ms"] for k, s in ks.items() if "wavefront" in k)
+                cells = sum(s["units"] for k, s in ks.items() if "wavefront" in k)
+                if not np.array_equal(got.view(np.int32), ref[:len(sel)].view(np.int32)):
+                    raise AssertionError("shape (%d, %d) changes %s values in bucket %d" % (W, U, v, b))
+                return ms, cells
+
             for (W, U) in shapes:
                 # no more than half of each row's chunks idle
                 if (hi + U - 1) // U < W // 2:
                     continue
                 for g in waves:
-                    eng.set_dp_shape(vi, b, W, U, g)
+                    eng.set_dp_shape(vi, b, W, U, g)  # both tables
                     f0 = eng.dp_fallbacks()
                     eng.pair_values(db, qs, p, idx[:256], idx[:256])  # warm-up
                     if eng.dp_fallbacks() != f0:
                         break  # this shape's halo does not fit the LDS for these pairs
-                    eng.reset_stats()
-                    got = eng.pair_values(db, qs, p, idx, idx)
-                    ks = eng.kernel_stats()
-                    ms = sum(s["ms"] for k, s in ks.items() if "wavefront" in k)
-                    cells = sum(s["units"] for k, s in ks.items() if "wavefront" in k)
-                    if not np.array_equal(got.view(np.int32), ref.view(np.int32)):
-                        raise AssertionError("shape (%d, %d) changes %s values in bucket %d" % (W, U, v, b))
+                    ms, cells = dp_time(idx)
                     rate = cells / (ms * 1e-3)
                     if best is None or rate > best[0]:
                         best = (rate, W, U, g, ms)
-            eng.set_dp_shape(vi, b, *default)
+                    lms, _ = dp_time(lat_idx)  # one small refine: its latency decides
+                    if best_lat is None or lms < best_lat[0]:
+                        best_lat = (lms, W, U, g)
+            eng.set_dp_shape(vi, b, *default, table=0)
+            eng.set_dp_shape(vi, b, *default_lat, table=1)
             rate, W, U, g, ms = best
             results[(v, b)] = best
             log("%-12s bucket %d: lanes %2d cells %d waves/CU %2d  %.1f G cells/s (%d pairs, %.3f ms)%s" %
                 (v, b, W, U, g, rate / 1e9, npairs, ms, "" if (W, U, g) != default else "  [default]"))
             lines.append("%s %d %d %d %d  # %.1f G cells/s" % (v, b, W, U, g, rate / 1e9))
+            lms, W, U, g = best_lat
+            log("%-12s bucket %d: small refines (%d pairs): lanes %2d cells %d waves/CU %2d  %.3f ms" %
+                (v, b, NLAT, W, U, g, lms))
+            lines.append("%s %d %d %d %d lat  # %d pairs in %.3f ms" % (v, b, W, U, g, NLAT, lms))
     return lines, results
 
 
