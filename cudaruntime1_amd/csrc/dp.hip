@@ -57,13 +57,21 @@ struct DpArgs {
   float eps;
 };
 
-// Value of the previous lane (lane 0 receives 0). Lane 0 of every group
-// overrides what it receives, so one wave shift serves every group width;
-// W = 16 uses row_shr:1, whose 16-lane rows are exactly the groups.
+// v of the previous lane of the group; the group's lane 0 gets first0.
+// W = 16: row_shr:1, whose 16-lane rows are the groups; W = 64: wave_shr:1.
+// In both, a lane without a source lane keeps the DPP's old operand, so
+// first0 lands in lane 0 with no select. W = 32: wave_shr:1, and lanes 0
+// and 32 select first0.
 template <int W>
-__device__ __forceinline__ float shift_from_prev_lane(float v) {
-  constexpr int ctrl = (W == 16) ? 0x111 /* row_shr:1 */ : 0x138 /* wave_shr:1 */;
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xf, 0xf, true));
+__device__ __forceinline__ float from_prev_lane(float v, float first0, int lg) {
+  if constexpr (W == 32) {
+    const float s = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
+    return lg == 0 ? first0 : s;
+  } else {
+    constexpr int ctrl = (W == 16) ? 0x111 /* row_shr:1 */ : 0x138 /* wave_shr:1 */;
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(first0), __float_as_int(v), ctrl, 0xf, 0xf, false));
+  }
 }
 
 template <int U>
@@ -141,10 +149,14 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
   // so a slot is rewritten only after lane 0 of the next strip read it).
   const int hmask = RANGED ? a.hmask : -1;
   extern __shared__ float4 lds_dyn4[];
+  // LDS: the groups' fetched item slots, one border row of 8 values per wave
+  // (what lane 0 reads above row 0 and outside a band), then the group
+  // areas, all on 16-byte boundaries (cmax is a multiple of 4).
+  const int hdr = ((nwaves * G + 3) >> 2) << 2;
   int* slot_item = (int*)lds_dyn4 + wv * G + g;
-  // Group areas start on 16-byte boundaries (cmax is a multiple of 4).
-  float* gbase = (float*)lds_dyn4 + (((nwaves * G + 3) >> 2) << 2) +
-                 (size_t)(wv * G + g) * dp_group_floats(cmax, STAGE);
+  float* brow = (float*)lds_dyn4 + hdr + wv * 8;
+  float* gbase = (float*)lds_dyn4 + hdr + nwaves * 8 + (size_t)(wv * G + g) * dp_group_floats(cmax, STAGE);
+  if (lane < 8) brow[lane] = BND;
   float* bnd = gbase;
   float* sbx = gbase + cmax;      // [2][cmax]
   float* sby = gbase + 3 * cmax;  // [2][cmax]
@@ -269,6 +281,7 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
         }
         nax = Ax[min(lg, R - 1)];
         nay = Ay[min(lg, R - 1)];
+        load_vec<U>(brow, hv);  // lane 0 starts on row 0: the border row above it
       }
     }
     if (__all(done)) break;
@@ -293,16 +306,7 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
     float up[U], on[U];
     const bool first = ch == cstart;
 #pragma unroll
-    for (int j = 0; j < U; ++j) {
-      const float sh = shift_from_prev_lane<W>(o[j]);
-      float hu = hv[j];
-      if (RANGED) {
-        // Above the previous strip's range: DTW cells there are out of band
-        // (infinite); an in-window LCSS cell there reads its diagonal instead.
-        if (ch >= pcend) hu = BND;
-      }
-      up[j] = (lg == 0) ? ((row == 0) ? BND : hu) : sh;
-    }
+    for (int j = 0; j < U; ++j) up[j] = from_prev_lane<W>(o[j], hv[j], lg);
     float d, l;
     if (RANGED) {
       d = first ? ((row == 0) ? (cstart == 0 ? 0.0f : BND) : (lg == 0 ? hdiag : BND)) : up_last;
@@ -389,7 +393,11 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
       load_vec<U>(Bx + lc, cx);
       load_vec<U>(By + lc, cy);
     }
-    load_vec<U>(bnd + (lc & hmask), hv);
+    // Lane 0 above row 0, or above the previous strip's range (DTW cells
+    // there are out of band, i.e. infinite; an in-window LCSS cell there
+    // reads its diagonal instead): the border row.
+    const bool border = row == 0 || (RANGED && ch >= pcend);
+    load_vec<U>(border ? brow : bnd + (lc & hmask), hv);
   }
 }
 
@@ -420,11 +428,11 @@ template <int MEAS, int W, int U, bool BAND, bool DELTA, bool STAGE>
 static hipError_t launch_dp_t(hipStream_t st, const DpArgs& a, int grid_waves) {
   constexpr int G = 64 / W;
   // Waves per block bounded so that the group areas fit in 64 KiB of LDS.
-  const size_t per_wave = (size_t)G * dp_group_floats(a.cmax, STAGE) * sizeof(float) + G * sizeof(int);
+  const size_t per_wave = (size_t)G * dp_group_floats(a.cmax, STAGE) * sizeof(float) + G * sizeof(int) + 32;
   int wpb = 4;
   while (wpb > 1 && per_wave * wpb + 64 > 65536) --wpb;
   if (per_wave + 64 > 65536) return hipErrorInvalidValue;
-  const size_t lds = (size_t)(((wpb * G + 3) >> 2) << 2) * sizeof(int) +
+  const size_t lds = (size_t)(((wpb * G + 3) >> 2) << 2) * sizeof(int) + (size_t)wpb * 8 * sizeof(float) +
                      (size_t)wpb * G * dp_group_floats(a.cmax, STAGE) * sizeof(float);
   int blocks = (grid_waves + wpb - 1) / wpb;
   if (blocks < 1) blocks = 1;
@@ -437,7 +445,7 @@ static hipError_t launch_dp_t(hipStream_t st, const DpArgs& a, int grid_waves) {
 // in 20 KiB per wave).
 bool dp_fits(int shape, DpHalo halo) {
   const int G = 64 / kDpShapeList[shape].W;
-  return (size_t)G * dp_group_floats(halo.len, false) * sizeof(float) + G * sizeof(int) + 64 <= 65536;
+  return (size_t)G * dp_group_floats(halo.len, false) * sizeof(float) + G * sizeof(int) + 32 + 64 <= 65536;
 }
 
 // Staging the column trajectory in LDS pays while it leaves room for 8
