@@ -303,7 +303,8 @@ hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetVie
 //   DTW       >= max( d(q0,c0) + sum_{0<i<n-1} d(q_i, B_c) + d(q_n-1,c_m-1),
 //                     the same over c's points against B_q ),
 //             discounted by (L+72)*2^-23 for the f32 sums (tree order here,
-//             running order in the DP);
+//             running order in the DP); with a band, B_c is the box of c's
+//             points inside row i's band (band_side_sum), and likewise;
 //   Frechet   >= max( max_i d(q_i,B_c), max_j d(c_j,B_q), endpoint costs );
 //   Hausdorff >= max( max_i d(q_i,B_c), max_j d(c_j,B_q) );
 //   Euclidean >= sqrt(sum_i d(q_i,B_c)^2) (double, discounted 1e-6);
@@ -350,6 +351,43 @@ __device__ void side_bound(const float* ax, const float* ay, int na, float4 b, f
       }
     }
   }
+}
+
+// Banded DTW form: row i of `a` (na points) has a path cell only in the
+// columns j of `b` (nb points) with |i*nb - j*na| <= W (W = w*max(na,nb)),
+// so it costs at least the distance from a_i to the box of b's points there;
+// the union of the 64-point superblock and 8-point block boxes covering that
+// window contains it. Sum over the points of a except the first and last,
+// 64 rows at a time; returns early (the partial sum) once that partial,
+// plus `base` and scaled by `disc`, exceeds `limit`.
+__device__ float band_side_sum(const float* ax, const float* ay, int na, const float4* bblk, const float4* bsblk,
+                               int nb, int64_t W, float base, float disc, float limit) {
+  const int lane = threadIdx.x & 63;
+  float total = 0.0f;
+  for (int i0 = 1; i0 < na - 1; i0 += 64) {
+    const int i = i0 + lane;
+    float acc = 0.0f;
+    if (i < na - 1) {
+      const int64_t lo = (int64_t)i * nb - W;
+      const int jlo = lo <= 0 ? 0 : (int)((lo + na - 1) / na);
+      const int jhi = (int)min<int64_t>(nb - 1, ((int64_t)i * nb + W) / na);
+      int kb = jlo >> 3;
+      const int ke = jhi >> 3;
+      float4 bx = bblk[kb];
+      for (; kb + 7 <= ke; kb += 8) {  // superblock kb covers blocks kb..kb+7
+        const float4 t = bsblk[kb];
+        bx.x = fminf(bx.x, t.x); bx.y = fmaxf(bx.y, t.y); bx.z = fminf(bx.z, t.z); bx.w = fmaxf(bx.w, t.w);
+      }
+      for (; kb <= ke; ++kb) {
+        const float4 t = bblk[kb];
+        bx.x = fminf(bx.x, t.x); bx.y = fmaxf(bx.y, t.y); bx.z = fminf(bx.z, t.z); bx.w = fmaxf(bx.w, t.w);
+      }
+      acc = sqrtf(pt_box(ax[i], ay[i], bx));
+    }
+    total += wave_sum(acc);
+    if ((total + base) * disc > limit) break;  // uniform: total is the same in every lane
+  }
+  return total;
 }
 
 template <int MEAS>
@@ -401,11 +439,28 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
       const float e0 = sqrtf(dx0 * dx0 + dy0 * dy0), e1 = sqrtf(dx1 * dx1 + dy1 * dy1);
       // Row form: rows 0 and n-1 contain (0,0) and (n-1,m-1); with n == 1 the
       // single row holds both when m >= 2. Column form likewise.
-      const float r1 = n >= 2 ? a1 + e0 + e1 : (m >= 2 ? e0 + e1 : e0);
-      const float r2 = m >= 2 ? a2 + e0 + e1 : (n >= 2 ? e0 + e1 : e0);
       const int L = max(n, m);
-      const float b = fmaxf(r1, r2) * (1.0f - (float)(L + 72) * 1.1920929e-7f);
-      pass = b <= t;
+      const float disc = 1.0f - (float)(L + 72) * 1.1920929e-7f;
+      auto bound = [&](float s1, float s2) {
+        const float r1 = n >= 2 ? s1 + e0 + e1 : (m >= 2 ? e0 + e1 : e0);
+        const float r2 = m >= 2 ? s2 + e0 + e1 : (n >= 2 ? e0 + e1 : e0);
+        return fmaxf(r1, r2) * disc;
+      };
+      pass = bound(a1, a2) <= t;
+      if (pass && ma.band != 0 && n >= 3 && m >= 3) {
+        // The whole-MBR form passed: the banded form (row windows of the
+        // band) is tighter and costs O(window / 64) per point. Each side alone
+        // bounds the value; the row side runs first and may end early.
+        const int64_t w = ma.band > 0 ? ma.band : ((int64_t)-ma.band * L + 99) / 100;
+        const float b1 = band_side_sum(q.x + qo, q.y + qo, n, db.blk + (co >> 3), db.sblk + (co >> 3), m, w * L,
+                                       e0 + e1, disc, t);
+        pass = (b1 + e0 + e1) * disc <= t;
+        if (pass) {
+          const float b2 = band_side_sum(db.x + co, db.y + co, m, q.blk + (qo >> 3), q.sblk + (qo >> 3), n, w * L,
+                                         e0 + e1, disc, t);
+          pass = bound(b1, b2) <= t;
+        }
+      }
     } else {
       float a1 = 0.0f, a2 = 0.0f;
       double fd = 0.0;

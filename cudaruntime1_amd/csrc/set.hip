@@ -74,9 +74,53 @@ __global__ void __launch_bounds__(256) k_summaries(SetView s, float4* mbr, float
   }
 }
 
+static_assert(kPad == 8, "block boxes take trajectories aligned and padded to 8 points");
+
+// Box of each 8-point block (trajectories start on multiples of 8 points
+// and are padded with their last point, so a block holds one trajectory's
+// points only).
+__global__ void __launch_bounds__(256) k_blocks(const float* x, const float* y, int64_t nblk, float4* blk) {
+  const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (b >= nblk) return;
+  float mnx = x[8 * b], mxx = mnx, mny = y[8 * b], mxy = mny;
+#pragma unroll
+  for (int i = 1; i < 8; ++i) {
+    const float px = x[8 * b + i], py = y[8 * b + i];
+    mnx = fminf(mnx, px); mxx = fmaxf(mxx, px);
+    mny = fminf(mny, py); mxy = fmaxf(mxy, py);
+  }
+  blk[b] = make_float4(mnx, mxx, mny, mxy);
+}
+
 hipError_t launch_summaries(hipStream_t st, SetView s, float4* mbr, float4* ends) {
   if (s.count == 0) return hipSuccess;
   hipLaunchKernelGGL(k_summaries, dim3((s.count + 3) / 4), dim3(256), 0, st, s, mbr, ends);
+  return hipGetLastError();
+}
+
+// Superblock g: the union of blocks g..g+7 that belong to g's trajectory
+// (fewer near its end). One wave per trajectory.
+__global__ void __launch_bounds__(256) k_superblocks(SetView s, float4* sblk) {
+  const int lane = threadIdx.x & 63;
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= s.count) return;
+  const int64_t b0 = s.off[t] >> 3;
+  const int nb = (s.len[t] + 7) >> 3;
+  for (int r = lane; r < nb; r += 64) {
+    float4 u = s.blk[b0 + r];
+    for (int i = 1; i < 8 && r + i < nb; ++i) {
+      const float4 v = s.blk[b0 + r + i];
+      u.x = fminf(u.x, v.x); u.y = fmaxf(u.y, v.y); u.z = fminf(u.z, v.z); u.w = fmaxf(u.w, v.w);
+    }
+    sblk[b0 + r] = u;
+  }
+}
+
+hipError_t launch_blocks(hipStream_t st, SetView s, int64_t points, float4* blk, float4* sblk) {
+  const int64_t nblk = points / 8;
+  if (nblk == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_blocks, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, st, s.x, s.y, nblk, blk);
+  hipLaunchKernelGGL(k_superblocks, dim3((s.count + 3) / 4), dim3(256), 0, st, s, sblk);
   return hipGetLastError();
 }
 
@@ -116,8 +160,8 @@ ts_status set_load(ts_set* s, const ts_point* xy, const int64_t* offsets, int32_
   if (P > s->cap_points || count > s->cap_count) {
     if (st) TS_HIP(hipStreamSynchronize(st));
     (void)hipFree(s->x); (void)hipFree(s->y); (void)hipFree(s->off); (void)hipFree(s->len);
-    (void)hipFree(s->mbr); (void)hipFree(s->ends);
-    s->x = s->y = nullptr; s->off = nullptr; s->len = nullptr; s->mbr = s->ends = nullptr;
+    (void)hipFree(s->mbr); (void)hipFree(s->ends); (void)hipFree(s->blk); (void)hipFree(s->sblk);
+    s->x = s->y = nullptr; s->off = nullptr; s->len = nullptr; s->mbr = s->ends = s->blk = s->sblk = nullptr;
     s->bytes = 0; s->cap_points = 0; s->cap_count = 0;
     hipError_t e;
 #define TS_ALLOC(ptr, nbytes)                                                        \
@@ -130,6 +174,8 @@ ts_status set_load(ts_set* s, const ts_point* xy, const int64_t* offsets, int32_
     TS_ALLOC(s->len, count * sizeof(int32_t));
     TS_ALLOC(s->mbr, count * sizeof(float4));
     TS_ALLOC(s->ends, count * sizeof(float4));
+    TS_ALLOC(s->blk, (P / 8) * sizeof(float4));
+    TS_ALLOC(s->sblk, (P / 8) * sizeof(float4));
 #undef TS_ALLOC
     s->cap_points = P;
     s->cap_count = count;
@@ -154,6 +200,7 @@ ts_status set_load(ts_set* s, const ts_point* xy, const int64_t* offsets, int32_
   TS_HIP(hipMemcpyAsync(s->off, doff.data(), (count + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
   TS_HIP(hipMemcpyAsync(s->len, s->h_len.data(), count * sizeof(int32_t), hipMemcpyHostToDevice, st));
   TS_HIP(launch_summaries(st, s->view(), s->mbr, s->ends));
+  TS_HIP(launch_blocks(st, s->view(), P, s->blk, s->sblk));
   TS_HIP(hipStreamSynchronize(st));
   return TS_OK;
 }
@@ -240,7 +287,7 @@ ts_status ts_set_reload(ts_set* s, const ts_point* xy, const int64_t* offsets, i
 ts_status ts_set_destroy(ts_set* s) {
   if (!s) return TS_OK;
   (void)hipFree(s->x); (void)hipFree(s->y); (void)hipFree(s->off); (void)hipFree(s->len);
-  (void)hipFree(s->mbr); (void)hipFree(s->ends);
+  (void)hipFree(s->mbr); (void)hipFree(s->ends); (void)hipFree(s->blk); (void)hipFree(s->sblk);
   delete s;
   return TS_OK;
 }

@@ -44,6 +44,8 @@ struct SetView {
   const int32_t* len;    // [count]
   const float4* mbr;     // [count] (minx, maxx, miny, maxy)
   const float4* ends;    // [count] (x0, y0, x_last, y_last)
+  const float4* blk;     // [points / 8] box of each 8-point block (blocks never straddle trajectories)
+  const float4* sblk;    // [points / 8] box of blocks g..g+7 of the same trajectory (64 points)
   int32_t count;
 };
 
@@ -95,13 +97,15 @@ struct ts_set {
   int32_t* len = nullptr;
   float4* mbr = nullptr;
   float4* ends = nullptr;
+  float4* blk = nullptr;
+  float4* sblk = nullptr;
   std::vector<int64_t> h_off;
   std::vector<int32_t> h_len;
   int64_t bytes = 0;
   // Capacity of the device arrays (sets reloaded in place by the server).
   int64_t cap_points = 0;
   int32_t cap_count = 0;
-  ts::SetView view() const { return {x, y, off, len, mbr, ends, count}; }
+  ts::SetView view() const { return {x, y, off, len, mbr, ends, blk, sblk, count}; }
 };
 
 namespace ts {

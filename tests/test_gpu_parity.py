@@ -203,6 +203,24 @@ def test_knn_small_all_pairs_match_oracle(eng, measure, n_db, n_q, k):
         assert np.array_equal(gv.view(np.int32), wv.view(np.int32)), (fn.__name__, gv, wv)
 
 
+@pytest.mark.parametrize("band", [-10, 3, -50])
+def test_knn_banded_dtw_match_oracle(eng, band):
+    """EXACT DTW with a band over 400 walks (many candidates near the k-th
+    value, pruned by the banded point-to-window bound) equals the oracle."""
+    say("knn banded dtw", band)
+    L = uniform_lengths(35, 400, 40, 300)
+    db_xy, db_off = walks(35, L, 4e-3)
+    q_xy, q_off = planted(db_xy, db_off, np.arange(0, 400, 20), (L[0:400:20] * 5) // 6 + 3, 2e-3, 36)
+    db, qs = ts.TrajSet(db_xy, db_off), ts.TrajSet(q_xy, q_off)
+    p = ts.params("dtw", mode=ts.EXACT, k=10, band=band)
+    prm = oracle.params(p.measure, p.mode, k=p.k, band=p.band)
+    wi, wv, _ = oracle.knn(prm, db_xy, db_off, q_xy, q_off)
+    for fn in (eng.knn, eng.knn_plain):
+        gi, gv, st = fn(db, qs, p)
+        assert np.array_equal(gi, wi), (fn.__name__, gi, wi)
+        assert np.array_equal(gv.view(np.int32), wv.view(np.int32)), (fn.__name__, gv, wv)
+
+
 @pytest.mark.parametrize("measure,kw", [("dtw", {"band": -10}), ("lcss", {"epsilon": 2e-3, "delta": 64})])
 def test_knn_long_banded_match_oracle(eng, measure, kw):
     """Long-trajectory shape (lengths U[2048,4096]) with a per-pair 10% band
