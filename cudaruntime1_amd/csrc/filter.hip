@@ -309,7 +309,9 @@ hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetVie
 //   Hausdorff >= max( max_i d(q_i,B_c), max_j d(c_j,B_q) );
 //   Euclidean >= sqrt(sum_i d(q_i,B_c)^2) (double, discounted 1e-6);
 //   LCSS      <= min(#q points in c's eps-box, #c points in q's eps-box,
-//                    min(n,m)) / min(n,m).
+//                    min(n,m)) / min(n,m); with a window delta, the box of
+//                    the other trajectory's points within delta of the
+//                    point's index (window_side_count).
 // min/max do not round, so the Frechet/Hausdorff/LCSS forms are exact.
 __device__ __forceinline__ float pt_box(float px, float py, float4 b) {
   const float gx = fmaxf(0.0f, fmaxf(b.x - px, px - b.y));
@@ -390,6 +392,35 @@ __device__ float band_side_sum(const float* ax, const float* ay, int na, const f
   return total;
 }
 
+// LCSS window form: point i of `a` can only match points j of `b` with
+// |i - j| <= delta, so it counts only if it lies inside the eps-inflated box
+// of b's points there (superblocks + blocks covering the window; f32
+// subtraction is monotone, so every real match passes this test).
+__device__ int window_side_count(const float* ax, const float* ay, int na, const float4* bblk, const float4* bsblk,
+                                 int nb, int delta, float eps) {
+  const int lane = threadIdx.x & 63;
+  int cnt = 0;
+  for (int i = lane; i < na; i += 64) {
+    const int jlo = max(0, i - delta), jhi = min(nb - 1, i + delta);
+    if (jlo > jhi) continue;
+    int kb = jlo >> 3;
+    const int ke = jhi >> 3;
+    float4 bx = bblk[kb];
+    for (; kb + 7 <= ke; kb += 8) {
+      const float4 t = bsblk[kb];
+      bx.x = fminf(bx.x, t.x); bx.y = fmaxf(bx.y, t.y); bx.z = fminf(bx.z, t.z); bx.w = fmaxf(bx.w, t.w);
+    }
+    for (; kb <= ke; ++kb) {
+      const float4 t = bblk[kb];
+      bx.x = fminf(bx.x, t.x); bx.y = fmaxf(bx.y, t.y); bx.z = fminf(bx.z, t.z); bx.w = fmaxf(bx.w, t.w);
+    }
+    const float px = ax[i], py = ay[i];
+    cnt += (px - bx.y < eps && bx.x - px < eps && py - bx.w < eps && bx.z - py < eps) ? 1 : 0;
+  }
+  for (int sh = 32; sh > 0; sh >>= 1) cnt += __shfl_xor(cnt, sh);
+  return cnt;
+}
+
 template <int MEAS>
 __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k, SetView q, int32_t q0, SetView db,
                                                       const float* tau, const Item* items, const int32_t* nitems,
@@ -419,6 +450,15 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
       const int mn = min(n, m);
       const float ub = (float)min(min(c1, c2), mn) / (float)mn;
       pass = (ub >= t) && (ub > 0.0f || it.d < k);
+      if (pass && ma.delta >= 0 && ub > 0.0f) {
+        // with an index window, count against the window's box instead
+        const int w1 = window_side_count(q.x + qo, q.y + qo, n, db.blk + (co >> 3), db.sblk + (co >> 3), m,
+                                         ma.delta, ma.eps);
+        const int w2 = window_side_count(db.x + co, db.y + co, m, q.blk + (qo >> 3), q.sblk + (qo >> 3), n,
+                                         ma.delta, ma.eps);
+        const float ubw = (float)min(min(w1, w2), mn) / (float)mn;
+        pass = (ubw >= t) && (ubw > 0.0f || it.d < k);
+      }
     } else if (MEAS == TS_EUCLIDEAN) {
       float f = 0.0f;
       double s = 0.0;

@@ -221,6 +221,24 @@ def test_knn_banded_dtw_match_oracle(eng, band):
         assert np.array_equal(gv.view(np.int32), wv.view(np.int32)), (fn.__name__, gv, wv)
 
 
+@pytest.mark.parametrize("delta", [0, 5, 64])
+def test_knn_windowed_lcss_match_oracle(eng, delta):
+    """EXACT LCSS with an index window over 400 walks (pruned by the window
+    box count) equals the oracle, ties at similarity 0 included."""
+    say("knn windowed lcss", delta)
+    L = uniform_lengths(37, 400, 40, 300)
+    db_xy, db_off = walks(37, L, 4e-3)
+    q_xy, q_off = planted(db_xy, db_off, np.arange(0, 400, 20), L[0:400:20], 2e-3, 38)
+    db, qs = ts.TrajSet(db_xy, db_off), ts.TrajSet(q_xy, q_off)
+    p = ts.params("lcss", mode=ts.EXACT, k=10, epsilon=5e-3, delta=delta)
+    prm = oracle.params(p.measure, p.mode, k=p.k, delta=p.delta, epsilon=p.epsilon)
+    wi, wv, _ = oracle.knn(prm, db_xy, db_off, q_xy, q_off)
+    for fn in (eng.knn, eng.knn_plain):
+        gi, gv, st = fn(db, qs, p)
+        assert np.array_equal(gi, wi), (fn.__name__, gi, wi)
+        assert np.array_equal(gv.view(np.int32), wv.view(np.int32)), (fn.__name__, gv, wv)
+
+
 @pytest.mark.parametrize("measure,kw", [("dtw", {"band": -10}), ("lcss", {"epsilon": 2e-3, "delta": 64})])
 def test_knn_long_banded_match_oracle(eng, measure, kw):
     """Long-trajectory shape (lengths U[2048,4096]) with a per-pair 10% band
