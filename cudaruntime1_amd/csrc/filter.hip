@@ -307,6 +307,9 @@ hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetVie
 //             points inside row i's band (band_side_sum), and likewise;
 //   Frechet   >= max( max_i d(q_i,B_c), max_j d(c_j,B_q), endpoint costs );
 //   Hausdorff >= max( max_i d(q_i,B_c), max_j d(c_j,B_q) );
+//             and then, with B_c the nearest of the 64-point tiles of c
+//             instead of its MBR, the same forms again (tile_side_*; DTW
+//             without a band likewise by tile_side_sum);
 //   Euclidean >= sqrt(sum_i d(q_i,B_c)^2) (double, discounted 1e-6);
 //   LCSS      <= min(#q points in c's eps-box, #c points in q's eps-box,
 //                    min(n,m)) / min(n,m); with a window delta, the box of
@@ -392,6 +395,69 @@ __device__ float band_side_sum(const float* ax, const float* ay, int na, const f
   return total;
 }
 
+// Unbanded DTW form: row i of `a` has a path cell somewhere in b, so it
+// costs at least the distance from a_i to the nearest of the boxes that tile
+// b (64-point superblocks at blocks 0, 8, 16, ...). Sum over the points of a
+// except the first and last, 64 rows at a time, with the same early return.
+__device__ float tile_side_sum(const float* ax, const float* ay, int na, const float4* bsblk, int nb, float base,
+                               float disc, float limit) {
+  const int lane = threadIdx.x & 63;
+  const int nblk = (nb + 7) >> 3;
+  float total = 0.0f;
+  for (int i0 = 1; i0 < na - 1; i0 += 64) {
+    const int i = i0 + lane;
+    float acc = 0.0f;
+    if (i < na - 1) {
+      const float px = ax[i], py = ay[i];
+      float best = pt_box(px, py, bsblk[0]);
+      for (int kb = 8; kb < nblk; kb += 8) best = fminf(best, pt_box(px, py, bsblk[kb]));
+      acc = sqrtf(best);
+    }
+    total += wave_sum(acc);
+    if ((total + base) * disc > limit) break;
+  }
+  return total;
+}
+
+// Tiled max form (Frechet, Hausdorff): every point of a is matched to some
+// point of b, so max_i of the squared distance from a_i to the nearest tile
+// box of b bounds the squared value from below. Stops early once the running
+// maximum exceeds limit2.
+__device__ float tile_side_max(const float* ax, const float* ay, int na, const float4* bsblk, int nb, float limit2) {
+  const int lane = threadIdx.x & 63;
+  const int nblk = (nb + 7) >> 3;
+  float mx = 0.0f;
+  for (int i0 = 0; i0 < na; i0 += 64) {
+    const int i = i0 + lane;
+    if (i < na) {
+      const float px = ax[i], py = ay[i];
+      float best = pt_box(px, py, bsblk[0]);
+      for (int kb = 8; kb < nblk; kb += 8) best = fminf(best, pt_box(px, py, bsblk[kb]));
+      mx = fmaxf(mx, best);
+    }
+    if (wave_max(mx) > limit2) break;
+  }
+  return wave_max(mx);
+}
+
+// Tiled LCSS count: points of a inside the eps-inflated box of some tile of b.
+__device__ int tile_side_count(const float* ax, const float* ay, int na, const float4* bsblk, int nb, float eps) {
+  const int lane = threadIdx.x & 63;
+  const int nblk = (nb + 7) >> 3;
+  int cnt = 0;
+  for (int i = lane; i < na; i += 64) {
+    const float px = ax[i], py = ay[i];
+    bool in = false;
+    for (int kb = 0; kb < nblk && !in; kb += 8) {
+      const float4 bx = bsblk[kb];
+      in = px - bx.y < eps && bx.x - px < eps && py - bx.w < eps && bx.z - py < eps;
+    }
+    cnt += in ? 1 : 0;
+  }
+  for (int sh = 32; sh > 0; sh >>= 1) cnt += __shfl_xor(cnt, sh);
+  return cnt;
+}
+
 // LCSS window form: point i of `a` can only match points j of `b` with
 // |i - j| <= delta, so it counts only if it lies inside the eps-inflated box
 // of b's points there (superblocks + blocks covering the window; f32
@@ -450,6 +516,13 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
       const int mn = min(n, m);
       const float ub = (float)min(min(c1, c2), mn) / (float)mn;
       pass = (ub >= t) && (ub > 0.0f || it.d < k);
+      if (pass && ma.delta < 0 && ub > 0.0f) {
+        // count against the tiles of the other trajectory instead of its MBR
+        const int w1 = tile_side_count(q.x + qo, q.y + qo, n, db.sblk + (co >> 3), m, ma.eps);
+        const int w2 = tile_side_count(db.x + co, db.y + co, m, q.sblk + (qo >> 3), n, ma.eps);
+        const float ubt = (float)min(min(w1, w2), mn) / (float)mn;
+        pass = (ubt >= t) && (ubt > 0.0f || it.d < k);
+      }
       if (pass && ma.delta >= 0 && ub > 0.0f) {
         // with an index window, count against the window's box instead
         const int w1 = window_side_count(q.x + qo, q.y + qo, n, db.blk + (co >> 3), db.sblk + (co >> 3), m,
@@ -487,6 +560,16 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
         return fmaxf(r1, r2) * disc;
       };
       pass = bound(a1, a2) <= t;
+      if (pass && ma.band == 0 && n >= 3 && m >= 3) {
+        // The whole-MBR form passed: the tiled form (nearest superblock box)
+        // is tighter and costs O(m / 64) per point.
+        const float b1 = tile_side_sum(q.x + qo, q.y + qo, n, db.sblk + (co >> 3), m, e0 + e1, disc, t);
+        pass = (b1 + e0 + e1) * disc <= t;
+        if (pass) {
+          const float b2 = tile_side_sum(db.x + co, db.y + co, m, q.sblk + (qo >> 3), n, e0 + e1, disc, t);
+          pass = bound(b1, b2) <= t;
+        }
+      }
       if (pass && ma.band != 0 && n >= 3 && m >= 3) {
         // The whole-MBR form passed: the banded form (row windows of the
         // band) is tighter and costs O(window / 64) per point. Each side alone
@@ -513,6 +596,18 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
         b2 = fmaxf(b2, fmaxf(dx0 * dx0 + dy0 * dy0, dx1 * dx1 + dy1 * dy1));
       }
       pass = sqrtf(b2) <= t;
+      if (pass) {
+        // nearest tile box instead of the whole MBR (sqrtf is monotone, so
+        // comparing squares against the threshold's square loses nothing
+        // only where t*t rounds; the test stays on sqrtf)
+        const float lim = t * t * 1.0000005f;
+        float c1 = tile_side_max(q.x + qo, q.y + qo, n, db.sblk + (co >> 3), m, lim);
+        if (sqrtf(fmaxf(b2, c1)) <= t) {
+          const float c2 = tile_side_max(db.x + co, db.y + co, m, q.sblk + (qo >> 3), n, lim);
+          c1 = fmaxf(c1, c2);
+        }
+        pass = sqrtf(fmaxf(b2, c1)) <= t;
+      }
     }
     if (pass && lane == 0) out[atomicAdd(nout, 1)] = it;
   }
