@@ -101,7 +101,7 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 
 enum BufId {
   B_ENV, B_SCORES, B_KEYS, B_KEYS2, B_SCA, B_SCB, B_CAND, B_CANDIDX, B_ITEMS, B_ITEMS3, B_VALS, B_REGION,
-  B_RVALS, B_COUNTS, B_CTR, B_TAU, B_TOP, B_OIDX, B_OVAL, B_CELLS, B_HOSTIN, B_ITEMS2, B_RITEMS, B_SEGC, B_NEED,
+  B_RVALS, B_COUNTS, B_CTR, B_TAU, B_TOP, B_OIDX, B_OVAL, B_CELLS, B_HOSTIN, B_ITEMS2, B_RITEMS, B_SEGC, B_NEED, B_BCNT,
   B_NBUF
 };
 
@@ -552,10 +552,13 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   // counts read back to the host; graph-captured calls cannot read back, so
   // the device sizes them within the engine's candidate capacity and the
   // call is repeated with a larger capacity if the total exceeded it.
-  TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, counts, 0, nq); }));
+  const int32_t gx = compact_blocks(N);
+  int32_t* bcounts = nullptr;
+  TS_TRY(ensure(e, B_BCNT, (int64_t)nq * gx, &bcounts));
   TS_TRY(run(e, "bound_compact", pairs * 36.0, pairs, [&] {
-    return launch_bound_compact(e->st, ma, k, qv, q0, nq, dv, tau, nullptr, counts, nullptr, 0);
+    return launch_bound_compact(e->st, ma, k, qv, q0, nq, dv, tau, nullptr, bcounts, nullptr, 0);
   }));
+  TS_TRY(run(e, "block_prefix", 0, 0, [&] { return launch_block_prefix(e->st, bcounts, nq, gx, counts); }));
   const bool dev_sized = e->capturing;
   int64_t total = 0;  // candidates written (eager) or the capacity (device-sized)
   int64_t cap = 0;    // length of the candidate buffers
@@ -604,9 +607,9 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   TS_TRY(ensure(e, B_RITEMS, cap, &items2));
   TS_TRY(ensure(e, B_RVALS, cap, &rvals));
   TS_TRY(ensure(e, B_SCORES, 2 * cap, (float**)&rkeys));
-  TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, counts, 0, nq + 1); }));
+  TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, counts + nq, 0, 1); }));
   TS_TRY(run(e, "bound_compact", pairs * 36.0, pairs, [&] {
-    return launch_bound_compact(e->st, ma, k, qv, q0, nq, dv, tau, d_off, counts, citems, cap);
+    return launch_bound_compact(e->st, ma, k, qv, q0, nq, dv, tau, d_off, bcounts, citems, cap);
   }));
   // 3b. tighter O(n+m) bound on the survivors
   int32_t* nitems2 = counts + nq;
@@ -618,9 +621,8 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   TS_TRY(refine(e, p, db, qs, items2, nitems2, cap, rvals));
   // 4. top-k over each query's refined candidates (slot = position in citems)
   TS_TRY(run(e, "keys", 0, 0, [&] { return launch_keys_from_items(e->st, rvals, citems, cap, desc, rkeys); }));
-  // Region lengths: clamped to the capacity (device-sized), or the write
-  // cursors, which end at the counted lengths (both passes test the same
-  // bounds).
+  // Region lengths: clamped to the capacity (device-sized), or the counted
+  // totals.
   return topk(e, rkeys, 0, seg_len, dev_sized ? segc : counts, nq, k, top, d_off);
 }
 

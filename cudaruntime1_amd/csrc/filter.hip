@@ -169,6 +169,15 @@ __device__ __forceinline__ float pair_bound(const MeasureArgs& ma, const QSum& q
   return 0.0f;
 }
 
+// pair_bound with the measure known at compile time (the switch folds).
+template <int MEAS>
+__device__ __forceinline__ float pair_bound_m(const MeasureArgs& ma, const QSum& qs, const SetView& q,
+                                              const float4 cm, const float4 ce, int m) {
+  MeasureArgs mc = ma;
+  mc.measure = MEAS;
+  return pair_bound(mc, qs, q, cm, ce, m);
+}
+
 constexpr int kBoundChunk = 2048;  // DB trajectories per seeding block
 constexpr int kSeedPerThread = kBoundChunk / 256;
 
@@ -193,7 +202,7 @@ __device__ void bitonic_sort_lds(uint64_t* s, int n, int tid, int nthreads) {
 // not be the exact top-s by bound: each thread keeps the best key of its
 // kSeedPerThread trajectories and the block keeps the best `s` of those
 // (s <= 256). Larger s sorts the whole chunk.
-template <bool WHOLE>
+template <bool WHOLE, int MEAS>
 __global__ void __launch_bounds__(256) k_bound_chunks(MeasureArgs ma, SetView q, int32_t q0, SetView db,
                                                       int32_t s, uint64_t* keys) {
   constexpr int NS = WHOLE ? kBoundChunk : 256;
@@ -207,7 +216,7 @@ __global__ void __launch_bounds__(256) k_bound_chunks(MeasureArgs ma, SetView q,
     const int i = t * 256 + threadIdx.x;
     const int j = base + i;
     uint64_t key = ~0ull;
-    if (j < db.count) key = make_key(pair_bound(ma, qs, q, db.mbr[j], db.ends[j], db.len[j]), j, desc);
+    if (j < db.count) key = make_key(pair_bound_m<MEAS>(ma, qs, q, db.mbr[j], db.ends[j], db.len[j]), j, desc);
     if (WHOLE) sk[i] = key;
     else best = key < best ? key : best;
   }
@@ -223,8 +232,18 @@ hipError_t launch_bound_chunks(hipStream_t st, MeasureArgs m, int32_t k, SetView
   (void)k;
   (void)chunk;
   dim3 grid((db.count + kBoundChunk - 1) / kBoundChunk, nq);
-  if (s <= 256) hipLaunchKernelGGL(k_bound_chunks<false>, grid, dim3(256), 0, st, m, q, q0, db, s, keys);
-  else hipLaunchKernelGGL(k_bound_chunks<true>, grid, dim3(256), 0, st, m, q, q0, db, s, keys);
+#define TS_CHUNKS(MEAS)                                                                              \
+  if (s <= 256) hipLaunchKernelGGL((k_bound_chunks<false, MEAS>), grid, dim3(256), 0, st, m, q, q0, db, s, keys); \
+  else hipLaunchKernelGGL((k_bound_chunks<true, MEAS>), grid, dim3(256), 0, st, m, q, q0, db, s, keys);
+  switch (m.measure) {
+    case TS_DTW: TS_CHUNKS(TS_DTW) break;
+    case TS_FRECHET: TS_CHUNKS(TS_FRECHET) break;
+    case TS_LCSS: TS_CHUNKS(TS_LCSS) break;
+    case TS_HAUSDORFF: TS_CHUNKS(TS_HAUSDORFF) break;
+    case TS_EUCLIDEAN: TS_CHUNKS(TS_EUCLIDEAN) break;
+    default: return hipErrorInvalidValue;
+  }
+#undef TS_CHUNKS
   return hipGetLastError();
 }
 
@@ -233,23 +252,32 @@ hipError_t launch_bound_chunks(hipStream_t st, MeasureArgs m, int32_t k, SetView
 // every query of the block. A pair survives when its bound can still reach
 // the query's k-th value tau: distances bound <= tau; LCSS ub >= tau and,
 // because zero-similarity ties are broken by index, (ub > 0 or j < k).
-// Two passes: COUNT adds each query's survivors to counts[q]; WRITE appends
-// them as Items at off[q] + cursor[q] (slot = their position). Both use a
-// wave ballot + prefix count, one atomic per wave and query.
+// Two passes, no global atomics (one hot counter per query serialises in
+// L2): COUNT reduces each block's survivors per query in LDS and stores them
+// in bcounts[q * gridDim.x + block]; after k_block_prefix has turned those
+// into exclusive offsets within each query's region, WRITE places the block's
+// survivors at off[q] + bcounts[...] + an LDS cursor (slot = position).
 constexpr int kCompactQ = 8, kCompactJ = 2;
 
-template <bool WRITE>
+int32_t compact_blocks(int32_t n_db) { return (n_db + 256 * kCompactJ - 1) / (256 * kCompactJ); }
+
+template <bool WRITE, int MEAS>
 __global__ void __launch_bounds__(256) k_bound_compact(MeasureArgs ma, int32_t k, SetView q, int32_t q0, int32_t nq,
                                                        SetView db, const float* tau, const int64_t* off,
-                                                       int32_t* counts, Item* items, int64_t cap) {
+                                                       int32_t* bcounts, Item* items, int64_t cap) {
   __shared__ QSum sq[kCompactQ];
   __shared__ float st[kCompactQ];
+  __shared__ int bc[kCompactQ];
+  __shared__ int64_t bbase[kCompactQ];
   const int qb = blockIdx.y * kCompactQ;
   const int nqb = min(kCompactQ, nq - qb);
+  const int gx = gridDim.x, bx = blockIdx.x;
   if (threadIdx.x < nqb) {
     const int qi = q0 + qb + threadIdx.x;
     sq[threadIdx.x] = QSum{q.mbr[qi], q.ends[qi], q.len[qi], q.off[qi]};
     st[threadIdx.x] = tau[qb + threadIdx.x];
+    bc[threadIdx.x] = 0;
+    if (WRITE) bbase[threadIdx.x] = off[qb + threadIdx.x] + bcounts[(int64_t)(qb + threadIdx.x) * gx + bx];
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -262,35 +290,83 @@ __global__ void __launch_bounds__(256) k_bound_compact(MeasureArgs ma, int32_t k
     for (int u = 0; u < nqb; ++u) {
       bool pass = false;
       if (valid) {
-        const float b = pair_bound(ma, sq[u], q, cm, ce, m);
+        const float b = pair_bound_m<MEAS>(ma, sq[u], q, cm, ce, m);
         const float tq = st[u];
-        if (ma.measure == TS_LCSS) pass = (b >= tq) && (b > 0.0f || j < k);
+        if (MEAS == TS_LCSS) pass = (b >= tq) && (b > 0.0f || j < k);
         else pass = isfinite(b) && (b <= tq);
       }
       const uint64_t bal = __ballot(pass);
       if (bal == 0) continue;
       const int leader = __ffsll((unsigned long long)bal) - 1;
       int base = 0;
-      if (lane == leader) base = atomicAdd(&counts[qb + u], __popcll(bal));
+      if (lane == leader) base = atomicAdd(&bc[u], __popcll(bal));  // LDS
       if (WRITE) {
         base = __shfl(base, leader);
         if (pass) {
-          const int64_t pos = off[qb + u] + base + __popcll(bal & ((1ull << lane) - 1ull));
+          const int64_t pos = bbase[u] + base + __popcll(bal & ((1ull << lane) - 1ull));
           if (pos < cap) items[pos] = Item{q0 + qb + u, j, (int32_t)pos, 0};
         }
       }
     }
   }
+  if (!WRITE) {
+    __syncthreads();
+    if (threadIdx.x < nqb) bcounts[(int64_t)(qb + threadIdx.x) * gx + bx] = bc[threadIdx.x];
+  }
+}
+
+// Per query (one block each): exclusive prefix of its gx block counts in
+// place, and the query's total into counts[q].
+__global__ void __launch_bounds__(1024) k_block_prefix(int32_t* bcounts, int32_t gx, int32_t* counts) {
+  __shared__ int32_t part[1024];
+  int32_t* row = bcounts + (int64_t)blockIdx.x * gx;
+  const int per = (gx + 1023) / 1024;
+  const int b = threadIdx.x * per;
+  int32_t sum = 0;
+  for (int i = b; i < min(gx, b + per); ++i) sum += row[i];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const int32_t v = threadIdx.x >= d ? part[threadIdx.x - d] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int32_t run = part[threadIdx.x] - sum;
+  for (int i = b; i < min(gx, b + per); ++i) {
+    const int32_t c = row[i];
+    row[i] = run;
+    run += c;
+  }
+  if (threadIdx.x == 1023) counts[blockIdx.x] = part[1023];
+}
+
+hipError_t launch_block_prefix(hipStream_t st, int32_t* bcounts, int32_t nq, int32_t gx, int32_t* counts) {
+  if (nq == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_block_prefix, dim3(nq), dim3(1024), 0, st, bcounts, gx, counts);
+  return hipGetLastError();
 }
 
 hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, int32_t nq,
                                 SetView db, const float* tau, const int64_t* off, int32_t* counts, Item* items,
                                 int64_t cap) {
-  dim3 grid((db.count + 256 * kCompactJ - 1) / (256 * kCompactJ), (nq + kCompactQ - 1) / kCompactQ);
-  if (items)
-    hipLaunchKernelGGL(k_bound_compact<true>, grid, dim3(256), 0, st, m, k, q, q0, nq, db, tau, off, counts, items, cap);
-  else
-    hipLaunchKernelGGL(k_bound_compact<false>, grid, dim3(256), 0, st, m, k, q, q0, nq, db, tau, off, counts, items, cap);
+  dim3 grid(compact_blocks(db.count), (nq + kCompactQ - 1) / kCompactQ);
+#define TS_COMPACT(MEAS)                                                                                       \
+  if (items)                                                                                                   \
+    hipLaunchKernelGGL((k_bound_compact<true, MEAS>), grid, dim3(256), 0, st, m, k, q, q0, nq, db, tau, off, counts, \
+                       items, cap);                                                                            \
+  else                                                                                                         \
+    hipLaunchKernelGGL((k_bound_compact<false, MEAS>), grid, dim3(256), 0, st, m, k, q, q0, nq, db, tau, off,  \
+                       counts, items, cap);
+  switch (m.measure) {
+    case TS_DTW: TS_COMPACT(TS_DTW) break;
+    case TS_FRECHET: TS_COMPACT(TS_FRECHET) break;
+    case TS_LCSS: TS_COMPACT(TS_LCSS) break;
+    case TS_HAUSDORFF: TS_COMPACT(TS_HAUSDORFF) break;
+    case TS_EUCLIDEAN: TS_COMPACT(TS_EUCLIDEAN) break;
+    default: return hipErrorInvalidValue;
+  }
+#undef TS_COMPACT
   return hipGetLastError();
 }
 

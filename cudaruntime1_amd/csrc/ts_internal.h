@@ -130,13 +130,17 @@ hipError_t launch_filter_ref(hipStream_t st, int32_t measure, SetView q, int32_t
 hipError_t launch_bound_chunks(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0,
                                int32_t nq, SetView db, int32_t s, int32_t chunk, uint64_t* keys);
 // Threshold compaction of every DB trajectory whose bound can still reach
-// the query's threshold tau[q]: with items == null counts survivors into
-// counts[q]; otherwise appends them as Items at off[q] + counts[q] (counts
-// then serve as cursors and must start at zero); positions >= cap are
-// dropped.
+// the query's threshold tau[q], in compact_blocks(N) blocks per query:
+// with items == null stores each block's survivors in
+// bcounts[q * compact_blocks(N) + block]; launch_block_prefix then turns
+// them into exclusive offsets within each query (counts[q] = total); with
+// items, writes the survivors as Items at off[q] + bcounts[...] + rank;
+// positions >= cap are dropped.
+int32_t compact_blocks(int32_t n_db);
 hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, int32_t nq,
-                                SetView db, const float* tau, const int64_t* off, int32_t* counts, Item* items,
+                                SetView db, const float* tau, const int64_t* off, int32_t* bcounts, Item* items,
                                 int64_t cap);
+hipError_t launch_block_prefix(hipStream_t st, int32_t* bcounts, int32_t nq, int32_t gx, int32_t* counts);
 // Second, O(n+m) admissible bound over compacted pairs (one wave per pair);
 // survivors are appended to `out` with count *nout (zeroed by the caller).
 hipError_t launch_tight_filter(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, SetView db,
