@@ -563,10 +563,19 @@ __device__ int window_side_count(const float* ax, const float* ay, int na, const
   return cnt;
 }
 
+// Survivors are staged in an LDS buffer per block and appended to `out` with
+// one atomic per block at the end (a single hot counter per pair serialises
+// in L2); a full buffer sends the rest directly.
+constexpr int kTightBuf = 256;
+
 template <int MEAS>
 __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k, SetView q, int32_t q0, SetView db,
                                                       const float* tau, const Item* items, const int32_t* nitems,
                                                       int64_t max_items, Item* out, int32_t* nout) {
+  __shared__ Item sbuf[kTightBuf];
+  __shared__ int scnt, sbase;
+  if (threadIdx.x == 0) scnt = 0;
+  __syncthreads();
   const int lane = threadIdx.x & 63;
   const int64_t n_items = nitems ? min<int64_t>(max_items, (int64_t)*nitems) : max_items;
   const int64_t nw = (int64_t)gridDim.x * 4;
@@ -685,8 +694,17 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
         pass = sqrtf(fmaxf(b2, c1)) <= t;
       }
     }
-    if (pass && lane == 0) out[atomicAdd(nout, 1)] = it;
+    if (pass && lane == 0) {
+      const int p = atomicAdd(&scnt, 1);
+      if (p < kTightBuf) sbuf[p] = it;
+      else out[atomicAdd(nout, 1)] = it;
+    }
   }
+  __syncthreads();
+  const int n = min(scnt, kTightBuf);
+  if (threadIdx.x == 0) sbase = n > 0 ? atomicAdd(nout, n) : 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) out[sbase + i] = sbuf[i];
 }
 
 hipError_t launch_tight_filter(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, SetView db,
