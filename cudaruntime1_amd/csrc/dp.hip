@@ -27,6 +27,12 @@
 //    into one of two LDS buffers (sets pad every trajectory to a multiple of
 //    4 points, so the copy is aligned 16-byte vectors); per-step operands
 //    are then LDS reads. Rows of 1024+ columns read global memory instead.
+//    When it fits as well, the group also copies the row trajectory into
+//    one LDS buffer (a lane reads its row point once per strip, and every
+//    lane has read its last one before lane 0 starts the next pair), so the
+//    steady state issues no global loads: lanes start rows on consecutive
+//    steps, and a global load per row start would cost a memory round trip
+//    on every step.
 //    A row has at least W chunks so the halo is always written at least one
 //    step before it is read; shorter rows are padded with idle chunks.
 //  * Strips and pairs follow each other without draining the pipeline: after
@@ -38,6 +44,7 @@
 //    minimises a pair's latency (R*C/(W*U) steps) when pairs are few, and
 //    its own halo length, so short pairs do not pay for long rows' LDS.
 #include <algorithm>
+#include <climits>
 
 #include "ts_internal.h"
 
@@ -52,6 +59,7 @@ struct DpArgs {
   float* out;
   int32_t cmax;   // LDS halo floats per group (and staged column capacity)
   int32_t hmask;  // halo index mask: -1, or ring length - 1
+  int32_t rmax;   // staged row buffer floats per coordinate (STAGE == 2)
   int32_t band;
   int32_t delta;
   float eps;
@@ -105,9 +113,39 @@ __device__ __forceinline__ void store_vec(float* p, const float (&v)[U]) {
   }
 }
 
-// LDS per lane group, in floats: the halo row, and with STAGE two buffers
-// each of the column trajectory's x and y (double-buffered per pair).
-__host__ __device__ constexpr int dp_group_floats(int cmax, bool stage) { return stage ? 5 * cmax : cmax; }
+// Correctly rounded sqrtf of U values. v_sqrt_f32 is within 1 ulp; for
+// x >= 2^-96, x = 0 and x = +inf, testing its two neighbours with fma
+// residuals (exact in that range) selects the correctly rounded root. That
+// is the expansion the compiler emits for sqrtf minus its rescaling of tiny
+// inputs and its 0/inf select, which these inputs do not need (0 and +inf
+// come out of the tests unchanged). A wave holding any x in (0, 2^-96)
+// takes sqrtf itself; the branch is uniform.
+template <int U>
+__device__ __forceinline__ void sqrt_cr(float (&x)[U]) {
+  bool tiny = false;
+#pragma unroll
+  for (int j = 0; j < U; ++j) tiny |= (__float_as_uint(x[j]) - 1u) < 0x0f7fffffu;
+  if (__builtin_expect(__any(tiny), 0)) {
+#pragma unroll
+    for (int j = 0; j < U; ++j) x[j] = sqrtf(x[j]);
+    return;
+  }
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    const float s = __builtin_amdgcn_sqrtf(x[j]);
+    const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+    const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+    const float t = __builtin_fmaf(-sm, s, x[j]) <= 0.0f ? sm : s;
+    x[j] = __builtin_fmaf(-sp, s, x[j]) > 0.0f ? sp : t;
+  }
+}
+
+// LDS per lane group, in floats: the halo row; with STAGE >= 1 two buffers
+// each of the column trajectory's x and y (double-buffered per pair); with
+// STAGE == 2 one buffer each of the row trajectory's x and y (rmax floats).
+__host__ __device__ constexpr int dp_group_floats(int cmax, int stage, int rmax) {
+  return stage == 0 ? cmax : 5 * cmax + (stage == 2 ? 2 * rmax : 0);
+}
 
 // Chunk range [clo, chi] (chunks of U columns) that holds every in-band cell
 // of strip s (rows s*W .. s*W+W-1). DTW band: |row*C - col*R| <= bw, so row
@@ -134,7 +172,7 @@ __device__ __forceinline__ void strip_range(int s, int R, int C, int bw, int del
   chi = hi / U;
 }
 
-template <int MEAS, int W, int U, bool BAND, bool DELTA, bool STAGE>
+template <int MEAS, int W, int U, bool BAND, bool DELTA, int STAGE>
 __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
   constexpr int G = 64 / W;
   constexpr float BND = (MEAS == TS_LCSS) ? 0.0f : kInf;
@@ -155,11 +193,13 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
   const int hdr = ((nwaves * G + 3) >> 2) << 2;
   int* slot_item = (int*)lds_dyn4 + wv * G + g;
   float* brow = (float*)lds_dyn4 + hdr + wv * 8;
-  float* gbase = (float*)lds_dyn4 + hdr + nwaves * 8 + (size_t)(wv * G + g) * dp_group_floats(cmax, STAGE);
+  float* gbase = (float*)lds_dyn4 + hdr + nwaves * 8 + (size_t)(wv * G + g) * dp_group_floats(cmax, STAGE, a.rmax);
   if (lane < 8) brow[lane] = BND;
   float* bnd = gbase;
   float* sbx = gbase + cmax;      // [2][cmax]
   float* sby = gbase + 3 * cmax;  // [2][cmax]
+  float* sax = gbase + 5 * cmax;  // [rmax], STAGE == 2
+  float* say = sax + a.rmax;      // [rmax]
 
   int64_t n_items = a.max_items;
   if (a.nitems) n_items = min<int64_t>(n_items, (int64_t)*a.nitems);
@@ -177,6 +217,11 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
   int par = 0, gpar = 0;  // buffer of this lane's pair / of the group's next pair
   const float *Ax = a.q.x, *Ay = a.q.y, *Bx = a.q.x, *By = a.q.y;
   float ax = 0.0f, ay = 0.0f, nax = 0.0f, nay = 0.0f;
+  // descriptor of the group's next pair (see the fetch below)
+  const float *fAx = a.q.x, *fAy = a.q.y, *fBx = a.q.x, *fBy = a.q.y;
+  int fR = 1, fC = 1, f_slot = 0;
+  bool f_ok = false, f_live = false;
+  float f_ax = 0.0f, f_ay = 0.0f;
   int bd = 0, bw = 0;
   const int delta = a.delta;
   float o[U], cx[U], cy[U], hv[U];
@@ -193,21 +238,45 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
     // previous pair read the other buffer; the one written here was last
     // used two pairs ago, which every lane has finished (a pair lasts at
     // least W steps and lanes trail lane 0 by at most W-1).
+    // The whole group reads the pair's descriptor here, once, and keeps it
+    // in registers (f*) until each lane starts the pair: lanes start on
+    // consecutive steps, so per-lane reads would stall W steps in a row on
+    // dependent global loads. The next fetch comes after lane 0 finishes
+    // this pair, i.e. after every lane has started it.
     if (!done && rem == lg) {
       if (lg == 0) *slot_item = atomicAdd(a.next, 1);
       __builtin_amdgcn_wave_barrier();
       const int it_idx = *slot_item;
-      if (STAGE && it_idx < n_items) {
+      f_ok = it_idx < n_items;
+      f_live = false;
+      if (f_ok) {
         const Item it = a.items[it_idx];
-        if (it.d >= 0) {
+        f_live = it.d >= 0;
+        if (f_live) {
           const int n = a.q.len[it.q], m = a.db.len[it.d];
-          const float *gx, *gy;
-          if (n >= m) { gx = a.db.x + a.db.off[it.d]; gy = a.db.y + a.db.off[it.d]; }
-          else { gx = a.q.x + a.q.off[it.q]; gy = a.q.y + a.q.off[it.q]; }
-          const int c4 = (min(n, m) + 3) >> 2;
-          for (int c = lg; c < c4; c += W) {
-            *(float4*)(sbx + gpar * cmax + 4 * c) = *(const float4*)(gx + 4 * c);
-            *(float4*)(sby + gpar * cmax + 4 * c) = *(const float4*)(gy + 4 * c);
+          const int64_t qo = a.q.off[it.q], dof = a.db.off[it.d];
+          if (n >= m) {
+            fAx = a.q.x + qo; fAy = a.q.y + qo; fBx = a.db.x + dof; fBy = a.db.y + dof; fR = n; fC = m;
+          } else {
+            fAx = a.db.x + dof; fAy = a.db.y + dof; fBx = a.q.x + qo; fBy = a.q.y + qo; fR = m; fC = n;
+          }
+          f_slot = it.slot;
+          if (STAGE == 2) {
+            for (int c = lg; c < ((fR + 3) >> 2); c += W) {
+              *(float4*)(sax + 4 * c) = *(const float4*)(fAx + 4 * c);
+              *(float4*)(say + 4 * c) = *(const float4*)(fAy + 4 * c);
+            }
+          } else {
+            // this lane's first row point
+            f_ax = fAx[min(lg, fR - 1)];
+            f_ay = fAy[min(lg, fR - 1)];
+          }
+          if (STAGE) {
+            const int c4 = (fC + 3) >> 2;
+            for (int c = lg; c < c4; c += W) {
+              *(float4*)(sbx + gpar * cmax + 4 * c) = *(const float4*)(fBx + 4 * c);
+              *(float4*)(sby + gpar * cmax + 4 * c) = *(const float4*)(fBy + 4 * c);
+            }
           }
         }
       }
@@ -215,23 +284,24 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
     }
     // ---- this lane starts the pair its group's lane 0 fetched lg steps ago
     if (!done && rem == 0) {
-      const int it_idx = *slot_item;
-      if (it_idx >= n_items) {
+      if (!f_ok) {
         done = true;
       } else {
-        const Item it = a.items[it_idx];
-        live = it.d >= 0;
+        live = f_live;
         if (live) {
-          const int n = a.q.len[it.q], m = a.db.len[it.d];
-          const int64_t qo = a.q.off[it.q], dof = a.db.off[it.d];
-          if (n >= m) {
-            Ax = a.q.x + qo; Ay = a.q.y + qo; Bx = a.db.x + dof; By = a.db.y + dof; R = n; C = m;
+          Ax = fAx; Ay = fAy; Bx = fBx; By = fBy; R = fR; C = fC;
+          slot = f_slot;
+          if (STAGE == 2) {
+            nax = sax[min(lg, R - 1)];
+            nay = say[min(lg, R - 1)];
           } else {
-            Ax = a.db.x + dof; Ay = a.db.y + dof; Bx = a.q.x + qo; By = a.q.y + qo; R = m; C = n;
+            nax = f_ax;
+            nay = f_ay;
           }
-          slot = it.slot;
         } else {
-          R = 1; C = 1;
+          R = 1; C = 1;  // a dead item: computed on stale operands, never stored
+          nax = 0.0f;
+          nay = 0.0f;
         }
         par = gpar ^ 1;
         lastch = (C - 1) / U;
@@ -279,8 +349,6 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
           load_vec<U>(Bx + lc0, cx);
           load_vec<U>(By + lc0, cy);
         }
-        nax = Ax[min(lg, R - 1)];
-        nay = Ay[min(lg, R - 1)];
         load_vec<U>(brow, hv);  // lane 0 starts on row 0: the border row above it
       }
     }
@@ -291,8 +359,13 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
       ax = nax;
       ay = nay;
       const int nr = min(row + W, R - 1);
-      nax = Ax[nr];
-      nay = Ay[nr];
+      if (STAGE == 2) {
+        nax = sax[nr];
+        nay = say[nr];
+      } else {
+        nax = Ax[nr];
+        nay = Ay[nr];
+      }
       if (BAND) bd = row * C - cstart * U * R;
       // Lane 0's diagonal for the strip's first cell: the halo row's value
       // left of the range when the previous strip computed it.
@@ -316,12 +389,21 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
       d = first ? ((row == 0) ? 0.0f : BND) : up_last;
       l = first ? BND : o[U - 1];
     }
+    float dist[U];
+    if (MEAS == TS_DTW) {
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const float dx = ax - cx[j], dy = ay - cy[j];
+        dist[j] = dx * dx + dy * dy;
+      }
+      sqrt_cr<U>(dist);
+    }
 #pragma unroll
     for (int j = 0; j < U; ++j) {
       const float dx = ax - cx[j], dy = ay - cy[j];
       float v;
       if (MEAS == TS_DTW) {
-        v = sqrtf(dx * dx + dy * dy) + fminf(fminf(up[j], l), d);
+        v = dist[j] + fminf(fminf(up[j], l), d);
         if (BAND) {
           const int bj = bd - j * R;
           if ((bj < 0 ? -bj : bj) > bw) v = kInf;
@@ -401,6 +483,199 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
   }
 }
 
+// Unbanded DP with both trajectories staged in LDS: the k_dp schedule with
+// its per-step control cut down. A lone wave issues one instruction every
+// four cycles whatever its kind, so a step's latency is its instruction
+// count; the common step is the cells plus straight-line bookkeeping:
+//  * pair fetch, pair start and the exit test sit behind one wave-uniform
+//    branch taken only on the steps where some lane fetches or starts;
+//  * the result is emitted on the step where the lane's countdown of steps
+//    left in the pair equals the value it has on the result cell;
+//  * lane W-1 writes every chunk of its row to the halo, idle ones
+//    included: lane 0 of the next strip reads chunk k D >= W steps after
+//    the strip started, after that write (made W-1+k steps after), and the
+//    next write of chunk k comes W-1 steps after that read;
+//  * row changes are selects, with the next row point read from LDS every
+//    step (rows are single-buffered: only the start of a pair reads a row
+//    of the new pair, and a lane still on the previous pair uses a row
+//    point it read before the buffer was refilled).
+template <int MEAS, int W, int U>
+__global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
+  constexpr int G = 64 / W;
+  constexpr float BND = (MEAS == TS_LCSS) ? 0.0f : kInf;
+  constexpr int kNoResult = INT_MIN;
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int nwaves = blockDim.x >> 6;
+  const int g = lane / W, lg = lane & (W - 1);
+  const int cmax = a.cmax, rmax = a.rmax;
+  extern __shared__ float4 lds_dyn4[];
+  const int hdr = ((nwaves * G + 3) >> 2) << 2;
+  int* slot_item = (int*)lds_dyn4 + wv * G + g;
+  float* brow = (float*)lds_dyn4 + hdr + wv * 8;
+  float* gbase = (float*)lds_dyn4 + hdr + nwaves * 8 + (size_t)(wv * G + g) * dp_group_floats(cmax, 2, rmax);
+  if (lane < 8) brow[lane] = BND;
+  float* bnd = gbase;
+  float* sbx = gbase + cmax;      // [2][cmax]
+  float* sby = gbase + 3 * cmax;  // [2][cmax]
+  float* sax = gbase + 5 * cmax;  // [rmax]
+  float* say = sax + rmax;        // [rmax]
+
+  int64_t n_items = a.max_items;
+  if (a.nitems) n_items = min<int64_t>(n_items, (int64_t)*a.nitems);
+
+  // Current pair of this lane. Before its first pair lane lg idles lg steps
+  // on chunks 0..lg-1 (D = W, so k does not wrap).
+  int R = 1, C = 1, lastch = 0, D = W, k = 0, row = lg, rem = lg, res_rem = kNoResult, slot = 0, par = 0;
+  int gpar = 0;
+  bool done = false, wrapped = false;
+  // The group's next pair, read once by the whole group at the fetch.
+  int fR = 1, fC = 1, f_slot = 0;
+  bool f_ok = false, f_live = false;
+  float o[U], cx[U], cy[U], hv[U];
+#pragma unroll
+  for (int j = 0; j < U; ++j) { o[j] = 0.0f; hv[j] = 0.0f; cx[j] = 0.0f; cy[j] = 0.0f; }
+  float ax = 0.0f, ay = 0.0f, rx = 0.0f, ry = 0.0f, up_last = 0.0f;
+
+  for (;;) {
+    __builtin_amdgcn_wave_barrier();
+    if (__any(rem == lg || rem == 0)) {
+      // ---- fetch: the group's lane 0 finished its pair (rem of lane lg is
+      // lg exactly then); the whole group reads the next pair's descriptor
+      // and copies both trajectories into LDS.
+      if (rem == lg) {
+        if (lg == 0) *slot_item = atomicAdd(a.next, 1);
+        __builtin_amdgcn_wave_barrier();
+        const int it_idx = *slot_item;
+        f_ok = it_idx < n_items;
+        f_live = false;
+        if (f_ok) {
+          const Item it = a.items[it_idx];
+          f_live = it.d >= 0;
+          if (f_live) {
+            const int n = a.q.len[it.q], m = a.db.len[it.d];
+            const int64_t qo = a.q.off[it.q], dof = a.db.off[it.d];
+            const float *Ax, *Ay, *Bx, *By;
+            if (n >= m) {
+              Ax = a.q.x + qo; Ay = a.q.y + qo; Bx = a.db.x + dof; By = a.db.y + dof; fR = n; fC = m;
+            } else {
+              Ax = a.db.x + dof; Ay = a.db.y + dof; Bx = a.q.x + qo; By = a.q.y + qo; fR = m; fC = n;
+            }
+            f_slot = it.slot;
+            for (int c = lg; c < ((fC + 3) >> 2); c += W) {
+              *(float4*)(sbx + gpar * cmax + 4 * c) = *(const float4*)(Bx + 4 * c);
+              *(float4*)(sby + gpar * cmax + 4 * c) = *(const float4*)(By + 4 * c);
+            }
+            for (int c = lg; c < ((fR + 3) >> 2); c += W) {
+              *(float4*)(sax + 4 * c) = *(const float4*)(Ax + 4 * c);
+              *(float4*)(say + 4 * c) = *(const float4*)(Ay + 4 * c);
+            }
+          }
+        }
+        gpar ^= 1;
+      }
+      // ---- start: this lane begins the pair fetched lg steps ago
+      if (rem == 0) {
+        if (!f_ok) {
+          done = true;
+          rem = -2;  // no further events
+          res_rem = kNoResult;
+        } else {
+          R = f_live ? fR : 1;
+          C = f_live ? fC : 1;
+          slot = f_slot;
+          lastch = (C - 1) / U;
+          D = max(lastch + 1, W);
+          const int S = (R + W - 1) / W;
+          rem = S * D;
+          // the lane holding row R-1 in the last strip meets the result on
+          // chunk lastch, when D - lastch steps are left
+          res_rem = (f_live && lg == R - 1 - (S - 1) * W) ? D - lastch : kNoResult;
+          k = 0;
+          row = lg;
+          par = gpar ^ 1;
+          wrapped = false;
+          ax = sax[min(lg, R - 1)];
+          ay = say[min(lg, R - 1)];
+          load_vec<U>(sbx + par * cmax, cx);
+          load_vec<U>(sby + par * cmax, cy);
+          load_vec<U>(brow, hv);
+        }
+      }
+      if (__all(done)) break;
+    }
+    if (wrapped) {
+      ax = rx;
+      ay = ry;
+    }
+
+    // ---- U cells
+    float up[U], on[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) up[j] = from_prev_lane<W>(o[j], hv[j], lg);
+    const bool first = k == 0;
+    float d = first ? ((MEAS == TS_LCSS || row == 0) ? 0.0f : BND) : up_last;
+    float l = first ? BND : o[U - 1];
+    float dist[U];
+    if (MEAS == TS_DTW) {
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const float dx = ax - cx[j], dy = ay - cy[j];
+        dist[j] = dx * dx + dy * dy;
+      }
+      sqrt_cr<U>(dist);
+    }
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const float dx = ax - cx[j], dy = ay - cy[j];
+      float v;
+      if (MEAS == TS_DTW) {
+        v = dist[j] + fminf(fminf(up[j], l), d);
+      } else if (MEAS == TS_FRECHET) {
+        v = fmaxf(dx * dx + dy * dy, fminf(fminf(up[j], l), d));
+      } else {
+        const bool match = fabsf(dx) < a.eps && fabsf(dy) < a.eps;
+        v = match ? d + 1.0f : fmaxf(up[j], l);
+      }
+      on[j] = v;
+      d = up[j];
+      l = v;
+    }
+    up_last = up[U - 1];
+#pragma unroll
+    for (int j = 0; j < U; ++j) o[j] = on[j];
+
+    // ---- halo row for the next strip, and the pair's result
+    if (lg == W - 1) store_vec<U>(bnd + k * U, on);
+    if (__any(rem == res_rem)) {
+      if (rem == res_rem) {
+        const int rc = (C - 1) - lastch * U;
+        float v = on[0];
+#pragma unroll
+        for (int j = 1; j < U; ++j)
+          if (rc == j) v = on[j];
+        if (MEAS == TS_FRECHET) v = sqrtf(v);
+        if (MEAS == TS_LCSS) v = v / (float)C;
+        a.out[slot] = v;
+      }
+    }
+
+    // ---- advance, and the operands of the next step
+    --rem;
+    ++k;
+    wrapped = k == D;
+    k = wrapped ? 0 : k;
+    row = wrapped ? row + W : row;
+    const int rr = min(row, R - 1);
+    rx = sax[rr];
+    ry = say[rr];
+    const int lc = U * min(k, lastch);
+    load_vec<U>(sbx + par * cmax + lc, cx);
+    load_vec<U>(sby + par * cmax + lc, cy);
+    load_vec<U>(row == 0 ? brow : bnd + k * U, hv);
+  }
+}
+
 // Halo of a launch of shape `shape` over bucket b. Without a band: a whole
 // row of the longest C in the bucket (at least W chunks). With a band: a row
 // of the longest C when that is short, else a ring of 2^k floats covering a
@@ -424,19 +699,24 @@ DpHalo dp_halo(int shape, int b, const MeasureArgs& m, int32_t cmax_c, int32_t w
   return DpHalo{h, h - 1};
 }
 
-template <int MEAS, int W, int U, bool BAND, bool DELTA, bool STAGE>
+template <int MEAS, int W, int U, bool BAND, bool DELTA, int STAGE, bool LEAN = false>
 static hipError_t launch_dp_t(hipStream_t st, const DpArgs& a, int grid_waves) {
   constexpr int G = 64 / W;
   // Waves per block bounded so that the group areas fit in 64 KiB of LDS.
-  const size_t per_wave = (size_t)G * dp_group_floats(a.cmax, STAGE) * sizeof(float) + G * sizeof(int) + 32;
+  const size_t per_wave = (size_t)G * dp_group_floats(a.cmax, STAGE, a.rmax) * sizeof(float) + G * sizeof(int) + 32;
   int wpb = 4;
   while (wpb > 1 && per_wave * wpb + 64 > 65536) --wpb;
   if (per_wave + 64 > 65536) return hipErrorInvalidValue;
   const size_t lds = (size_t)(((wpb * G + 3) >> 2) << 2) * sizeof(int) + (size_t)wpb * 8 * sizeof(float) +
-                     (size_t)wpb * G * dp_group_floats(a.cmax, STAGE) * sizeof(float);
+                     (size_t)wpb * G * dp_group_floats(a.cmax, STAGE, a.rmax) * sizeof(float);
   int blocks = (grid_waves + wpb - 1) / wpb;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL((k_dp<MEAS, W, U, BAND, DELTA, STAGE>), dim3(blocks), dim3(64 * wpb), lds, st, a);
+  if constexpr (LEAN) {
+    static_assert(STAGE == 2 && !BAND && !DELTA, "the lean kernel stages both trajectories, unbanded");
+    hipLaunchKernelGGL((k_dp_lean<MEAS, W, U>), dim3(blocks), dim3(64 * wpb), lds, st, a);
+  } else {
+    hipLaunchKernelGGL((k_dp<MEAS, W, U, BAND, DELTA, STAGE>), dim3(blocks), dim3(64 * wpb), lds, st, a);
+  }
   return hipGetLastError();
 }
 
@@ -445,18 +725,28 @@ static hipError_t launch_dp_t(hipStream_t st, const DpArgs& a, int grid_waves) {
 // in 20 KiB per wave).
 bool dp_fits(int shape, DpHalo halo) {
   const int G = 64 / kDpShapeList[shape].W;
-  return (size_t)G * dp_group_floats(halo.len, false) * sizeof(float) + G * sizeof(int) + 32 + 64 <= 65536;
+  return (size_t)G * dp_group_floats(halo.len, 0, 0) * sizeof(float) + G * sizeof(int) + 32 + 64 <= 65536;
 }
 
 // Staging the column trajectory in LDS pays while it leaves room for 8
-// waves per CU (20 KiB per wave); banded rows never stage (a pair's column
-// trajectory can be far longer than the band the LDS holds).
+// waves per CU (20 KiB per wave), and the row trajectory too when both fit
+// that; banded rows never stage (a pair's column trajectory can be far
+// longer than the band the LDS holds).
 template <int MEAS, int W, int U, bool BAND, bool DELTA>
-static hipError_t launch_dp_s(hipStream_t st, const DpArgs& a, int grid_waves) {
+static hipError_t launch_dp_s(hipStream_t st, DpArgs a, int grid_waves) {
   constexpr int G = 64 / W;
-  const bool stage = !(BAND || DELTA) && (size_t)G * dp_group_floats(a.cmax, true) * sizeof(float) <= 20480;
-  if (BAND || DELTA || !stage) return launch_dp_t<MEAS, W, U, BAND, DELTA, false>(st, a, grid_waves);
-  return launch_dp_t<MEAS, W, U, BAND, DELTA, true>(st, a, grid_waves);
+  if constexpr (BAND || DELTA) {
+    return launch_dp_t<MEAS, W, U, BAND, DELTA, 0>(st, a, grid_waves);
+  } else {
+    if ((size_t)G * dp_group_floats(a.cmax, 2, a.rmax) * sizeof(float) <= 20480)
+      return launch_dp_t<MEAS, W, U, BAND, DELTA, 2, true>(st, a, grid_waves);
+    if ((size_t)G * dp_group_floats(a.cmax, 1, 0) * sizeof(float) <= 20480) {
+      a.rmax = 0;
+      return launch_dp_t<MEAS, W, U, BAND, DELTA, 1>(st, a, grid_waves);
+    }
+    a.rmax = 0;
+    return launch_dp_t<MEAS, W, U, BAND, DELTA, 0>(st, a, grid_waves);
+  }
 }
 
 template <int MEAS, bool BAND, bool DELTA>
@@ -477,8 +767,10 @@ static hipError_t launch_dp_b(hipStream_t st, const DpArgs& a, int shape, int gr
 
 hipError_t launch_dp(hipStream_t st, MeasureArgs m, int shape, SetView q, SetView db, const Item* items,
                      const int32_t* nitems, int64_t max_items, int32_t* next, float* out, DpHalo halo,
-                     int grid_waves) {
-  DpArgs a{q, db, items, nitems, max_items, next, out, halo.len, halo.mask, m.band, m.delta, m.eps};
+                     int32_t rmax_c, int grid_waves) {
+  // row buffer: the longest trajectory of either set, in whole float4s
+  const int32_t rmax = ((std::max(rmax_c, 1) + 3) / 4) * 4;
+  DpArgs a{q, db, items, nitems, max_items, next, out, halo.len, halo.mask, rmax, m.band, m.delta, m.eps};
   switch (m.measure) {
     case TS_DTW:
       return m.band != 0 ? launch_dp_b<TS_DTW, true, false>(st, a, shape, grid_waves)

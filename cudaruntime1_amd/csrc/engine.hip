@@ -386,6 +386,7 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   const int32_t lo = std::min(qs->min_len, db->min_len), hi = std::min(qs->max_len, db->max_len);
   const int32_t wlo = dp_width_of(qs->min_len, db->min_len, ma), whi = dp_width_of(qs->max_len, db->max_len, ma);
   const int b_lo = dp_bucket_of(wlo), b_hi = dp_bucket_of(whi);
+  const int32_t rmax = std::max(qs->max_len, db->max_len);  // longest row of any pair
   if (p.measure == TS_DTW && p.band != 0 && (int64_t)std::max(qs->max_len, db->max_len) * std::max(qs->max_len, db->max_len) >= (1ll << 31))
     return fail(TS_ERR_UNSUPPORTED, "band needs trajectory lengths below 46341");
   static const char* kNames[2][3] = {{"dtw_wavefront", "frechet_wavefront", "lcss_wavefront"},
@@ -398,7 +399,7 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
     const DpHalo halo = shape_halo(e, sh, b_lo, ma, hi, whi);
     const std::string label = dp_label(nm, b_lo, sh);
     return run(e, label.c_str(), 0, 0, [&] {
-      return launch_dp(e->st, ma, sh, qv, dv, items, nitems_dev, max_items, ctr, vals, halo,
+      return launch_dp(e->st, ma, sh, qv, dv, items, nitems_dev, max_items, ctr, vals, halo, rmax,
                        cus * tune[b_lo].waves_per_cu);
     }, slot + b_lo);
   }
@@ -419,7 +420,7 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
     TS_HIP(hipStreamWaitEvent(e->aux[b], e->fork, 0));
     TS_TRY(run_on(e, e->aux[b], label.c_str(), 0, 0, [&] {
       return launch_dp(e->aux[b], ma, sh, qv, dv, itemsb + (int64_t)b * max_items, ctr + kDpBuckets + b, max_items,
-                       ctr + b, vals, halo, cus * tune[b].waves_per_cu);
+                       ctr + b, vals, halo, rmax, cus * tune[b].waves_per_cu);
     }, slot + b));
     TS_HIP(hipEventRecord(e->join[b], e->aux[b]));
     TS_HIP(hipStreamWaitEvent(e->st, e->join[b], 0));

@@ -122,6 +122,46 @@ struct QSum {
   int64_t off;
 };
 
+// LCSS box tests against c's eps-inflated box cm (same comparisons as the
+// per-point test px - cm.y < eps && cm.x - px < eps && ...).
+__device__ __forceinline__ bool lcss_box_out(const float4 b, const float4 cm, float eps) {
+  return b.x - cm.y >= eps || cm.x - b.y >= eps || b.z - cm.w >= eps || cm.z - b.w >= eps;
+}
+__device__ __forceinline__ bool lcss_box_in(const float4 b, const float4 cm, float eps) {
+  return b.y - cm.y < eps && cm.x - b.x < eps && b.w - cm.w < eps && cm.z - b.z < eps;
+}
+
+// Points of the trajectory at `off` (n points) inside cm inflated by eps.
+__device__ int lcss_box_count(const SetView& q, int64_t off, int n, const float4 cm, float eps) {
+  const float4* sb = q.sblk + (off >> 3);
+  const float4* bl = q.blk + (off >> 3);
+  const int nb = (n + 7) >> 3;
+  int cnt = 0;
+  for (int g = 0; g < nb; g += 8) {
+    const float4 s = sb[g];
+    if (lcss_box_out(s, cm, eps)) continue;
+    if (lcss_box_in(s, cm, eps)) {
+      cnt += min(n, 8 * g + 64) - 8 * g;
+      continue;
+    }
+    const int ge = min(nb, g + 8);
+    for (int b = g; b < ge; ++b) {
+      const float4 bb = bl[b];
+      if (lcss_box_out(bb, cm, eps)) continue;
+      const int np = min(n - 8 * b, 8);
+      if (lcss_box_in(bb, cm, eps)) {
+        cnt += np;
+        continue;
+      }
+      for (int i = 0; i < np; ++i) {
+        const float px = q.x[off + 8 * b + i], py = q.y[off + 8 * b + i];
+        cnt += (px - cm.y < eps && cm.x - px < eps && py - cm.w < eps && cm.z - py < eps) ? 1 : 0;
+      }
+    }
+  }
+  return cnt;
+}
+
 __device__ __forceinline__ float pair_bound(const MeasureArgs& ma, const QSum& qs, const SetView& q,
                                             const float4 cm, const float4 ce, int m) {
   const float4 a = qs.mbr;
@@ -152,16 +192,14 @@ __device__ __forceinline__ float pair_bound(const MeasureArgs& ma, const QSum& q
       return (float)(sqrt((double)n * (double)g2) * (1.0 - 1e-6));
     case TS_LCSS: {
       // Points of q that can match any point of c lie inside c's
-      // eps-inflated MBR (f32 subtraction is monotone). When q's MBR lies
-      // eps or more outside it on one side, no point of q does: the count
-      // is 0 without visiting them.
-      int cnt = 0;
+      // eps-inflated MBR. The count descends q's box hierarchy (MBR,
+      // 64-point superblocks, 8-point blocks, points): a box wholly outside
+      // adds none of its points and one wholly inside adds all of them.
+      // f32 subtraction is monotone, so each box test agrees with the test
+      // of every point in the box and the count equals the per-point count.
       const float eps = ma.eps;
-      if (a.x - cm.y >= eps || cm.x - a.y >= eps || a.z - cm.w >= eps || cm.z - a.w >= eps) return 0.0f;
-      for (int i = 0; i < n; ++i) {
-        float px = q.x[qs.off + i], py = q.y[qs.off + i];
-        cnt += (px - cm.y < eps && cm.x - px < eps && py - cm.w < eps && cm.z - py < eps) ? 1 : 0;
-      }
+      if (lcss_box_out(a, cm, eps)) return 0.0f;
+      const int cnt = lcss_box_in(a, cm, eps) ? n : lcss_box_count(q, qs.off, n, cm, eps);
       int mn = min(n, m);
       return (float)min(cnt, mn) / (float)mn;
     }

@@ -60,6 +60,22 @@ def test_pair_values_bit_exact_all_shapes(eng, shapes, measure):
     check_pairs(eng, shapes, measure, epsilon=3e-2)
 
 
+@pytest.mark.parametrize("scale", [1e-22, 1e-12, 1.0, 1e17])
+def test_dtw_extreme_scales_bit_exact(eng, scale):
+    """DTW point distances across the f32 range: squared distances below
+    2^-96 (1e-22: every wave takes plain sqrtf), around it (1e-12: waves of
+    both kinds), ordinary, and near the top (1e17), with zero distances on
+    the self pairs (the set is its own database)."""
+    xy, off = walks(31, LENGTHS, 2e-2)
+    xy = (xy.astype(np.float64) * scale).astype(np.float32)
+    t = ts.TrajSet(xy, off)
+    n = len(LENGTHS)
+    qi, di = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    s = dict(db=t, qs=t, db_xy=xy, db_off=off, q_xy=xy, q_off=off, qi=qi.ravel(), di=di.ravel())
+    say("dtw scale", scale)
+    check_pairs(eng, s, "dtw")
+
+
 @pytest.mark.parametrize("band", [1, 3, 10, 40, -1, -10, -55])
 def test_dtw_band_bit_exact(eng, shapes, band):
     say("dtw band", band)
