@@ -122,10 +122,10 @@ __device__ __forceinline__ void store_vec(float* p, const float (&v)[U]) {
 // takes sqrtf itself; the branch is uniform.
 template <int U>
 __device__ __forceinline__ void sqrt_cr(float (&x)[U]) {
-  bool tiny = false;
+  uint32_t mb = __float_as_uint(x[0]) - 1u;  // 0 maps to the top
 #pragma unroll
-  for (int j = 0; j < U; ++j) tiny |= (__float_as_uint(x[j]) - 1u) < 0x0f7fffffu;
-  if (__builtin_expect(__any(tiny), 0)) {
+  for (int j = 1; j < U; ++j) mb = min(mb, __float_as_uint(x[j]) - 1u);
+  if (__builtin_expect(__any(mb < 0x0f7fffffu), 0)) {
 #pragma unroll
     for (int j = 0; j < U; ++j) x[j] = sqrtf(x[j]);
     return;
@@ -487,18 +487,47 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
 // its per-step control cut down. A lone wave issues one instruction every
 // four cycles whatever its kind, so a step's latency is its instruction
 // count; the common step is the cells plus straight-line bookkeeping:
-//  * pair fetch, pair start and the exit test sit behind one wave-uniform
-//    branch taken only on the steps where some lane fetches or starts;
-//  * the result is emitted on the step where the lane's countdown of steps
-//    left in the pair equals the value it has on the result cell;
+//  * the pair's result, pair fetch, pair start and the exit test sit behind
+//    one wave-uniform branch taken only on the steps where some lane needs
+//    one; a lane's result is stored from its outputs of the step before,
+//    found by its countdown of steps left in the pair;
+//  * every LDS operand of step t+1 is read early in step t (LDS latency
+//    hides behind the cells). Both trajectories are staged as (x, y)
+//    pairs, so one 16-byte read brings two points;
 //  * lane W-1 writes every chunk of its row to the halo, idle ones
-//    included: lane 0 of the next strip reads chunk k D >= W steps after
-//    the strip started, after that write (made W-1+k steps after), and the
-//    next write of chunk k comes W-1 steps after that read;
-//  * row changes are selects, with the next row point read from LDS every
-//    step (rows are single-buffered: only the start of a pair reads a row
-//    of the new pair, and a lane still on the previous pair uses a row
-//    point it read before the buffer was refilled).
+//    included. A strip lasts D >= W+1 steps: lane 0 of the next strip reads
+//    chunk k early in step D+k-1 after the strip started, after lane W-1's
+//    write of it in step W-1+k, and the next write of chunk k comes in step
+//    D+W-1+k, after the read;
+//  * row changes are selects. Rows are single-buffered: a lane uses a row
+//    point read in the step before a strip starts, and every lane's last
+//    strip of a pair starts before lane 0 refills the buffer.
+template <int U>
+__device__ __forceinline__ void load_pts(const float2* p, float (&x)[U], float (&y)[U]) {
+  if constexpr (U == 1) {
+    const float2 t = *p;
+    x[0] = t.x;
+    y[0] = t.y;
+  } else {
+#pragma unroll
+    for (int j = 0; j < U; j += 2) {
+      const float4 t = *(const float4*)(p + j);
+      x[j] = t.x; y[j] = t.y; x[j + 1] = t.z; y[j + 1] = t.w;
+    }
+  }
+}
+
+// Copies n points of a trajectory (sets pad it to whole 16-byte vectors)
+// into LDS as (x, y) pairs, the group's W lanes 4 points each per round.
+template <int W>
+__device__ __forceinline__ void stage_pts(float2* dst, const float* gx, const float* gy, int n, int lg) {
+  for (int c = lg; c < ((n + 3) >> 2); c += W) {
+    const float4 x = *(const float4*)(gx + 4 * c), y = *(const float4*)(gy + 4 * c);
+    *(float4*)(dst + 4 * c) = make_float4(x.x, y.x, x.y, y.y);
+    *(float4*)(dst + 4 * c + 2) = make_float4(x.z, y.z, x.w, y.w);
+  }
+}
+
 template <int MEAS, int W, int U>
 __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
   constexpr int G = 64 / W;
@@ -515,31 +544,44 @@ __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
   float* brow = (float*)lds_dyn4 + hdr + wv * 8;
   float* gbase = (float*)lds_dyn4 + hdr + nwaves * 8 + (size_t)(wv * G + g) * dp_group_floats(cmax, 2, rmax);
   if (lane < 8) brow[lane] = BND;
-  float* bnd = gbase;
-  float* sbx = gbase + cmax;      // [2][cmax]
-  float* sby = gbase + 3 * cmax;  // [2][cmax]
-  float* sax = gbase + 5 * cmax;  // [rmax]
-  float* say = sax + rmax;        // [rmax]
+  float* bnd = gbase;                           // [cmax] halo row
+  float2* cols = (float2*)(gbase + cmax);       // [2][cmax] column points
+  float2* rows = (float2*)(gbase + 5 * cmax);   // [rmax] row points
 
   int64_t n_items = a.max_items;
   if (a.nitems) n_items = min<int64_t>(n_items, (int64_t)*a.nitems);
 
   // Current pair of this lane. Before its first pair lane lg idles lg steps
-  // on chunks 0..lg-1 (D = W, so k does not wrap).
-  int R = 1, C = 1, lastch = 0, D = W, k = 0, row = lg, rem = lg, res_rem = kNoResult, slot = 0, par = 0;
+  // on chunks 0..lg-1 (D = W+1, so k does not wrap).
+  int R = 1, C = 1, lastch = 0, D = W + 1, k = 0, row = lg, rem = lg, res_rem = kNoResult, slot = 0;
   int gpar = 0;
+  const float2* cb = cols;  // this lane's column buffer
   bool done = false, wrapped = false;
   // The group's next pair, read once by the whole group at the fetch.
   int fR = 1, fC = 1, f_slot = 0;
   bool f_ok = false, f_live = false;
   float o[U], cx[U], cy[U], hv[U];
 #pragma unroll
-  for (int j = 0; j < U; ++j) { o[j] = 0.0f; hv[j] = 0.0f; cx[j] = 0.0f; cy[j] = 0.0f; }
-  float ax = 0.0f, ay = 0.0f, rx = 0.0f, ry = 0.0f, up_last = 0.0f;
+  for (int j = 0; j < U; ++j) { o[j] = 0.0f; hv[j] = BND; cx[j] = 0.0f; cy[j] = 0.0f; }
+  float ax = 0.0f, ay = 0.0f, up_last = 0.0f;
+  float2 rp = make_float2(0.0f, 0.0f);  // row point of the next strip, read a step ahead
+  // diagonal of a row's first cell: D[r-1][-1], 0 on row 0 (and for LCSS)
+  float dfirst = (MEAS == TS_LCSS || lg == 0) ? 0.0f : BND;
 
   for (;;) {
     __builtin_amdgcn_wave_barrier();
-    if (__any(rem == lg || rem == 0)) {
+    if (__any(rem == lg || rem == 0 || rem + 1 == res_rem)) {
+      // ---- result: the lane's last step covered the result cell
+      if (rem + 1 == res_rem) {
+        const int rc = (C - 1) - lastch * U;
+        float v = o[0];
+#pragma unroll
+        for (int j = 1; j < U; ++j)
+          if (rc == j) v = o[j];
+        if (MEAS == TS_FRECHET) v = sqrtf(v);
+        if (MEAS == TS_LCSS) v = v / (float)C;
+        a.out[slot] = v;
+      }
       // ---- fetch: the group's lane 0 finished its pair (rem of lane lg is
       // lg exactly then); the whole group reads the next pair's descriptor
       // and copies both trajectories into LDS.
@@ -562,14 +604,8 @@ __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
               Ax = a.db.x + dof; Ay = a.db.y + dof; Bx = a.q.x + qo; By = a.q.y + qo; fR = m; fC = n;
             }
             f_slot = it.slot;
-            for (int c = lg; c < ((fC + 3) >> 2); c += W) {
-              *(float4*)(sbx + gpar * cmax + 4 * c) = *(const float4*)(Bx + 4 * c);
-              *(float4*)(sby + gpar * cmax + 4 * c) = *(const float4*)(By + 4 * c);
-            }
-            for (int c = lg; c < ((fR + 3) >> 2); c += W) {
-              *(float4*)(sax + 4 * c) = *(const float4*)(Ax + 4 * c);
-              *(float4*)(say + 4 * c) = *(const float4*)(Ay + 4 * c);
-            }
+            stage_pts<W>(cols + gpar * cmax, Bx, By, fC, lg);
+            stage_pts<W>(rows, Ax, Ay, fR, lg);
           }
         }
         gpar ^= 1;
@@ -585,28 +621,29 @@ __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
           C = f_live ? fC : 1;
           slot = f_slot;
           lastch = (C - 1) / U;
-          D = max(lastch + 1, W);
+          D = max(lastch + 1, W + 1);
           const int S = (R + W - 1) / W;
           rem = S * D;
-          // the lane holding row R-1 in the last strip meets the result on
-          // chunk lastch, when D - lastch steps are left
+          // the lane holding row R-1 in the last strip covers the result
+          // cell on chunk lastch, when D - lastch steps are left
           res_rem = (f_live && lg == R - 1 - (S - 1) * W) ? D - lastch : kNoResult;
           k = 0;
           row = lg;
-          par = gpar ^ 1;
           wrapped = false;
-          ax = sax[min(lg, R - 1)];
-          ay = say[min(lg, R - 1)];
-          load_vec<U>(sbx + par * cmax, cx);
-          load_vec<U>(sby + par * cmax, cy);
+          cb = cols + (gpar ^ 1) * cmax;
+          const float2 p0 = rows[min(lg, R - 1)];
+          ax = p0.x;
+          ay = p0.y;
+          load_pts<U>(cb, cx, cy);
           load_vec<U>(brow, hv);
+          dfirst = (MEAS == TS_LCSS || lg == 0) ? 0.0f : BND;
         }
       }
       if (__all(done)) break;
     }
     if (wrapped) {
-      ax = rx;
-      ay = ry;
+      ax = rp.x;
+      ay = rp.y;
     }
 
     // ---- U cells
@@ -614,7 +651,7 @@ __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
 #pragma unroll
     for (int j = 0; j < U; ++j) up[j] = from_prev_lane<W>(o[j], hv[j], lg);
     const bool first = k == 0;
-    float d = first ? ((MEAS == TS_LCSS || row == 0) ? 0.0f : BND) : up_last;
+    float d = first ? dfirst : up_last;
     float l = first ? BND : o[U - 1];
     float dist[U];
     if (MEAS == TS_DTW) {
@@ -645,39 +682,26 @@ __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
 #pragma unroll
     for (int j = 0; j < U; ++j) o[j] = on[j];
 
-    // ---- halo row for the next strip, and the pair's result
-    if (lg == W - 1) store_vec<U>(bnd + k * U, on);
-    if (__any(rem == res_rem)) {
-      if (rem == res_rem) {
-        const int rc = (C - 1) - lastch * U;
-        float v = on[0];
-#pragma unroll
-        for (int j = 1; j < U; ++j)
-          if (rc == j) v = on[j];
-        if (MEAS == TS_FRECHET) v = sqrtf(v);
-        if (MEAS == TS_LCSS) v = v / (float)C;
-        a.out[slot] = v;
-      }
-    }
-
-    // ---- advance, and the operands of the next step
-    --rem;
-    ++k;
-    wrapped = k == D;
-    k = wrapped ? 0 : k;
+    // ---- operands of the next step, read into the registers the cells
+    // are done with; the halo chunk (see above for its timing) before this
+    // step's halo store
+    const int kh = k;
+    wrapped = k + 1 == D;
+    k = wrapped ? 0 : k + 1;
     row = wrapped ? row + W : row;
-    const int rr = min(row, R - 1);
-    rx = sax[rr];
-    ry = say[rr];
-    const int lc = U * min(k, lastch);
-    load_vec<U>(sbx + par * cmax + lc, cx);
-    load_vec<U>(sby + par * cmax + lc, cy);
-    load_vec<U>(row == 0 ? brow : bnd + k * U, hv);
+    rp = rows[min(row, R - 1)];
+    load_pts<U>(cb + k * U, cx, cy);
+    const bool top = row == 0;
+    load_vec<U>(top ? brow : bnd + k * U, hv);
+    dfirst = (MEAS == TS_LCSS || top) ? 0.0f : BND;
+    if (lg == W - 1) store_vec<U>(bnd + kh * U, on);
+    --rem;
   }
 }
 
 // Halo of a launch of shape `shape` over bucket b. Without a band: a whole
-// row of the longest C in the bucket (at least W chunks). With a band: a row
+// row of the longest C in the bucket (at least W+1 chunks, the shortest
+// strip of k_dp_lean). With a band: a row
 // of the longest C when that is short, else a ring of 2^k floats covering a
 // strip's range, which spans at most W-1+2w+2U columns (band half-width w,
 // 2w+1 <= width) or W+1 chunks plus the range's shift, plus the chunk read
@@ -686,7 +710,7 @@ DpHalo dp_halo(int shape, int b, const MeasureArgs& m, int32_t cmax_c, int32_t w
   const DpShape& k = kDpShapeList[shape];
   if (!dp_ranged(m)) {
     const int32_t c = std::min<int32_t>(cmax_c, kDpBucketCap[b]);
-    const int32_t chunks = std::max((c + k.U - 1) / k.U, k.W);
+    const int32_t chunks = std::max((c + k.U - 1) / k.U, k.W + 1);
     return DpHalo{((chunks * k.U + 3) / 4) * 4, -1};
   }
   const int32_t chunks = std::max((cmax_c + k.U - 1) / k.U, k.W);
