@@ -122,8 +122,13 @@ struct DpTune {
   int waves_per_cu;
 };
 constexpr int kDpVariants = 5;  // DTW, DTW banded, Frechet, LCSS, LCSS windowed
-// Refines of at most this many pairs use the small-refine shape table.
+// DP shape tables by refine size (pairs the refine can hold): small refines
+// (one round of the grid or less: a pair's latency decides), mid-size ones
+// (a few rounds; c2's main refine), and throughput.
 constexpr int64_t kLatencyPairs = 4096;
+constexpr int64_t kMidPairs = 65536;
+constexpr int kTables = 3;  // 0: throughput, 1: small, 2: mid-size
+int dp_table_of(int64_t max_items) { return max_items <= kLatencyPairs ? 1 : (max_items <= kMidPairs ? 2 : 0); }
 
 struct Pending {
   int kid;
@@ -166,10 +171,9 @@ struct ts_engine {
   std::vector<unsigned long long> cells_host;
   int cells_used = 0;
   int64_t overflow_retries = 0;
-  // DP shape per (variant, bucket): defaults, then the tuning table.
-  // [0]: throughput table; [1]: small refines (at most kLatencyPairs pairs,
-  // one round of the grid or less: per-pair latency decides)
-  DpTune tune[2][kDpVariants][kDpBuckets];
+  // DP shape per (table, variant, bucket): defaults, then the tuning table
+  // (tables as in dp_table_of).
+  DpTune tune[kTables][kDpVariants][kDpBuckets];
   std::string tuning_source = "defaults";
   int64_t dp_fallbacks = 0;  // launches that fell back to the default shape
   // hipGraph replay of batches (off by default; never with timing or for
@@ -392,7 +396,7 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   static const char* kNames[2][3] = {{"dtw_wavefront", "frechet_wavefront", "lcss_wavefront"},
                                      {"dtw_wavefront_seed", "frechet_wavefront_seed", "lcss_wavefront_seed"}};
   const char* nm = kNames[seed ? 1 : 0][p.measure == TS_DTW ? 0 : (p.measure == TS_FRECHET ? 1 : 2)];
-  const DpTune* tune = e->tune[max_items <= kLatencyPairs ? 1 : 0][dp_variant(p)];
+  const DpTune* tune = e->tune[dp_table_of(max_items)][dp_variant(p)];
   if (b_lo == b_hi) {
     TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, ctr, 0, 1); }));
     int sh = tune[b_lo].shape;
@@ -829,7 +833,7 @@ void tune_defaults(ts_engine* e) {
   const char* v = std::getenv("TS_DP_WAVES_PER_CU");
   int w = v ? std::atoi(v) : 0;
   if (w <= 0) w = 8;  // best of a 2..32 sweep on 64 x 100k DTW
-  for (int m = 0; m < 2; ++m)
+  for (int m = 0; m < kTables; ++m)
     for (int var = 0; var < kDpVariants; ++var)
       for (int b = 0; b < kDpBuckets; ++b) e->tune[m][var][b] = DpTune{kDpDefaultShape[b], w};
 }
@@ -841,18 +845,18 @@ int shape_of(int W, int U) {
 }
 
 // Tuning table: text, one line per (variant, bucket[, table]),
-//   <variant> <bucket> <W> <U> <waves_per_cu> [lat]
+//   <variant> <bucket> <W> <U> <waves_per_cu> [lat|mid]
 // variant one of dtw, dtw_band, frechet, lcss, lcss_window; a trailing
-// "lat" fills the small-refine table (else the throughput table; small
-// refines then use the same line); '#' comments. Lines not listed keep their
-// current values.
+// "lat" fills the small-refine table, "mid" the mid-size one, nothing the
+// throughput table (and the other two where no line of theirs came
+// first); '#' comments. Lines not listed keep their current values.
 ts_status load_tuning(ts_engine* e, const char* path) {
   FILE* f = std::fopen(path, "r");
   if (!f) return fail(TS_ERR_INVALID, std::string("cannot open tuning table ") + path);
   char line[256];
   int lineno = 0;
   ts_status st = TS_OK;
-  bool lat_set[kDpVariants][kDpBuckets] = {};
+  bool set[kTables][kDpVariants][kDpBuckets] = {};
   while (std::fgets(line, sizeof line, f)) {
     ++lineno;
     char* hash = std::strchr(line, '#');
@@ -866,16 +870,20 @@ ts_status load_tuning(ts_engine* e, const char* path) {
       if (n >= 5 && std::strcmp(name, kVariantNames[i]) == 0) var = i;
     const int sh = shape_of(W, U);
     const bool lat = n == 6 && std::strcmp(mode, "lat") == 0;
-    if (n < 5 || (n == 6 && !lat) || var < 0 || b < 0 || b >= kDpBuckets || sh < 0 || waves < 1 || waves > 64) {
+    const bool mid = n == 6 && std::strcmp(mode, "mid") == 0;
+    if (n < 5 || (n == 6 && !lat && !mid) || var < 0 || b < 0 || b >= kDpBuckets || sh < 0 || waves < 1 ||
+        waves > 64) {
       st = fail(TS_ERR_INVALID, std::string(path) + ":" + std::to_string(lineno) + ": bad tuning line");
       break;
     }
-    if (lat) {
-      e->tune[1][var][b] = DpTune{sh, waves};
-      lat_set[var][b] = true;
+    if (lat || mid) {
+      const int t = lat ? 1 : 2;
+      e->tune[t][var][b] = DpTune{sh, waves};
+      set[t][var][b] = true;
     } else {
       e->tune[0][var][b] = DpTune{sh, waves};
-      if (!lat_set[var][b]) e->tune[1][var][b] = DpTune{sh, waves};
+      for (int t = 1; t < kTables; ++t)
+        if (!set[t][var][b]) e->tune[t][var][b] = DpTune{sh, waves};
     }
   }
   std::fclose(f);
@@ -964,19 +972,21 @@ ts_status ts_engine_set_dp_shape(ts_engine* e, int32_t table, int32_t variant, i
                                  int32_t cells, int32_t waves_per_cu) {
   if (!e) return fail(TS_ERR_INVALID, "null engine");
   const int sh = shape_of(lanes, cells);
-  if (table < 0 || table > 2 || variant < 0 || variant >= kDpVariants || bucket < 0 || bucket >= kDpBuckets ||
+  if (table < 0 || table > 3 || variant < 0 || variant >= kDpVariants || bucket < 0 || bucket >= kDpBuckets ||
       sh < 0 || waves_per_cu < 1 || waves_per_cu > 64)
     return fail(TS_ERR_INVALID, "ts_engine_set_dp_shape arguments");
-  for (int m = 0; m < 2; ++m)
-    if (table == 2 || table == m) e->tune[m][variant][bucket] = DpTune{sh, waves_per_cu};
+  // table 0: throughput, 1: small, 2: every table, 3: mid-size
+  for (int m = 0; m < kTables; ++m)
+    if (table == 2 || table == (m == 2 ? 3 : m)) e->tune[m][variant][bucket] = DpTune{sh, waves_per_cu};
   return TS_OK;
 }
 
 ts_status ts_engine_get_dp_shape(ts_engine* e, int32_t table, int32_t variant, int32_t bucket, int32_t* lanes,
                                  int32_t* cells, int32_t* waves_per_cu) {
-  if (!e || table < 0 || table > 1 || variant < 0 || variant >= kDpVariants || bucket < 0 || bucket >= kDpBuckets)
+  if (!e || table < 0 || table > 3 || table == 2 || variant < 0 || variant >= kDpVariants || bucket < 0 ||
+      bucket >= kDpBuckets)
     return fail(TS_ERR_INVALID, "ts_engine_get_dp_shape arguments");
-  const DpTune& t = e->tune[table][variant][bucket];
+  const DpTune& t = e->tune[table == 3 ? 2 : table][variant][bucket];
   if (lanes) *lanes = kDpShapeList[t.shape].W;
   if (cells) *cells = kDpShapeList[t.shape].U;
   if (waves_per_cu) *waves_per_cu = t.waves_per_cu;

@@ -1,7 +1,8 @@
 """Autotuner for the wavefront DP: per DP variant and length bucket, the
 lane-group shape (lanes per pair, cells per lane step) and the persistent
 grid (waves per CU) with the highest measured cell rate, and for small
-refines (one round of pairs, up to 4096) the one with the lowest time.
+refines (one round of pairs, up to 4096) and mid-size refines (4096 pairs)
+the ones with the lowest time.
 
     python -m cudaruntime1_amd.tune [--out PATH] [--quick]
 
@@ -32,6 +33,9 @@ MIN_PAIRS = 16384
 WAVES = [4, 6, 8, 12, 16]
 # pairs of the small-refine measurement (a seed round; at most kLatencyPairs)
 NLAT = 1024
+# pairs of the mid-size measurement (a few rounds of the grid: the main refine
+# of a batch of tens of queries)
+NMID = 4096
 
 
 def variant_params(v, width):
@@ -100,9 +104,11 @@ def tune(quick=False, log=print):
                 (v, b, widths.min(), widths.max())
             default = eng.get_dp_shape(vi, b, 0)
             default_lat = eng.get_dp_shape(vi, b, 1)
+            default_mid = eng.get_dp_shape(vi, b, 3)
             ref = eng.pair_values(db, qs, p, idx, idx)
             lat_idx = idx[:NLAT]
-            best, best_lat = None, None
+            mid_idx = idx[:NMID]
+            best, best_lat, best_mid = None, None, None
 
             def dp_time(sel):
                 eng.reset_stats()
@@ -119,7 +125,7 @@ def tune(quick=False, log=print):
                 if (hi + U - 1) // U < W // 2:
                     continue
                 for g in waves:
-                    eng.set_dp_shape(vi, b, W, U, g)  # both tables
+                    eng.set_dp_shape(vi, b, W, U, g)  # every table
                     f0 = eng.dp_fallbacks()
                     eng.pair_values(db, qs, p, idx[:256], idx[:256])  # warm-up
                     if eng.dp_fallbacks() != f0:
@@ -131,8 +137,12 @@ def tune(quick=False, log=print):
                     lms, _ = dp_time(lat_idx)  # one small refine: its latency decides
                     if best_lat is None or lms < best_lat[0]:
                         best_lat = (lms, W, U, g)
+                    mms, _ = dp_time(mid_idx)
+                    if best_mid is None or mms < best_mid[0]:
+                        best_mid = (mms, W, U, g)
             eng.set_dp_shape(vi, b, *default, table=0)
             eng.set_dp_shape(vi, b, *default_lat, table=1)
+            eng.set_dp_shape(vi, b, *default_mid, table=3)
             rate, W, U, g, ms = best
             results[(v, b)] = best
             log("%-12s bucket %d: lanes %2d cells %d waves/CU %2d  %.1f G cells/s (%d pairs, %.3f ms)%s" %
@@ -142,6 +152,10 @@ def tune(quick=False, log=print):
             log("%-12s bucket %d: small refines (%d pairs): lanes %2d cells %d waves/CU %2d  %.3f ms" %
                 (v, b, NLAT, W, U, g, lms))
             lines.append("%s %d %d %d %d lat  # %d pairs in %.3f ms" % (v, b, W, U, g, NLAT, lms))
+            mms, W, U, g = best_mid
+            log("%-12s bucket %d: mid-size refines (%d pairs): lanes %2d cells %d waves/CU %2d  %.3f ms" %
+                (v, b, NMID, W, U, g, mms))
+            lines.append("%s %d %d %d %d mid  # %d pairs in %.3f ms" % (v, b, W, U, g, NMID, mms))
     return lines, results
 
 

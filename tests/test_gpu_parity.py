@@ -297,3 +297,32 @@ def test_compat_reproduces_reference_fixtures(eng):
         else:
             fns[meas](P(t1.ctypes.data), P(t2.ctypes.data), P(out.ctypes.data), num_t, n, m, ctypes.byref(ms))
         assert np.array_equal(out.view(np.int32), z["h_results"].view(np.int32)), f
+
+
+def test_tuning_table_lines_fill_their_tables(tmp_path):
+    """Plain lines fill every table not yet given its own line; "lat" and
+    "mid" lines fill the small- and mid-size tables; bad lines are refused."""
+    e = ts.Engine()
+    try:
+        t = tmp_path / "table.txt"
+        t.write_text("dtw 3 32 4 8\n"
+                     "dtw 2 16 2 6 lat\n"
+                     "dtw 2 64 4 12 mid\n"
+                     "dtw 2 32 8 16  # throughput after lat and mid\n")
+        e.load_tuning(str(t))
+        assert e.get_dp_shape(0, 3, 0) == (32, 4, 8)
+        assert e.get_dp_shape(0, 3, 1) == (32, 4, 8)
+        assert e.get_dp_shape(0, 3, 3) == (32, 4, 8)
+        assert e.get_dp_shape(0, 2, 0) == (32, 8, 16)
+        assert e.get_dp_shape(0, 2, 1) == (16, 2, 6)
+        assert e.get_dp_shape(0, 2, 3) == (64, 4, 12)
+        e.set_dp_shape(0, 2, 64, 2, 4, table=3)
+        assert e.get_dp_shape(0, 2, 3) == (64, 2, 4) and e.get_dp_shape(0, 2, 1) == (16, 2, 6)
+        bad = tmp_path / "bad.txt"
+        bad.write_text("dtw 2 64 4 12 fast\n")
+        with pytest.raises(ts.TrajsimError):
+            e.load_tuning(str(bad))
+        with pytest.raises(ts.TrajsimError):
+            e.get_dp_shape(0, 2, 2)
+    finally:
+        e.close()
