@@ -192,6 +192,10 @@ struct ts_engine {
   // copies from them stay valid
   std::vector<int32_t> h_counts;
   std::vector<int64_t> h_off;
+  // Pinned host staging for a call's results and counters: one stream
+  // synchronisation brings them all back.
+  char* h_stage = nullptr;
+  size_t h_stage_bytes = 0;
 };
 
 namespace ts {
@@ -778,14 +782,38 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
   }
   TS_TRY(run(e, "decode", 0, 0, [&] { return launch_decode_keys(e->st, top, (int64_t)Q * k, desc, d_idx, d_val); }));
   TS_HIP(hipEventRecord(e->call_b, e->st));
-  TS_HIP(hipMemcpyAsync(out_idx, d_idx, sizeof(int32_t) * Q * k, hipMemcpyDeviceToHost, e->st));
-  TS_HIP(hipMemcpyAsync(out_val, d_val, sizeof(float) * Q * k, hipMemcpyDeviceToHost, e->st));
+  // Results, the capacity check and the cell counters come back through the
+  // pinned staging buffer, copied asynchronously behind the last kernel.
+  const bool check_need = use_graphs && p.mode == TS_MODE_EXACT && !all_pairs;
+  const int used = stats ? std::min(e->cells_used, kCellSlots) : 0;
+  const size_t res_bytes = (((size_t)Q * k * 8) + 15) / 16 * 16;
+  const size_t stage_bytes = res_bytes + sizeof(unsigned long long) * (2 * kCellSlots + 1);
+  if (e->h_stage_bytes < stage_bytes) {
+    if (e->h_stage) TS_HIP(hipHostFree(e->h_stage));
+    e->h_stage = nullptr;
+    e->h_stage_bytes = 0;
+    TS_HIP(hipHostMalloc((void**)&e->h_stage, stage_bytes, hipHostMallocDefault));
+    e->h_stage_bytes = stage_bytes;
+  }
+  int32_t* h_idx = (int32_t*)e->h_stage;
+  float* h_val = (float*)(e->h_stage + (size_t)Q * k * 4);
+  unsigned long long* h_cells = (unsigned long long*)(e->h_stage + res_bytes);
+  unsigned long long* h_need = h_cells + 2 * kCellSlots;
+  TS_HIP(hipMemcpyAsync(h_idx, d_idx, sizeof(int32_t) * Q * k, hipMemcpyDeviceToHost, e->st));
+  TS_HIP(hipMemcpyAsync(h_val, d_val, sizeof(float) * Q * k, hipMemcpyDeviceToHost, e->st));
+  if (check_need) TS_HIP(hipMemcpyAsync(h_need, e->need_dev, sizeof(unsigned long long), hipMemcpyDeviceToHost, e->st));
+  if (used > 0) {
+    TS_HIP(hipMemcpyAsync(h_cells, cells, sizeof(unsigned long long) * used, hipMemcpyDeviceToHost, e->st));
+    TS_HIP(hipMemcpyAsync(h_cells + kCellSlots, cells + kCellSlots, sizeof(unsigned long long) * used,
+                          hipMemcpyDeviceToHost, e->st));
+  }
   TS_HIP(hipStreamSynchronize(e->st));
-  if (use_graphs && p.mode == TS_MODE_EXACT && !all_pairs) {
+  std::memcpy(out_idx, h_idx, sizeof(int32_t) * Q * k);
+  std::memcpy(out_val, h_val, sizeof(float) * Q * k);
+  if (check_need) {
     // A replayed batch sized its candidates on the device: if any total
     // exceeded the capacity its results are incomplete; grow and repeat.
-    unsigned long long need = 0;
-    TS_HIP(hipMemcpy(&need, e->need_dev, sizeof(need), hipMemcpyDeviceToHost));
+    const unsigned long long need = *h_need;
     if ((int64_t)need > e->exact_cap) {
       if (retry > 0) return fail(TS_ERR_HIP, "candidate capacity still exceeded after growth");
       e->exact_cap = std::min<int64_t>((int64_t)need + (int64_t)need / 4 + 4096, (1ll << 31) - 1);
@@ -794,12 +822,10 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
     }
   }
   if (stats) {
-    const int used = std::min(e->cells_used, kCellSlots);
     std::vector<unsigned long long> hc(2 * kCellSlots, 0ull);
-    if (used > 0) {
-      TS_HIP(hipMemcpy(hc.data(), cells, sizeof(unsigned long long) * used, hipMemcpyDeviceToHost));
-      TS_HIP(hipMemcpy(hc.data() + kCellSlots, cells + kCellSlots, sizeof(unsigned long long) * used,
-                       hipMemcpyDeviceToHost));
+    for (int i = 0; i < used; ++i) {
+      hc[i] = h_cells[i];
+      hc[kCellSlots + i] = h_cells[kCellSlots + i];
     }
     float dms = 0;
     TS_HIP(hipEventElapsedTime(&dms, e->call_a, e->call_b));
@@ -944,6 +970,7 @@ ts_status ts_engine_destroy(ts_engine* e) {
   for (GraphEntry& g : e->gcache) (void)hipGraphExecDestroy(g.exec);
   for (auto& b : e->bufs)
     if (b.p) (void)hipFree(b.p);
+  if (e->h_stage) (void)hipHostFree(e->h_stage);
   for (auto ev : e->evpool) (void)hipEventDestroy(ev);
   if (e->call_a) (void)hipEventDestroy(e->call_a);
   if (e->call_b) (void)hipEventDestroy(e->call_b);
