@@ -63,8 +63,10 @@ __device__ unsigned long long pair_cells(int64_t a, int64_t b, int32_t measure, 
 // DP count into bucket 0.
 __global__ void k_count_cells(const Item* items, const int32_t* nitems, int64_t max_items, SetView q, SetView db,
                               MeasureArgs ma, int32_t by_bucket, unsigned long long* acc,
-                              unsigned long long* acc_pairs) {
+                              unsigned long long* acc_pairs, int32_t* zero, int32_t nzero) {
   __shared__ unsigned long long part[kDpBuckets], partp[kDpBuckets];
+  // the refine's DP work counters start at zero
+  if (blockIdx.x == 0 && threadIdx.x < nzero) zero[threadIdx.x] = 0;
   if (threadIdx.x < kDpBuckets) {
     part[threadIdx.x] = 0;
     partp[threadIdx.x] = 0;
@@ -371,13 +373,15 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   if (slot + kDpBuckets <= kCellSlots) e->cells_used += kDpBuckets;
   else slot = kCellSlots - kDpBuckets;
   const bool dp = p.measure == TS_DTW || p.measure == TS_FRECHET || p.measure == TS_LCSS;
-  TS_TRY(run(e, "count_cells", 0, 0, [&] {
-    hipLaunchKernelGGL(k_count_cells, dim3((unsigned)((max_items + 255) / 256)), dim3(256), 0, e->st, items,
-                       nitems_dev, max_items, qv, dv, ma, dp ? 1 : 0, cells + slot, cells + kCellSlots + slot);
-    return hipGetLastError();
-  }));
   int32_t* ctr = nullptr;
   TS_TRY(ensure(e, B_CTR, 16, &ctr));
+  static_assert(2 * kDpBuckets <= 16, "DP work counters");
+  TS_TRY(run(e, "count_cells", 0, 0, [&] {
+    hipLaunchKernelGGL(k_count_cells, dim3((unsigned)((max_items + 255) / 256)), dim3(256), 0, e->st, items,
+                       nitems_dev, max_items, qv, dv, ma, dp ? 1 : 0, cells + slot, cells + kCellSlots + slot, ctr,
+                       dp ? 2 * kDpBuckets : 0);
+    return hipGetLastError();
+  }));
   if (p.measure == TS_HAUSDORFF) {
     return run(e, seed ? "hausdorff_refine_seed" : "hausdorff_refine", 0, 0, [&] {
       return launch_hausdorff(e->st, qv, dv, items, nitems_dev, max_items, vals);
@@ -402,7 +406,6 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   const char* nm = kNames[seed ? 1 : 0][p.measure == TS_DTW ? 0 : (p.measure == TS_FRECHET ? 1 : 2)];
   const DpTune* tune = e->tune[dp_table_of(max_items)][dp_variant(p)];
   if (b_lo == b_hi) {
-    TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, ctr, 0, 1); }));
     int sh = tune[b_lo].shape;
     const DpHalo halo = shape_halo(e, sh, b_lo, ma, hi, whi);
     const std::string label = dp_label(nm, b_lo, sh);
@@ -413,7 +416,6 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   }
   Item* itemsb = nullptr;
   TS_TRY(ensure(e, B_ITEMS3, (int64_t)kDpBuckets * max_items, &itemsb));
-  TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, ctr, 0, 2 * kDpBuckets); }));
   TS_TRY(run(e, "items_by_bucket", 0, 0, [&] {
     return launch_items_by_bucket(e->st, items, nitems_dev, max_items, qv, dv, ma, itemsb, max_items,
                                   ctr + kDpBuckets);
@@ -482,9 +484,7 @@ ts_status knn_ref_batch(ts_engine* e, const ts_params& p, const ts_set* db, cons
   }));
   TS_TRY(run(e, "keys", 0, 0, [&] { return launch_keys_from_rows(e->st, scores, nq, N, desc, keys); }));
   TS_TRY(topk(e, keys, N, N, nullptr, nq, kc, cand));
-  TS_TRY(run(e, "items", 0, 0, [&] { return launch_items_from_keys(e->st, cand, nq, kc, q0, items, nullptr); }));
-  TS_TRY(run(e, "decode", 0, 0, [&] { return launch_decode_keys(e->st, cand, (int64_t)nq * kc, desc, cand_idx, nullptr); }));
-  TS_TRY(fill_nan(e, vals, (int64_t)nq * kc));
+  TS_TRY(run(e, "items", 0, 0, [&] { return launch_items_from_keys(e->st, cand, nq, kc, q0, items, cand_idx, vals); }));
   TS_TRY(refine(e, p, db, qs, items, nullptr, (int64_t)nq * kc, vals));
   TS_TRY(run(e, "keys", 0, 0, [&] {
     return launch_keys_from_regions(e->st, vals, cand_idx, nullptr, nq, kc, desc, keys2);
@@ -548,9 +548,7 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
     return launch_bound_chunks(e->st, ma, k, qv, q0, nq, dv, s, 2048, bkeys);
   }));
   TS_TRY(topk(e, bkeys, (int64_t)nch * s, nch * s, nullptr, nq, s, seeds));
-  TS_TRY(run(e, "items", 0, 0, [&] { return launch_items_from_keys(e->st, seeds, nq, s, q0, items, nullptr); }));
-  TS_TRY(run(e, "decode", 0, 0, [&] { return launch_decode_keys(e->st, seeds, (int64_t)nq * s, desc, seed_idx, nullptr); }));
-  TS_TRY(fill_nan(e, vals, (int64_t)nq * s));
+  TS_TRY(run(e, "items", 0, 0, [&] { return launch_items_from_keys(e->st, seeds, nq, s, q0, items, seed_idx, vals); }));
   TS_TRY(refine(e, p, db, qs, items, nullptr, (int64_t)nq * s, vals, true));
   // 2. threshold: k-th best exact value among the seeds
   TS_TRY(run(e, "keys", 0, 0, [&] { return launch_keys_from_regions(e->st, vals, seed_idx, nullptr, nq, s, desc, skeys); }));
@@ -616,17 +614,15 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   TS_TRY(ensure(e, B_RITEMS, cap, &items2));
   TS_TRY(ensure(e, B_RVALS, cap, &rvals));
   TS_TRY(ensure(e, B_SCORES, 2 * cap, (float**)&rkeys));
-  TS_TRY(run(e, "fill", 0, 0, [&] { return launch_fill_i32(e->st, counts + nq, 0, 1); }));
   TS_TRY(run(e, "bound_compact", pairs * 36.0, pairs, [&] {
     return launch_bound_compact(e->st, ma, k, qv, q0, nq, dv, tau, d_off, bcounts, citems, cap);
   }));
   // 3b. tighter O(n+m) bound on the survivors
   int32_t* nitems2 = counts + nq;
   TS_TRY(run(e, "tight_filter", 0, 0, [&] {
-    return launch_tight_filter(e->st, ma, k, qv, q0, dv, tau, citems, ntot, total, items2, nitems2,
+    return launch_tight_filter(e->st, ma, k, qv, q0, dv, tau, citems, ntot, total, items2, nitems2, rvals,
                                device_cu_count() * 16);
   }));
-  TS_TRY(fill_nan(e, rvals, cap));
   TS_TRY(refine(e, p, db, qs, items2, nitems2, cap, rvals));
   // 4. top-k over each query's refined candidates (slot = position in citems)
   TS_TRY(run(e, "keys", 0, 0, [&] { return launch_keys_from_items(e->st, rvals, citems, cap, desc, rkeys); }));

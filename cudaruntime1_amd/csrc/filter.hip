@@ -377,6 +377,8 @@ __global__ void __launch_bounds__(1024) k_block_prefix(int32_t* bcounts, int32_t
     run += c;
   }
   if (threadIdx.x == 1023) counts[blockIdx.x] = part[1023];
+  // counts[nq] is the tight filter's survivor counter, cleared here
+  if (blockIdx.x == 0 && threadIdx.x == 0) counts[gridDim.x] = 0;
 }
 
 hipError_t launch_block_prefix(hipStream_t st, int32_t* bcounts, int32_t nq, int32_t gx, int32_t* counts) {
@@ -609,7 +611,7 @@ constexpr int kTightBuf = 256;
 template <int MEAS>
 __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k, SetView q, int32_t q0, SetView db,
                                                       const float* tau, const Item* items, const int32_t* nitems,
-                                                      int64_t max_items, Item* out, int32_t* nout) {
+                                                      int64_t max_items, Item* out, int32_t* nout, float* rej) {
   __shared__ Item sbuf[kTightBuf];
   __shared__ int scnt, sbase;
   if (threadIdx.x == 0) scnt = 0;
@@ -619,7 +621,10 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
   const int64_t nw = (int64_t)gridDim.x * 4;
   for (int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); w < n_items; w += nw) {
     const Item it = items[w];
-    if (it.d < 0) continue;
+    if (it.d < 0) {
+      if (lane == 0) rej[it.slot] = __int_as_float(0x7fc00000);
+      continue;
+    }
     const int n = q.len[it.q], m = db.len[it.d];
     const int64_t qo = q.off[it.q], co = db.off[it.d];
     const float4 qb = q.mbr[it.q], cb = db.mbr[it.d];
@@ -659,7 +664,10 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
       float f = 0.0f;
       double s = 0.0;
       int c = 0;
-      if (n != m) continue;
+      if (n != m) {
+        if (lane == 0) rej[it.slot] = __int_as_float(0x7fc00000);
+        continue;
+      }
       side_bound<MEAS>(q.x + qo, q.y + qo, n, cb, ma.eps, false, f, s, c);
       s = wave_sum_d(s);
       pass = (float)(sqrt(s) * (1.0 - 1e-6)) <= t;
@@ -732,6 +740,8 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
         pass = sqrtf(fmaxf(b2, c1)) <= t;
       }
     }
+    // a rejected pair's value slot holds NaN (no refine writes it)
+    if (!pass && lane == 0) rej[it.slot] = __int_as_float(0x7fc00000);
     if (pass && lane == 0) {
       const int p = atomicAdd(&scnt, 1);
       if (p < kTightBuf) sbuf[p] = it;
@@ -747,14 +757,14 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
 
 hipError_t launch_tight_filter(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, SetView db,
                                const float* tau, const Item* items, const int32_t* nitems, int64_t max_items,
-                               Item* out, int32_t* nout, int grid_waves) {
+                               Item* out, int32_t* nout, float* rej, int grid_waves) {
   const dim3 grid((grid_waves + 3) / 4);
   switch (m.measure) {
-    case TS_DTW: hipLaunchKernelGGL(k_tight_filter<TS_DTW>, grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout); break;
-    case TS_FRECHET: hipLaunchKernelGGL(k_tight_filter<TS_FRECHET>, grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout); break;
-    case TS_LCSS: hipLaunchKernelGGL(k_tight_filter<TS_LCSS>, grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout); break;
-    case TS_HAUSDORFF: hipLaunchKernelGGL(k_tight_filter<TS_HAUSDORFF>, grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout); break;
-    case TS_EUCLIDEAN: hipLaunchKernelGGL(k_tight_filter<TS_EUCLIDEAN>, grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout); break;
+    case TS_DTW: hipLaunchKernelGGL(k_tight_filter<TS_DTW>, grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout, rej); break;
+    case TS_FRECHET: hipLaunchKernelGGL(k_tight_filter<TS_FRECHET>, grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout, rej); break;
+    case TS_LCSS: hipLaunchKernelGGL(k_tight_filter<TS_LCSS>, grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout, rej); break;
+    case TS_HAUSDORFF: hipLaunchKernelGGL(k_tight_filter<TS_HAUSDORFF>, grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout, rej); break;
+    case TS_EUCLIDEAN: hipLaunchKernelGGL(k_tight_filter<TS_EUCLIDEAN>, grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout, rej); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -70,13 +70,17 @@ __global__ void __launch_bounds__(256) k_topk_stage(const uint64_t* in, int64_t 
       for (int t = threadIdx.x; t < k; t += 256) dst[t] = ~0ull;
       continue;
     }
+    // Sort only the smallest power of two (>= 64) holding the chunk's keys:
+    // the rest would be empty keys, which sort last.
+    int cs = ch;
+    while (cs > 64 && (cs >> 1) >= nh) cs >>= 1;
     __syncthreads();  // the previous chunk's results are out of s
     const uint64_t* src = base + begin;
-    for (int i = threadIdx.x; i < ch; i += 256) s[i] = i < nh ? src[i] : ~0ull;
+    for (int i = threadIdx.x; i < cs; i += 256) s[i] = i < nh ? src[i] : ~0ull;
     __syncthreads();
-    for (int kk = 2; kk <= ch; kk <<= 1) {
+    for (int kk = 2; kk <= cs; kk <<= 1) {
       for (int j = kk >> 1; j > 0; j >>= 1) {
-        for (int i = threadIdx.x; i < ch; i += 256) {
+        for (int i = threadIdx.x; i < cs; i += 256) {
           int ixj = i ^ j;
           if (ixj > i) {
             uint64_t a = s[i], b = s[ixj];
@@ -87,7 +91,7 @@ __global__ void __launch_bounds__(256) k_topk_stage(const uint64_t* in, int64_t 
         __syncthreads();
       }
     }
-    for (int t = threadIdx.x; t < k; t += 256) dst[t] = t < ch ? s[t] : ~0ull;
+    for (int t = threadIdx.x; t < k; t += 256) dst[t] = t < cs ? s[t] : ~0ull;
   }
 }
 
@@ -173,7 +177,7 @@ hipError_t launch_kth_threshold(hipStream_t st, const uint64_t* keys, int32_t ns
 }
 
 __global__ void __launch_bounds__(256) k_items_from_keys(const uint64_t* keys, int32_t rows, int32_t s, int32_t q0,
-                                                         Item* items) {
+                                                         Item* items, int32_t* idx, float* nan_vals) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= (int64_t)rows * s) return;
   uint64_t k = keys[i];
@@ -183,15 +187,16 @@ __global__ void __launch_bounds__(256) k_items_from_keys(const uint64_t* keys, i
   it.slot = (int32_t)i;
   it.pad = 0;
   items[i] = it;
+  if (idx) idx[i] = it.d;
+  if (nan_vals) nan_vals[i] = __int_as_float(0x7fc00000);
 }
 
 hipError_t launch_items_from_keys(hipStream_t st, const uint64_t* keys, int32_t rows, int32_t s, int32_t q0,
-                                  Item* items, int32_t* nitems) {
+                                  Item* items, int32_t* idx, float* nan_vals) {
   int64_t n = (int64_t)rows * s;
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_items_from_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys, rows, s, q0,
-                     items);
-  (void)nitems;
+                     items, idx, nan_vals);
   return hipGetLastError();
 }
 

@@ -142,10 +142,12 @@ hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetVie
                                 int64_t cap);
 hipError_t launch_block_prefix(hipStream_t st, int32_t* bcounts, int32_t nq, int32_t gx, int32_t* counts);
 // Second, O(n+m) admissible bound over compacted pairs (one wave per pair);
-// survivors are appended to `out` with count *nout (zeroed by the caller).
+// survivors are appended to `out` with count *nout (zeroed by the caller:
+// launch_block_prefix clears counts[nq]); a rejected pair's rej[slot] is set
+// to NaN.
 hipError_t launch_tight_filter(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, SetView db,
                                const float* tau, const Item* items, const int32_t* nitems, int64_t max_items,
-                               Item* out, int32_t* nout, int grid_waves);
+                               Item* out, int32_t* nout, float* rej, int grid_waves);
 
 // select.hip
 // Key order: ascending u64. enc_asc(v,i) = ordered(v)<<32|i, desc uses ~ordered(v).
@@ -171,9 +173,11 @@ hipError_t launch_decode_keys(hipStream_t st, const uint64_t* keys, int64_t n, i
 // if the segment has fewer keys.
 hipError_t launch_kth_threshold(hipStream_t st, const uint64_t* keys, int32_t nseg, int32_t s,
                                 int32_t k, int32_t descending, float* tau);
-// Build Items (q0+row, d=decoded idx, slot=row*s+t) from per-row sorted keys.
+// Build Items (q0+row, d=decoded idx, slot=row*s+t) from per-row sorted keys;
+// with idx, also the decoded index (-1 for an empty key) per slot, and with
+// nan_vals, NaN into every value slot (the refine overwrites the live ones).
 hipError_t launch_items_from_keys(hipStream_t st, const uint64_t* keys, int32_t rows, int32_t s,
-                                  int32_t q0, Item* items, int32_t* nitems);
+                                  int32_t q0, Item* items, int32_t* idx, float* nan_vals);
 // Keys of refined values: key(value[slot], d) for every slot of the query
 // regions, ~0 for empty slots.
 hipError_t launch_keys_from_regions(hipStream_t st, const float* vals, const int32_t* region_d,
