@@ -172,21 +172,21 @@ __device__ __forceinline__ float pair_bound(const MeasureArgs& ma, const QSum& q
   switch (ma.measure) {
     case TS_DTW: {
       float dx = qs.ends.x - ce.x, dy = qs.ends.y - ce.y;
-      float e = sqrtf(dx * dx + dy * dy);
+      float e = sqrt_lo(dx * dx + dy * dy);
       int L = max(n, m);
       if (L >= 2) {
         float ex = qs.ends.z - ce.z, ey = qs.ends.w - ce.w;
-        e = e + sqrtf(ex * ex + ey * ey);
+        e = e + sqrt_lo(ex * ex + ey * ey);
       }
-      float s = e + (float)max(0, L - 2) * sqrtf(g2);
+      float s = e + (float)max(0, L - 2) * sqrt_lo(g2);
       return s * (1.0f - (float)(L + 8) * 1.1920929e-7f);
     }
     case TS_FRECHET: {
       float dx = qs.ends.x - ce.x, dy = qs.ends.y - ce.y;
       float ex = qs.ends.z - ce.z, ey = qs.ends.w - ce.w;
-      return fmaxf(sqrtf(g2), fmaxf(sqrtf(dx * dx + dy * dy), sqrtf(ex * ex + ey * ey)));
+      return fmaxf(sqrt_lo(g2), fmaxf(sqrt_lo(dx * dx + dy * dy), sqrt_lo(ex * ex + ey * ey)));
     }
-    case TS_HAUSDORFF: return sqrtf(g2);
+    case TS_HAUSDORFF: return sqrt_lo(g2);
     case TS_EUCLIDEAN:
       if (m != n) return __int_as_float(0x7f800000);
       return (float)(sqrt((double)n * (double)g2) * (1.0 - 1e-6));
@@ -466,7 +466,7 @@ __device__ void side_bound(const float* ax, const float* ay, int na, float4 b, f
     } else {
       const float d2 = pt_box(px, py, b);
       if (MEAS == TS_DTW) {
-        if (!(skip_ends && (i == 0 || i == na - 1))) acc += sqrtf(d2);
+        if (!(skip_ends && (i == 0 || i == na - 1))) acc += sqrt_lo(d2);
       } else {
         acc = fmaxf(acc, d2);
       }
@@ -503,7 +503,7 @@ __device__ float band_side_sum(const float* ax, const float* ay, int na, const f
         const float4 t = bblk[kb];
         bx.x = fminf(bx.x, t.x); bx.y = fmaxf(bx.y, t.y); bx.z = fminf(bx.z, t.z); bx.w = fmaxf(bx.w, t.w);
       }
-      acc = sqrtf(pt_box(ax[i], ay[i], bx));
+      acc = sqrt_lo(pt_box(ax[i], ay[i], bx));
     }
     total += wave_sum(acc);
     if ((total + base) * disc > limit) break;  // uniform: total is the same in every lane
@@ -527,7 +527,7 @@ __device__ float tile_side_sum(const float* ax, const float* ay, int na, const f
       const float px = ax[i], py = ay[i];
       float best = pt_box(px, py, bsblk[0]);
       for (int kb = 8; kb < nblk; kb += 8) best = fminf(best, pt_box(px, py, bsblk[kb]));
-      acc = sqrtf(best);
+      acc = sqrt_lo(best);
     }
     total += wave_sum(acc);
     if ((total + base) * disc > limit) break;
@@ -680,7 +680,7 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
       a1 = wave_sum(a1);
       a2 = wave_sum(a2);
       const float dx0 = qe.x - ce.x, dy0 = qe.y - ce.y, dx1 = qe.z - ce.z, dy1 = qe.w - ce.w;
-      const float e0 = sqrtf(dx0 * dx0 + dy0 * dy0), e1 = sqrtf(dx1 * dx1 + dy1 * dy1);
+      const float e0 = sqrt_lo(dx0 * dx0 + dy0 * dy0), e1 = sqrt_lo(dx1 * dx1 + dy1 * dy1);
       // Row form: rows 0 and n-1 contain (0,0) and (n-1,m-1); with n == 1 the
       // single row holds both when m >= 2. Column form likewise.
       const int L = max(n, m);

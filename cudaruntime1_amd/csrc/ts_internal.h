@@ -13,6 +13,15 @@ namespace ts {
 // DP border value: the reference's "infinity" (dtw.cpp:68-69, dtw.cu:59).
 constexpr float kInf = 1e20f;
 
+// A lower bound of sqrt(x) for admissible filter bounds (never for a value
+// that is returned): v_sqrt_f32 is within 1 ulp, so scaling it by 1 - 2^-21
+// stays below the true root (by at least 2^-21 - 2^-23 - 2^-24 relative);
+// below 2^-96, where v_sqrt_f32 may flush, 0. Four instructions instead of
+// the correctly rounded sqrtf's sixteen.
+__device__ __forceinline__ float sqrt_lo(float x) {
+  return x < 0x1p-96f ? 0.0f : __builtin_amdgcn_sqrtf(x) * (1.0f - 0x1p-21f);
+}
+
 // Thread-local error message behind ts_last_error().
 void set_error(const std::string& msg);
 ts_status fail(ts_status s, const std::string& msg);
