@@ -1,0 +1,33 @@
+"""CPU checks of two parts of the DP refine that GPU runs only sample: the
+lean kernel's schedule, modelled lane by lane against the oracle
+(tests/emu/dp_lean_schedule.cpp: lane stagger, pair fetch and start, halo
+timing, result countdown, staged operands' provenance, LDS bounds), and the
+neighbour fixup that makes v_sqrt_f32 a correctly rounded square root
+(tests/emu/sqrt_fixup.c)."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_lean_dp_schedule_model_matches_oracle(tmp_path):
+    exe = str(tmp_path / "dp_lean_schedule")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fopenmp", "-o", exe,
+                    os.path.join(HERE, "emu", "dp_lean_schedule.cpp"), os.path.join(ROOT, "oracle", "oracle.cpp")],
+                   check=True, timeout=300)
+    r = subprocess.run([exe, "12"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
+def test_sqrt_neighbour_fixup_is_correctly_rounded(tmp_path):
+    exe = str(tmp_path / "sqrt_fixup")
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-o", exe, os.path.join(HERE, "emu", "sqrt_fixup.c"), "-lm"],
+                   check=True, timeout=120)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and " 0 mismatches" in r.stdout, r.stdout + r.stderr
