@@ -53,7 +53,8 @@ namespace ts {
 struct DpArgs {
   SetView q, db;
   const Item* items;
-  const int32_t* nitems;
+  const int32_t* nitems;  // items [0, *nitems) ...
+  const int32_t* ntail;   // ... then, when set, *ntail items stored backwards from items[max_items - 1]
   int64_t max_items;
   int32_t* next;
   float* out;
@@ -203,6 +204,8 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
 
   int64_t n_items = a.max_items;
   if (a.nitems) n_items = min<int64_t>(n_items, (int64_t)*a.nitems);
+  const int64_t n_head = n_items;
+  if (a.ntail) n_items = min<int64_t>(a.max_items, n_head + (int64_t)*a.ntail);
 
   // rem: steps left in this lane's current pair (its first pair starts after
   // lg idle steps). Lane 0 of the group starts each pair exactly lg steps
@@ -250,7 +253,7 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
       f_ok = it_idx < n_items;
       f_live = false;
       if (f_ok) {
-        const Item it = a.items[it_idx];
+        const Item it = a.items[it_idx < n_head ? (int64_t)it_idx : a.max_items - 1 - (it_idx - n_head)];
         f_live = it.d >= 0;
         if (f_live) {
           const int n = a.q.len[it.q], m = a.db.len[it.d];
@@ -550,6 +553,8 @@ __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
 
   int64_t n_items = a.max_items;
   if (a.nitems) n_items = min<int64_t>(n_items, (int64_t)*a.nitems);
+  const int64_t n_head = n_items;
+  if (a.ntail) n_items = min<int64_t>(a.max_items, n_head + (int64_t)*a.ntail);
 
   // Current pair of this lane. Before its first pair lane lg idles lg steps
   // on chunks 0..lg-1 (D = W+1, so k does not wrap).
@@ -592,7 +597,7 @@ __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
         f_ok = it_idx < n_items;
         f_live = false;
         if (f_ok) {
-          const Item it = a.items[it_idx];
+          const Item it = a.items[it_idx < n_head ? (int64_t)it_idx : a.max_items - 1 - (it_idx - n_head)];
           f_live = it.d >= 0;
           if (f_live) {
             const int n = a.q.len[it.q], m = a.db.len[it.d];
@@ -790,11 +795,11 @@ static hipError_t launch_dp_b(hipStream_t st, const DpArgs& a, int shape, int gr
 }
 
 hipError_t launch_dp(hipStream_t st, MeasureArgs m, int shape, SetView q, SetView db, const Item* items,
-                     const int32_t* nitems, int64_t max_items, int32_t* next, float* out, DpHalo halo,
-                     int32_t rmax_c, int grid_waves) {
+                     const int32_t* nitems, const int32_t* ntail, int64_t max_items, int32_t* next, float* out,
+                     DpHalo halo, int32_t rmax_c, int grid_waves) {
   // row buffer: the longest trajectory of either set, in whole float4s
   const int32_t rmax = ((std::max(rmax_c, 1) + 3) / 4) * 4;
-  DpArgs a{q, db, items, nitems, max_items, next, out, halo.len, halo.mask, rmax, m.band, m.delta, m.eps};
+  DpArgs a{q, db, items, nitems, ntail, max_items, next, out, halo.len, halo.mask, rmax, m.band, m.delta, m.eps};
   switch (m.measure) {
     case TS_DTW:
       return m.band != 0 ? launch_dp_b<TS_DTW, true, false>(st, a, shape, grid_waves)

@@ -373,13 +373,15 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   if (slot + kDpBuckets <= kCellSlots) e->cells_used += kDpBuckets;
   else slot = kCellSlots - kDpBuckets;
   const bool dp = p.measure == TS_DTW || p.measure == TS_FRECHET || p.measure == TS_LCSS;
+  // DP counters: [0, B) work counters of the bucket launches, [B, 2B) and
+  // [2B, 3B) the heads and tails of the bucket lists
   int32_t* ctr = nullptr;
-  TS_TRY(ensure(e, B_CTR, 16, &ctr));
-  static_assert(2 * kDpBuckets <= 16, "DP work counters");
+  TS_TRY(ensure(e, B_CTR, 32, &ctr));
+  static_assert(3 * kDpBuckets <= 32, "DP counters");
   TS_TRY(run(e, "count_cells", 0, 0, [&] {
     hipLaunchKernelGGL(k_count_cells, dim3((unsigned)((max_items + 255) / 256)), dim3(256), 0, e->st, items,
                        nitems_dev, max_items, qv, dv, ma, dp ? 1 : 0, cells + slot, cells + kCellSlots + slot, ctr,
-                       dp ? 2 * kDpBuckets : 0);
+                       dp ? 3 * kDpBuckets : 0);
     return hipGetLastError();
   }));
   if (p.measure == TS_HAUSDORFF) {
@@ -410,15 +412,22 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
     const DpHalo halo = shape_halo(e, sh, b_lo, ma, hi, whi);
     const std::string label = dp_label(nm, b_lo, sh);
     return run(e, label.c_str(), 0, 0, [&] {
-      return launch_dp(e->st, ma, sh, qv, dv, items, nitems_dev, max_items, ctr, vals, halo, rmax,
+      return launch_dp(e->st, ma, sh, qv, dv, items, nitems_dev, nullptr, max_items, ctr, vals, halo, rmax,
                        cus * tune[b_lo].waves_per_cu);
     }, slot + b_lo);
   }
   Item* itemsb = nullptr;
   TS_TRY(ensure(e, B_ITEMS3, (int64_t)kDpBuckets * max_items, &itemsb));
+  // Longest pairs first: in each bucket, pairs of at least half the bucket's
+  // largest n*m lead its list.
+  BucketCells long_cells;
+  for (int b = 0; b < kDpBuckets; ++b) {
+    const int64_t cb = dp_ranged(ma) ? hi : std::min<int64_t>(kDpBucketCap[b], hi);
+    long_cells.t[b] = cb * rmax / 2;
+  }
   TS_TRY(run(e, "items_by_bucket", 0, 0, [&] {
     return launch_items_by_bucket(e->st, items, nitems_dev, max_items, qv, dv, ma, itemsb, max_items,
-                                  ctr + kDpBuckets);
+                                  ctr + kDpBuckets, ctr + 2 * kDpBuckets, long_cells);
   }));
   // Buckets run concurrently on their own streams (fork/join by events), so
   // the latency-bound tail of one overlaps the others.
@@ -429,8 +438,8 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
     const std::string label = dp_label(nm, b, sh);
     TS_HIP(hipStreamWaitEvent(e->aux[b], e->fork, 0));
     TS_TRY(run_on(e, e->aux[b], label.c_str(), 0, 0, [&] {
-      return launch_dp(e->aux[b], ma, sh, qv, dv, itemsb + (int64_t)b * max_items, ctr + kDpBuckets + b, max_items,
-                       ctr + b, vals, halo, rmax, cus * tune[b].waves_per_cu);
+      return launch_dp(e->aux[b], ma, sh, qv, dv, itemsb + (int64_t)b * max_items, ctr + kDpBuckets + b,
+                       ctr + 2 * kDpBuckets + b, max_items, ctr + b, vals, halo, rmax, cus * tune[b].waves_per_cu);
     }, slot + b));
     TS_HIP(hipEventRecord(e->join[b], e->aux[b]));
     TS_HIP(hipStreamWaitEvent(e->st, e->join[b], 0));

@@ -304,35 +304,44 @@ hipError_t launch_fill_i32(hipStream_t st, int32_t* p, int32_t v, int64_t n) {
 __global__ void __launch_bounds__(256) k_items_by_bucket(const Item* items, const int32_t* nitems,
                                                          int64_t max_items, SetView q, SetView db,
                                                          MeasureArgs m, Item* outb, int64_t cap_each,
-                                                         int32_t* countsb) {
+                                                         int32_t* heads, int32_t* tails, BucketCells long_cells) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   int64_t n = nitems ? (int64_t)*nitems : max_items;
   if (n > max_items) n = max_items;
   Item it{0, -1, 0, 0};
   if (i < n) it = items[i];
-  const int b = it.d >= 0 ? dp_bucket_of(dp_width_of(q.len[it.q], db.len[it.d], m)) : -1;
+  int b = -1;
+  bool lng = false;
+  if (it.d >= 0) {
+    const int na = q.len[it.q], nb = db.len[it.d];
+    b = dp_bucket_of(dp_width_of(na, nb, m));
+    lng = (int64_t)na * nb >= long_cells.t[b];
+  }
   const int lane = threadIdx.x & 63;
-  // One atomic per wave and bucket (wave ballot + prefix count).
+  // One atomic per wave, bucket and end (wave ballot + prefix count).
   for (int bb = 0; bb < kDpBuckets; ++bb) {
-    const uint64_t bal = __ballot(b == bb);
-    if (bal == 0) continue;
-    const int leader = __ffsll((unsigned long long)bal) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&countsb[bb], __popcll(bal));
-    base = __shfl(base, leader);
-    if (b == bb) {
-      const int pos = base + __popcll(bal & ((1ull << lane) - 1ull));
-      if (pos < cap_each) outb[bb * cap_each + pos] = it;
+    for (int e = 0; e < 2; ++e) {
+      const bool mine = b == bb && lng == (e == 0);
+      const uint64_t bal = __ballot(mine);
+      if (bal == 0) continue;
+      const int leader = __ffsll((unsigned long long)bal) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(e == 0 ? &heads[bb] : &tails[bb], __popcll(bal));
+      base = __shfl(base, leader);
+      if (mine) {
+        const int64_t pos = base + __popcll(bal & ((1ull << lane) - 1ull));
+        if (pos < cap_each) outb[bb * cap_each + (e == 0 ? pos : cap_each - 1 - pos)] = it;
+      }
     }
   }
 }
 
 hipError_t launch_items_by_bucket(hipStream_t st, const Item* items, const int32_t* nitems, int64_t max_items,
                                   SetView q, SetView db, MeasureArgs m, Item* outb, int64_t cap_each,
-                                  int32_t* countsb) {
+                                  int32_t* heads, int32_t* tails, BucketCells long_cells) {
   if (max_items == 0) return hipSuccess;
   hipLaunchKernelGGL(k_items_by_bucket, dim3((unsigned)((max_items + 255) / 256)), dim3(256), 0, st, items, nitems,
-                     max_items, q, db, m, outb, cap_each, countsb);
+                     max_items, q, db, m, outb, cap_each, heads, tails, long_cells);
   return hipGetLastError();
 }
 

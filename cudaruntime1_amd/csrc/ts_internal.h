@@ -195,9 +195,14 @@ hipError_t launch_keys_from_regions(hipStream_t st, const float* vals, const int
 hipError_t launch_fill_i32(hipStream_t st, int32_t* p, int32_t v, int64_t n);
 // Split items by DP bucket (dp_bucket_of the visited row width,
 // dp_width_of) into kDpBuckets lists of cap_each entries.
+// Per-bucket work lists for the DP, longest pairs first: a pair of bucket b
+// with n*m >= long_cells.t[b] is appended at the front of list b (count
+// heads[b]), any other from the back (count tails[b], stored backwards from
+// the list's last slot), so a persistent DP launch starts the long pairs
+// first and the short ones fill in behind them.
 hipError_t launch_items_by_bucket(hipStream_t st, const Item* items, const int32_t* nitems, int64_t max_items,
                                   SetView q, SetView db, MeasureArgs m, Item* outb, int64_t cap_each,
-                                  int32_t* countsb);
+                                  int32_t* heads, int32_t* tails, struct BucketCells long_cells);
 // Exclusive prefix of counts into off[nq+1] on the device; *ntot = total
 // clamped to cap, segc[q] = region length clamped to cap, *need = max(*need,
 // total).
@@ -214,6 +219,10 @@ hipError_t launch_items_from_regions(hipStream_t st, const int32_t* region_d, co
 // lane-group shape (lanes per pair W, cells per lane step U) out of
 // kDpShapes; a tuning table picks the shape and grid per bucket.
 constexpr int kDpBuckets = 6;
+// Per-bucket n*m threshold of the DP work lists' long pairs (launch_items_by_bucket).
+struct BucketCells {
+  int64_t t[kDpBuckets];
+};
 constexpr int kDpShapes = 9;
 struct DpShape {
   int W, U;
@@ -237,8 +246,8 @@ struct DpHalo {
 DpHalo dp_halo(int shape, int b, const MeasureArgs& m, int32_t cmax_c, int32_t wmax_c);
 bool dp_fits(int shape, DpHalo halo);
 hipError_t launch_dp(hipStream_t st, MeasureArgs m, int shape, SetView q, SetView db, const Item* items,
-                     const int32_t* nitems, int64_t max_items, int32_t* next, float* out, DpHalo halo,
-                     int32_t rmax_c, int grid_waves);
+                     const int32_t* nitems, const int32_t* ntail, int64_t max_items, int32_t* next, float* out,
+                     DpHalo halo, int32_t rmax_c, int grid_waves);
 
 // reduce.hip -- Hausdorff (wave per pair) and lock-step Euclidean (lane per pair).
 hipError_t launch_hausdorff(hipStream_t st, SetView q, SetView db, const Item* items, const int32_t* nitems,
