@@ -141,6 +141,14 @@ __device__ __forceinline__ void sqrt_cr(float (&x)[U]) {
   }
 }
 
+// Band half-width in rows of a pair (R rows, C columns): band cells, or
+// -band percent of R when negative; a width of C or more admits every cell,
+// so it is clamped there (w*R then stays below 2^31).
+__device__ __forceinline__ int band_half_width(int band, int R, int C) {
+  const int w = band > 0 ? band : (-band * R + 99) / 100;
+  return min(w, C);
+}
+
 // LDS per lane group, in floats: the halo row; with STAGE >= 1 two buffers
 // each of the column trajectory's x and y (double-buffered per pair); with
 // STAGE == 2 one buffer each of the row trajectory's x and y (rmax floats).
@@ -152,8 +160,11 @@ __host__ __device__ constexpr int dp_group_floats(int cmax, int stage, int rmax)
 // of strip s (rows s*W .. s*W+W-1). DTW band: |row*C - col*R| <= bw, so row
 // r spans columns ceil((rC-bw)/R) .. floor((rC+bw)/R); LCSS window: r-delta
 // .. r+delta; otherwise the whole row.
+// With bw = wb*R the band's limits are ceil(r0*C/R) - wb and
+// floor(r1*C/R) + wb (wb an integer), and r*C < R*C < 2^31 (the host refuses
+// longer banded pairs), so 32-bit divisions give them.
 template <bool BAND, bool DELTA, int W, int U>
-__device__ __forceinline__ void strip_range(int s, int R, int C, int bw, int delta, int lastch, int& clo, int& chi) {
+__device__ __forceinline__ void strip_range(int s, int R, int C, int wb, int delta, int lastch, int& clo, int& chi) {
   if (!BAND && !DELTA) {
     clo = 0;
     chi = lastch;
@@ -162,9 +173,9 @@ __device__ __forceinline__ void strip_range(int s, int R, int C, int bw, int del
   const int r0 = s * W, r1 = min(s * W + W - 1, R - 1);
   int lo, hi;
   if (BAND) {
-    const int64_t a = (int64_t)r0 * C - bw;
-    lo = a <= 0 ? 0 : (int)((a + R - 1) / R);
-    hi = (int)min<int64_t>(C - 1, ((int64_t)r1 * C + bw) / R);
+    const uint32_t a0 = (uint32_t)r0 * (uint32_t)C, a1 = (uint32_t)r1 * (uint32_t)C;
+    lo = max(0, (int)((a0 + (uint32_t)R - 1u) / (uint32_t)R) - wb);
+    hi = min(C - 1, (int)(a1 / (uint32_t)R) + wb);
   } else {
     lo = max(0, r0 - delta);
     hi = min(C - 1, r1 + delta);
@@ -225,7 +236,8 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
   int fR = 1, fC = 1, f_slot = 0;
   bool f_ok = false, f_live = false;
   float f_ax = 0.0f, f_ay = 0.0f;
-  int bd = 0, bw = 0;
+  int bd = 0, bw = 0, wb = 0;
+  int f_rem = 0;  // steps of the fetched pair (ranged: summed over its strips by the group)
   const int delta = a.delta;
   float o[U], cx[U], cy[U], hv[U];
 #pragma unroll
@@ -283,6 +295,24 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
           }
         }
       }
+      if (RANGED) {
+        // the fetched pair's steps, its strips shared out over the group
+        const int R2 = f_live ? fR : 1, C2 = f_live ? fC : 1;
+        const int lastch2 = (C2 - 1) / U;
+        const int wb2 = BAND ? band_half_width(a.band, R2, C2) : 0;
+        const int S2 = DELTA ? min(R2 - 1, C2 - 1 + delta) / W + 1 : (R2 + W - 1) / W;
+        int part = 0;
+        for (int s2 = lg; s2 < S2; s2 += W) {
+          int clo, chi, nlo, nhi;
+          strip_range<BAND, DELTA, W, U>(s2, R2, C2, wb2, delta, lastch2, clo, chi);
+          nlo = clo;
+          if (s2 + 1 < S2) strip_range<BAND, DELTA, W, U>(s2 + 1, R2, C2, wb2, delta, lastch2, nlo, nhi);
+          part += max(chi - clo + 1, W + (nlo - clo));
+        }
+#pragma unroll
+        for (int sh = W / 2; sh > 0; sh >>= 1) part += __shfl_xor(part, sh);  // within the group
+        f_rem = part;
+      }
       gpar ^= 1;
     }
     // ---- this lane starts the pair its group's lane 0 fetched lg steps ago
@@ -310,7 +340,10 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
         lastch = (C - 1) / U;
         const int SW = ((R + W - 1) / W) * W;
         // band < 0: a half-width of -band percent of the longer length.
-        if (BAND) bw = (a.band > 0 ? a.band : (-a.band * R + 99) / 100) * R;
+        if (BAND) {
+          wb = band_half_width(a.band, R, C);
+          bw = wb * R;
+        }
         // The result cell: the corner, or for an LCSS window whose band
         // misses it, the band's last cell in the last column.
         res_col = C - 1;
@@ -321,23 +354,15 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
         // least one step earlier.
         S = DELTA ? res_row / W + 1 : SW / W;
         int clo0, chi0;
-        strip_range<BAND, DELTA, W, U>(0, R, C, bw, delta, lastch, clo0, chi0);
+        strip_range<BAND, DELTA, W, U>(0, R, C, wb, delta, lastch, clo0, chi0);
         if (RANGED) {
-          int total = 0, clo = clo0, chi = chi0;
-          for (int s2 = 0; s2 < S; ++s2) {
-            int nlo = clo, nhi = chi;
-            if (s2 + 1 < S) strip_range<BAND, DELTA, W, U>(s2 + 1, R, C, bw, delta, lastch, nlo, nhi);
-            total += max(chi - clo + 1, W + (nlo - clo));
-            clo = nlo;
-            chi = nhi;
-          }
-          rem = total;
+          rem = f_rem;
         } else {
           rem = S * max(lastch + 1, W);
         }
         strip = 0;
         int nlo = clo0, nhi = chi0;
-        if (S > 1) strip_range<BAND, DELTA, W, U>(1, R, C, bw, delta, lastch, nlo, nhi);
+        if (S > 1) strip_range<BAND, DELTA, W, U>(1, R, C, wb, delta, lastch, nlo, nhi);
         cstart = clo0;
         cend = clo0 + max(chi0 - clo0 + 1, W + (nlo - clo0));
         pclo = 0;
@@ -459,9 +484,9 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
       pclo = cstart;
       pcend = cend;
       int nlo = 0, nhi = 0, nnlo = 0, nnhi = 0;
-      strip_range<BAND, DELTA, W, U>(strip, R, C, bw, delta, lastch, nlo, nhi);
+      strip_range<BAND, DELTA, W, U>(strip, R, C, wb, delta, lastch, nlo, nhi);
       nnlo = nlo;
-      if (strip + 1 < S) strip_range<BAND, DELTA, W, U>(strip + 1, R, C, bw, delta, lastch, nnlo, nnhi);
+      if (strip + 1 < S) strip_range<BAND, DELTA, W, U>(strip + 1, R, C, wb, delta, lastch, nnlo, nnhi);
       cstart = nlo;
       cend = nlo + max(nhi - nlo + 1, W + (nnlo - nlo));
       ch = cstart;
