@@ -536,7 +536,7 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   const int32_t nch = (N + 2047) / 2048;
   const SetView qv = qs->view(), dv = db->view();
   const MeasureArgs ma = margs(p);
-  uint64_t *bkeys = nullptr, *seeds = nullptr, *skeys = nullptr, *stop = nullptr;
+  uint64_t *bkeys = nullptr, *seeds = nullptr;
   int32_t *seed_idx = nullptr, *counts = nullptr;
   int64_t* d_off = nullptr;
   Item* items = nullptr;
@@ -544,8 +544,6 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   TS_TRY(ensure(e, B_KEYS, (int64_t)nq * nch * s, &bkeys));
   TS_TRY(ensure(e, B_CAND, (int64_t)nq * s, &seeds));
   TS_TRY(ensure(e, B_CANDIDX, (int64_t)nq * s, &seed_idx));
-  TS_TRY(ensure(e, B_KEYS2, (int64_t)nq * s, &skeys));
-  TS_TRY(ensure(e, B_TOP, (int64_t)nq * k, &stop));
   TS_TRY(ensure(e, B_TAU, nq, &tau));
   TS_TRY(ensure(e, B_VALS, (int64_t)nq * s, &vals));
   TS_TRY(ensure(e, B_ITEMS, (int64_t)nq * s, &items));
@@ -560,9 +558,7 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   TS_TRY(run(e, "items", 0, 0, [&] { return launch_items_from_keys(e->st, seeds, nq, s, q0, items, seed_idx, vals); }));
   TS_TRY(refine(e, p, db, qs, items, nullptr, (int64_t)nq * s, vals, true));
   // 2. threshold: k-th best exact value among the seeds
-  TS_TRY(run(e, "keys", 0, 0, [&] { return launch_keys_from_regions(e->st, vals, seed_idx, nullptr, nq, s, desc, skeys); }));
-  TS_TRY(topk(e, skeys, s, s, nullptr, nq, k, stop));
-  TS_TRY(run(e, "threshold", 0, 0, [&] { return launch_kth_threshold(e->st, stop, nq, k, k, desc, tau); }));
+  TS_TRY(run(e, "threshold", 0, 0, [&] { return launch_seed_tau(e->st, vals, seed_idx, nq, s, k, desc, tau); }));
   // 3. compaction of every pair that can still reach tau: count, size each
   // query's region, write. Eager calls size the regions exactly from the
   // counts read back to the host; graph-captured calls cannot read back, so
@@ -582,7 +578,9 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   int32_t *segc = nullptr, *ntot = nullptr;
   if (dev_sized) {
     total = cap = e->exact_cap;
-    seg_len = N;
+    // a region holds at most N candidates, and at most the capacity (the
+    // scan clamps region lengths to it)
+    seg_len = (int32_t)std::min<int64_t>(N, cap);
     TS_TRY(ensure(e, B_SEGC, nq + 2, &segc));
     ntot = segc + nq;
     TS_TRY(run(e, "scan", 0, 0, [&] {

@@ -95,6 +95,36 @@ __global__ void __launch_bounds__(256) k_topk_stage(const uint64_t* in, int64_t 
   }
 }
 
+// Seed threshold, one block per query: keys of the query's s refined seed
+// values (index -1 or a non-finite value: empty), sorted in LDS; tau is the
+// k-th best value, or +-inf when fewer than k seeds have one.
+__global__ void __launch_bounds__(256) k_seed_tau(const float* vals, const int32_t* seed_idx, int32_t s, int32_t k,
+                                                  int32_t cs, int32_t desc, float* tau) {
+  __shared__ uint64_t sk[kSelChunk];
+  const int q = blockIdx.x;
+  for (int i = threadIdx.x; i < cs; i += 256)
+    sk[i] = i < s ? key_of(vals[(int64_t)q * s + i], seed_idx[(int64_t)q * s + i], desc != 0) : ~0ull;
+  __syncthreads();
+  for (int kk = 2; kk <= cs; kk <<= 1) {
+    for (int j = kk >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < cs; i += 256) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const uint64_t a = sk[i], b = sk[ixj];
+          const bool asc = (i & kk) == 0;
+          if ((a > b) == asc) { sk[i] = b; sk[ixj] = a; }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (threadIdx.x == 0) {
+    const uint64_t key = sk[k - 1];
+    const float inf = __int_as_float(0x7f800000);
+    tau[q] = key == ~0ull ? (desc ? -inf : inf) : value_of(key, desc != 0);
+  }
+}
+
 static int chunk_for(int32_t len) {
   int ch = 64;
   while (ch < len && ch < kSelChunk) ch <<= 1;
@@ -167,6 +197,15 @@ __global__ void k_kth(const uint64_t* keys, int32_t nseg, int32_t s, int32_t k, 
   uint64_t key = keys[(int64_t)i * s + (k - 1)];
   float inf = __int_as_float(0x7f800000);
   tau[i] = key == ~0ull ? (desc ? -inf : inf) : value_of(key, desc != 0);
+}
+
+hipError_t launch_seed_tau(hipStream_t st, const float* vals, const int32_t* seed_idx, int32_t nq, int32_t s,
+                           int32_t k, int32_t descending, float* tau) {
+  if (nq == 0) return hipSuccess;
+  if (s > kSelChunk || k < 1 || k > s) return hipErrorInvalidValue;
+  const int cs = chunk_for(s);
+  hipLaunchKernelGGL(k_seed_tau, dim3(nq), dim3(256), 0, st, vals, seed_idx, s, k, cs, descending, tau);
+  return hipGetLastError();
 }
 
 hipError_t launch_kth_threshold(hipStream_t st, const uint64_t* keys, int32_t nseg, int32_t s, int32_t k,
