@@ -556,7 +556,16 @@ __device__ __forceinline__ void stage_pts(float2* dst, const float* gx, const fl
   }
 }
 
-template <int MEAS, int W, int U>
+// LDS per lane group of k_dp_lean, in floats: the halo row, with SC the two
+// column buffers of (x, y) points, with SR the row buffer.
+__host__ __device__ constexpr int lean_group_floats(int cmax, int rmax, bool sc, bool sr) {
+  return cmax + (sc ? 4 * cmax : 0) + (sr ? 2 * rmax : 0);
+}
+
+// SC / SR: columns / rows staged in LDS; otherwise read from global memory
+// (long trajectories: the launch then needs several waves per SIMD to hide
+// the reads, which the throughput regime has).
+template <int MEAS, int W, int U, bool SC, bool SR>
 __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
   constexpr int G = 64 / W;
   constexpr float BND = (MEAS == TS_LCSS) ? 0.0f : kInf;
@@ -570,11 +579,11 @@ __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
   const int hdr = ((nwaves * G + 3) >> 2) << 2;
   int* slot_item = (int*)lds_dyn4 + wv * G + g;
   float* brow = (float*)lds_dyn4 + hdr + wv * 8;
-  float* gbase = (float*)lds_dyn4 + hdr + nwaves * 8 + (size_t)(wv * G + g) * dp_group_floats(cmax, 2, rmax);
+  float* gbase = (float*)lds_dyn4 + hdr + nwaves * 8 + (size_t)(wv * G + g) * lean_group_floats(cmax, rmax, SC, SR);
   if (lane < 8) brow[lane] = BND;
-  float* bnd = gbase;                           // [cmax] halo row
-  float2* cols = (float2*)(gbase + cmax);       // [2][cmax] column points
-  float2* rows = (float2*)(gbase + 5 * cmax);   // [rmax] row points
+  float* bnd = gbase;                                         // [cmax] halo row
+  float2* cols = (float2*)(gbase + cmax);                     // SC: [2][cmax] column points
+  float2* rows = (float2*)(gbase + (SC ? 5 * cmax : cmax));   // SR: [rmax] row points
 
   int64_t n_items = a.max_items;
   if (a.nitems) n_items = min<int64_t>(n_items, (int64_t)*a.nitems);
@@ -585,10 +594,13 @@ __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
   // on chunks 0..lg-1 (D = W+1, so k does not wrap).
   int R = 1, C = 1, lastch = 0, D = W + 1, k = 0, row = lg, rem = lg, res_rem = kNoResult, slot = 0;
   int gpar = 0;
-  const float2* cb = cols;  // this lane's column buffer
+  const float2* cb = cols;  // SC: this lane's column buffer
+  // !SC / !SR: the pair's column / row trajectory in global memory
+  const float *Bx = a.q.x, *By = a.q.y, *Ax = a.q.x, *Ay = a.q.y;
   bool done = false, wrapped = false;
   // The group's next pair, read once by the whole group at the fetch.
   int fR = 1, fC = 1, f_slot = 0;
+  const float *fAx = a.q.x, *fAy = a.q.y, *fBx = a.q.x, *fBy = a.q.y;
   bool f_ok = false, f_live = false;
   float o[U], cx[U], cy[U], hv[U];
 #pragma unroll
@@ -627,15 +639,14 @@ __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
           if (f_live) {
             const int n = a.q.len[it.q], m = a.db.len[it.d];
             const int64_t qo = a.q.off[it.q], dof = a.db.off[it.d];
-            const float *Ax, *Ay, *Bx, *By;
             if (n >= m) {
-              Ax = a.q.x + qo; Ay = a.q.y + qo; Bx = a.db.x + dof; By = a.db.y + dof; fR = n; fC = m;
+              fAx = a.q.x + qo; fAy = a.q.y + qo; fBx = a.db.x + dof; fBy = a.db.y + dof; fR = n; fC = m;
             } else {
-              Ax = a.db.x + dof; Ay = a.db.y + dof; Bx = a.q.x + qo; By = a.q.y + qo; fR = m; fC = n;
+              fAx = a.db.x + dof; fAy = a.db.y + dof; fBx = a.q.x + qo; fBy = a.q.y + qo; fR = m; fC = n;
             }
             f_slot = it.slot;
-            stage_pts<W>(cols + gpar * cmax, Bx, By, fC, lg);
-            stage_pts<W>(rows, Ax, Ay, fR, lg);
+            if (SC) stage_pts<W>(cols + gpar * cmax, fBx, fBy, fC, lg);
+            if (SR) stage_pts<W>(rows, fAx, fAy, fR, lg);
           }
         }
         gpar ^= 1;
@@ -660,11 +671,24 @@ __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
           k = 0;
           row = lg;
           wrapped = false;
-          cb = cols + (gpar ^ 1) * cmax;
-          const float2 p0 = rows[min(lg, R - 1)];
-          ax = p0.x;
-          ay = p0.y;
-          load_pts<U>(cb, cx, cy);
+          if (f_live) {
+            Ax = fAx; Ay = fAy; Bx = fBx; By = fBy;
+          }
+          if (SC) {
+            cb = cols + (gpar ^ 1) * cmax;
+            load_pts<U>(cb, cx, cy);
+          } else {
+            load_vec<U>(Bx, cx);
+            load_vec<U>(By, cy);
+          }
+          if (SR) {
+            const float2 p0 = rows[min(lg, R - 1)];
+            ax = p0.x;
+            ay = p0.y;
+          } else {
+            ax = Ax[min(lg, R - 1)];
+            ay = Ay[min(lg, R - 1)];
+          }
           load_vec<U>(brow, hv);
           dfirst = (MEAS == TS_LCSS || lg == 0) ? 0.0f : BND;
         }
@@ -719,8 +743,18 @@ __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
     wrapped = k + 1 == D;
     k = wrapped ? 0 : k + 1;
     row = wrapped ? row + W : row;
-    rp = rows[min(row, R - 1)];
-    load_pts<U>(cb + k * U, cx, cy);
+    if (SR) {
+      rp = rows[min(row, R - 1)];
+    } else {
+      rp = make_float2(Ax[min(row, R - 1)], Ay[min(row, R - 1)]);
+    }
+    if (SC) {
+      load_pts<U>(cb + k * U, cx, cy);
+    } else {
+      // idle chunks re-read the last one (sets pad trajectories to 8 points)
+      load_vec<U>(Bx + U * min(k, lastch), cx);
+      load_vec<U>(By + U * min(k, lastch), cy);
+    }
     const bool top = row == 0;
     load_vec<U>(top ? brow : bnd + k * U, hv);
     dfirst = (MEAS == TS_LCSS || top) ? 0.0f : BND;
@@ -753,21 +787,26 @@ DpHalo dp_halo(int shape, int b, const MeasureArgs& m, int32_t cmax_c, int32_t w
   return DpHalo{h, h - 1};
 }
 
-template <int MEAS, int W, int U, bool BAND, bool DELTA, int STAGE, bool LEAN = false>
+// Launches k_dp (LEAN 0, with STAGE) or k_dp_lean (LEAN 1: both trajectories
+// staged, 2: columns staged and rows from global memory, 3: both from global
+// memory) as a persistent grid of grid_waves waves, blocks of up to 4 waves
+// whose group areas fit in 64 KiB of LDS.
+template <int MEAS, int W, int U, bool BAND, bool DELTA, int STAGE, int LEAN = 0>
 static hipError_t launch_dp_t(hipStream_t st, const DpArgs& a, int grid_waves) {
   constexpr int G = 64 / W;
-  // Waves per block bounded so that the group areas fit in 64 KiB of LDS.
-  const size_t per_wave = (size_t)G * dp_group_floats(a.cmax, STAGE, a.rmax) * sizeof(float) + G * sizeof(int) + 32;
+  const size_t gf = LEAN == 0 ? (size_t)dp_group_floats(a.cmax, STAGE, a.rmax)
+                              : (size_t)lean_group_floats(a.cmax, a.rmax, LEAN != 3, LEAN == 1);
+  const size_t per_wave = (size_t)G * gf * sizeof(float) + G * sizeof(int) + 32;
   int wpb = 4;
   while (wpb > 1 && per_wave * wpb + 64 > 65536) --wpb;
   if (per_wave + 64 > 65536) return hipErrorInvalidValue;
   const size_t lds = (size_t)(((wpb * G + 3) >> 2) << 2) * sizeof(int) + (size_t)wpb * 8 * sizeof(float) +
-                     (size_t)wpb * G * dp_group_floats(a.cmax, STAGE, a.rmax) * sizeof(float);
+                     (size_t)wpb * G * gf * sizeof(float);
   int blocks = (grid_waves + wpb - 1) / wpb;
   if (blocks < 1) blocks = 1;
-  if constexpr (LEAN) {
-    static_assert(STAGE == 2 && !BAND && !DELTA, "the lean kernel stages both trajectories, unbanded");
-    hipLaunchKernelGGL((k_dp_lean<MEAS, W, U>), dim3(blocks), dim3(64 * wpb), lds, st, a);
+  if constexpr (LEAN != 0) {
+    static_assert(!BAND && !DELTA, "the lean kernel is unbanded");
+    hipLaunchKernelGGL((k_dp_lean<MEAS, W, U, LEAN != 3, LEAN == 1>), dim3(blocks), dim3(64 * wpb), lds, st, a);
   } else {
     hipLaunchKernelGGL((k_dp<MEAS, W, U, BAND, DELTA, STAGE>), dim3(blocks), dim3(64 * wpb), lds, st, a);
   }
@@ -782,24 +821,22 @@ bool dp_fits(int shape, DpHalo halo) {
   return (size_t)G * dp_group_floats(halo.len, 0, 0) * sizeof(float) + G * sizeof(int) + 32 + 64 <= 65536;
 }
 
-// Staging the column trajectory in LDS pays while it leaves room for 8
-// waves per CU (20 KiB per wave), and the row trajectory too when both fit
-// that; banded rows never stage (a pair's column trajectory can be far
-// longer than the band the LDS holds).
+// Unbanded pairs run k_dp_lean, staging in LDS what fits in 20 KiB per wave
+// (room for 8 waves per CU): both trajectories, else the columns, else
+// neither. Bands and windows run k_dp, columns from global memory (a pair's
+// column trajectory can be far longer than the band the LDS holds).
 template <int MEAS, int W, int U, bool BAND, bool DELTA>
 static hipError_t launch_dp_s(hipStream_t st, DpArgs a, int grid_waves) {
   constexpr int G = 64 / W;
   if constexpr (BAND || DELTA) {
     return launch_dp_t<MEAS, W, U, BAND, DELTA, 0>(st, a, grid_waves);
   } else {
-    if ((size_t)G * dp_group_floats(a.cmax, 2, a.rmax) * sizeof(float) <= 20480)
-      return launch_dp_t<MEAS, W, U, BAND, DELTA, 2, true>(st, a, grid_waves);
-    if ((size_t)G * dp_group_floats(a.cmax, 1, 0) * sizeof(float) <= 20480) {
-      a.rmax = 0;
-      return launch_dp_t<MEAS, W, U, BAND, DELTA, 1>(st, a, grid_waves);
-    }
+    if ((size_t)G * lean_group_floats(a.cmax, a.rmax, true, true) * sizeof(float) <= 20480)
+      return launch_dp_t<MEAS, W, U, BAND, DELTA, 0, 1>(st, a, grid_waves);
     a.rmax = 0;
-    return launch_dp_t<MEAS, W, U, BAND, DELTA, 0>(st, a, grid_waves);
+    if ((size_t)G * lean_group_floats(a.cmax, 0, true, false) * sizeof(float) <= 20480)
+      return launch_dp_t<MEAS, W, U, BAND, DELTA, 0, 2>(st, a, grid_waves);
+    return launch_dp_t<MEAS, W, U, BAND, DELTA, 0, 3>(st, a, grid_waves);
   }
 }
 

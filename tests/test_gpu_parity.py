@@ -132,6 +132,23 @@ def test_long_pairs_bit_exact(eng):
         assert np.array_equal(got.view(np.int32), want.view(np.int32)), (measure, got, want)
 
 
+def test_long_rows_short_columns_bit_exact(eng):
+    """Rows too long to stage beside the columns (the unbanded DP then reads
+    row points from global memory), and rows and columns both too long."""
+    say("long rows")
+    db_xy, db_off = walks(28, [200, 150, 3000, 1100])
+    q_xy, q_off = walks(29, [2500, 250, 40])
+    db, qs = ts.TrajSet(db_xy, db_off), ts.TrajSet(q_xy, q_off)
+    qi = np.array([0, 0, 1, 1, 2, 0, 1])
+    di = np.array([0, 1, 2, 3, 2, 3, 0])
+    for measure in ("dtw", "frechet", "lcss"):
+        p = ts.params(measure, epsilon=2e-3)
+        prm = oracle.params(p.measure, epsilon=p.epsilon)
+        got = eng.pair_values(db, qs, p, qi, di)
+        want = oracle.pair_values(prm, db_xy, db_off, q_xy, q_off, qi, di)
+        assert np.array_equal(got.view(np.int32), want.view(np.int32)), (measure, got, want)
+
+
 @pytest.mark.parametrize("measure", ["dtw", "frechet", "lcss", "hausdorff", "euclidean"])
 def test_filter_ref_bit_exact(eng, measure):
     say("filter_ref", measure)
