@@ -22,19 +22,14 @@
 //    previous step; the first cell's diagonal is the last value that arrived
 //    on the step before. Lane 0 of a group reads the row above from an LDS
 //    halo row written by lane W-1 during the previous strip of W rows.
-//  * Columns run along the shorter trajectory, U per step. When a group's
-//    lane 0 starts a pair, the group copies that pair's column trajectory
-//    into one of two LDS buffers (sets pad every trajectory to a multiple of
-//    4 points, so the copy is aligned 16-byte vectors); per-step operands
-//    are then LDS reads. Rows of 1024+ columns read global memory instead.
-//    When it fits as well, the group also copies the row trajectory into
-//    one LDS buffer (a lane reads its row point once per strip, and every
-//    lane has read its last one before lane 0 starts the next pair), so the
-//    steady state issues no global loads: lanes start rows on consecutive
-//    steps, and a global load per row start would cost a memory round trip
-//    on every step.
-//    A row has at least W chunks so the halo is always written at least one
-//    step before it is read; shorter rows are padded with idle chunks.
+//  * Columns run along the shorter trajectory, U per step; a row has at
+//    least W (k_dp) or W+1 (k_dp_lean) chunks so the halo is written before
+//    it is read; shorter rows are padded with idle chunks.
+//  * Two kernels. k_dp_lean runs unbanded pairs with a per-step path cut to
+//    the cells and straight-line bookkeeping, staging the pair's columns and
+//    rows in LDS when they fit (see above k_dp_lean). k_dp runs DTW bands and
+//    LCSS windows: each strip visits only its band's chunk range, the halo is
+//    an LDS ring, columns come from global memory.
 //  * Strips and pairs follow each other without draining the pipeline: after
 //    its last row a lane starts its first row of the next pair, which the
 //    group's lane 0 fetched from a device-wide atomic work counter. The
@@ -60,7 +55,7 @@ struct DpArgs {
   float* out;
   int32_t cmax;   // LDS halo floats per group (and staged column capacity)
   int32_t hmask;  // halo index mask: -1, or ring length - 1
-  int32_t rmax;   // staged row buffer floats per coordinate (STAGE == 2)
+  int32_t rmax;   // k_dp_lean: staged row buffer floats per coordinate
   int32_t band;
   int32_t delta;
   float eps;
@@ -149,12 +144,8 @@ __device__ __forceinline__ int band_half_width(int band, int R, int C) {
   return min(w, C);
 }
 
-// LDS per lane group, in floats: the halo row; with STAGE >= 1 two buffers
-// each of the column trajectory's x and y (double-buffered per pair); with
-// STAGE == 2 one buffer each of the row trajectory's x and y (rmax floats).
-__host__ __device__ constexpr int dp_group_floats(int cmax, int stage, int rmax) {
-  return stage == 0 ? cmax : 5 * cmax + (stage == 2 ? 2 * rmax : 0);
-}
+// LDS per lane group of k_dp, in floats: the halo row (or ring).
+__host__ __device__ constexpr int dp_group_floats(int cmax) { return cmax; }
 
 // Chunk range [clo, chi] (chunks of U columns) that holds every in-band cell
 // of strip s (rows s*W .. s*W+W-1). DTW band: |row*C - col*R| <= bw, so row
@@ -184,11 +175,11 @@ __device__ __forceinline__ void strip_range(int s, int R, int C, int wb, int del
   chi = hi / U;
 }
 
-template <int MEAS, int W, int U, bool BAND, bool DELTA, int STAGE>
+template <int MEAS, int W, int U, bool BAND, bool DELTA>
 __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
+  static_assert(BAND || DELTA, "unbanded pairs run k_dp_lean");
   constexpr int G = 64 / W;
   constexpr float BND = (MEAS == TS_LCSS) ? 0.0f : kInf;
-  constexpr bool RANGED = BAND || DELTA;
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int nwaves = blockDim.x >> 6;
@@ -197,7 +188,7 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
   // Halo slot of a column: with a band the halo is a ring that holds one
   // strip's range (its length is at least the longest range plus a chunk,
   // so a slot is rewritten only after lane 0 of the next strip read it).
-  const int hmask = RANGED ? a.hmask : -1;
+  const int hmask = a.hmask;
   extern __shared__ float4 lds_dyn4[];
   // LDS: the groups' fetched item slots, one border row of 8 values per wave
   // (what lane 0 reads above row 0 and outside a band), then the group
@@ -205,13 +196,9 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
   const int hdr = ((nwaves * G + 3) >> 2) << 2;
   int* slot_item = (int*)lds_dyn4 + wv * G + g;
   float* brow = (float*)lds_dyn4 + hdr + wv * 8;
-  float* gbase = (float*)lds_dyn4 + hdr + nwaves * 8 + (size_t)(wv * G + g) * dp_group_floats(cmax, STAGE, a.rmax);
+  float* gbase = (float*)lds_dyn4 + hdr + nwaves * 8 + (size_t)(wv * G + g) * dp_group_floats(cmax);
   if (lane < 8) brow[lane] = BND;
   float* bnd = gbase;
-  float* sbx = gbase + cmax;      // [2][cmax]
-  float* sby = gbase + 3 * cmax;  // [2][cmax]
-  float* sax = gbase + 5 * cmax;  // [rmax], STAGE == 2
-  float* say = sax + a.rmax;      // [rmax]
 
   int64_t n_items = a.max_items;
   if (a.nitems) n_items = min<int64_t>(n_items, (int64_t)*a.nitems);
@@ -228,7 +215,6 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
   // reads its halo only inside it).
   int strip = 0, cstart = 0, cend = W, pclo = 0, pcend = 0;
   int res_row = 0, res_col = 0;
-  int par = 0, gpar = 0;  // buffer of this lane's pair / of the group's next pair
   const float *Ax = a.q.x, *Ay = a.q.y, *Bx = a.q.x, *By = a.q.y;
   float ax = 0.0f, ay = 0.0f, nax = 0.0f, nay = 0.0f;
   // descriptor of the group's next pair (see the fetch below)
@@ -248,11 +234,7 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
     // LDS is shared by the lanes of this wave only; the barrier keeps the
     // compiler from moving LDS accesses across steps.
     __builtin_amdgcn_wave_barrier();
-    // ---- the group's lane 0 starts a pair: fetch it, and with STAGE load its
-    // column trajectory into the group's spare buffer. Lanes still on the
-    // previous pair read the other buffer; the one written here was last
-    // used two pairs ago, which every lane has finished (a pair lasts at
-    // least W steps and lanes trail lane 0 by at most W-1).
+    // ---- the group's lane 0 starts a pair: fetch it.
     // The whole group reads the pair's descriptor here, once, and keeps it
     // in registers (f*) until each lane starts the pair: lanes start on
     // consecutive steps, so per-lane reads would stall W steps in a row on
@@ -276,26 +258,12 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
             fAx = a.db.x + dof; fAy = a.db.y + dof; fBx = a.q.x + qo; fBy = a.q.y + qo; fR = m; fC = n;
           }
           f_slot = it.slot;
-          if (STAGE == 2) {
-            for (int c = lg; c < ((fR + 3) >> 2); c += W) {
-              *(float4*)(sax + 4 * c) = *(const float4*)(fAx + 4 * c);
-              *(float4*)(say + 4 * c) = *(const float4*)(fAy + 4 * c);
-            }
-          } else {
-            // this lane's first row point
-            f_ax = fAx[min(lg, fR - 1)];
-            f_ay = fAy[min(lg, fR - 1)];
-          }
-          if (STAGE) {
-            const int c4 = (fC + 3) >> 2;
-            for (int c = lg; c < c4; c += W) {
-              *(float4*)(sbx + gpar * cmax + 4 * c) = *(const float4*)(fBx + 4 * c);
-              *(float4*)(sby + gpar * cmax + 4 * c) = *(const float4*)(fBy + 4 * c);
-            }
-          }
+          // this lane's first row point
+          f_ax = fAx[min(lg, fR - 1)];
+          f_ay = fAy[min(lg, fR - 1)];
         }
       }
-      if (RANGED) {
+      {
         // the fetched pair's steps, its strips shared out over the group
         const int R2 = f_live ? fR : 1, C2 = f_live ? fC : 1;
         const int lastch2 = (C2 - 1) / U;
@@ -313,7 +281,6 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
         for (int sh = W / 2; sh > 0; sh >>= 1) part += __shfl_xor(part, sh);  // within the group
         f_rem = part;
       }
-      gpar ^= 1;
     }
     // ---- this lane starts the pair its group's lane 0 fetched lg steps ago
     if (!done && rem == 0) {
@@ -324,21 +291,14 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
         if (live) {
           Ax = fAx; Ay = fAy; Bx = fBx; By = fBy; R = fR; C = fC;
           slot = f_slot;
-          if (STAGE == 2) {
-            nax = sax[min(lg, R - 1)];
-            nay = say[min(lg, R - 1)];
-          } else {
-            nax = f_ax;
-            nay = f_ay;
-          }
+          nax = f_ax;
+          nay = f_ay;
         } else {
           R = 1; C = 1;  // a dead item: computed on stale operands, never stored
           nax = 0.0f;
           nay = 0.0f;
         }
-        par = gpar ^ 1;
         lastch = (C - 1) / U;
-        const int SW = ((R + W - 1) / W) * W;
         // band < 0: a half-width of -band percent of the longer length.
         if (BAND) {
           wb = band_half_width(a.band, R, C);
@@ -352,14 +312,10 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
         // its steps: every strip lasts at least W steps plus the shift of the
         // next strip's range, so the halo chunk a lane 0 needs was written at
         // least one step earlier.
-        S = DELTA ? res_row / W + 1 : SW / W;
+        S = DELTA ? res_row / W + 1 : (R + W - 1) / W;
         int clo0, chi0;
         strip_range<BAND, DELTA, W, U>(0, R, C, wb, delta, lastch, clo0, chi0);
-        if (RANGED) {
-          rem = f_rem;
-        } else {
-          rem = S * max(lastch + 1, W);
-        }
+        rem = f_rem;
         strip = 0;
         int nlo = clo0, nhi = chi0;
         if (S > 1) strip_range<BAND, DELTA, W, U>(1, R, C, wb, delta, lastch, nlo, nhi);
@@ -370,13 +326,8 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
         row = lg;
         ch = cstart;
         const int lc0 = U * min(cstart, lastch);
-        if (STAGE) {
-          load_vec<U>(sbx + par * cmax + lc0, cx);
-          load_vec<U>(sby + par * cmax + lc0, cy);
-        } else {
-          load_vec<U>(Bx + lc0, cx);
-          load_vec<U>(By + lc0, cy);
-        }
+        load_vec<U>(Bx + lc0, cx);
+        load_vec<U>(By + lc0, cy);
         load_vec<U>(brow, hv);  // lane 0 starts on row 0: the border row above it
       }
     }
@@ -387,20 +338,13 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
       ax = nax;
       ay = nay;
       const int nr = min(row + W, R - 1);
-      if (STAGE == 2) {
-        nax = sax[nr];
-        nay = say[nr];
-      } else {
-        nax = Ax[nr];
-        nay = Ay[nr];
-      }
+      nax = Ax[nr];
+      nay = Ay[nr];
       if (BAND) bd = row * C - cstart * U * R;
       // Lane 0's diagonal for the strip's first cell: the halo row's value
       // left of the range when the previous strip computed it.
-      if (RANGED) {
-        const int dc = cstart * U - 1;
-        hdiag = (dc >= 0 && dc >= pclo * U && dc < pcend * U && dc < C) ? bnd[dc & hmask] : BND;
-      }
+      const int dc = cstart * U - 1;
+      hdiag = (dc >= 0 && dc >= pclo * U && dc < pcend * U && dc < C) ? bnd[dc & hmask] : BND;
     }
 
     // ---- U cells
@@ -408,15 +352,9 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
     const bool first = ch == cstart;
 #pragma unroll
     for (int j = 0; j < U; ++j) up[j] = from_prev_lane<W>(o[j], hv[j], lg);
-    float d, l;
-    if (RANGED) {
-      d = first ? ((row == 0) ? (cstart == 0 ? 0.0f : BND) : (lg == 0 ? hdiag : BND)) : up_last;
-      if (MEAS == TS_LCSS && first && cstart == 0) d = 0.0f;
-      l = first ? BND : o[U - 1];
-    } else {
-      d = first ? ((row == 0) ? 0.0f : BND) : up_last;
-      l = first ? BND : o[U - 1];
-    }
+    float d = first ? ((row == 0) ? (cstart == 0 ? 0.0f : BND) : (lg == 0 ? hdiag : BND)) : up_last;
+    if (MEAS == TS_LCSS && first && cstart == 0) d = 0.0f;
+    float l = first ? BND : o[U - 1];
     float dist[U];
     if (MEAS == TS_DTW) {
 #pragma unroll
@@ -491,22 +429,16 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
       cend = nlo + max(nhi - nlo + 1, W + (nnlo - nlo));
       ch = cstart;
     }
-    // ---- operands of the next step: columns of chunk ch (from the staged
-    // buffer, or from global memory for the longest rows), and the halo
-    // chunk ch, written by lane W-1 at least one step ago or earlier in this
-    // step, which precedes this read in program order.
+    // ---- operands of the next step: columns of chunk ch (global memory),
+    // and the halo chunk ch, written by lane W-1 at least one step ago or
+    // earlier in this step, which precedes this read in program order.
     const int lc = U * min(max(ch, 0), lastch);
-    if (STAGE) {
-      load_vec<U>(sbx + par * cmax + lc, cx);
-      load_vec<U>(sby + par * cmax + lc, cy);
-    } else {
-      load_vec<U>(Bx + lc, cx);
-      load_vec<U>(By + lc, cy);
-    }
+    load_vec<U>(Bx + lc, cx);
+    load_vec<U>(By + lc, cy);
     // Lane 0 above row 0, or above the previous strip's range (DTW cells
     // there are out of band, i.e. infinite; an in-window LCSS cell there
     // reads its diagonal instead): the border row.
-    const bool border = row == 0 || (RANGED && ch >= pcend);
+    const bool border = row == 0 || ch >= pcend;
     load_vec<U>(border ? brow : bnd + (lc & hmask), hv);
   }
 }
@@ -787,14 +719,14 @@ DpHalo dp_halo(int shape, int b, const MeasureArgs& m, int32_t cmax_c, int32_t w
   return DpHalo{h, h - 1};
 }
 
-// Launches k_dp (LEAN 0, with STAGE) or k_dp_lean (LEAN 1: both trajectories
+// Launches k_dp (LEAN 0: bands and windows) or k_dp_lean (LEAN 1: both trajectories
 // staged, 2: columns staged and rows from global memory, 3: both from global
 // memory) as a persistent grid of grid_waves waves, blocks of up to 4 waves
 // whose group areas fit in 64 KiB of LDS.
-template <int MEAS, int W, int U, bool BAND, bool DELTA, int STAGE, int LEAN = 0>
+template <int MEAS, int W, int U, bool BAND, bool DELTA, int LEAN>
 static hipError_t launch_dp_t(hipStream_t st, const DpArgs& a, int grid_waves) {
   constexpr int G = 64 / W;
-  const size_t gf = LEAN == 0 ? (size_t)dp_group_floats(a.cmax, STAGE, a.rmax)
+  const size_t gf = LEAN == 0 ? (size_t)dp_group_floats(a.cmax)
                               : (size_t)lean_group_floats(a.cmax, a.rmax, LEAN != 3, LEAN == 1);
   const size_t per_wave = (size_t)G * gf * sizeof(float) + G * sizeof(int) + 32;
   int wpb = 4;
@@ -808,7 +740,7 @@ static hipError_t launch_dp_t(hipStream_t st, const DpArgs& a, int grid_waves) {
     static_assert(!BAND && !DELTA, "the lean kernel is unbanded");
     hipLaunchKernelGGL((k_dp_lean<MEAS, W, U, LEAN != 3, LEAN == 1>), dim3(blocks), dim3(64 * wpb), lds, st, a);
   } else {
-    hipLaunchKernelGGL((k_dp<MEAS, W, U, BAND, DELTA, STAGE>), dim3(blocks), dim3(64 * wpb), lds, st, a);
+    hipLaunchKernelGGL((k_dp<MEAS, W, U, BAND, DELTA>), dim3(blocks), dim3(64 * wpb), lds, st, a);
   }
   return hipGetLastError();
 }
@@ -818,7 +750,7 @@ static hipError_t launch_dp_t(hipStream_t st, const DpArgs& a, int grid_waves) {
 // in 20 KiB per wave).
 bool dp_fits(int shape, DpHalo halo) {
   const int G = 64 / kDpShapeList[shape].W;
-  return (size_t)G * dp_group_floats(halo.len, 0, 0) * sizeof(float) + G * sizeof(int) + 32 + 64 <= 65536;
+  return (size_t)G * dp_group_floats(halo.len) * sizeof(float) + G * sizeof(int) + 32 + 64 <= 65536;
 }
 
 // Unbanded pairs run k_dp_lean, staging in LDS what fits in 20 KiB per wave
@@ -832,11 +764,11 @@ static hipError_t launch_dp_s(hipStream_t st, DpArgs a, int grid_waves) {
     return launch_dp_t<MEAS, W, U, BAND, DELTA, 0>(st, a, grid_waves);
   } else {
     if ((size_t)G * lean_group_floats(a.cmax, a.rmax, true, true) * sizeof(float) <= 20480)
-      return launch_dp_t<MEAS, W, U, BAND, DELTA, 0, 1>(st, a, grid_waves);
+      return launch_dp_t<MEAS, W, U, BAND, DELTA, 1>(st, a, grid_waves);
     a.rmax = 0;
     if ((size_t)G * lean_group_floats(a.cmax, 0, true, false) * sizeof(float) <= 20480)
-      return launch_dp_t<MEAS, W, U, BAND, DELTA, 0, 2>(st, a, grid_waves);
-    return launch_dp_t<MEAS, W, U, BAND, DELTA, 0, 3>(st, a, grid_waves);
+      return launch_dp_t<MEAS, W, U, BAND, DELTA, 2>(st, a, grid_waves);
+    return launch_dp_t<MEAS, W, U, BAND, DELTA, 3>(st, a, grid_waves);
   }
 }
 
