@@ -56,6 +56,8 @@ struct DpArgs {
   int32_t cmax;   // LDS halo floats per group (and staged column capacity)
   int32_t hmask;  // halo index mask: -1, or ring length - 1
   int32_t rmax;   // k_dp_lean: staged row buffer floats per coordinate
+  int32_t ccap;   // k_dp_lean_r: staged column capacity in points (the halo is a ring)
+  int32_t smax;   // k_dp_lean_r: strip table entries
   int32_t band;
   int32_t delta;
   float eps;
@@ -695,6 +697,272 @@ __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
   }
 }
 
+// Banded DTW and windowed LCSS with the lean step. The strips' chunk ranges
+// come from a table the group writes at the fetch (the same strip_range as
+// k_dp, each strip lasting max(range, W + shift + 1) steps, so the early
+// halo read of chunk k of strip s+1 and lane 0's diagonal at a strip's
+// start follow lane W-1's write of them in strip s); a lane reads its next
+// strip's entry every step and takes it on a wrap. The table, like the row
+// buffer, is single-buffered: every lane has read its last entry before
+// lane 0 refills it. Band cells, window cells and the halo ring as in k_dp;
+// the halo chunk of a step is its chunk clamped to the row's last one.
+// ST: both trajectories staged in LDS (short pairs), else both read from
+// global memory.
+__host__ __device__ constexpr int lean_r_group_floats(int cmax, int ccap, int rmax, int smax, bool st) {
+  return cmax + (st ? 4 * ccap + 2 * rmax : 0) + ((2 * smax + 3) & ~3);  // whole float4s per group
+}
+
+template <int MEAS, int W, int U, bool BAND, bool DELTA, bool ST>
+__global__ void __launch_bounds__(256) k_dp_lean_r(DpArgs a) {
+  static_assert(BAND || DELTA, "ranged pairs only");
+  constexpr int G = 64 / W;
+  constexpr float BND = (MEAS == TS_LCSS) ? 0.0f : kInf;
+  constexpr int kNoResult = INT_MIN;
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int nwaves = blockDim.x >> 6;
+  const int g = lane / W, lg = lane & (W - 1);
+  const int cmax = a.cmax, hmask = a.hmask, ccap = a.ccap, rmax = a.rmax, smax = a.smax;
+  const int delta = a.delta;
+  extern __shared__ float4 lds_dyn4[];
+  const int hdr = ((nwaves * G + 3) >> 2) << 2;
+  int* slot_item = (int*)lds_dyn4 + wv * G + g;
+  float* brow = (float*)lds_dyn4 + hdr + wv * 8;
+  float* gbase = (float*)lds_dyn4 + hdr + nwaves * 8 +
+                 (size_t)(wv * G + g) * lean_r_group_floats(cmax, ccap, rmax, smax, ST);
+  if (lane < 8) brow[lane] = BND;
+  float* bnd = gbase;                                              // [cmax] halo ring
+  float2* cols = (float2*)(gbase + cmax);                          // ST: [2][ccap] column points
+  float2* rows = (float2*)(gbase + cmax + 4 * ccap);               // ST: [rmax] row points
+  int2* tbl = (int2*)(gbase + cmax + (ST ? 4 * ccap + 2 * rmax : 0));  // [smax] (cstart, cend)
+
+  int64_t n_items = a.max_items;
+  if (a.nitems) n_items = min<int64_t>(n_items, (int64_t)*a.nitems);
+  const int64_t n_head = n_items;
+  if (a.ntail) n_items = min<int64_t>(a.max_items, n_head + (int64_t)*a.ntail);
+
+  // Current pair of this lane; before its first pair lane lg idles lg steps
+  // on one strip of W+1 chunks.
+  int R = 1, C = 1, lastch = 0, S = 1, rem = lg, res_rem = kNoResult, slot = 0;
+  int strip = 0, k = 0, cs = 0, ce = W + 1, pcs = 0, pce = 0, row = lg;
+  int bw = 0, bd = 0;
+  int2 nxt = make_int2(0, W + 1);  // the next strip's (cstart, cend)
+  int gpar = 0;
+  const float2* cb = cols;
+  const float *Ax = a.q.x, *Ay = a.q.y, *Bx = a.q.x, *By = a.q.y;
+  bool done = false, wrapped = false;
+  // the group's next pair (read once by the whole group at the fetch)
+  int fR = 1, fC = 1, f_slot = 0, f_total = 0, f_ires = 0, f_lgres = -1;
+  const float *fAx = a.q.x, *fAy = a.q.y, *fBx = a.q.x, *fBy = a.q.y;
+  bool f_ok = false, f_live = false;
+  float o[U], cx[U], cy[U], hv[U];
+#pragma unroll
+  for (int j = 0; j < U; ++j) { o[j] = 0.0f; hv[j] = BND; cx[j] = 0.0f; cy[j] = 0.0f; }
+  float ax = 0.0f, ay = 0.0f, up_last = 0.0f, hdiag = BND;
+  float2 rp = make_float2(0.0f, 0.0f);
+
+  for (;;) {
+    __builtin_amdgcn_wave_barrier();
+    if (__any(rem == lg || rem == 0 || rem + 1 == res_rem)) {
+      // ---- result: the lane's last step covered the result cell
+      if (rem + 1 == res_rem) {
+        const int rc = (C - 1) - lastch * U;
+        float v = o[0];
+#pragma unroll
+        for (int j = 1; j < U; ++j)
+          if (rc == j) v = o[j];
+        if (MEAS == TS_LCSS) v = v / (float)C;
+        a.out[slot] = v;
+      }
+      // ---- fetch (the whole group): descriptor, staged trajectories, the
+      // strip table, the pair's step total and its result step
+      if (rem == lg) {
+        if (lg == 0) *slot_item = atomicAdd(a.next, 1);
+        __builtin_amdgcn_wave_barrier();
+        const int it_idx = *slot_item;
+        f_ok = it_idx < n_items;
+        f_live = false;
+        if (f_ok) {
+          const Item it = a.items[it_idx < n_head ? (int64_t)it_idx : a.max_items - 1 - (it_idx - n_head)];
+          f_live = it.d >= 0;
+          if (f_live) {
+            const int n = a.q.len[it.q], m = a.db.len[it.d];
+            const int64_t qo = a.q.off[it.q], dof = a.db.off[it.d];
+            if (n >= m) {
+              fAx = a.q.x + qo; fAy = a.q.y + qo; fBx = a.db.x + dof; fBy = a.db.y + dof; fR = n; fC = m;
+            } else {
+              fAx = a.db.x + dof; fAy = a.db.y + dof; fBx = a.q.x + qo; fBy = a.q.y + qo; fR = m; fC = n;
+            }
+            f_slot = it.slot;
+            if (ST) {
+              stage_pts<W>(cols + gpar * ccap, fBx, fBy, fC, lg);
+              stage_pts<W>(rows, fAx, fAy, fR, lg);
+            }
+          }
+        }
+        const int R2 = f_live ? fR : 1, C2 = f_live ? fC : 1;
+        const int lastch2 = (C2 - 1) / U;
+        const int wb2 = BAND ? band_half_width(a.band, R2, C2) : 0;
+        const int res_row2 = DELTA ? min(R2 - 1, C2 - 1 + delta) : R2 - 1;
+        const int S2 = DELTA ? res_row2 / W + 1 : (R2 + W - 1) / W;
+        const int sres = res_row2 / W;
+        int part = 0, before = 0, at_res = 0;
+        for (int s2 = lg; s2 < S2; s2 += W) {
+          int clo, chi, nlo, nhi;
+          strip_range<BAND, DELTA, W, U>(s2, R2, C2, wb2, delta, lastch2, clo, chi);
+          nlo = clo;
+          if (s2 + 1 < S2) strip_range<BAND, DELTA, W, U>(s2 + 1, R2, C2, wb2, delta, lastch2, nlo, nhi);
+          const int dur = max(chi - clo + 1, W + (nlo - clo) + 1);
+          tbl[s2] = make_int2(clo, clo + dur);
+          part += dur;
+          if (s2 < sres) before += dur;
+          if (s2 == sres) at_res = lastch2 - clo;  // the result chunk's step within its strip
+        }
+#pragma unroll
+        for (int sh = W / 2; sh > 0; sh >>= 1) {  // within the group
+          part += __shfl_xor(part, sh);
+          before += __shfl_xor(before, sh);
+          at_res += __shfl_xor(at_res, sh);
+        }
+        f_total = part;
+        f_ires = before + at_res;
+        f_lgres = f_live ? res_row2 - sres * W : -1;
+        gpar ^= 1;
+      }
+      // ---- start: this lane begins the pair fetched lg steps ago
+      if (rem == 0) {
+        if (!f_ok) {
+          done = true;
+          rem = -2;
+          res_rem = kNoResult;
+        } else {
+          R = f_live ? fR : 1;
+          C = f_live ? fC : 1;
+          slot = f_slot;
+          lastch = (C - 1) / U;
+          if (BAND) bw = band_half_width(a.band, R, C) * R;
+          S = DELTA ? min(R - 1, C - 1 + delta) / W + 1 : (R + W - 1) / W;
+          rem = f_total;
+          res_rem = lg == f_lgres ? f_total - f_ires : kNoResult;
+          strip = 0;
+          row = lg;
+          wrapped = false;
+          const int2 t0 = tbl[0];
+          cs = t0.x;
+          ce = t0.y;
+          pcs = 0;
+          pce = 0;
+          k = cs;
+          nxt = tbl[min(1, S - 1)];
+          if (f_live) {
+            Ax = fAx; Ay = fAy; Bx = fBx; By = fBy;
+          }
+          if (BAND) bd = row * C - cs * U * R;
+          const int lc = U * min(cs, lastch);
+          if (ST) {
+            cb = cols + (gpar ^ 1) * ccap;
+            load_pts<U>(cb + lc, cx, cy);
+            const float2 p0 = rows[min(lg, R - 1)];
+            ax = p0.x;
+            ay = p0.y;
+          } else {
+            load_vec<U>(Bx + lc, cx);
+            load_vec<U>(By + lc, cy);
+            ax = Ax[min(lg, R - 1)];
+            ay = Ay[min(lg, R - 1)];
+          }
+          load_vec<U>(brow, hv);
+        }
+      }
+      if (__all(done)) break;
+    }
+    if (wrapped) {
+      ax = rp.x;
+      ay = rp.y;
+    }
+
+    // ---- U cells (as in k_dp)
+    float up[U], on[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) up[j] = from_prev_lane<W>(o[j], hv[j], lg);
+    const bool first = k == cs;
+    float d = first ? ((row == 0) ? (cs == 0 ? 0.0f : BND) : (lg == 0 ? hdiag : BND)) : up_last;
+    if (MEAS == TS_LCSS && first && cs == 0) d = 0.0f;
+    float l = first ? BND : o[U - 1];
+    float dist[U];
+    if (MEAS == TS_DTW) {
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const float dx = ax - cx[j], dy = ay - cy[j];
+        dist[j] = dx * dx + dy * dy;
+      }
+      sqrt_cr<U>(dist);
+    }
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const float dx = ax - cx[j], dy = ay - cy[j];
+      float v;
+      if (MEAS == TS_DTW) {
+        v = dist[j] + fminf(fminf(up[j], l), d);
+        if (BAND) {
+          const int bj = bd - j * R;
+          if ((bj < 0 ? -bj : bj) > bw) v = kInf;
+        }
+      } else {
+        bool match = fabsf(dx) < a.eps && fabsf(dy) < a.eps;
+        float uu = up[j], ll = l;
+        if (DELTA) {
+          const int dd = row - (k * U + j);
+          match = match && (dd < 0 ? -dd : dd) <= delta;
+          if (dd - 1 > delta || 1 - dd > delta) uu = d;  // (row-1, col) outside
+          if (dd + 1 > delta || -1 - dd > delta) ll = d;  // (row, col-1) outside
+        }
+        v = match ? d + 1.0f : fmaxf(uu, ll);
+      }
+      on[j] = v;
+      d = up[j];
+      l = v;
+    }
+    up_last = up[U - 1];
+#pragma unroll
+    for (int j = 0; j < U; ++j) o[j] = on[j];
+
+    // ---- advance; the next step's operands (before this step's halo store)
+    const int kh = k;
+    wrapped = k + 1 == ce;
+    if (wrapped) {
+      ++strip;
+      row += W;
+      pcs = cs;
+      pce = ce;
+      cs = nxt.x;
+      ce = nxt.y;
+      k = cs;
+      if (BAND) bd = row * C - cs * U * R;
+      // lane 0's diagonal for the new strip's first cell: the halo value
+      // left of its range, when the previous strip computed it
+      const int dc = cs * U - 1;
+      hdiag = (dc >= 0 && dc >= pcs * U && dc < pce * U && dc < C) ? bnd[dc & hmask] : BND;
+    } else {
+      ++k;
+      if (BAND) bd -= U * R;
+    }
+    nxt = tbl[min(strip + 1, S - 1)];
+    const int lc = U * min(k, lastch);
+    if (ST) {
+      rp = rows[min(row, R - 1)];
+      load_pts<U>(cb + lc, cx, cy);
+    } else {
+      rp = make_float2(Ax[min(row, R - 1)], Ay[min(row, R - 1)]);
+      load_vec<U>(Bx + lc, cx);
+      load_vec<U>(By + lc, cy);
+    }
+    load_vec<U>((row == 0 || k >= pce) ? brow : bnd + (lc & hmask), hv);
+    if (lg == W - 1 && kh <= lastch) store_vec<U>(bnd + ((kh * U) & hmask), on);
+    --rem;
+  }
+}
+
 // Halo of a launch of shape `shape` over bucket b. Without a band: a whole
 // row of the longest C in the bucket (at least W+1 chunks, the shortest
 // strip of k_dp_lean). With a band: a row
@@ -726,8 +994,9 @@ DpHalo dp_halo(int shape, int b, const MeasureArgs& m, int32_t cmax_c, int32_t w
 template <int MEAS, int W, int U, bool BAND, bool DELTA, int LEAN>
 static hipError_t launch_dp_t(hipStream_t st, const DpArgs& a, int grid_waves) {
   constexpr int G = 64 / W;
-  const size_t gf = LEAN == 0 ? (size_t)dp_group_floats(a.cmax)
-                              : (size_t)lean_group_floats(a.cmax, a.rmax, LEAN != 3, LEAN == 1);
+  const size_t gf = LEAN == 0   ? (size_t)dp_group_floats(a.cmax)
+                    : LEAN >= 4 ? (size_t)lean_r_group_floats(a.cmax, a.ccap, a.rmax, a.smax, LEAN == 4)
+                                : (size_t)lean_group_floats(a.cmax, a.rmax, LEAN != 3, LEAN == 1);
   const size_t per_wave = (size_t)G * gf * sizeof(float) + G * sizeof(int) + 32;
   int wpb = 4;
   while (wpb > 1 && per_wave * wpb + 64 > 65536) --wpb;
@@ -736,7 +1005,9 @@ static hipError_t launch_dp_t(hipStream_t st, const DpArgs& a, int grid_waves) {
                      (size_t)wpb * G * gf * sizeof(float);
   int blocks = (grid_waves + wpb - 1) / wpb;
   if (blocks < 1) blocks = 1;
-  if constexpr (LEAN != 0) {
+  if constexpr (LEAN >= 4) {
+    hipLaunchKernelGGL((k_dp_lean_r<MEAS, W, U, BAND, DELTA, LEAN == 4>), dim3(blocks), dim3(64 * wpb), lds, st, a);
+  } else if constexpr (LEAN != 0) {
     static_assert(!BAND && !DELTA, "the lean kernel is unbanded");
     hipLaunchKernelGGL((k_dp_lean<MEAS, W, U, LEAN != 3, LEAN == 1>), dim3(blocks), dim3(64 * wpb), lds, st, a);
   } else {
@@ -761,6 +1032,14 @@ template <int MEAS, int W, int U, bool BAND, bool DELTA>
 static hipError_t launch_dp_s(hipStream_t st, DpArgs a, int grid_waves) {
   constexpr int G = 64 / W;
   if constexpr (BAND || DELTA) {
+    // k_dp_lean_r: both trajectories staged, else both from global memory;
+    // k_dp when even the halo ring and strip table do not fit
+    a.smax = (a.rmax + W - 1) / W;
+    a.ccap = ((a.ccap + U - 1) / U) * U;  // whole chunks: the last one is read U points wide
+    if ((size_t)G * lean_r_group_floats(a.cmax, a.ccap, a.rmax, a.smax, true) * sizeof(float) <= 20480)
+      return launch_dp_t<MEAS, W, U, BAND, DELTA, 4>(st, a, grid_waves);
+    if ((size_t)G * lean_r_group_floats(a.cmax, 0, 0, a.smax, false) * sizeof(float) <= 20480)
+      return launch_dp_t<MEAS, W, U, BAND, DELTA, 5>(st, a, grid_waves);
     return launch_dp_t<MEAS, W, U, BAND, DELTA, 0>(st, a, grid_waves);
   } else {
     if ((size_t)G * lean_group_floats(a.cmax, a.rmax, true, true) * sizeof(float) <= 20480)
@@ -790,10 +1069,13 @@ static hipError_t launch_dp_b(hipStream_t st, const DpArgs& a, int shape, int gr
 
 hipError_t launch_dp(hipStream_t st, MeasureArgs m, int shape, SetView q, SetView db, const Item* items,
                      const int32_t* nitems, const int32_t* ntail, int64_t max_items, int32_t* next, float* out,
-                     DpHalo halo, int32_t rmax_c, int grid_waves) {
-  // row buffer: the longest trajectory of either set, in whole float4s
+                     DpHalo halo, int32_t rmax_c, int32_t ccap_c, int grid_waves) {
+  // row buffer: the longest trajectory of either set, in whole float4s;
+  // column buffer (ranged): the longest shorter trajectory, likewise
   const int32_t rmax = ((std::max(rmax_c, 1) + 3) / 4) * 4;
-  DpArgs a{q, db, items, nitems, ntail, max_items, next, out, halo.len, halo.mask, rmax, m.band, m.delta, m.eps};
+  const int32_t ccap = ((std::max(ccap_c, 1) + 3) / 4) * 4;
+  DpArgs a{q, db, items, nitems, ntail, max_items, next, out, halo.len, halo.mask, rmax, ccap, 0, m.band, m.delta,
+           m.eps};
   switch (m.measure) {
     case TS_DTW:
       return m.band != 0 ? launch_dp_b<TS_DTW, true, false>(st, a, shape, grid_waves)

@@ -412,7 +412,7 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
     const DpHalo halo = shape_halo(e, sh, b_lo, ma, hi, whi);
     const std::string label = dp_label(nm, b_lo, sh);
     return run(e, label.c_str(), 0, 0, [&] {
-      return launch_dp(e->st, ma, sh, qv, dv, items, nitems_dev, nullptr, max_items, ctr, vals, halo, rmax,
+      return launch_dp(e->st, ma, sh, qv, dv, items, nitems_dev, nullptr, max_items, ctr, vals, halo, rmax, hi,
                        cus * tune[b_lo].waves_per_cu);
     }, slot + b_lo);
   }
@@ -439,7 +439,8 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
     TS_HIP(hipStreamWaitEvent(e->aux[b], e->fork, 0));
     TS_TRY(run_on(e, e->aux[b], label.c_str(), 0, 0, [&] {
       return launch_dp(e->aux[b], ma, sh, qv, dv, itemsb + (int64_t)b * max_items, ctr + kDpBuckets + b,
-                       ctr + 2 * kDpBuckets + b, max_items, ctr + b, vals, halo, rmax, cus * tune[b].waves_per_cu);
+                       ctr + 2 * kDpBuckets + b, max_items, ctr + b, vals, halo, rmax, hi,
+                       cus * tune[b].waves_per_cu);
     }, slot + b));
     TS_HIP(hipEventRecord(e->join[b], e->aux[b]));
     TS_HIP(hipStreamWaitEvent(e->st, e->join[b], 0));

@@ -251,7 +251,7 @@ DpHalo dp_halo(int shape, int b, const MeasureArgs& m, int32_t cmax_c, int32_t w
 bool dp_fits(int shape, DpHalo halo);
 hipError_t launch_dp(hipStream_t st, MeasureArgs m, int shape, SetView q, SetView db, const Item* items,
                      const int32_t* nitems, const int32_t* ntail, int64_t max_items, int32_t* next, float* out,
-                     DpHalo halo, int32_t rmax_c, int grid_waves);
+                     DpHalo halo, int32_t rmax_c, int32_t ccap_c, int grid_waves);
 
 // reduce.hip -- Hausdorff (wave per pair) and lock-step Euclidean (lane per pair).
 hipError_t launch_hausdorff(hipStream_t st, SetView q, SetView db, const Item* items, const int32_t* nitems,

@@ -1,9 +1,10 @@
-"""CPU checks of two parts of the DP refine that GPU runs only sample: the
-lean kernel's schedule, modelled lane by lane against the oracle
-(tests/emu/dp_lean_schedule.cpp: lane stagger, pair fetch and start, halo
-timing, result countdown, staged operands' provenance, LDS bounds), and the
-neighbour fixup that makes v_sqrt_f32 a correctly rounded square root
-(tests/emu/sqrt_fixup.c)."""
+"""CPU checks of parts of the DP refine that GPU runs only sample: the lean
+kernels' schedules, modelled lane by lane against the oracle
+(tests/emu/dp_lean_schedule.cpp, unbanded; tests/emu/dp_lean_ranged_schedule.cpp,
+banded DTW and windowed LCSS: lane stagger, pair fetch and start, strip table,
+halo and ring timing, result countdown, staged operands' provenance, LDS
+bounds), and the neighbour fixup that makes v_sqrt_f32 a correctly rounded
+square root (tests/emu/sqrt_fixup.c)."""
 import os
 import shutil
 import subprocess
@@ -22,6 +23,19 @@ def test_lean_dp_schedule_model_matches_oracle(tmp_path):
                    check=True, timeout=300)
     r = subprocess.run([exe, "12"], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_lean_ranged_dp_schedule_model_matches_oracle(tmp_path):
+    exe = str(tmp_path / "dp_lean_ranged_schedule")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fopenmp", "-o", exe,
+                    os.path.join(HERE, "emu", "dp_lean_ranged_schedule.cpp"),
+                    os.path.join(ROOT, "oracle", "oracle.cpp")], check=True, timeout=300)
+    r = subprocess.run([exe, "40"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+    # a ring half the size the launch gives is caught
+    r = subprocess.run([exe, "20", "2"], capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0, r.stdout[-2000:]
 
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
