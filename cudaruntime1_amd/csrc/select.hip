@@ -50,6 +50,60 @@ hipError_t launch_keys_from_rows(hipStream_t st, const float* vals, int32_t rows
   return hipGetLastError();
 }
 
+// Selections of at most kWaveK keys per chunk take the minimum key k times
+// instead of sorting the chunk.
+constexpr int kWaveK = 32;
+
+// One DPP step of a wave-wide minimum of 64-bit keys: lanes whose source is
+// outside the row or in a masked row see the empty key.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint64_t min_dpp(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_update_dpp(0xffffffffu, (uint32_t)v, CTRL, ROWS, 0xf, false);
+  const uint32_t hi = __builtin_amdgcn_update_dpp(0xffffffffu, (uint32_t)(v >> 32), CTRL, ROWS, 0xf, false);
+  const uint64_t s = ((uint64_t)hi << 32) | lo;
+  return s < v ? s : v;
+}
+
+// Minimum over the wave's 64 lanes (all lanes active): prefix minima within
+// rows of 16 (row_shr 1, 2, 4, 8), rows combined by row_bcast 15 and 31, so
+// lane 63 holds the minimum; returned to every lane.
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+  v = min_dpp<0x111, 0xf>(v);
+  v = min_dpp<0x112, 0xf>(v);
+  v = min_dpp<0x114, 0xf>(v);
+  v = min_dpp<0x118, 0xf>(v);
+  v = min_dpp<0x142, 0xa>(v);
+  v = min_dpp<0x143, 0xc>(v);
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, 63), hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), 63);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Takes the k smallest of the wave's T keys per lane, in ascending order, into
+// dst[0..k) (lane 0 writes; missing keys are empty). One occurrence is taken
+// per round: the lowest lane holding the minimum gives up its first copy.
+template <int T>
+__device__ __forceinline__ void wave_extract(uint64_t (&v)[T], int k, uint64_t* dst) {
+  const int lane = threadIdx.x & 63;
+  for (int r = 0; r < k; ++r) {
+    uint64_t lm = v[0];
+    int at = 0;
+#pragma unroll
+    for (int t = 1; t < T; ++t)
+      if (v[t] < lm) { lm = v[t]; at = t; }
+    const uint64_t m = wave_min_u64(lm);
+    if (m == ~0ull) {  // wave-uniform: no keys left
+      for (int t = r + lane; t < k; t += 64) dst[t] = ~0ull;
+      return;
+    }
+    if (lane == 0) dst[r] = m;
+    const uint64_t holders = __ballot(lm == m);
+    const bool mine = lane == __ffsll((unsigned long long)holders) - 1;
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+      if (mine && t == at) v[t] = ~0ull;
+  }
+}
+
 // One selection stage: segment seg's keys [0, len) in chunks of ch; chunk c
 // writes its best k to out[(seg * nch + c) * k]. Blocks stride over the
 // chunks (the grid's x is capped), so an upper-bound seg_len costs little:
@@ -59,6 +113,41 @@ __global__ void __launch_bounds__(256) k_topk_stage(const uint64_t* in, int64_t 
                                                     uint64_t* out, const int64_t* seg_off) {
   __shared__ uint64_t s[kSelChunk];
   const int seg = blockIdx.y;
+  if (k <= kWaveK) {
+    // Each wave takes the best k of its quarter of the chunk (keys i with
+    // i % 256 in the wave's 64), then wave 0 takes the best k of those 4k.
+    uint64_t* part = s;  // [4][kWaveK]
+    int len = seg_len;
+    if (counts) len = min(counts[seg], seg_len);
+    const uint64_t* base = in + (seg_off ? seg_off[seg] : (int64_t)seg * seg_stride);
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int c = blockIdx.x; c < nch; c += gridDim.x) {
+      const int begin = c * ch;
+      const int nh = max(0, min(ch, len - begin));
+      uint64_t* dst = out + ((int64_t)seg * nch + c) * k;
+      if (nh == 0) {  // block-uniform
+        for (int t = threadIdx.x; t < k; t += 256) dst[t] = ~0ull;
+        continue;
+      }
+      const uint64_t* src = base + begin;
+      uint64_t v[kSelChunk / 256];
+#pragma unroll
+      for (int t = 0; t < kSelChunk / 256; ++t) {
+        const int i = t * 256 + threadIdx.x;
+        v[t] = i < nh ? src[i] : ~0ull;
+      }
+      __syncthreads();  // the previous chunk's merge is done with part
+      wave_extract(v, k, part + wv * kWaveK);
+      __syncthreads();
+      if (wv == 0) {
+        uint64_t w[2];
+        w[0] = lane < 2 * k ? part[(lane / k) * kWaveK + lane % k] : ~0ull;
+        w[1] = lane < 2 * k ? part[(2 + lane / k) * kWaveK + lane % k] : ~0ull;
+        wave_extract(w, k, dst);
+      }
+    }
+    return;
+  }
   int len = seg_len;
   if (counts) len = min(counts[seg], seg_len);
   const uint64_t* base = in + (seg_off ? seg_off[seg] : (int64_t)seg * seg_stride);
