@@ -104,7 +104,7 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 enum BufId {
   B_ENV, B_SCORES, B_KEYS, B_KEYS2, B_SCA, B_SCB, B_CAND, B_CANDIDX, B_ITEMS, B_ITEMS3, B_VALS, B_REGION,
   B_RVALS, B_COUNTS, B_CTR, B_TAU, B_TOP, B_OIDX, B_OVAL, B_CELLS, B_HOSTIN, B_ITEMS2, B_RITEMS, B_SEGC, B_NEED, B_BCNT,
-  B_NBUF
+  B_BITS, B_BOUNDS, B_NBUF
 };
 
 struct Buf {
@@ -527,6 +527,10 @@ ts_status knn_all_batch(ts_engine* e, const ts_params& p, const ts_set* db, cons
   return topk(e, keys, N, N, nullptr, nq, p.k, top);
 }
 
+// Largest batch (queries x DB trajectories) whose seed-pass bounds are kept
+// for the compaction (4 bytes a pair).
+constexpr int64_t kStoredBoundsMax = 1ll << 26;
+
 // EXACT pipeline for queries [q0, q0+nq).
 ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, const ts_set* qs, int32_t q0,
                           int32_t nq, uint64_t* top) {
@@ -534,7 +538,7 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   const int32_t k = p.k;
   const bool desc = desc_measure(p.measure);
   const int32_t s = std::min<int32_t>(std::max<int32_t>(2 * k, 16), 1024);
-  const int32_t nch = (N + 2047) / 2048;
+  const int32_t nch = (N + bound_chunk_items(s) - 1) / bound_chunk_items(s);
   const SetView qv = qs->view(), dv = db->view();
   const MeasureArgs ma = margs(p);
   uint64_t *bkeys = nullptr, *seeds = nullptr;
@@ -551,9 +555,13 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   TS_TRY(ensure(e, B_COUNTS, nq + 1, &counts));
   TS_TRY(ensure(e, B_REGION, 2 * (int64_t)(nq + 1), (int32_t**)&d_off));
   const double pairs = (double)nq * N;
+  // The seed pass keeps every pair's bound for the compaction's count pass
+  // when the batch is small enough; larger batches evaluate them again.
+  float* bvals = nullptr;
+  if ((int64_t)nq * N <= kStoredBoundsMax) TS_TRY(ensure(e, B_BOUNDS, (int64_t)nq * N, &bvals));
   // 1. seeds: good candidates by bound, refined exactly
   TS_TRY(run(e, "bound_seed", pairs * 36.0, pairs, [&] {
-    return launch_bound_chunks(e->st, ma, k, qv, q0, nq, dv, s, 2048, bkeys);
+    return launch_bound_chunks(e->st, ma, k, qv, q0, nq, dv, s, bound_chunk_items(s), bkeys, bvals);
   }));
   TS_TRY(topk(e, bkeys, (int64_t)nch * s, nch * s, nullptr, nq, s, seeds));
   TS_TRY(run(e, "items", 0, 0, [&] { return launch_items_from_keys(e->st, seeds, nq, s, q0, items, seed_idx, vals); }));
@@ -567,9 +575,11 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   // call is repeated with a larger capacity if the total exceeded it.
   const int32_t gx = compact_blocks(N);
   int32_t* bcounts = nullptr;
+  uint64_t* bits = nullptr;
   TS_TRY(ensure(e, B_BCNT, (int64_t)nq * gx, &bcounts));
+  TS_TRY(ensure(e, B_BITS, (int64_t)nq * compact_mask_words(N), &bits));
   TS_TRY(run(e, "bound_compact", pairs * 36.0, pairs, [&] {
-    return launch_bound_compact(e->st, ma, k, qv, q0, nq, dv, tau, nullptr, bcounts, nullptr, 0);
+    return launch_bound_compact(e->st, ma, k, qv, q0, nq, dv, tau, nullptr, bvals, bcounts, bits, nullptr, 0);
   }));
   TS_TRY(run(e, "block_prefix", 0, 0, [&] { return launch_block_prefix(e->st, bcounts, nq, gx, counts); }));
   const bool dev_sized = e->capturing;
@@ -622,8 +632,8 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   TS_TRY(ensure(e, B_RITEMS, cap, &items2));
   TS_TRY(ensure(e, B_RVALS, cap, &rvals));
   TS_TRY(ensure(e, B_SCORES, 2 * cap, (float**)&rkeys));
-  TS_TRY(run(e, "bound_compact", pairs * 36.0, pairs, [&] {
-    return launch_bound_compact(e->st, ma, k, qv, q0, nq, dv, tau, d_off, bcounts, citems, cap);
+  TS_TRY(run(e, "compact_write", 0, 0, [&] {
+    return launch_bound_compact(e->st, ma, k, qv, q0, nq, dv, tau, d_off, nullptr, bcounts, bits, citems, cap);
   }));
   // 3b. tighter O(n+m) bound on the survivors
   int32_t* nitems2 = counts + nq;
@@ -751,8 +761,8 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
   } else if (p.mode == TS_MODE_REF) {
     bq = (int32_t)std::max<int64_t>(1, std::min<int64_t>(Q, (int64_t)(256ll << 20) / (8ll * std::max(1, N))));
   } else {
-    int64_t nch = (N + 2047) / 2048;
     int64_t s = std::min<int64_t>(std::max<int64_t>(2 * k, 16), 1024);
+    int64_t nch = (N + bound_chunk_items((int32_t)s) - 1) / bound_chunk_items((int32_t)s);
     bq = (int32_t)std::max<int64_t>(1, std::min<int64_t>({(int64_t)Q, 4096, (int64_t)(512ll << 20) / (8 * nch * s)}));
   }
   // Small EXACT problems refine every pair (Euclidean keeps its filter,

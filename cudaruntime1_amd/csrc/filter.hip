@@ -216,8 +216,11 @@ __device__ __forceinline__ float pair_bound_m(const MeasureArgs& ma, const QSum&
   return pair_bound(mc, qs, q, cm, ce, m);
 }
 
-constexpr int kBoundChunk = 2048;  // DB trajectories per seeding block
-constexpr int kSeedPerThread = kBoundChunk / 256;
+// DB trajectories per seeding block and thread: 16 when the block keeps the
+// best key of each thread (s <= 256: the chunk's blocks fit the chip in one
+// round at the 64x100k scale), 8 when it sorts every key of the chunk.
+constexpr int kSeedPerThread = 16, kSeedPerThreadWhole = 8;
+int32_t bound_chunk_items(int32_t s) { return 256 * (s <= 256 ? kSeedPerThread : kSeedPerThreadWhole); }
 
 __device__ void bitonic_sort_lds(uint64_t* s, int n, int tid, int nthreads) {
   for (int k = 2; k <= n; k <<= 1) {
@@ -235,28 +238,38 @@ __device__ void bitonic_sort_lds(uint64_t* s, int n, int tid, int nthreads) {
   }
 }
 
-// Seeds: good candidates per (query, DB chunk of kBoundChunk trajectories).
-// Any real pair's exact value bounds the k-th best from above, so seeds need
-// not be the exact top-s by bound: each thread keeps the best key of its
-// kSeedPerThread trajectories and the block keeps the best `s` of those
-// (s <= 256). Larger s sorts the whole chunk.
+// Seeds: good candidates per (query, DB chunk of bound_chunk_items(s)
+// trajectories). Any real pair's exact value bounds the k-th best from above,
+// so seeds need not be the exact top-s by bound: each thread keeps the best
+// key of its kSeedPerThread trajectories and the block keeps the best `s` of
+// those (s <= 256). Larger s sorts the whole chunk.
 template <bool WHOLE, int MEAS>
 __global__ void __launch_bounds__(256) k_bound_chunks(MeasureArgs ma, SetView q, int32_t q0, SetView db,
-                                                      int32_t s, uint64_t* keys) {
-  constexpr int NS = WHOLE ? kBoundChunk : 256;
+                                                      int32_t s, uint64_t* keys, float* bstore) {
+  constexpr int PER = WHOLE ? kSeedPerThreadWhole : kSeedPerThread;
+  constexpr int NS = WHOLE ? 256 * PER : 256;
   __shared__ uint64_t sk[NS];
   const int qi = q0 + blockIdx.y;
   QSum qs{q.mbr[qi], q.ends[qi], q.len[qi], q.off[qi]};
   const bool desc = ma.measure == TS_LCSS;
-  const int base = blockIdx.x * kBoundChunk;
+  const int base = blockIdx.x * 256 * PER;
   uint64_t best = ~0ull;
-  for (int t = 0; t < kSeedPerThread; ++t) {
-    const int i = t * 256 + threadIdx.x;
-    const int j = base + i;
-    uint64_t key = ~0ull;
-    if (j < db.count) key = make_key(pair_bound_m<MEAS>(ma, qs, q, db.mbr[j], db.ends[j], db.len[j]), j, desc);
-    if (WHOLE) sk[i] = key;
-    else best = key < best ? key : best;
+  // eight trajectories' summaries in flight at a time
+#pragma unroll 1
+  for (int t0 = 0; t0 < PER; t0 += 8) {
+#pragma unroll
+    for (int t = t0; t < t0 + 8; ++t) {
+      const int i = t * 256 + threadIdx.x;
+      const int j = base + i;
+      uint64_t key = ~0ull;
+      if (j < db.count) {
+        const float b = pair_bound_m<MEAS>(ma, qs, q, db.mbr[j], db.ends[j], db.len[j]);
+        if (bstore) bstore[(int64_t)blockIdx.y * db.count + j] = b;  // for the compaction's count pass
+        key = make_key(b, j, desc);
+      }
+      if (WHOLE) sk[i] = key;
+      else best = key < best ? key : best;
+    }
   }
   if (!WHOLE) sk[threadIdx.x] = best;
   __syncthreads();
@@ -266,13 +279,15 @@ __global__ void __launch_bounds__(256) k_bound_chunks(MeasureArgs ma, SetView q,
 }
 
 hipError_t launch_bound_chunks(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, int32_t nq,
-                               SetView db, int32_t s, int32_t chunk, uint64_t* keys) {
+                               SetView db, int32_t s, int32_t chunk, uint64_t* keys, float* bstore) {
   (void)k;
   (void)chunk;
-  dim3 grid((db.count + kBoundChunk - 1) / kBoundChunk, nq);
+  const int32_t items = bound_chunk_items(s);
+  dim3 grid((db.count + items - 1) / items, nq);
 #define TS_CHUNKS(MEAS)                                                                              \
-  if (s <= 256) hipLaunchKernelGGL((k_bound_chunks<false, MEAS>), grid, dim3(256), 0, st, m, q, q0, db, s, keys); \
-  else hipLaunchKernelGGL((k_bound_chunks<true, MEAS>), grid, dim3(256), 0, st, m, q, q0, db, s, keys);
+  if (s <= 256)                                                                                     \
+    hipLaunchKernelGGL((k_bound_chunks<false, MEAS>), grid, dim3(256), 0, st, m, q, q0, db, s, keys, bstore); \
+  else hipLaunchKernelGGL((k_bound_chunks<true, MEAS>), grid, dim3(256), 0, st, m, q, q0, db, s, keys, bstore);
   switch (m.measure) {
     case TS_DTW: TS_CHUNKS(TS_DTW) break;
     case TS_FRECHET: TS_CHUNKS(TS_FRECHET) break;
@@ -292,17 +307,21 @@ hipError_t launch_bound_chunks(hipStream_t st, MeasureArgs m, int32_t k, SetView
 // because zero-similarity ties are broken by index, (ub > 0 or j < k).
 // Two passes, no global atomics (one hot counter per query serialises in
 // L2): COUNT reduces each block's survivors per query in LDS and stores them
-// in bcounts[q * gridDim.x + block]; after k_block_prefix has turned those
-// into exclusive offsets within each query's region, WRITE places the block's
+// in bcounts[q * gridDim.x + block], and each wave's pass mask (one bit per
+// DB trajectory) in bits[q * words + word]; after k_block_prefix has turned
+// the counts into exclusive offsets within each query's region, WRITE reads
+// the masks back (no bound is evaluated twice) and places the block's
 // survivors at off[q] + bcounts[...] + an LDS cursor (slot = position).
 constexpr int kCompactQ = 8, kCompactJ = 2;
 
 int32_t compact_blocks(int32_t n_db) { return (n_db + 256 * kCompactJ - 1) / (256 * kCompactJ); }
+int64_t compact_mask_words(int32_t n_db) { return (int64_t)compact_blocks(n_db) * kCompactJ * 4; }
 
 template <bool WRITE, int MEAS>
 __global__ void __launch_bounds__(256) k_bound_compact(MeasureArgs ma, int32_t k, SetView q, int32_t q0, int32_t nq,
                                                        SetView db, const float* tau, const int64_t* off,
-                                                       int32_t* bcounts, Item* items, int64_t cap) {
+                                                       const float* bvals, int32_t* bcounts, uint64_t* bits,
+                                                       Item* items, int64_t cap) {
   __shared__ QSum sq[kCompactQ];
   __shared__ float st[kCompactQ];
   __shared__ int bc[kCompactQ];
@@ -319,21 +338,47 @@ __global__ void __launch_bounds__(256) k_bound_compact(MeasureArgs ma, int32_t k
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
+  const int64_t words = (int64_t)gx * kCompactJ * 4;
   for (int t = 0; t < kCompactJ; ++t) {
     const int j = (blockIdx.x * kCompactJ + t) * 256 + threadIdx.x;
+    const int64_t word = (int64_t)(blockIdx.x * kCompactJ + t) * 4 + (threadIdx.x >> 6);
     const bool valid = j < db.count;
     const int jj = valid ? j : 0;
-    const float4 cm = db.mbr[jj], ce = db.ends[jj];
-    const int m = db.len[jj];
-    for (int u = 0; u < nqb; ++u) {
-      bool pass = false;
-      if (valid) {
-        const float b = pair_bound_m<MEAS>(ma, sq[u], q, cm, ce, m);
-        const float tq = st[u];
-        if (MEAS == TS_LCSS) pass = (b >= tq) && (b > 0.0f || j < k);
-        else pass = isfinite(b) && (b <= tq);
+    float4 cm = make_float4(0.0f, 0.0f, 0.0f, 0.0f), ce = cm;
+    int m = 0;
+    if (!WRITE && !bvals) {
+      cm = db.mbr[jj];
+      ce = db.ends[jj];
+      m = db.len[jj];
+    }
+    // every load of the block's queries up front (independent, so their
+    // latencies overlap)
+    uint64_t wbits[kCompactQ];
+    float sb[kCompactQ];
+#pragma unroll
+    for (int u = 0; u < kCompactQ; ++u) {
+      if (WRITE) wbits[u] = u < nqb ? bits[(int64_t)(qb + u) * words + word] : 0ull;
+      else sb[u] = (bvals && valid && u < nqb) ? bvals[(int64_t)(qb + u) * db.count + j] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < kCompactQ; ++u) {
+      if (u >= nqb) break;
+      uint64_t bal;
+      bool pass;
+      if (WRITE) {
+        bal = wbits[u];
+        pass = (bal >> lane) & 1ull;
+      } else {
+        pass = false;
+        if (valid) {
+          const float b = bvals ? sb[u] : pair_bound_m<MEAS>(ma, sq[u], q, cm, ce, m);
+          const float tq = st[u];
+          if (MEAS == TS_LCSS) pass = (b >= tq) && (b > 0.0f || j < k);
+          else pass = isfinite(b) && (b <= tq);
+        }
+        bal = __ballot(pass);
+        if (lane == 0) bits[(int64_t)(qb + u) * words + word] = bal;
       }
-      const uint64_t bal = __ballot(pass);
       if (bal == 0) continue;
       const int leader = __ffsll((unsigned long long)bal) - 1;
       int base = 0;
@@ -388,16 +433,16 @@ hipError_t launch_block_prefix(hipStream_t st, int32_t* bcounts, int32_t nq, int
 }
 
 hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, int32_t nq,
-                                SetView db, const float* tau, const int64_t* off, int32_t* counts, Item* items,
-                                int64_t cap) {
+                                SetView db, const float* tau, const int64_t* off, const float* bvals, int32_t* counts,
+                                uint64_t* bits, Item* items, int64_t cap) {
   dim3 grid(compact_blocks(db.count), (nq + kCompactQ - 1) / kCompactQ);
 #define TS_COMPACT(MEAS)                                                                                       \
   if (items)                                                                                                   \
-    hipLaunchKernelGGL((k_bound_compact<true, MEAS>), grid, dim3(256), 0, st, m, k, q, q0, nq, db, tau, off, counts, \
-                       items, cap);                                                                            \
+    hipLaunchKernelGGL((k_bound_compact<true, MEAS>), grid, dim3(256), 0, st, m, k, q, q0, nq, db, tau, off, bvals, \
+                       counts, bits, items, cap);                                                                     \
   else                                                                                                         \
     hipLaunchKernelGGL((k_bound_compact<false, MEAS>), grid, dim3(256), 0, st, m, k, q, q0, nq, db, tau, off,  \
-                       counts, items, cap);
+                       bvals, counts, bits, items, cap);
   switch (m.measure) {
     case TS_DTW: TS_COMPACT(TS_DTW) break;
     case TS_FRECHET: TS_COMPACT(TS_FRECHET) break;
@@ -425,7 +470,7 @@ hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetVie
 //   Hausdorff >= max( max_i d(q_i,B_c), max_j d(c_j,B_q) );
 //             and then, with B_c the nearest of the 64-point tiles of c
 //             instead of its MBR, the same forms again (tile_side_*; DTW
-//             without a band likewise by tile_side_sum);
+//             without a band likewise by tile_side_sums2);
 //   Euclidean >= sqrt(sum_i d(q_i,B_c)^2) (double, discounted 1e-6);
 //   LCSS      <= min(#q points in c's eps-box, #c points in q's eps-box,
 //                    min(n,m)) / min(n,m); with a window delta, the box of
@@ -511,28 +556,62 @@ __device__ float band_side_sum(const float* ax, const float* ay, int na, const f
   return total;
 }
 
+// Sum over the wave (all lanes active) by DPP: prefix sums within rows of 16,
+// rows combined by row_bcast 15 and 31; lane 63's total returned to every lane.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float add_dpp(float v) {
+  return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xf, false));
+}
+__device__ __forceinline__ float wave_sum_dpp(float v) {
+  v = add_dpp<0x111, 0xf>(v);
+  v = add_dpp<0x112, 0xf>(v);
+  v = add_dpp<0x114, 0xf>(v);
+  v = add_dpp<0x118, 0xf>(v);
+  v = add_dpp<0x142, 0xa>(v);
+  v = add_dpp<0x143, 0xc>(v);
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 // Unbanded DTW form: row i of `a` has a path cell somewhere in b, so it
 // costs at least the distance from a_i to the nearest of the boxes that tile
-// b (64-point superblocks at blocks 0, 8, 16, ...). Sum over the points of a
-// except the first and last, 64 rows at a time, with the same early return.
-__device__ float tile_side_sum(const float* ax, const float* ay, int na, const float4* bsblk, int nb, float base,
-                               float disc, float limit) {
+// b (64-point superblocks at blocks 0, 8, 16, ...); likewise every column.
+// Both sides in one pass: the interior points of a (all but the first and
+// last) against b's superblock boxes into sa, those of b against a's into sb. The
+// nearest superblock box lies inside the MBR, so each side also bounds the
+// MBR form from above. Loads of both trajectories and of up to 8 superblock
+// boxes of each go out together; one reduction per side. The f32 sums are in
+// a different order than a serial sum; the caller's factor disc covers any
+// order.
+__device__ void tile_side_sums2(const float* ax, const float* ay, int na, const float4* asb, const float* bx,
+                                const float* by, int nb, const float4* bsb, float& sa, float& sb) {
   const int lane = threadIdx.x & 63;
-  const int nblk = (nb + 7) >> 3;
-  float total = 0.0f;
-  for (int i0 = 1; i0 < na - 1; i0 += 64) {
+  const int nsa = (((na + 7) >> 3) + 7) >> 3, nsbx = (((nb + 7) >> 3) + 7) >> 3;  // superblocks
+  const int L = max(na, nb);
+  float a = 0.0f, b = 0.0f;
+  for (int i0 = 0; i0 < L; i0 += 64) {
     const int i = i0 + lane;
-    float acc = 0.0f;
-    if (i < na - 1) {
-      const float px = ax[i], py = ay[i];
-      float best = pt_box(px, py, bsblk[0]);
-      for (int kb = 8; kb < nblk; kb += 8) best = fminf(best, pt_box(px, py, bsblk[kb]));
-      acc = sqrt_lo(best);
+    const bool ia = i >= 1 && i < na - 1, ib = i >= 1 && i < nb - 1;
+    const float px = ia ? ax[i] : 0.0f, py = ia ? ay[i] : 0.0f;
+    const float rx = ib ? bx[i] : 0.0f, ry = ib ? by[i] : 0.0f;
+    float ba = 3.4e38f, bbest = 3.4e38f;
+    for (int s0 = 0; s0 < max(nsa, nsbx); s0 += 8) {
+      float4 tb[8], ta[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        tb[u] = s0 + u < nsbx ? bsb[8 * (s0 + u)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        ta[u] = s0 + u < nsa ? asb[8 * (s0 + u)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (s0 + u < nsbx) ba = fminf(ba, pt_box(px, py, tb[u]));
+        if (s0 + u < nsa) bbest = fminf(bbest, pt_box(rx, ry, ta[u]));
+      }
     }
-    total += wave_sum(acc);
-    if ((total + base) * disc > limit) break;
+    if (ia) a += sqrt_lo(ba);
+    if (ib) b += sqrt_lo(bbest);
   }
-  return total;
+  sa = wave_sum_dpp(a);
+  sb = wave_sum_dpp(b);
 }
 
 // Tiled max form (Frechet, Hausdorff): every point of a is matched to some
@@ -671,6 +750,15 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
       side_bound<MEAS>(q.x + qo, q.y + qo, n, cb, ma.eps, false, f, s, c);
       s = wave_sum_d(s);
       pass = (float)(sqrt(s) * (1.0 - 1e-6)) <= t;
+    } else if (MEAS == TS_DTW && ma.band == 0 && n >= 3 && m >= 3) {
+      // tiled form of both sides directly (it implies the MBR form)
+      const float dx0 = qe.x - ce.x, dy0 = qe.y - ce.y, dx1 = qe.z - ce.z, dy1 = qe.w - ce.w;
+      const float e = sqrt_lo(dx0 * dx0 + dy0 * dy0) + sqrt_lo(dx1 * dx1 + dy1 * dy1);
+      const float disc = 1.0f - (float)(max(n, m) + 72) * 1.1920929e-7f;
+      float b1, b2;
+      tile_side_sums2(q.x + qo, q.y + qo, n, q.sblk + (qo >> 3), db.x + co, db.y + co, m, db.sblk + (co >> 3), b1,
+                      b2);
+      pass = fmaxf(b1 + e, b2 + e) * disc <= t;
     } else if (MEAS == TS_DTW) {
       float a1 = 0.0f, a2 = 0.0f;
       double fd = 0.0;
@@ -691,16 +779,6 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
         return fmaxf(r1, r2) * disc;
       };
       pass = bound(a1, a2) <= t;
-      if (pass && ma.band == 0 && n >= 3 && m >= 3) {
-        // The whole-MBR form passed: the tiled form (nearest superblock box)
-        // is tighter and costs O(m / 64) per point.
-        const float b1 = tile_side_sum(q.x + qo, q.y + qo, n, db.sblk + (co >> 3), m, e0 + e1, disc, t);
-        pass = (b1 + e0 + e1) * disc <= t;
-        if (pass) {
-          const float b2 = tile_side_sum(db.x + co, db.y + co, m, q.sblk + (qo >> 3), n, e0 + e1, disc, t);
-          pass = bound(b1, b2) <= t;
-        }
-      }
       if (pass && ma.band != 0 && n >= 3 && m >= 3) {
         // The whole-MBR form passed: the banded form (row windows of the
         // band) is tighter and costs O(window / 64) per point. Each side alone

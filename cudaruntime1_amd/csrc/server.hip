@@ -71,7 +71,9 @@ int64_t estimate_bytes(const ts_server* s, int32_t len, const ts_params& p) {
   if (p.mode == TS_MODE_REF) {
     b += N * 16 + (int64_t)p.kc * 64;
   } else {
-    const int64_t nch = (N + 2047) / 2048, sd = std::max<int64_t>(2 * p.k, 16);
+    const int64_t sd = std::max<int64_t>(2 * p.k, 16);
+    const int64_t nch = (N + ts::bound_chunk_items((int32_t)std::min<int64_t>(sd, 1024)) - 1) /
+                        ts::bound_chunk_items((int32_t)std::min<int64_t>(sd, 1024));
     b += nch * sd * 8 * 2 + std::min<int64_t>(N, 4096) * 48 + sd * 64;
   }
   return b;

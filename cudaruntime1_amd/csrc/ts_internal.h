@@ -134,21 +134,27 @@ hipError_t launch_envelope(hipStream_t st, SetView q, int32_t max_len, int32_t r
 hipError_t launch_filter_ref(hipStream_t st, int32_t measure, SetView q, int32_t q0, int32_t nq,
                              const float4* env, SetView db, float eps, float* scores);
 // Admissible summary bounds (EXACT mode) of queries [q0,q0+nq) against the
-// whole DB, reduced per DB chunk to the best `s` keys per query:
-// keys[(q * nchunks + chunk) * s + t].
+// whole DB, reduced per DB chunk of bound_chunk_items(s) trajectories to the
+// best `s` keys per query: keys[(q * nchunks + chunk) * s + t].
+int32_t bound_chunk_items(int32_t s);
+// With bstore, also stores every pair's bound at bstore[q * N + j].
 hipError_t launch_bound_chunks(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0,
-                               int32_t nq, SetView db, int32_t s, int32_t chunk, uint64_t* keys);
+                               int32_t nq, SetView db, int32_t s, int32_t chunk, uint64_t* keys, float* bstore);
 // Threshold compaction of every DB trajectory whose bound can still reach
-// the query's threshold tau[q], in compact_blocks(N) blocks per query:
+// the query's threshold tau[q] (bounds read from bvals[q * N + j] when given,
+// else evaluated), in compact_blocks(N) blocks per query:
 // with items == null stores each block's survivors in
-// bcounts[q * compact_blocks(N) + block]; launch_block_prefix then turns
-// them into exclusive offsets within each query (counts[q] = total); with
-// items, writes the survivors as Items at off[q] + bcounts[...] + rank;
-// positions >= cap are dropped.
+// bcounts[q * compact_blocks(N) + block] and the pass masks (64 DB
+// trajectories a word) in bits[q * compact_mask_words(N) + word];
+// launch_block_prefix then turns the counts into exclusive offsets within
+// each query (counts[q] = total); with items, reads the masks and writes the
+// survivors as Items at off[q] + bcounts[...] + rank; positions >= cap are
+// dropped.
 int32_t compact_blocks(int32_t n_db);
+int64_t compact_mask_words(int32_t n_db);
 hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, int32_t nq,
-                                SetView db, const float* tau, const int64_t* off, int32_t* bcounts, Item* items,
-                                int64_t cap);
+                                SetView db, const float* tau, const int64_t* off, const float* bvals,
+                                int32_t* bcounts, uint64_t* bits, Item* items, int64_t cap);
 hipError_t launch_block_prefix(hipStream_t st, int32_t* bcounts, int32_t nq, int32_t gx, int32_t* counts);
 // Second, O(n+m) admissible bound over compacted pairs (one wave per pair);
 // survivors are appended to `out` with count *nout (zeroed by the caller:
