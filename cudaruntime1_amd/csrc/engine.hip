@@ -735,11 +735,16 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
   if (p.mode == TS_MODE_REF && (p.measure == TS_DTW || p.measure == TS_FRECHET || p.measure == TS_LCSS)) {
     TS_TRY(ensure(e, B_ENV, qs->dev_points, &env));
   }
-  int32_t* d_idx = nullptr;
-  float* d_val = nullptr;
+  // Results block, laid out as the host staging buffer: indices, values
+  // (padded to 16 bytes), then a copy of the cell / pair counters and the
+  // candidate total, so one copy brings everything back.
+  const int64_t res_words = ((int64_t)Q * k * 2 + 3) / 4 * 4;
+  int32_t* d_out = nullptr;
   uint64_t* top = nullptr;
-  TS_TRY(ensure(e, B_OIDX, (int64_t)Q * k, &d_idx));
-  TS_TRY(ensure(e, B_OVAL, (int64_t)Q * k, &d_val));
+  TS_TRY(ensure(e, B_OIDX, res_words + 2 * kCellBuf, &d_out));
+  int32_t* d_idx = d_out;
+  float* d_val = (float*)(d_out + (int64_t)Q * k);
+  unsigned long long* d_tail = (unsigned long long*)(d_out + res_words);
   TS_TRY(ensure(e, B_HOSTIN, (int64_t)Q * k, &top));
   unsigned long long* cells = nullptr;
   TS_TRY(ensure(e, B_CELLS, kCellBuf, &cells));
@@ -794,13 +799,17 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
       TS_TRY(body());
     }
   }
-  TS_TRY(run(e, "decode", 0, 0, [&] { return launch_decode_keys(e->st, top, (int64_t)Q * k, desc, d_idx, d_val); }));
-  TS_HIP(hipEventRecord(e->call_b, e->st));
   // Results, the capacity check and the cell counters come back through the
-  // pinned staging buffer, copied asynchronously behind the last kernel.
+  // pinned staging buffer in one copy behind the last kernel.
   const bool check_need = use_graphs && p.mode == TS_MODE_EXACT && !all_pairs;
   const int used = stats ? std::min(e->cells_used, kCellSlots) : 0;
-  const size_t res_bytes = (((size_t)Q * k * 8) + 15) / 16 * 16;
+  const bool tail = check_need || used > 0;
+  TS_TRY(run(e, "decode", 0, 0, [&] {
+    return launch_decode_keys(e->st, top, (int64_t)Q * k, desc, d_idx, d_val, tail ? cells : nullptr, d_tail,
+                              kCellBuf);
+  }));
+  TS_HIP(hipEventRecord(e->call_b, e->st));
+  const size_t res_bytes = (size_t)res_words * 4;
   const size_t stage_bytes = res_bytes + sizeof(unsigned long long) * (2 * kCellSlots + 1);
   if (e->h_stage_bytes < stage_bytes) {
     if (e->h_stage) TS_HIP(hipHostFree(e->h_stage));
@@ -813,14 +822,7 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
   float* h_val = (float*)(e->h_stage + (size_t)Q * k * 4);
   unsigned long long* h_cells = (unsigned long long*)(e->h_stage + res_bytes);
   unsigned long long* h_need = h_cells + 2 * kCellSlots;
-  TS_HIP(hipMemcpyAsync(h_idx, d_idx, sizeof(int32_t) * Q * k, hipMemcpyDeviceToHost, e->st));
-  TS_HIP(hipMemcpyAsync(h_val, d_val, sizeof(float) * Q * k, hipMemcpyDeviceToHost, e->st));
-  if (check_need) TS_HIP(hipMemcpyAsync(h_need, e->need_dev, sizeof(unsigned long long), hipMemcpyDeviceToHost, e->st));
-  if (used > 0) {
-    TS_HIP(hipMemcpyAsync(h_cells, cells, sizeof(unsigned long long) * used, hipMemcpyDeviceToHost, e->st));
-    TS_HIP(hipMemcpyAsync(h_cells + kCellSlots, cells + kCellSlots, sizeof(unsigned long long) * used,
-                          hipMemcpyDeviceToHost, e->st));
-  }
+  TS_HIP(hipMemcpyAsync(e->h_stage, d_out, tail ? stage_bytes : res_bytes, hipMemcpyDeviceToHost, e->st));
   TS_HIP(hipStreamSynchronize(e->st));
   std::memcpy(out_idx, h_idx, sizeof(int32_t) * Q * k);
   std::memcpy(out_val, h_val, sizeof(float) * Q * k);

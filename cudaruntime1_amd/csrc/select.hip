@@ -7,9 +7,10 @@
 // (value descending, index ascending) for LCSS (std::stable_sort, lcss.cpp:
 // 67-70). Keys are unique, so the top-k set is well defined under ties.
 //
-// Selection is a tree of chunk sorts: each block bitonic-sorts up to 2048
-// keys of one segment in LDS and keeps the best k; stages repeat until one
-// chunk per segment remains.
+// Selection is a tree of chunk selections: each block keeps the best k of up
+// to 2048 keys of one segment (k <= 32: k rounds of a wave-wide minimum per
+// quarter of the chunk, then over the quarters' 4k; larger k: a bitonic sort
+// in LDS); stages repeat until one chunk per segment remains.
 #include "ts_internal.h"
 
 namespace ts {
@@ -258,10 +259,16 @@ hipError_t select_topk(hipStream_t st, const uint64_t* in, int64_t seg_stride, i
   }
 }
 
+// Threads past n copy tail_src[0, ntail) to tail_dst (counters that leave
+// the device in the same copy as the results).
 __global__ void __launch_bounds__(256) k_decode(const uint64_t* keys, int64_t n, int32_t desc, int32_t* idx,
-                                                float* val) {
+                                                float* val, const unsigned long long* tail_src,
+                                                unsigned long long* tail_dst, int32_t ntail) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n) {
+    if (i - n < ntail) tail_dst[i - n] = tail_src[i - n];
+    return;
+  }
   uint64_t k = keys[i];
   if (k == ~0ull) {
     if (idx) idx[i] = -1;
@@ -273,10 +280,12 @@ __global__ void __launch_bounds__(256) k_decode(const uint64_t* keys, int64_t n,
 }
 
 hipError_t launch_decode_keys(hipStream_t st, const uint64_t* keys, int64_t n, int32_t descending, int32_t* out_idx,
-                              float* out_val) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_decode, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys, n, descending, out_idx,
-                     out_val);
+                              float* out_val, const unsigned long long* tail_src, unsigned long long* tail_dst,
+                              int32_t ntail) {
+  if (!tail_src) ntail = 0;
+  if (n + ntail == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_decode, dim3((unsigned)((n + ntail + 255) / 256)), dim3(256), 0, st, keys, n, descending,
+                     out_idx, out_val, tail_src, tail_dst, ntail);
   return hipGetLastError();
 }
 

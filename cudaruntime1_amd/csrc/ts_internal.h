@@ -181,9 +181,11 @@ hipError_t select_topk(hipStream_t st, const uint64_t* in, int64_t seg_stride, i
 hipError_t launch_keys_from_items(hipStream_t st, const float* vals, const Item* items, int64_t n, int32_t descending,
                                   uint64_t* keys);
 int64_t select_scratch_elems(int32_t seg_len, int32_t nseg, int32_t k);
-// Decode sorted keys into (idx, value) with optional index remap.
+// Decode sorted keys into (idx, value); with tail_src, also copies ntail
+// counters to tail_dst.
 hipError_t launch_decode_keys(hipStream_t st, const uint64_t* keys, int64_t n, int32_t descending,
-                              int32_t* out_idx, float* out_val);
+                              int32_t* out_idx, float* out_val, const unsigned long long* tail_src = nullptr,
+                              unsigned long long* tail_dst = nullptr, int32_t ntail = 0);
 // Threshold per segment: value of key t (the k-th smallest), or +inf / -inf
 // if the segment has fewer keys.
 // tau[q] = k-th best of query q's s refined seed values (vals[q*s + i],
