@@ -91,6 +91,74 @@ __global__ void k_count_cells(const Item* items, const int32_t* nitems, int64_t 
   }
 }
 
+// Everything a refine needs before its DP launches, in one pass over its
+// items: with FROM_KEYS the items are first built from top-k keys (key i of
+// query q0 + i / s -> Item{q0 + i / s, index or -1, slot i}; seed_idx[i] =
+// the index, vals[i] = NaN); then cells and pairs per DP bucket as in
+// k_count_cells and, with outb, each item's place in its bucket's two-ended
+// list as in k_items_by_bucket. The list counters must be zero on entry.
+template <bool FROM_KEYS>
+__global__ void __launch_bounds__(256) k_refine_prep(const uint64_t* keys, int32_t s, int32_t q0, int32_t* seed_idx,
+                                                     float* nan_vals, Item* items, const int32_t* nitems,
+                                                     int64_t max_items, SetView q, SetView db, MeasureArgs ma,
+                                                     int32_t by_bucket, unsigned long long* acc,
+                                                     unsigned long long* acc_pairs, Item* outb, int64_t cap_each,
+                                                     int32_t* heads, int32_t* tails, BucketCells long_cells) {
+  __shared__ unsigned long long part[kDpBuckets], partp[kDpBuckets];
+  if (threadIdx.x < kDpBuckets) {
+    part[threadIdx.x] = 0;
+    partp[threadIdx.x] = 0;
+  }
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  int64_t n = max_items;
+  if (nitems) n = min<int64_t>(n, (int64_t)*nitems);
+  Item it{0, -1, 0, 0};
+  if (i < n) {
+    if (FROM_KEYS) {
+      const uint64_t k = keys[i];
+      it = Item{q0 + (int32_t)(i / s), k == ~0ull ? -1 : (int32_t)(uint32_t)k, (int32_t)i, 0};
+      items[i] = it;
+      seed_idx[i] = it.d;
+      nan_vals[i] = __int_as_float(0x7fc00000);
+    } else {
+      it = items[i];
+    }
+  }
+  int b = -1;
+  bool lng = false;
+  if (it.d >= 0) {
+    const int na = q.len[it.q], nb = db.len[it.d];
+    b = by_bucket ? dp_bucket_of(dp_width_of(na, nb, ma)) : 0;
+    atomicAdd(&part[b], pair_cells(na, nb, ma.measure, ma.band, ma.delta));
+    atomicAdd(&partp[b], 1ull);
+    lng = (int64_t)na * nb >= long_cells.t[b];
+  }
+  if (outb) {  // grid-uniform
+    const int lane = threadIdx.x & 63;
+    for (int bb = 0; bb < kDpBuckets; ++bb) {
+      for (int e = 0; e < 2; ++e) {
+        const bool mine = b == bb && lng == (e == 0);
+        const uint64_t bal = __ballot(mine);
+        if (bal == 0) continue;
+        const int leader = __ffsll((unsigned long long)bal) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(e == 0 ? &heads[bb] : &tails[bb], __popcll(bal));
+        base = __shfl(base, leader);
+        if (mine) {
+          const int64_t pos = base + __popcll(bal & ((1ull << lane) - 1ull));
+          if (pos < cap_each) outb[bb * cap_each + (e == 0 ? pos : cap_each - 1 - pos)] = it;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kDpBuckets && partp[threadIdx.x] != 0) {
+    atomicAdd(acc + threadIdx.x, part[threadIdx.x]);
+    atomicAdd(acc_pairs + threadIdx.x, partp[threadIdx.x]);
+  }
+}
+
 __global__ void k_fill_f32(float* p, float v, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i < n) p[i] = v;
@@ -357,11 +425,24 @@ ts_status check_params(const ts_params* p, const ts_set* db, const ts_set* qs) {
   return TS_OK;
 }
 
+// Items a seed refine builds on the device from the seeds' top-k keys
+// (k_refine_prep<true>): into `items`, the DB indices into seed_idx.
+struct KeyItems {
+  const uint64_t* keys;
+  int32_t s, q0;
+  int32_t* seed_idx;
+};
+
 // Refine a device item list into vals[slot]. nitems_dev may be null (all
-// max_items valid positions; items with d < 0 are skipped).
-ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_set* qs, const Item* items,
-                 const int32_t* nitems_dev, int64_t max_items, float* vals, bool seed = false) {
+// max_items valid positions; items with d < 0 are skipped). ctr_zeroed: a
+// set of 32 DP counters the caller has already zeroed, which lets one
+// kernel prepare the refine (required with from_keys); otherwise the
+// refine's own set is zeroed by its cell count.
+ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_set* qs, Item* items,
+                 const int32_t* nitems_dev, int64_t max_items, float* vals, bool seed = false,
+                 int32_t* ctr_zeroed = nullptr, const KeyItems* from_keys = nullptr) {
   if (max_items == 0) return TS_OK;
+  if (from_keys && !ctr_zeroed) return fail(TS_ERR_INVALID, "items from keys need zeroed counters");
   const SetView qv = qs->view(), dv = db->view();
   const MeasureArgs ma = margs(p);
   const int cus = device_cu_count();
@@ -375,15 +456,57 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   const bool dp = p.measure == TS_DTW || p.measure == TS_FRECHET || p.measure == TS_LCSS;
   // DP counters: [0, B) work counters of the bucket launches, [B, 2B) and
   // [2B, 3B) the heads and tails of the bucket lists
-  int32_t* ctr = nullptr;
-  TS_TRY(ensure(e, B_CTR, 32, &ctr));
+  int32_t* ctr = ctr_zeroed;
+  if (!ctr) TS_TRY(ensure(e, B_CTR, 32, &ctr));
   static_assert(3 * kDpBuckets <= 32, "DP counters");
-  TS_TRY(run(e, "count_cells", 0, 0, [&] {
-    hipLaunchKernelGGL(k_count_cells, dim3((unsigned)((max_items + 255) / 256)), dim3(256), 0, e->st, items,
-                       nitems_dev, max_items, qv, dv, ma, dp ? 1 : 0, cells + slot, cells + kCellSlots + slot, ctr,
-                       dp ? 3 * kDpBuckets : 0);
-    return hipGetLastError();
-  }));
+  // Buckets of the row width the DP visits: one persistent launch per bucket
+  // that can occur for these two sets (the width grows with both lengths).
+  const int32_t hi = std::min(qs->max_len, db->max_len);
+  const int32_t wlo = dp_width_of(qs->min_len, db->min_len, ma), whi = dp_width_of(qs->max_len, db->max_len, ma);
+  const int b_lo = dp_bucket_of(wlo), b_hi = dp_bucket_of(whi);
+  const int32_t rmax = std::max(qs->max_len, db->max_len);  // longest row of any pair
+  if (dp && p.measure == TS_DTW && p.band != 0 && (int64_t)rmax * rmax >= (1ll << 31))
+    return fail(TS_ERR_UNSUPPORTED, "band needs trajectory lengths below 46341");
+  // With several buckets, each gets a work list, longest pairs first: pairs
+  // of at least half the bucket's largest n*m lead it.
+  const bool lists = dp && b_lo != b_hi;
+  Item* itemsb = nullptr;
+  BucketCells long_cells{};
+  if (lists) {
+    TS_TRY(ensure(e, B_ITEMS3, (int64_t)kDpBuckets * max_items, &itemsb));
+    for (int b = 0; b < kDpBuckets; ++b) {
+      const int64_t cb = dp_ranged(ma) ? hi : std::min<int64_t>(kDpBucketCap[b], hi);
+      long_cells.t[b] = cb * rmax / 2;
+    }
+  }
+  if (ctr_zeroed) {
+    TS_TRY(run(e, "refine_prep", 0, 0, [&] {
+      const dim3 grid((unsigned)((max_items + 255) / 256));
+      if (from_keys)
+        hipLaunchKernelGGL(k_refine_prep<true>, grid, dim3(256), 0, e->st, from_keys->keys, from_keys->s,
+                           from_keys->q0, from_keys->seed_idx, vals, items, nitems_dev, max_items, qv, dv, ma,
+                           dp ? 1 : 0, cells + slot, cells + kCellSlots + slot, lists ? itemsb : nullptr, max_items,
+                           ctr + kDpBuckets, ctr + 2 * kDpBuckets, long_cells);
+      else
+        hipLaunchKernelGGL(k_refine_prep<false>, grid, dim3(256), 0, e->st, nullptr, 1, 0, nullptr, nullptr, items,
+                           nitems_dev, max_items, qv, dv, ma, dp ? 1 : 0, cells + slot, cells + kCellSlots + slot,
+                           lists ? itemsb : nullptr, max_items, ctr + kDpBuckets, ctr + 2 * kDpBuckets, long_cells);
+      return hipGetLastError();
+    }));
+  } else {
+    TS_TRY(run(e, "count_cells", 0, 0, [&] {
+      hipLaunchKernelGGL(k_count_cells, dim3((unsigned)((max_items + 255) / 256)), dim3(256), 0, e->st, items,
+                         nitems_dev, max_items, qv, dv, ma, dp ? 1 : 0, cells + slot, cells + kCellSlots + slot,
+                         ctr, dp ? 3 * kDpBuckets : 0);
+      return hipGetLastError();
+    }));
+    if (lists) {
+      TS_TRY(run(e, "items_by_bucket", 0, 0, [&] {
+        return launch_items_by_bucket(e->st, items, nitems_dev, max_items, qv, dv, ma, itemsb, max_items,
+                                      ctr + kDpBuckets, ctr + 2 * kDpBuckets, long_cells);
+      }));
+    }
+  }
   if (p.measure == TS_HAUSDORFF) {
     return run(e, seed ? "hausdorff_refine_seed" : "hausdorff_refine", 0, 0, [&] {
       return launch_hausdorff(e->st, qv, dv, items, nitems_dev, max_items, vals);
@@ -394,15 +517,7 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
       return launch_euclidean(e->st, qv, dv, items, nitems_dev, max_items, vals);
     }, slot);
   }
-  // Wavefront DP: split by bucket of the row width the DP visits, one
-  // persistent launch per bucket that can occur for these two sets (the
-  // width grows with both lengths).
-  const int32_t lo = std::min(qs->min_len, db->min_len), hi = std::min(qs->max_len, db->max_len);
-  const int32_t wlo = dp_width_of(qs->min_len, db->min_len, ma), whi = dp_width_of(qs->max_len, db->max_len, ma);
-  const int b_lo = dp_bucket_of(wlo), b_hi = dp_bucket_of(whi);
-  const int32_t rmax = std::max(qs->max_len, db->max_len);  // longest row of any pair
-  if (p.measure == TS_DTW && p.band != 0 && (int64_t)std::max(qs->max_len, db->max_len) * std::max(qs->max_len, db->max_len) >= (1ll << 31))
-    return fail(TS_ERR_UNSUPPORTED, "band needs trajectory lengths below 46341");
+  // Wavefront DP, one launch per bucket.
   static const char* kNames[2][3] = {{"dtw_wavefront", "frechet_wavefront", "lcss_wavefront"},
                                      {"dtw_wavefront_seed", "frechet_wavefront_seed", "lcss_wavefront_seed"}};
   const char* nm = kNames[seed ? 1 : 0][p.measure == TS_DTW ? 0 : (p.measure == TS_FRECHET ? 1 : 2)];
@@ -416,19 +531,6 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
                        cus * tune[b_lo].waves_per_cu);
     }, slot + b_lo);
   }
-  Item* itemsb = nullptr;
-  TS_TRY(ensure(e, B_ITEMS3, (int64_t)kDpBuckets * max_items, &itemsb));
-  // Longest pairs first: in each bucket, pairs of at least half the bucket's
-  // largest n*m lead its list.
-  BucketCells long_cells;
-  for (int b = 0; b < kDpBuckets; ++b) {
-    const int64_t cb = dp_ranged(ma) ? hi : std::min<int64_t>(kDpBucketCap[b], hi);
-    long_cells.t[b] = cb * rmax / 2;
-  }
-  TS_TRY(run(e, "items_by_bucket", 0, 0, [&] {
-    return launch_items_by_bucket(e->st, items, nitems_dev, max_items, qv, dv, ma, itemsb, max_items,
-                                  ctr + kDpBuckets, ctr + 2 * kDpBuckets, long_cells);
-  }));
   // Buckets run concurrently on their own streams (fork/join by events), so
   // the latency-bound tail of one overlaps the others.
   TS_HIP(hipEventRecord(e->fork, e->st));
@@ -554,6 +656,10 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   TS_TRY(ensure(e, B_ITEMS, (int64_t)nq * s, &items));
   TS_TRY(ensure(e, B_COUNTS, nq + 1, &counts));
   TS_TRY(ensure(e, B_REGION, 2 * (int64_t)(nq + 1), (int32_t**)&d_off));
+  // DP counter sets of the seed refine [0, 32) and the main refine [32, 64),
+  // zeroed by the seed pass
+  int32_t* ctrs = nullptr;
+  TS_TRY(ensure(e, B_CTR, 64, &ctrs));
   const double pairs = (double)nq * N;
   // The seed pass keeps every pair's bound for the compaction's count pass
   // when the batch is small enough; larger batches evaluate them again.
@@ -561,11 +667,11 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   if ((int64_t)nq * N <= kStoredBoundsMax) TS_TRY(ensure(e, B_BOUNDS, (int64_t)nq * N, &bvals));
   // 1. seeds: good candidates by bound, refined exactly
   TS_TRY(run(e, "bound_seed", pairs * 36.0, pairs, [&] {
-    return launch_bound_chunks(e->st, ma, k, qv, q0, nq, dv, s, bound_chunk_items(s), bkeys, bvals);
+    return launch_bound_chunks(e->st, ma, k, qv, q0, nq, dv, s, bound_chunk_items(s), bkeys, bvals, ctrs, 64);
   }));
   TS_TRY(topk(e, bkeys, (int64_t)nch * s, nch * s, nullptr, nq, s, seeds));
-  TS_TRY(run(e, "items", 0, 0, [&] { return launch_items_from_keys(e->st, seeds, nq, s, q0, items, seed_idx, vals); }));
-  TS_TRY(refine(e, p, db, qs, items, nullptr, (int64_t)nq * s, vals, true));
+  const KeyItems seed_items{seeds, s, q0, seed_idx};
+  TS_TRY(refine(e, p, db, qs, items, nullptr, (int64_t)nq * s, vals, true, ctrs, &seed_items));
   // 2. threshold: k-th best exact value among the seeds
   TS_TRY(run(e, "threshold", 0, 0, [&] { return launch_seed_tau(e->st, vals, seed_idx, nq, s, k, desc, tau); }));
   // 3. compaction of every pair that can still reach tau: count, size each
@@ -641,7 +747,7 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
     return launch_tight_filter(e->st, ma, k, qv, q0, dv, tau, citems, ntot, total, items2, nitems2, rvals,
                                device_cu_count() * 16);
   }));
-  TS_TRY(refine(e, p, db, qs, items2, nitems2, cap, rvals));
+  TS_TRY(refine(e, p, db, qs, items2, nitems2, cap, rvals, false, ctrs + 32));
   // 4. top-k over each query's refined candidates (slot = position in citems)
   TS_TRY(run(e, "keys", 0, 0, [&] { return launch_keys_from_items(e->st, rvals, citems, cap, desc, rkeys); }));
   // Region lengths: clamped to the capacity (device-sized), or the counted

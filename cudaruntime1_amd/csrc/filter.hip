@@ -245,10 +245,13 @@ __device__ void bitonic_sort_lds(uint64_t* s, int n, int tid, int nthreads) {
 // those (s <= 256). Larger s sorts the whole chunk.
 template <bool WHOLE, int MEAS>
 __global__ void __launch_bounds__(256) k_bound_chunks(MeasureArgs ma, SetView q, int32_t q0, SetView db,
-                                                      int32_t s, uint64_t* keys, float* bstore) {
+                                                      int32_t s, uint64_t* keys, float* bstore, int32_t* zero,
+                                                      int32_t nzero) {
   constexpr int PER = WHOLE ? kSeedPerThreadWhole : kSeedPerThread;
   constexpr int NS = WHOLE ? 256 * PER : 256;
   __shared__ uint64_t sk[NS];
+  // counters the batch's later kernels expect to start at zero
+  if (blockIdx.x == 0 && blockIdx.y == 0 && (int)threadIdx.x < nzero) zero[threadIdx.x] = 0;
   const int qi = q0 + blockIdx.y;
   QSum qs{q.mbr[qi], q.ends[qi], q.len[qi], q.off[qi]};
   const bool desc = ma.measure == TS_LCSS;
@@ -279,15 +282,18 @@ __global__ void __launch_bounds__(256) k_bound_chunks(MeasureArgs ma, SetView q,
 }
 
 hipError_t launch_bound_chunks(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, int32_t nq,
-                               SetView db, int32_t s, int32_t chunk, uint64_t* keys, float* bstore) {
+                               SetView db, int32_t s, int32_t chunk, uint64_t* keys, float* bstore, int32_t* zero,
+                               int32_t nzero) {
   (void)k;
   (void)chunk;
   const int32_t items = bound_chunk_items(s);
   dim3 grid((db.count + items - 1) / items, nq);
 #define TS_CHUNKS(MEAS)                                                                              \
   if (s <= 256)                                                                                     \
-    hipLaunchKernelGGL((k_bound_chunks<false, MEAS>), grid, dim3(256), 0, st, m, q, q0, db, s, keys, bstore); \
-  else hipLaunchKernelGGL((k_bound_chunks<true, MEAS>), grid, dim3(256), 0, st, m, q, q0, db, s, keys, bstore);
+    hipLaunchKernelGGL((k_bound_chunks<false, MEAS>), grid, dim3(256), 0, st, m, q, q0, db, s, keys, bstore, zero, \
+                       nzero);                                                                          \
+  else hipLaunchKernelGGL((k_bound_chunks<true, MEAS>), grid, dim3(256), 0, st, m, q, q0, db, s, keys, bstore,   \
+                          zero, nzero);
   switch (m.measure) {
     case TS_DTW: TS_CHUNKS(TS_DTW) break;
     case TS_FRECHET: TS_CHUNKS(TS_FRECHET) break;
@@ -470,7 +476,7 @@ hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetVie
 //   Hausdorff >= max( max_i d(q_i,B_c), max_j d(c_j,B_q) );
 //             and then, with B_c the nearest of the 64-point tiles of c
 //             instead of its MBR, the same forms again (tile_side_*; DTW
-//             without a band likewise by tile_side_sums2);
+//             without a band likewise by tile_side_sum / tile_side_sums2);
 //   Euclidean >= sqrt(sum_i d(q_i,B_c)^2) (double, discounted 1e-6);
 //   LCSS      <= min(#q points in c's eps-box, #c points in q's eps-box,
 //                    min(n,m)) / min(n,m); with a window delta, the box of
@@ -556,6 +562,31 @@ __device__ float band_side_sum(const float* ax, const float* ay, int na, const f
   return total;
 }
 
+// Unbanded DTW form of one side for long pairs: row i of `a` has a path cell
+// somewhere in b, so it costs at least the distance from a_i to the nearest
+// of the boxes that tile b (64-point superblocks at blocks 0, 8, 16, ...).
+// Sum over the points of a except the first and last, 64 rows at a time,
+// returning early once (total + base) * disc exceeds limit.
+__device__ float tile_side_sum(const float* ax, const float* ay, int na, const float4* bsblk, int nb, float base,
+                               float disc, float limit) {
+  const int lane = threadIdx.x & 63;
+  const int nblk = (nb + 7) >> 3;
+  float total = 0.0f;
+  for (int i0 = 1; i0 < na - 1; i0 += 64) {
+    const int i = i0 + lane;
+    float acc = 0.0f;
+    if (i < na - 1) {
+      const float px = ax[i], py = ay[i];
+      float best = pt_box(px, py, bsblk[0]);
+      for (int kb = 8; kb < nblk; kb += 8) best = fminf(best, pt_box(px, py, bsblk[kb]));
+      acc = sqrt_lo(best);
+    }
+    total += wave_sum(acc);
+    if ((total + base) * disc > limit) break;  // uniform: total is the same in every lane
+  }
+  return total;
+}
+
 // Sum over the wave (all lanes active) by DPP: prefix sums within rows of 16,
 // rows combined by row_bcast 15 and 31; lane 63's total returned to every lane.
 template <int CTRL, int ROWS>
@@ -572,11 +603,9 @@ __device__ __forceinline__ float wave_sum_dpp(float v) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
-// Unbanded DTW form: row i of `a` has a path cell somewhere in b, so it
-// costs at least the distance from a_i to the nearest of the boxes that tile
-// b (64-point superblocks at blocks 0, 8, 16, ...); likewise every column.
-// Both sides in one pass: the interior points of a (all but the first and
-// last) against b's superblock boxes into sa, those of b against a's into sb. The
+// The same form for short pairs, both sides in one pass: the interior points
+// of a (all but the first and last) against b's superblock boxes into sa,
+// those of b against a's into sb. The
 // nearest superblock box lies inside the MBR, so each side also bounds the
 // MBR form from above. Loads of both trajectories and of up to 8 superblock
 // boxes of each go out together; one reduction per side. The f32 sums are in
@@ -686,6 +715,8 @@ __device__ int window_side_count(const float* ax, const float* ay, int na, const
 // one atomic per block at the end (a single hot counter per pair serialises
 // in L2); a full buffer sends the rest directly.
 constexpr int kTightBuf = 256;
+// Unbanded DTW pairs up to this length skip the whole-MBR form.
+constexpr int kTiledDirect = 256;
 
 template <int MEAS>
 __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k, SetView q, int32_t q0, SetView db,
@@ -750,8 +781,9 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
       side_bound<MEAS>(q.x + qo, q.y + qo, n, cb, ma.eps, false, f, s, c);
       s = wave_sum_d(s);
       pass = (float)(sqrt(s) * (1.0 - 1e-6)) <= t;
-    } else if (MEAS == TS_DTW && ma.band == 0 && n >= 3 && m >= 3) {
-      // tiled form of both sides directly (it implies the MBR form)
+    } else if (MEAS == TS_DTW && ma.band == 0 && n >= 3 && m >= 3 && max(n, m) <= kTiledDirect) {
+      // short pairs: the tiled form of both sides directly (it implies the
+      // MBR form, and a point meets at most 4 tiles)
       const float dx0 = qe.x - ce.x, dy0 = qe.y - ce.y, dx1 = qe.z - ce.z, dy1 = qe.w - ce.w;
       const float e = sqrt_lo(dx0 * dx0 + dy0 * dy0) + sqrt_lo(dx1 * dx1 + dy1 * dy1);
       const float disc = 1.0f - (float)(max(n, m) + 72) * 1.1920929e-7f;
@@ -779,6 +811,16 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
         return fmaxf(r1, r2) * disc;
       };
       pass = bound(a1, a2) <= t;
+      if (pass && ma.band == 0 && n >= 3 && m >= 3) {
+        // The whole-MBR form passed: the tiled form (nearest superblock box)
+        // is tighter and costs O(m / 64) per point.
+        const float b1 = tile_side_sum(q.x + qo, q.y + qo, n, db.sblk + (co >> 3), m, e0 + e1, disc, t);
+        pass = (b1 + e0 + e1) * disc <= t;
+        if (pass) {
+          const float b2 = tile_side_sum(db.x + co, db.y + co, m, q.sblk + (qo >> 3), n, e0 + e1, disc, t);
+          pass = bound(b1, b2) <= t;
+        }
+      }
       if (pass && ma.band != 0 && n >= 3 && m >= 3) {
         // The whole-MBR form passed: the banded form (row windows of the
         // band) is tighter and costs O(window / 64) per point. Each side alone

@@ -137,9 +137,11 @@ hipError_t launch_filter_ref(hipStream_t st, int32_t measure, SetView q, int32_t
 // whole DB, reduced per DB chunk of bound_chunk_items(s) trajectories to the
 // best `s` keys per query: keys[(q * nchunks + chunk) * s + t].
 int32_t bound_chunk_items(int32_t s);
-// With bstore, also stores every pair's bound at bstore[q * N + j].
+// With bstore, also stores every pair's bound at bstore[q * N + j]; zeroes
+// zero[0, nzero) (nzero <= 256) for the batch's later kernels.
 hipError_t launch_bound_chunks(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0,
-                               int32_t nq, SetView db, int32_t s, int32_t chunk, uint64_t* keys, float* bstore);
+                               int32_t nq, SetView db, int32_t s, int32_t chunk, uint64_t* keys, float* bstore,
+                               int32_t* zero = nullptr, int32_t nzero = 0);
 // Threshold compaction of every DB trajectory whose bound can still reach
 // the query's threshold tau[q] (bounds read from bvals[q * N + j] when given,
 // else evaluated), in compact_blocks(N) blocks per query:
