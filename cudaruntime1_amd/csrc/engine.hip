@@ -826,6 +826,19 @@ std::vector<int64_t> set_key(const ts_set* s) {
           (int64_t)(intptr_t)s->off};
 }
 
+// Waits for a k-NN call's stream by polling it: the call's host thread has
+// nothing else to do, and a poll sees the end of the last copy sooner than a
+// blocking wait's wake-up.
+ts_status wait_stream(hipStream_t st) {
+  for (;;) {
+    const hipError_t r = hipStreamQuery(st);
+    if (r == hipSuccess) return TS_OK;
+    if (r != hipErrorNotReady) {
+      TS_HIP(r);
+    }
+  }
+}
+
 ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_params* pp, int32_t* out_idx,
                    float* out_val, ts_stats* stats, bool plain, int retry = 0) {
   auto t0 = std::chrono::steady_clock::now();
@@ -929,7 +942,7 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
   unsigned long long* h_cells = (unsigned long long*)(e->h_stage + res_bytes);
   unsigned long long* h_need = h_cells + 2 * kCellSlots;
   TS_HIP(hipMemcpyAsync(e->h_stage, d_out, tail ? stage_bytes : res_bytes, hipMemcpyDeviceToHost, e->st));
-  TS_HIP(hipStreamSynchronize(e->st));
+  TS_TRY(wait_stream(e->st));
   std::memcpy(out_idx, h_idx, sizeof(int32_t) * Q * k);
   std::memcpy(out_val, h_val, sizeof(float) * Q * k);
   if (check_need) {
