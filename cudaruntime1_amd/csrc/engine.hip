@@ -868,14 +868,9 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
   unsigned long long* cells = nullptr;
   TS_TRY(ensure(e, B_CELLS, kCellBuf, &cells));
   e->need_dev = cells + 2 * kCellSlots;
-  TS_TRY(run(e, "fill", 0, 0, [&] { return zero_cells(e->st, cells); }));
   if (!e->call_a) {
     TS_HIP(hipEventCreate(&e->call_a));
     TS_HIP(hipEventCreate(&e->call_b));
-  }
-  TS_HIP(hipEventRecord(e->call_a, e->st));
-  if (env) {
-    TS_TRY(run(e, "envelope", 0, 0, [&] { return launch_envelope(e->st, qs->view(), qs->max_len, p.r, env); }));
   }
   // Queries per batch: bounded by the dense REF score matrix or by the
   // EXACT candidate regions.
@@ -897,37 +892,10 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
   // replays 1.28-1.48 s against 1.25 s eager, with the DP buckets' parallel
   // branches) batches launch eagerly.
   const bool use_graphs = e->graphs && !e->timing && !plain && (int64_t)bq * N <= kGraphMaxPairs;
-  for (int32_t q0 = 0; q0 < Q; q0 += bq) {
-    const int32_t nq = std::min(bq, Q - q0);
-    uint64_t* topq = top + (int64_t)q0 * k;
-    auto body = [&]() -> ts_status {
-      if (p.mode == TS_MODE_REF) return knn_ref_batch(e, p, db, qs, q0, nq, env, topq);
-      if (all_pairs) return knn_all_batch(e, p, db, qs, q0, nq, topq);
-      return knn_exact_batch(e, p, db, qs, q0, nq, topq);
-    };
-    if (use_graphs) {
-      std::vector<int64_t> key = set_key(db);
-      const std::vector<int64_t> qk = set_key(qs);
-      key.insert(key.end(), qk.begin(), qk.end());
-      int32_t eps_bits;
-      std::memcpy(&eps_bits, &p.epsilon, 4);
-      key.insert(key.end(), {p.measure, p.mode, p.k, p.kc, p.r, p.band, p.delta, eps_bits, q0, nq,
-                             (int64_t)(intptr_t)env, (int64_t)(intptr_t)topq});
-      TS_TRY(run_graph(e, key, body));
-    } else {
-      TS_TRY(body());
-    }
-  }
   // Results, the capacity check and the cell counters come back through the
   // pinned staging buffer in one copy behind the last kernel.
   const bool check_need = use_graphs && p.mode == TS_MODE_EXACT && !all_pairs;
-  const int used = stats ? std::min(e->cells_used, kCellSlots) : 0;
-  const bool tail = check_need || used > 0;
-  TS_TRY(run(e, "decode", 0, 0, [&] {
-    return launch_decode_keys(e->st, top, (int64_t)Q * k, desc, d_idx, d_val, tail ? cells : nullptr, d_tail,
-                              kCellBuf);
-  }));
-  TS_HIP(hipEventRecord(e->call_b, e->st));
+  const bool tail = check_need || stats != nullptr;
   const size_t res_bytes = (size_t)res_words * 4;
   const size_t stage_bytes = res_bytes + sizeof(unsigned long long) * (2 * kCellSlots + 1);
   if (e->h_stage_bytes < stage_bytes) {
@@ -937,11 +905,66 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
     TS_HIP(hipHostMalloc((void**)&e->h_stage, stage_bytes, hipHostMallocDefault));
     e->h_stage_bytes = stage_bytes;
   }
+  auto batch = [&](int32_t q0, int32_t nq) -> ts_status {
+    uint64_t* topq = top + (int64_t)q0 * k;
+    if (p.mode == TS_MODE_REF) return knn_ref_batch(e, p, db, qs, q0, nq, env, topq);
+    if (all_pairs) return knn_all_batch(e, p, db, qs, q0, nq, topq);
+    return knn_exact_batch(e, p, db, qs, q0, nq, topq);
+  };
+  auto prologue = [&]() -> ts_status {
+    TS_TRY(run(e, "fill", 0, 0, [&] { return zero_cells(e->st, cells); }));
+    if (env) {
+      TS_TRY(run(e, "envelope", 0, 0, [&] { return launch_envelope(e->st, qs->view(), qs->max_len, p.r, env); }));
+    }
+    return TS_OK;
+  };
+  auto epilogue = [&]() -> ts_status {
+    TS_TRY(run(e, "decode", 0, 0, [&] {
+      return launch_decode_keys(e->st, top, (int64_t)Q * k, desc, d_idx, d_val, tail ? cells : nullptr, d_tail,
+                                kCellBuf);
+    }));
+    TS_HIP(hipMemcpyAsync(e->h_stage, d_out, tail ? stage_bytes : res_bytes, hipMemcpyDeviceToHost, e->st));
+    return TS_OK;
+  };
+  auto graph_key = [&](int32_t q0, int32_t nq) {
+    std::vector<int64_t> key = set_key(db);
+    const std::vector<int64_t> qk = set_key(qs);
+    key.insert(key.end(), qk.begin(), qk.end());
+    int32_t eps_bits;
+    std::memcpy(&eps_bits, &p.epsilon, 4);
+    key.insert(key.end(), {p.measure, p.mode, p.k, p.kc, p.r, p.band, p.delta, eps_bits, q0, nq,
+                           (int64_t)(intptr_t)env, (int64_t)(intptr_t)(top + (int64_t)q0 * k)});
+    return key;
+  };
+  TS_HIP(hipEventRecord(e->call_a, e->st));
+  if (use_graphs && Q <= bq) {
+    // One batch: the whole call (counter clear, batch, decode, copy) is one
+    // recorded graph.
+    std::vector<int64_t> key = graph_key(0, Q);
+    key.insert(key.end(), {-1, (int64_t)(intptr_t)d_out, (int64_t)(intptr_t)e->h_stage, tail ? 1 : 0});
+    TS_TRY(run_graph(e, key, [&]() -> ts_status {
+      TS_TRY(prologue());
+      TS_TRY(batch(0, Q));
+      return epilogue();
+    }));
+  } else {
+    TS_TRY(prologue());
+    for (int32_t q0 = 0; q0 < Q; q0 += bq) {
+      const int32_t nq = std::min(bq, Q - q0);
+      if (use_graphs) {
+        TS_TRY(run_graph(e, graph_key(q0, nq), [&] { return batch(q0, nq); }));
+      } else {
+        TS_TRY(batch(q0, nq));
+      }
+    }
+    TS_TRY(epilogue());
+  }
+  TS_HIP(hipEventRecord(e->call_b, e->st));
+  const int used = stats ? std::min(e->cells_used, kCellSlots) : 0;
   int32_t* h_idx = (int32_t*)e->h_stage;
   float* h_val = (float*)(e->h_stage + (size_t)Q * k * 4);
   unsigned long long* h_cells = (unsigned long long*)(e->h_stage + res_bytes);
   unsigned long long* h_need = h_cells + 2 * kCellSlots;
-  TS_HIP(hipMemcpyAsync(e->h_stage, d_out, tail ? stage_bytes : res_bytes, hipMemcpyDeviceToHost, e->st));
   TS_TRY(wait_stream(e->st));
   std::memcpy(out_idx, h_idx, sizeof(int32_t) * Q * k);
   std::memcpy(out_val, h_val, sizeof(float) * Q * k);
