@@ -25,11 +25,13 @@
 //  * Columns run along the shorter trajectory, U per step; a row has at
 //    least W (k_dp) or W+1 (k_dp_lean) chunks so the halo is written before
 //    it is read; shorter rows are padded with idle chunks.
-//  * Two kernels. k_dp_lean runs unbanded pairs with a per-step path cut to
+//  * Three kernels. k_dp_lean runs unbanded pairs with a per-step path cut to
 //    the cells and straight-line bookkeeping, staging the pair's columns and
-//    rows in LDS when they fit (see above k_dp_lean). k_dp runs DTW bands and
-//    LCSS windows: each strip visits only its band's chunk range, the halo is
-//    an LDS ring, columns come from global memory.
+//    rows in LDS when they fit (see above k_dp_lean). k_dp_lean_r runs DTW
+//    bands and LCSS windows with the same step: each strip visits only its
+//    band's chunk range, taken from a strip table in LDS, and the halo is an
+//    LDS ring. k_dp runs the bands and windows whose ring and table do not
+//    fit a wave's LDS share, with columns from global memory.
 //  * Strips and pairs follow each other without draining the pipeline: after
 //    its last row a lane starts its first row of the next pair, which the
 //    group's lane 0 fetched from a device-wide atomic work counter. The
@@ -496,6 +498,19 @@ __host__ __device__ constexpr int lean_group_floats(int cmax, int rmax, bool sc,
   return cmax + (sc ? 4 * cmax : 0) + (sr ? 2 * rmax : 0);
 }
 
+// Next event of a lane of the lean kernels, tested at the top of its loop:
+// the group's fetch at rem == lg, the pair's start at rem == 0, the result at
+// rem + 1 == res_rem. rem falls by one a step, so the first of those it meets
+// is the largest below the count it leaves the event branch with; the loop
+// tests rem == ev instead of the three conditions. INT_MIN when none is left.
+__device__ __forceinline__ int next_event(int rem, int lg, int res_rem) {
+  int ev = INT_MIN;
+  if (0 < rem) ev = 0;
+  if (lg < rem) ev = max(ev, lg);
+  if (res_rem != INT_MIN && res_rem - 1 < rem) ev = max(ev, res_rem - 1);
+  return ev;
+}
+
 // SC / SR: columns / rows staged in LDS; otherwise read from global memory
 // (long trajectories: the launch then needs several waves per SIMD to hide
 // the reads, which the throughput regime has).
@@ -527,6 +542,7 @@ __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
   // Current pair of this lane. Before its first pair lane lg idles lg steps
   // on chunks 0..lg-1 (D = W+1, so k does not wrap).
   int R = 1, C = 1, lastch = 0, D = W + 1, k = 0, row = lg, rem = lg, res_rem = kNoResult, slot = 0;
+  int ev = lg;  // the first event: the group's first fetch
   int gpar = 0;
   const float2* cb = cols;  // SC: this lane's column buffer
   // !SC / !SR: the pair's column / row trajectory in global memory
@@ -546,7 +562,7 @@ __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
 
   for (;;) {
     __builtin_amdgcn_wave_barrier();
-    if (__any(rem == lg || rem == 0 || rem + 1 == res_rem)) {
+    if (__any(rem == ev)) {
       // ---- result: the lane's last step covered the result cell
       if (rem + 1 == res_rem) {
         const int rc = (C - 1) - lastch * U;
@@ -627,6 +643,7 @@ __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
           dfirst = (MEAS == TS_LCSS || lg == 0) ? 0.0f : BND;
         }
       }
+      ev = next_event(rem, lg, res_rem);
       if (__all(done)) break;
     }
     if (wrapped) {
@@ -744,6 +761,7 @@ __global__ void __launch_bounds__(256) k_dp_lean_r(DpArgs a) {
   // Current pair of this lane; before its first pair lane lg idles lg steps
   // on one strip of W+1 chunks.
   int R = 1, C = 1, lastch = 0, S = 1, rem = lg, res_rem = kNoResult, slot = 0;
+  int ev = lg;  // the first event: the group's first fetch
   int strip = 0, k = 0, cs = 0, ce = W + 1, pcs = 0, pce = 0, row = lg;
   int bw = 0, bd = 0;
   int2 nxt = make_int2(0, W + 1);  // the next strip's (cstart, cend)
@@ -763,7 +781,7 @@ __global__ void __launch_bounds__(256) k_dp_lean_r(DpArgs a) {
 
   for (;;) {
     __builtin_amdgcn_wave_barrier();
-    if (__any(rem == lg || rem == 0 || rem + 1 == res_rem)) {
+    if (__any(rem == ev)) {
       // ---- result: the lane's last step covered the result cell
       if (rem + 1 == res_rem) {
         const int rc = (C - 1) - lastch * U;
@@ -874,6 +892,7 @@ __global__ void __launch_bounds__(256) k_dp_lean_r(DpArgs a) {
           load_vec<U>(brow, hv);
         }
       }
+      ev = next_event(rem, lg, res_rem);
       if (__all(done)) break;
     }
     if (wrapped) {

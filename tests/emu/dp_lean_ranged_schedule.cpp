@@ -35,6 +35,16 @@ static const int kCaps[6] = {31, 63, 127, 255, 1023, 1 << 24};
     }                                                            \
   } while (0)
 
+// The kernels' next event (group fetch at rem == lg, pair start at rem == 0,
+// result at rem + 1 == res_rem): the largest trigger below rem, INT_MIN if none.
+static int next_event(int rem, int lg, int res_rem) {
+  int ev = INT_MIN;
+  if (0 < rem) ev = 0;
+  if (lg < rem) ev = std::max(ev, lg);
+  if (res_rem != INT_MIN && res_rem - 1 < rem) ev = std::max(ev, res_rem - 1);
+  return ev;
+}
+
 struct Pair { std::vector<float> a, b; bool dead; };
 struct Tag { float v; int pair, row, chunk; };
 
@@ -72,6 +82,7 @@ static std::vector<float> emulate(const std::vector<Pair>& pairs, int band, int 
   int next = 0;
   struct Lane {
     int R = 1, C = 1, lastch = 0, S = 1, rem, res_rem = INT_MIN, slot = 0, pair = -1;
+    int ev = 0;
     int strip = 0, k = 0, cs = 0, ce = W + 1, pcs = 0, pce = 0, row;
     int bw = 0, bd = 0;
     int nxs = 0, nxe = W + 1, nx_pair = -1;
@@ -97,6 +108,7 @@ static std::vector<float> emulate(const std::vector<Pair>& pairs, int band, int 
     for (int l = 0; l < 64; ++l) {
       w.L[l].row = l & (W - 1);
       w.L[l].rem = l & (W - 1);
+      w.L[l].ev = w.L[l].rem;
       w.L[l].hdiag.v = BND;
       for (int j = 0; j < U; ++j) { w.L[l].o[j] = 0; w.L[l].hv[j] = BND; w.L[l].cx[j] = w.L[l].cy[j] = {0, -1, -1, -1}; }
     }
@@ -117,9 +129,14 @@ static std::vector<float> emulate(const std::vector<Pair>& pairs, int band, int 
     for (auto& w : waves) {
       if (w.fin) continue;
       Lane* L = w.L;
+      // the kernel tests rem == ev; every lane's test must agree with the
+      // three conditions it stands for
       bool any_ev = false;
-      for (int l = 0; l < 64; ++l)
-        any_ev |= L[l].rem == lgof(l) || L[l].rem == 0 || L[l].rem + 1 == L[l].res_rem;
+      for (int l = 0; l < 64; ++l) {
+        const bool three = L[l].rem == lgof(l) || L[l].rem == 0 || L[l].rem + 1 == L[l].res_rem;
+        CHK(three == (L[l].rem == L[l].ev));
+        any_ev |= L[l].rem == L[l].ev;
+      }
       if (any_ev) {
         for (int l = 0; l < 64; ++l) {
           Lane& ln = L[l];
@@ -253,6 +270,7 @@ static std::vector<float> emulate(const std::vector<Pair>& pairs, int band, int 
           ln.ay = gr.rowy[std::min(lg, ln.R - 1)];
           for (int j = 0; j < U; ++j) { ln.hv[j] = BND; ln.hvt[j] = {BND, -2, -1, -1}; }
         }
+        for (int l = 0; l < 64; ++l) L[l].ev = next_event(L[l].rem, lgof(l), L[l].res_rem);
         bool all = true;
         for (int l = 0; l < 64; ++l) all &= L[l].done;
         if (all) {

@@ -33,6 +33,16 @@ enum { DTW = 0, LCSS = 1, FRECHET = 2 };
     }                                                            \
   } while (0)
 
+// The kernels' next event (group fetch at rem == lg, pair start at rem == 0,
+// result at rem + 1 == res_rem): the largest trigger below rem, INT_MIN if none.
+static int next_event(int rem, int lg, int res_rem) {
+  int ev = INT_MIN;
+  if (0 < rem) ev = 0;
+  if (lg < rem) ev = std::max(ev, lg);
+  if (res_rem != INT_MIN && res_rem - 1 < rem) ev = std::max(ev, res_rem - 1);
+  return ev;
+}
+
 struct Pair { std::vector<float> a, b; bool dead; };  // a: query (xy), b: db (xy)
 
 struct Tagged { float v; int pair; int idx; };
@@ -47,6 +57,7 @@ std::vector<float> emulate(const std::vector<Pair>& pairs, float eps, int cmax, 
   int next = 0;
   struct Lane {
     int R = 1, C = 1, lastch = 0, D = W + 1, k = 0, row, rem, res_rem = INT_MIN, slot = 0, par = 0, gpar = 0;
+    int ev = 0;
     bool done = false, wrapped = false;
     int fR = 1, fC = 1, f_slot = 0, f_pair = -1, pair = -1;
     bool f_ok = false, f_live = false, live = false;
@@ -69,6 +80,7 @@ std::vector<float> emulate(const std::vector<Pair>& pairs, float eps, int cmax, 
       Lane& ln = w.L[l];
       ln.row = l & (W - 1);
       ln.rem = l & (W - 1);
+      ln.ev = ln.rem;
       for (int j = 0; j < U; ++j) ln.o[j] = ln.cx[j] = ln.cy[j] = ln.hv[j] = 0;
       ln.dfirst = (MEAS == LCSS || (l & (W - 1)) == 0) ? 0.0f : BND;
     }
@@ -89,9 +101,14 @@ std::vector<float> emulate(const std::vector<Pair>& pairs, float eps, int cmax, 
       ++st;
       Lane* L = w.L;
       auto lgof = [](int l) { return l & (W - 1); };
+      // the kernel tests rem == ev; every lane's test must agree with the
+      // three conditions it stands for
       bool any_ev = false;
-      for (int l = 0; l < 64; ++l)
-        any_ev |= (L[l].rem == lgof(l) || L[l].rem == 0 || L[l].rem + 1 == L[l].res_rem);
+      for (int l = 0; l < 64; ++l) {
+        const bool three = L[l].rem == lgof(l) || L[l].rem == 0 || L[l].rem + 1 == L[l].res_rem;
+        CHK(three == (L[l].rem == L[l].ev));
+        any_ev |= L[l].rem == L[l].ev;
+      }
       if (any_ev) {
         // result from the outputs of the step before
         for (int l = 0; l < 64; ++l) {
@@ -197,6 +214,7 @@ std::vector<float> emulate(const std::vector<Pair>& pairs, float eps, int cmax, 
           ln.hvpair = -2;
           ln.dfirst = (MEAS == LCSS || lg == 0) ? 0.0f : BND;
         }
+        for (int l = 0; l < 64; ++l) L[l].ev = next_event(L[l].rem, lgof(l), L[l].res_rem);
         bool all = true;
         for (int l = 0; l < 64; ++l) all &= L[l].done;
         if (all) {
