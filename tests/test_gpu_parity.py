@@ -132,6 +132,33 @@ def test_long_pairs_bit_exact(eng):
         assert np.array_equal(got.view(np.int32), want.view(np.int32)), (measure, got, want)
 
 
+@pytest.mark.parametrize("variant,measure,kw", [(1, "dtw", {"band": -10}), (1, "dtw", {"band": 205}),
+                                                (4, "lcss", {"epsilon": 2e-3, "delta": 64}),
+                                                (4, "lcss", {"epsilon": 2e-3, "delta": 1000})])
+def test_long_ranged_pairs_every_shape_bit_exact(variant, measure, kw):
+    """Bands and windows on long pairs with every lane-group shape forced.
+    With 16-lane groups, a band 601 columns wide on 3000-point rows needs a
+    1024-float halo ring and a 376-float strip table per group, 22.4 KB for
+    the wave's 4 groups, more than the 20 KiB a wave of the lean banded kernel
+    may take, so those launches run k_dp; so do 16- and 32-lane groups with a
+    window of 1000 (a 2500-float halo row). The other shapes run k_dp_lean_r
+    with both trajectories read from global memory."""
+    db_xy, db_off = walks(24, [2048, 3000, 700])
+    q_xy, q_off = walks(25, [2500, 900])
+    db, qs = ts.TrajSet(db_xy, db_off), ts.TrajSet(q_xy, q_off)
+    qi, di = np.array([0, 0, 1, 1, 0]), np.array([0, 1, 0, 2, 2])
+    p = ts.params(measure, **kw)
+    prm = oracle.params(p.measure, band=p.band, delta=p.delta, epsilon=p.epsilon)
+    want = oracle.pair_values(prm, db_xy, db_off, q_xy, q_off, qi, di)
+    e = ts.Engine()
+    for (W, U) in ts.dp_shapes():
+        say("long ranged pairs", measure, kw, W, U)
+        for b in range(len(ts.DP_BUCKET_CAPS)):
+            e.set_dp_shape(variant, b, W, U, 4)
+        got = e.pair_values(db, qs, p, qi, di)
+        assert np.array_equal(got.view(np.int32), want.view(np.int32)), (W, U, got, want)
+
+
 def test_long_rows_short_columns_bit_exact(eng):
     """Rows too long to stage beside the columns (the unbanded DP then reads
     row points from global memory), and rows and columns both too long."""
