@@ -63,6 +63,11 @@ _sig = {
     "ts_set_reload": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32]),
     "ts_set_destroy": (ctypes.c_int, [_P]),
     "ts_set_info": (ctypes.c_int, [_P, _i32p, ctypes.POINTER(ctypes.c_int64), _i32p, _i32p]),
+    "ts_set_lengths": (ctypes.c_int, [_P, _P]),
+    "ts_set_append": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32]),
+    "ts_set_save": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "ts_set_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_P)]),
+    "ts_db_write": (ctypes.c_int, [ctypes.c_char_p, _P, _P, ctypes.c_int32]),
     "ts_set_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "ts_engine_create": (ctypes.c_int, [ctypes.POINTER(_P)]),
     "ts_engine_destroy": (ctypes.c_int, [_P]),
@@ -170,6 +175,13 @@ def offsets_of(lengths):
     return off
 
 
+def db_write(path, xy, offsets):
+    """Write a tsdb database file from host arrays (no GPU needed)."""
+    xy = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)
+    offsets = np.ascontiguousarray(offsets, np.int64)
+    _check(_lib.ts_db_write(os.fsencode(path), _ptr(xy), _ptr(offsets), len(offsets) - 1), "ts_db_write")
+
+
 def gen_walks(seed, lengths, sigma=1e-3):
     """Random walks: returns (xy float32 [P,2], offsets int64 [count+1])."""
     off = offsets_of(lengths)
@@ -203,6 +215,33 @@ class TrajSet:
         self.count = len(offsets) - 1
         self.offsets = offsets
         self.lengths = np.diff(offsets).astype(np.int32)
+
+    def append(self, xy, offsets):
+        """Append trajectories (offsets[0] == 0) behind the current ones."""
+        xy = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)
+        offsets = np.ascontiguousarray(offsets, np.int64)
+        _check(_lib.ts_set_append(self._h, _ptr(xy), _ptr(offsets), len(offsets) - 1), "ts_set_append")
+        self._sync_lengths()
+
+    def save(self, path):
+        """Write the set to a tsdb database file."""
+        _check(_lib.ts_set_save(self._h, os.fsencode(path)), "ts_set_save")
+
+    @classmethod
+    def load(cls, path):
+        """A device-resident set from a tsdb database file."""
+        self = cls.__new__(cls)
+        self._h = ctypes.c_void_p()
+        _check(_lib.ts_set_load(os.fsencode(path), ctypes.byref(self._h)), "ts_set_load")
+        self._sync_lengths()
+        return self
+
+    def _sync_lengths(self):
+        self.count = self.info()["count"]
+        self.lengths = np.empty(self.count, np.int32)
+        _check(_lib.ts_set_lengths(self._h, _ptr(self.lengths)), "ts_set_lengths")
+        self.offsets = np.zeros(self.count + 1, np.int64)
+        np.cumsum(self.lengths, out=self.offsets[1:])
 
     @property
     def handle(self):

@@ -50,9 +50,9 @@ int device_cu_count() {
 
 // One wave per trajectory: MBR by wave min/max, endpoints from the first and
 // last point.
-__global__ void __launch_bounds__(256) k_summaries(SetView s, float4* mbr, float4* ends) {
+__global__ void __launch_bounds__(256) k_summaries(SetView s, int32_t t0, float4* mbr, float4* ends) {
   const int lane = threadIdx.x & 63;
-  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int t = t0 + blockIdx.x * 4 + (threadIdx.x >> 6);
   if (t >= s.count) return;
   const int64_t o = s.off[t];
   const int n = s.len[t];
@@ -79,8 +79,9 @@ static_assert(kPad == 8, "block boxes take trajectories aligned and padded to 8 
 // Box of each 8-point block (trajectories start on multiples of 8 points
 // and are padded with their last point, so a block holds one trajectory's
 // points only).
-__global__ void __launch_bounds__(256) k_blocks(const float* x, const float* y, int64_t nblk, float4* blk) {
-  const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
+__global__ void __launch_bounds__(256) k_blocks(const float* x, const float* y, int64_t b0, int64_t nblk,
+                                                float4* blk) {
+  const int64_t b = b0 + (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (b >= nblk) return;
   float mnx = x[8 * b], mxx = mnx, mny = y[8 * b], mxy = mny;
 #pragma unroll
@@ -92,17 +93,17 @@ __global__ void __launch_bounds__(256) k_blocks(const float* x, const float* y, 
   blk[b] = make_float4(mnx, mxx, mny, mxy);
 }
 
-hipError_t launch_summaries(hipStream_t st, SetView s, float4* mbr, float4* ends) {
-  if (s.count == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_summaries, dim3((s.count + 3) / 4), dim3(256), 0, st, s, mbr, ends);
+hipError_t launch_summaries(hipStream_t st, SetView s, float4* mbr, float4* ends, int32_t t0) {
+  if (s.count <= t0) return hipSuccess;
+  hipLaunchKernelGGL(k_summaries, dim3((s.count - t0 + 3) / 4), dim3(256), 0, st, s, t0, mbr, ends);
   return hipGetLastError();
 }
 
 // Superblock g: the union of blocks g..g+7 that belong to g's trajectory
 // (fewer near its end). One wave per trajectory.
-__global__ void __launch_bounds__(256) k_superblocks(SetView s, float4* sblk) {
+__global__ void __launch_bounds__(256) k_superblocks(SetView s, int32_t t0, float4* sblk) {
   const int lane = threadIdx.x & 63;
-  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int t = t0 + blockIdx.x * 4 + (threadIdx.x >> 6);
   if (t >= s.count) return;
   const int64_t b0 = s.off[t] >> 3;
   const int nb = (s.len[t] + 7) >> 3;
@@ -116,11 +117,12 @@ __global__ void __launch_bounds__(256) k_superblocks(SetView s, float4* sblk) {
   }
 }
 
-hipError_t launch_blocks(hipStream_t st, SetView s, int64_t points, float4* blk, float4* sblk) {
-  const int64_t nblk = points / 8;
-  if (nblk == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_blocks, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, st, s.x, s.y, nblk, blk);
-  hipLaunchKernelGGL(k_superblocks, dim3((s.count + 3) / 4), dim3(256), 0, st, s, sblk);
+hipError_t launch_blocks(hipStream_t st, SetView s, int64_t points, float4* blk, float4* sblk, int32_t t0,
+                         int64_t point0) {
+  const int64_t nblk = points / 8, b0 = point0 / 8;
+  if (nblk <= b0 || s.count <= t0) return hipSuccess;
+  hipLaunchKernelGGL(k_blocks, dim3((unsigned)((nblk - b0 + 255) / 256)), dim3(256), 0, st, s.x, s.y, b0, nblk, blk);
+  hipLaunchKernelGGL(k_superblocks, dim3((s.count - t0 + 3) / 4), dim3(256), 0, st, s, t0, sblk);
   return hipGetLastError();
 }
 
@@ -138,63 +140,103 @@ static inline uint64_t ctr_hash(uint64_t seed, uint64_t a, uint64_t b) {
 // Uniform in [0,1) with 53 random bits.
 static inline double u01(uint64_t h) { return (double)(h >> 11) * (1.0 / 9007199254740992.0); }
 
-ts_status set_load(ts_set* s, const ts_point* xy, const int64_t* offsets, int32_t count, hipStream_t st) {
+ts_status pad_layout(const ts_point* xy, const int64_t* offsets, int32_t count, int64_t base,
+                     std::vector<int32_t>& lens, std::vector<int64_t>& doff, std::vector<float>& hx,
+                     std::vector<float>& hy) {
   if (!offsets || !xy || count <= 0) return fail(TS_ERR_INVALID, "set: need count > 0, offsets and points");
   if (offsets[0] != 0) return fail(TS_ERR_INVALID, "set: offsets[0] must be 0");
-  std::vector<int32_t> lens(count);
-  int32_t mn = INT32_MAX, mx = 0;
+  lens.resize(count);
   for (int32_t i = 0; i < count; ++i) {
     const int64_t l = offsets[i + 1] - offsets[i];
     if (l < 1 || l > (1 << 24))
       return fail(TS_ERR_INVALID, "set: trajectory " + std::to_string(i) + " has length " + std::to_string(l) +
                                       " (need 1..2^24)");
     lens[i] = (int32_t)l;
-    mn = std::min(mn, lens[i]);
-    mx = std::max(mx, lens[i]);
   }
-  // Device offsets padded to kPad points per trajectory.
-  std::vector<int64_t> doff(count + 1);
-  doff[0] = 0;
+  doff.resize(count + 1);
+  doff[0] = base;
   for (int32_t i = 0; i < count; ++i) doff[i + 1] = doff[i] + ((lens[i] + kPad - 1) / kPad) * kPad;
-  const int64_t P = doff[count];
-  if (P > s->cap_points || count > s->cap_count) {
-    if (st) TS_HIP(hipStreamSynchronize(st));
+  const int64_t P = doff[count] - base;
+  hx.resize(P);
+  hy.resize(P);
+#pragma omp parallel for schedule(static) if (count > 4096)
+  for (int32_t i = 0; i < count; ++i) {
+    const int64_t src = offsets[i], dst = doff[i] - base;
+    const int32_t n = lens[i];
+    for (int32_t j = 0; j < n; ++j) { hx[dst + j] = xy[src + j].x; hy[dst + j] = xy[src + j].y; }
+    for (int64_t j = n; j < doff[i + 1] - doff[i]; ++j) { hx[dst + j] = xy[src + n - 1].x; hy[dst + j] = xy[src + n - 1].y; }
+  }
+  return TS_OK;
+}
+
+ts_status set_reserve(ts_set* s, int64_t points_cap, int32_t count_cap, bool keep) {
+  if (points_cap <= s->cap_points && count_cap <= s->cap_count) return TS_OK;
+  points_cap = std::max(points_cap, s->cap_points);
+  count_cap = std::max(count_cap, s->cap_count);
+  TS_HIP(hipDeviceSynchronize());
+  if (!keep) {  // free first: the peak stays at the new size
     (void)hipFree(s->x); (void)hipFree(s->y); (void)hipFree(s->off); (void)hipFree(s->len);
     (void)hipFree(s->mbr); (void)hipFree(s->ends); (void)hipFree(s->blk); (void)hipFree(s->sblk);
     s->x = s->y = nullptr; s->off = nullptr; s->len = nullptr; s->mbr = s->ends = s->blk = s->sblk = nullptr;
     s->bytes = 0; s->cap_points = 0; s->cap_count = 0;
-    hipError_t e;
-#define TS_ALLOC(ptr, nbytes)                                                        \
-    e = hipMalloc((void**)&(ptr), (nbytes));                                         \
-    if (e != hipSuccess) return hip_fail(e, "hipMalloc(set)", __FILE__, __LINE__);    \
-    s->bytes += (int64_t)(nbytes);
-    TS_ALLOC(s->x, P * sizeof(float));
-    TS_ALLOC(s->y, P * sizeof(float));
-    TS_ALLOC(s->off, (count + 1) * sizeof(int64_t));
-    TS_ALLOC(s->len, count * sizeof(int32_t));
-    TS_ALLOC(s->mbr, count * sizeof(float4));
-    TS_ALLOC(s->ends, count * sizeof(float4));
-    TS_ALLOC(s->blk, (P / 8) * sizeof(float4));
-    TS_ALLOC(s->sblk, (P / 8) * sizeof(float4));
-#undef TS_ALLOC
-    s->cap_points = P;
-    s->cap_count = count;
+    s->count = 0; s->dev_points = 0;
   }
+  ts_set n;
+  hipError_t e;
+#define TS_ALLOC(ptr, nbytes)                                                        \
+  e = hipMalloc((void**)&(ptr), (nbytes));                                           \
+  if (e != hipSuccess) {                                                             \
+    (void)hipFree(n.x); (void)hipFree(n.y); (void)hipFree(n.off); (void)hipFree(n.len); \
+    (void)hipFree(n.mbr); (void)hipFree(n.ends); (void)hipFree(n.blk); (void)hipFree(n.sblk); \
+    return hip_fail(e, "hipMalloc(set)", __FILE__, __LINE__);                        \
+  }                                                                                  \
+  n.bytes += (int64_t)(nbytes);
+  TS_ALLOC(n.x, points_cap * sizeof(float));
+  TS_ALLOC(n.y, points_cap * sizeof(float));
+  TS_ALLOC(n.off, (count_cap + 1) * sizeof(int64_t));
+  TS_ALLOC(n.len, count_cap * sizeof(int32_t));
+  TS_ALLOC(n.mbr, count_cap * sizeof(float4));
+  TS_ALLOC(n.ends, count_cap * sizeof(float4));
+  TS_ALLOC(n.blk, (points_cap / 8) * sizeof(float4));
+  TS_ALLOC(n.sblk, (points_cap / 8) * sizeof(float4));
+#undef TS_ALLOC
+  if (keep && s->count > 0) {
+    const int64_t P = s->dev_points;
+    const int32_t C = s->count;
+    TS_HIP(hipMemcpy(n.x, s->x, P * sizeof(float), hipMemcpyDeviceToDevice));
+    TS_HIP(hipMemcpy(n.y, s->y, P * sizeof(float), hipMemcpyDeviceToDevice));
+    TS_HIP(hipMemcpy(n.off, s->off, (C + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice));
+    TS_HIP(hipMemcpy(n.len, s->len, C * sizeof(int32_t), hipMemcpyDeviceToDevice));
+    TS_HIP(hipMemcpy(n.mbr, s->mbr, C * sizeof(float4), hipMemcpyDeviceToDevice));
+    TS_HIP(hipMemcpy(n.ends, s->ends, C * sizeof(float4), hipMemcpyDeviceToDevice));
+    TS_HIP(hipMemcpy(n.blk, s->blk, (P / 8) * sizeof(float4), hipMemcpyDeviceToDevice));
+    TS_HIP(hipMemcpy(n.sblk, s->sblk, (P / 8) * sizeof(float4), hipMemcpyDeviceToDevice));
+  }
+  (void)hipFree(s->x); (void)hipFree(s->y); (void)hipFree(s->off); (void)hipFree(s->len);
+  (void)hipFree(s->mbr); (void)hipFree(s->ends); (void)hipFree(s->blk); (void)hipFree(s->sblk);
+  s->x = n.x; s->y = n.y; s->off = n.off; s->len = n.len;
+  s->mbr = n.mbr; s->ends = n.ends; s->blk = n.blk; s->sblk = n.sblk;
+  n.x = n.y = nullptr; n.off = nullptr; n.len = nullptr; n.mbr = n.ends = n.blk = n.sblk = nullptr;
+  s->bytes = n.bytes;
+  s->cap_points = points_cap;
+  s->cap_count = count_cap;
+  return TS_OK;
+}
+
+ts_status set_load(ts_set* s, const ts_point* xy, const int64_t* offsets, int32_t count, hipStream_t st) {
+  std::vector<int32_t> lens;
+  std::vector<int64_t> doff;
+  std::vector<float> hx, hy;
+  TS_TRY(pad_layout(xy, offsets, count, 0, lens, doff, hx, hy));
+  const int64_t P = doff[count];
+  TS_TRY(set_reserve(s, P, count, false));
   s->count = count;
   s->points = offsets[count];
   s->dev_points = P;
-  s->min_len = mn;
-  s->max_len = mx;
+  s->min_len = *std::min_element(lens.begin(), lens.end());
+  s->max_len = *std::max_element(lens.begin(), lens.end());
   s->h_off.assign(offsets, offsets + count + 1);
   s->h_len = lens;
-  std::vector<float> hx(P), hy(P);
-#pragma omp parallel for schedule(static) if (count > 4096)
-  for (int32_t i = 0; i < count; ++i) {
-    const int64_t src = offsets[i], dst = doff[i];
-    const int32_t n = lens[i];
-    for (int32_t j = 0; j < n; ++j) { hx[dst + j] = xy[src + j].x; hy[dst + j] = xy[src + j].y; }
-    for (int64_t j = n; j < doff[i + 1] - dst; ++j) { hx[dst + j] = xy[src + n - 1].x; hy[dst + j] = xy[src + n - 1].y; }
-  }
   TS_HIP(hipMemcpyAsync(s->x, hx.data(), P * sizeof(float), hipMemcpyHostToDevice, st));
   TS_HIP(hipMemcpyAsync(s->y, hy.data(), P * sizeof(float), hipMemcpyHostToDevice, st));
   TS_HIP(hipMemcpyAsync(s->off, doff.data(), (count + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
@@ -202,6 +244,42 @@ ts_status set_load(ts_set* s, const ts_point* xy, const int64_t* offsets, int32_
   TS_HIP(launch_summaries(st, s->view(), s->mbr, s->ends));
   TS_HIP(launch_blocks(st, s->view(), P, s->blk, s->sblk));
   TS_HIP(hipStreamSynchronize(st));
+  return TS_OK;
+}
+
+// Appends trajectories behind the set's current ones: device arrays grow (by
+// half again at least, keeping their contents) when the new ones do not fit;
+// summaries are computed for the new trajectories only.
+ts_status set_append(ts_set* s, const ts_point* xy, const int64_t* offsets, int32_t count) {
+  std::vector<int32_t> lens;
+  std::vector<int64_t> doff;
+  std::vector<float> hx, hy;
+  const int64_t P0 = s->dev_points;
+  const int32_t C0 = s->count;
+  TS_TRY(pad_layout(xy, offsets, count, P0, lens, doff, hx, hy));
+  const int64_t P = doff[count];
+  if ((int64_t)C0 + count > INT32_MAX - 1) return fail(TS_ERR_INVALID, "set: more than 2^31 trajectories");
+  const int32_t C = C0 + count;
+  if (P > s->cap_points || C > s->cap_count)
+    TS_TRY(set_reserve(s, std::max<int64_t>(P, s->cap_points + s->cap_points / 2),
+                       (int32_t)std::min<int64_t>(INT32_MAX - 1, std::max<int64_t>(C, (int64_t)s->cap_count * 3 / 2)),
+                       true));
+  TS_HIP(hipMemcpy(s->x + P0, hx.data(), (P - P0) * sizeof(float), hipMemcpyHostToDevice));
+  TS_HIP(hipMemcpy(s->y + P0, hy.data(), (P - P0) * sizeof(float), hipMemcpyHostToDevice));
+  TS_HIP(hipMemcpy(s->off + C0 + 1, doff.data() + 1, count * sizeof(int64_t), hipMemcpyHostToDevice));
+  TS_HIP(hipMemcpy(s->len + C0, lens.data(), count * sizeof(int32_t), hipMemcpyHostToDevice));
+  const int64_t pts0 = s->points;
+  s->h_off.reserve(C + 1);
+  for (int32_t i = 1; i <= count; ++i) s->h_off.push_back(pts0 + offsets[i]);
+  s->h_len.insert(s->h_len.end(), lens.begin(), lens.end());
+  s->min_len = std::min(C0 ? s->min_len : INT32_MAX, *std::min_element(lens.begin(), lens.end()));
+  s->max_len = std::max(C0 ? s->max_len : 0, *std::max_element(lens.begin(), lens.end()));
+  s->count = C;
+  s->points = pts0 + offsets[count];
+  s->dev_points = P;
+  TS_HIP(launch_summaries(nullptr, s->view(), s->mbr, s->ends, C0));
+  TS_HIP(launch_blocks(nullptr, s->view(), P, s->blk, s->sblk, C0, P0));
+  TS_HIP(hipStreamSynchronize(nullptr));
   return TS_OK;
 }
 
@@ -282,6 +360,14 @@ ts_status ts_set_reload(ts_set* s, const ts_point* xy, const int64_t* offsets, i
   (void)hipGetDevice(&dev);
   if (dev != s->device) return fail(TS_ERR_INVALID, "ts_set_reload: set lives on another device");
   return set_load(s, xy, offsets, count, nullptr);
+}
+
+ts_status ts_set_append(ts_set* s, const ts_point* xy, const int64_t* offsets, int32_t count) {
+  if (!s) return fail(TS_ERR_INVALID, "ts_set_append: null set");
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (dev != s->device) return fail(TS_ERR_INVALID, "ts_set_append: set lives on another device");
+  return set_append(s, xy, offsets, count);
 }
 
 ts_status ts_set_destroy(ts_set* s) {

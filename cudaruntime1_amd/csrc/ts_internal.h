@@ -123,10 +123,26 @@ namespace ts {
 // Each launcher enqueues its kernel(s) on `st` and returns the launch status.
 
 // set.hip
-hipError_t launch_summaries(hipStream_t st, SetView s, float4* mbr, float4* ends);
+// MBR and endpoints of trajectories [t0, count); 8-point block boxes from
+// point point0 on and superblock boxes of trajectories [t0, count).
+hipError_t launch_summaries(hipStream_t st, SetView s, float4* mbr, float4* ends, int32_t t0 = 0);
+hipError_t launch_blocks(hipStream_t st, SetView s, int64_t points, float4* blk, float4* sblk, int32_t t0 = 0,
+                         int64_t point0 = 0);
 // (Re)load a set in place on `st`: grows the device arrays only when the new
 // contents do not fit, so a warmed-up set reloads without hipMalloc/hipFree.
 ts_status set_load(ts_set* s, const ts_point* xy, const int64_t* offsets, int32_t count, hipStream_t st);
+// Append trajectories to a set (device arrays grow when needed, keeping the
+// contents); summaries are computed for the new trajectories only.
+ts_status set_append(ts_set* s, const ts_point* xy, const int64_t* offsets, int32_t count);
+// Device arrays for at least points_cap padded points and count_cap
+// trajectories; with keep, the set's current contents are copied over.
+ts_status set_reserve(ts_set* s, int64_t points_cap, int32_t count_cap, bool keep);
+// Points padded to kPad per trajectory (the device layout): padded offsets
+// doff[count+1] (from base), x and y with each trajectory's last point
+// repeated into its padding. Fails on a length outside 1..2^24.
+ts_status pad_layout(const ts_point* xy, const int64_t* offsets, int32_t count, int64_t base,
+                     std::vector<int32_t>& lens, std::vector<int64_t>& doff, std::vector<float>& hx,
+                     std::vector<float>& hy);
 
 // filter.hip
 hipError_t launch_envelope(hipStream_t st, SetView q, int32_t max_len, int32_t r, float4* env);

@@ -111,6 +111,29 @@ ts_status ts_set_reload(ts_set* s, const ts_point* xy, const int64_t* offsets, i
 ts_status ts_set_info(const ts_set* s, int32_t* count, int64_t* points, int32_t* min_len, int32_t* max_len);
 /* Device memory held by the set, in bytes. */
 ts_status ts_set_bytes(const ts_set* s, int64_t* bytes);
+/* Lengths of the set's trajectories into out_len[count]. */
+ts_status ts_set_lengths(const ts_set* s, int32_t* out_len);
+/* Append trajectories (offsets[0] == 0, as for ts_set_create) behind the
+ * set's current ones; they get the next indices. Device arrays grow when
+ * needed, keeping their contents; summaries are computed for the new
+ * trajectories only. No call that reads the set may be in flight. (No
+ * counterpart in the reference, which keeps no database between calls.) */
+ts_status ts_set_append(ts_set* s, const ts_point* xy, const int64_t* offsets, int32_t count);
+
+/* -------------------------------------------------------- database files
+ * A tsdb file holds a set in the device layout (lengths, padded SoA points,
+ * MBRs, endpoints, a length index; layout in csrc/dbfile.hip), so loading is
+ * a copy of sections into HBM. No counterpart in the reference, which keeps
+ * no database between calls (main.cu:68-74).
+ *   ts_set_save   writes a device-resident set;
+ *   ts_db_write   writes the same file from host arrays, without a GPU;
+ *   ts_set_load   maps a file, checks its header, lengths and length index,
+ *                 copies the points to the current device, recomputes the
+ *                 summaries there and rejects the file (TS_ERR_INVALID) when
+ *                 its stored MBRs or endpoints disagree with its points. */
+ts_status ts_set_save(const ts_set* s, const char* path);
+ts_status ts_db_write(const char* path, const ts_point* xy, const int64_t* offsets, int32_t count);
+ts_status ts_set_load(const char* path, ts_set** out);
 
 /* --------------------------------------------------------------- engine */
 typedef struct ts_engine ts_engine;
