@@ -22,7 +22,7 @@
 //
 // Each section starts on a 256-byte boundary. The loader checks the header,
 // the lengths against the sizes and the length index, copies len, x and y
-// into the set's device arrays, recomputes every summary there (box kernels
+// into the set's device arrays (the points through pinned staging buffers), recomputes every summary there (box kernels
 // run in well under a millisecond per million trajectories), and rejects the
 // file when the recomputed MBRs and endpoints differ from the stored ones.
 #include <fcntl.h>
@@ -173,6 +173,46 @@ struct Mapped {
   }
 };
 
+// Copies n bytes from pageable memory (the file's mapping) to the device
+// through two pinned staging buffers: host threads fill one while the copy
+// engine drains the other.
+hipError_t upload_staged(void* dst, const void* src, size_t n) {
+  constexpr size_t kChunk = (size_t)32 << 20;
+  if (n <= kChunk) return hipMemcpy(dst, src, n, hipMemcpyHostToDevice);
+  char* pin[2] = {nullptr, nullptr};
+  hipEvent_t free_again[2] = {nullptr, nullptr};
+  hipStream_t st = nullptr;
+  hipError_t e = hipHostMalloc((void**)&pin[0], kChunk, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&pin[1], kChunk, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&free_again[0], hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&free_again[1], hipEventDisableTiming);
+  for (size_t off = 0, i = 0; e == hipSuccess && off < n; off += kChunk, ++i) {
+    const size_t len = std::min(kChunk, n - off);
+    char* b = pin[i & 1];
+    if (i >= 2) e = hipEventSynchronize(free_again[i & 1]);
+    if (e != hipSuccess) break;
+    const char* from = (const char*)src + off;
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < 16; ++t) {
+      const size_t a = len * t / 16, z = len * (t + 1) / 16;
+      std::memcpy(b + a, from + a, z - a);
+    }
+    e = hipMemcpyAsync((char*)dst + off, b, len, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipEventRecord(free_again[i & 1], st);
+  }
+  if (st) {
+    const hipError_t e2 = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = e2;
+    (void)hipStreamDestroy(st);
+  }
+  for (hipEvent_t ev : free_again)
+    if (ev) (void)hipEventDestroy(ev);
+  for (char* b : pin)
+    if (b) (void)hipHostFree(b);
+  return e;
+}
+
 }  // namespace
 }  // namespace ts
 
@@ -260,11 +300,28 @@ ts_status ts_set_load(const char* path, ts_set** out) {
   }
   if (doff[h.count] != h.dev_points || hoff[h.count] != h.points || mn != h.min_len || mx != h.max_len)
     return fail(TS_ERR_INVALID, where + "lengths disagree with the header");
-  std::vector<int32_t> want_order;
-  std::vector<int64_t> want_cls;
-  length_index(std::vector<int32_t>(lens, lens + h.count), want_order, want_cls);
-  if (std::memcmp(want_order.data(), order, n * 4) != 0 || std::memcmp(want_cls.data(), cls, want_cls.size() * 8) != 0)
-    return fail(TS_ERR_INVALID, where + "length index disagrees with the lengths");
+  // the length index in one pass: a permutation in increasing (length, id)
+  // whose class starts fall where the classes change
+  {
+    std::vector<uint8_t> seen(n, 0);
+    bool ok = cls[0] == 0 && cls[kClasses] == h.count;
+    int c = 0;
+    for (int32_t i = 0; ok && i < h.count; ++i) {
+      const int32_t id = order[i];
+      ok = id >= 0 && id < h.count && !seen[id];
+      if (!ok) break;
+      seen[id] = 1;
+      if (i > 0) {
+        const int32_t prev = order[i - 1];
+        ok = lens[prev] < lens[id] || (lens[prev] == lens[id] && prev < id);
+      }
+      const int ci = length_class(lens[id]);
+      while (ok && c < ci) ok = cls[++c] == i;  // classes c+1..ci start here
+      ok = ok && cls[ci] <= i && i < cls[ci + 1];
+    }
+    for (int cc = c + 1; ok && cc < kClasses; ++cc) ok = cls[cc] == h.count;
+    if (!ok) return fail(TS_ERR_INVALID, where + "length index disagrees with the lengths");
+  }
   // device arrays, points, summaries recomputed from the points
   auto s = new ts_set();
   auto bail = [&](ts_status e) {
@@ -282,8 +339,8 @@ ts_status ts_set_load(const char* path, ts_set** out) {
   s->h_off = hoff;
   s->h_len.assign(lens, lens + h.count);
   hipError_t e = hipSuccess;
-  if (e == hipSuccess) e = hipMemcpy(s->x, fx, P * sizeof(float), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(s->y, fy, P * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = upload_staged(s->x, fx, P * sizeof(float));
+  if (e == hipSuccess) e = upload_staged(s->y, fy, P * sizeof(float));
   if (e == hipSuccess) e = hipMemcpy(s->off, doff.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(s->len, s->h_len.data(), n * sizeof(int32_t), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = launch_summaries(nullptr, s->view(), s->mbr, s->ends);
