@@ -21,6 +21,7 @@ LIB_PATH = os.path.join(_HERE, "libtrajsim.so")
 DTW, LCSS, FRECHET, HAUSDORFF, EUCLIDEAN = 0, 1, 2, 3, 4
 MEASURES = {"dtw": DTW, "lcss": LCSS, "frechet": FRECHET, "hausdorff": HAUSDORFF, "euclidean": EUCLIDEAN}
 REF, EXACT = 0, 1
+SET_TINY_COORD = 1  # TrajSet.flags(): a coordinate c with 0 < |c| < 2^-25
 
 if not os.path.exists(LIB_PATH):
     raise ImportError("libtrajsim.so is not built (run `python cudaruntime1_amd/build.py` or "
@@ -64,6 +65,7 @@ _sig = {
     "ts_set_destroy": (ctypes.c_int, [_P]),
     "ts_set_info": (ctypes.c_int, [_P, _i32p, ctypes.POINTER(ctypes.c_int64), _i32p, _i32p]),
     "ts_set_lengths": (ctypes.c_int, [_P, _P]),
+    "ts_set_flags": (ctypes.c_int, [_P, _i32p]),
     "ts_set_append": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32]),
     "ts_set_save": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "ts_set_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_P)]),
@@ -258,6 +260,13 @@ class TrajSet:
         b = ctypes.c_int64()
         _check(_lib.ts_set_bytes(self._h, ctypes.byref(b)), "ts_set_bytes")
         return b.value
+
+    def flags(self):
+        """Bits found in the points at load: SET_TINY_COORD (a coordinate c
+        with 0 < |c| < 2^-25)."""
+        f = ctypes.c_int32()
+        _check(_lib.ts_set_flags(self._h, ctypes.byref(f)), "ts_set_flags")
+        return f.value
 
     def close(self):
         if self._h:

@@ -343,7 +343,7 @@ ts_status ts_set_load(const char* path, ts_set** out) {
   if (e == hipSuccess) e = upload_staged(s->y, fy, P * sizeof(float));
   if (e == hipSuccess) e = hipMemcpy(s->off, doff.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(s->len, s->h_len.data(), n * sizeof(int32_t), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = launch_summaries(nullptr, s->view(), s->mbr, s->ends);
+  if (e == hipSuccess) e = launch_summaries(nullptr, s->view(), s->mbr, s->ends, s->tiny);
   if (e == hipSuccess) e = launch_blocks(nullptr, s->view(), h.dev_points, s->blk, s->sblk);
   std::vector<float4> gm(n), ge(n);
   if (e == hipSuccess) e = hipMemcpy(gm.data(), s->mbr, n * sizeof(float4), hipMemcpyDeviceToHost);

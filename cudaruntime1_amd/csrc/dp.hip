@@ -118,17 +118,21 @@ __device__ __forceinline__ void store_vec(float* p, const float (&v)[U]) {
 // residuals (exact in that range) selects the correctly rounded root. That
 // is the expansion the compiler emits for sqrtf minus its rescaling of tiny
 // inputs and its 0/inf select, which these inputs do not need (0 and +inf
-// come out of the tests unchanged). A wave holding any x in (0, 2^-96)
-// takes sqrtf itself; the branch is uniform.
+// come out of the tests unchanged). Inputs in (0, 2^-96) need a trajectory
+// with a tiny coordinate (0 < |c| < 2^-25); `tiny` says whether a pair of the
+// wave's lanes has one (uniform), and then a wave holding any such x takes
+// sqrtf itself.
 template <int U>
-__device__ __forceinline__ void sqrt_cr(float (&x)[U]) {
-  uint32_t mb = __float_as_uint(x[0]) - 1u;  // 0 maps to the top
+__device__ __forceinline__ void sqrt_cr(float (&x)[U], bool tiny) {
+  if (__builtin_expect(tiny, 0)) {
+    uint32_t mb = __float_as_uint(x[0]) - 1u;  // 0 maps to the top
 #pragma unroll
-  for (int j = 1; j < U; ++j) mb = min(mb, __float_as_uint(x[j]) - 1u);
-  if (__builtin_expect(__any(mb < 0x0f7fffffu), 0)) {
+    for (int j = 1; j < U; ++j) mb = min(mb, __float_as_uint(x[j]) - 1u);
+    if (__builtin_expect(__any(mb < 0x0f7fffffu), 0)) {
 #pragma unroll
-    for (int j = 0; j < U; ++j) x[j] = sqrtf(x[j]);
-    return;
+      for (int j = 0; j < U; ++j) x[j] = sqrtf(x[j]);
+      return;
+    }
   }
 #pragma unroll
   for (int j = 0; j < U; ++j) {
@@ -225,6 +229,7 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
   const float *fAx = a.q.x, *fAy = a.q.y, *fBx = a.q.x, *fBy = a.q.y;
   int fR = 1, fC = 1, f_slot = 0;
   bool f_ok = false, f_live = false;
+  bool f_tiny = false, pair_tiny = false;  // fetched / current pair has a tiny coordinate
   float f_ax = 0.0f, f_ay = 0.0f;
   int bd = 0, bw = 0, wb = 0;
   int f_rem = 0;  // steps of the fetched pair (ranged: summed over its strips by the group)
@@ -262,6 +267,7 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
             fAx = a.db.x + dof; fAy = a.db.y + dof; fBx = a.q.x + qo; fBy = a.q.y + qo; fR = m; fC = n;
           }
           f_slot = it.slot;
+          f_tiny = MEAS == TS_DTW && (a.q.tiny[it.q] | a.db.tiny[it.d]) != 0;
           // this lane's first row point
           f_ax = fAx[min(lg, fR - 1)];
           f_ay = fAy[min(lg, fR - 1)];
@@ -292,6 +298,7 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
         done = true;
       } else {
         live = f_live;
+        pair_tiny = f_live && f_tiny;
         if (live) {
           Ax = fAx; Ay = fAy; Bx = fBx; By = fBy; R = fR; C = fC;
           slot = f_slot;
@@ -366,7 +373,7 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
         const float dx = ax - cx[j], dy = ay - cy[j];
         dist[j] = dx * dx + dy * dy;
       }
-      sqrt_cr<U>(dist);
+      sqrt_cr<U>(dist, __any(pair_tiny));
     }
 #pragma unroll
     for (int j = 0; j < U; ++j) {
@@ -518,6 +525,7 @@ template <int MEAS, int W, int U, bool SC, bool SR>
 __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
   constexpr int G = 64 / W;
   constexpr float BND = (MEAS == TS_LCSS) ? 0.0f : kInf;
+  bool tiny = false;  // a lane's pair has a trajectory with a tiny coordinate (uniform)
   constexpr int kNoResult = INT_MIN;
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
@@ -552,6 +560,7 @@ __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
   int fR = 1, fC = 1, f_slot = 0;
   const float *fAx = a.q.x, *fAy = a.q.y, *fBx = a.q.x, *fBy = a.q.y;
   bool f_ok = false, f_live = false;
+  bool f_tiny = false, pair_tiny = false;  // fetched / current pair has a tiny coordinate
   float o[U], cx[U], cy[U], hv[U];
 #pragma unroll
   for (int j = 0; j < U; ++j) { o[j] = 0.0f; hv[j] = BND; cx[j] = 0.0f; cy[j] = 0.0f; }
@@ -595,6 +604,7 @@ __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
               fAx = a.db.x + dof; fAy = a.db.y + dof; fBx = a.q.x + qo; fBy = a.q.y + qo; fR = m; fC = n;
             }
             f_slot = it.slot;
+            f_tiny = MEAS == TS_DTW && (a.q.tiny[it.q] | a.db.tiny[it.d]) != 0;
             if (SC) stage_pts<W>(cols + gpar * cmax, fBx, fBy, fC, lg);
             if (SR) stage_pts<W>(rows, fAx, fAy, fR, lg);
           }
@@ -608,6 +618,7 @@ __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
           rem = -2;  // no further events
           res_rem = kNoResult;
         } else {
+          pair_tiny = f_live && f_tiny;
           R = f_live ? fR : 1;
           C = f_live ? fC : 1;
           slot = f_slot;
@@ -644,6 +655,7 @@ __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
         }
       }
       ev = next_event(rem, lg, res_rem);
+      tiny = __any(pair_tiny);
       if (__all(done)) break;
     }
     if (wrapped) {
@@ -665,7 +677,7 @@ __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
         const float dx = ax - cx[j], dy = ay - cy[j];
         dist[j] = dx * dx + dy * dy;
       }
-      sqrt_cr<U>(dist);
+      sqrt_cr<U>(dist, tiny);
     }
 #pragma unroll
     for (int j = 0; j < U; ++j) {
@@ -734,6 +746,7 @@ __global__ void __launch_bounds__(256) k_dp_lean_r(DpArgs a) {
   static_assert(BAND || DELTA, "ranged pairs only");
   constexpr int G = 64 / W;
   constexpr float BND = (MEAS == TS_LCSS) ? 0.0f : kInf;
+  bool tiny = false;  // a lane's pair has a trajectory with a tiny coordinate (uniform)
   constexpr int kNoResult = INT_MIN;
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
@@ -773,6 +786,7 @@ __global__ void __launch_bounds__(256) k_dp_lean_r(DpArgs a) {
   int fR = 1, fC = 1, f_slot = 0, f_total = 0, f_ires = 0, f_lgres = -1;
   const float *fAx = a.q.x, *fAy = a.q.y, *fBx = a.q.x, *fBy = a.q.y;
   bool f_ok = false, f_live = false;
+  bool f_tiny = false, pair_tiny = false;  // fetched / current pair has a tiny coordinate
   float o[U], cx[U], cy[U], hv[U];
 #pragma unroll
   for (int j = 0; j < U; ++j) { o[j] = 0.0f; hv[j] = BND; cx[j] = 0.0f; cy[j] = 0.0f; }
@@ -812,6 +826,7 @@ __global__ void __launch_bounds__(256) k_dp_lean_r(DpArgs a) {
               fAx = a.db.x + dof; fAy = a.db.y + dof; fBx = a.q.x + qo; fBy = a.q.y + qo; fR = m; fC = n;
             }
             f_slot = it.slot;
+            f_tiny = MEAS == TS_DTW && (a.q.tiny[it.q] | a.db.tiny[it.d]) != 0;
             if (ST) {
               stage_pts<W>(cols + gpar * ccap, fBx, fBy, fC, lg);
               stage_pts<W>(rows, fAx, fAy, fR, lg);
@@ -854,6 +869,7 @@ __global__ void __launch_bounds__(256) k_dp_lean_r(DpArgs a) {
           rem = -2;
           res_rem = kNoResult;
         } else {
+          pair_tiny = f_live && f_tiny;
           R = f_live ? fR : 1;
           C = f_live ? fC : 1;
           slot = f_slot;
@@ -893,6 +909,7 @@ __global__ void __launch_bounds__(256) k_dp_lean_r(DpArgs a) {
         }
       }
       ev = next_event(rem, lg, res_rem);
+      tiny = __any(pair_tiny);
       if (__all(done)) break;
     }
     if (wrapped) {
@@ -915,7 +932,7 @@ __global__ void __launch_bounds__(256) k_dp_lean_r(DpArgs a) {
         const float dx = ax - cx[j], dy = ay - cy[j];
         dist[j] = dx * dx + dy * dy;
       }
-      sqrt_cr<U>(dist);
+      sqrt_cr<U>(dist, tiny);
     }
 #pragma unroll
     for (int j = 0; j < U; ++j) {

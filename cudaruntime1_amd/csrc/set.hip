@@ -48,20 +48,27 @@ int device_cu_count() {
   return cus;
 }
 
+// Nonzero and below 2^-25 in magnitude.
+__device__ __forceinline__ bool tiny_coord(float c) { return c != 0.0f && fabsf(c) < 0x1p-25f; }
+
 // One wave per trajectory: MBR by wave min/max, endpoints from the first and
-// last point.
-__global__ void __launch_bounds__(256) k_summaries(SetView s, int32_t t0, float4* mbr, float4* ends) {
+// last point, whether a point has a tiny coordinate.
+__global__ void __launch_bounds__(256) k_summaries(SetView s, int32_t t0, float4* mbr, float4* ends,
+                                                   uint8_t* tiny) {
   const int lane = threadIdx.x & 63;
   const int t = t0 + blockIdx.x * 4 + (threadIdx.x >> 6);
   if (t >= s.count) return;
   const int64_t o = s.off[t];
   const int n = s.len[t];
   float mnx = s.x[o], mxx = mnx, mny = s.y[o], mxy = mny;
+  bool tc = false;
   for (int i = lane; i < n; i += 64) {
     float px = s.x[o + i], py = s.y[o + i];
     mnx = fminf(mnx, px); mxx = fmaxf(mxx, px);
     mny = fminf(mny, py); mxy = fmaxf(mxy, py);
+    tc |= tiny_coord(px) || tiny_coord(py);
   }
+  const bool any_tc = __any(tc);
   for (int sh = 32; sh > 0; sh >>= 1) {
     mnx = fminf(mnx, __shfl_xor(mnx, sh));
     mxx = fmaxf(mxx, __shfl_xor(mxx, sh));
@@ -71,6 +78,7 @@ __global__ void __launch_bounds__(256) k_summaries(SetView s, int32_t t0, float4
   if (lane == 0) {
     mbr[t] = make_float4(mnx, mxx, mny, mxy);
     ends[t] = make_float4(s.x[o], s.y[o], s.x[o + n - 1], s.y[o + n - 1]);
+    tiny[t] = any_tc ? 1 : 0;
   }
 }
 
@@ -93,9 +101,9 @@ __global__ void __launch_bounds__(256) k_blocks(const float* x, const float* y, 
   blk[b] = make_float4(mnx, mxx, mny, mxy);
 }
 
-hipError_t launch_summaries(hipStream_t st, SetView s, float4* mbr, float4* ends, int32_t t0) {
+hipError_t launch_summaries(hipStream_t st, SetView s, float4* mbr, float4* ends, uint8_t* tiny, int32_t t0) {
   if (s.count <= t0) return hipSuccess;
-  hipLaunchKernelGGL(k_summaries, dim3((s.count - t0 + 3) / 4), dim3(256), 0, st, s, t0, mbr, ends);
+  hipLaunchKernelGGL(k_summaries, dim3((s.count - t0 + 3) / 4), dim3(256), 0, st, s, t0, mbr, ends, tiny);
   return hipGetLastError();
 }
 
@@ -177,7 +185,9 @@ ts_status set_reserve(ts_set* s, int64_t points_cap, int32_t count_cap, bool kee
   if (!keep) {  // free first: the peak stays at the new size
     (void)hipFree(s->x); (void)hipFree(s->y); (void)hipFree(s->off); (void)hipFree(s->len);
     (void)hipFree(s->mbr); (void)hipFree(s->ends); (void)hipFree(s->blk); (void)hipFree(s->sblk);
+    (void)hipFree(s->tiny);
     s->x = s->y = nullptr; s->off = nullptr; s->len = nullptr; s->mbr = s->ends = s->blk = s->sblk = nullptr;
+    s->tiny = nullptr;
     s->bytes = 0; s->cap_points = 0; s->cap_count = 0;
     s->count = 0; s->dev_points = 0;
   }
@@ -188,6 +198,7 @@ ts_status set_reserve(ts_set* s, int64_t points_cap, int32_t count_cap, bool kee
   if (e != hipSuccess) {                                                             \
     (void)hipFree(n.x); (void)hipFree(n.y); (void)hipFree(n.off); (void)hipFree(n.len); \
     (void)hipFree(n.mbr); (void)hipFree(n.ends); (void)hipFree(n.blk); (void)hipFree(n.sblk); \
+    (void)hipFree(n.tiny);                                                           \
     return hip_fail(e, "hipMalloc(set)", __FILE__, __LINE__);                        \
   }                                                                                  \
   n.bytes += (int64_t)(nbytes);
@@ -199,6 +210,7 @@ ts_status set_reserve(ts_set* s, int64_t points_cap, int32_t count_cap, bool kee
   TS_ALLOC(n.ends, count_cap * sizeof(float4));
   TS_ALLOC(n.blk, (points_cap / 8) * sizeof(float4));
   TS_ALLOC(n.sblk, (points_cap / 8) * sizeof(float4));
+  TS_ALLOC(n.tiny, count_cap * sizeof(uint8_t));
 #undef TS_ALLOC
   if (keep && s->count > 0) {
     const int64_t P = s->dev_points;
@@ -211,12 +223,15 @@ ts_status set_reserve(ts_set* s, int64_t points_cap, int32_t count_cap, bool kee
     TS_HIP(hipMemcpy(n.ends, s->ends, C * sizeof(float4), hipMemcpyDeviceToDevice));
     TS_HIP(hipMemcpy(n.blk, s->blk, (P / 8) * sizeof(float4), hipMemcpyDeviceToDevice));
     TS_HIP(hipMemcpy(n.sblk, s->sblk, (P / 8) * sizeof(float4), hipMemcpyDeviceToDevice));
+    TS_HIP(hipMemcpy(n.tiny, s->tiny, C * sizeof(uint8_t), hipMemcpyDeviceToDevice));
   }
   (void)hipFree(s->x); (void)hipFree(s->y); (void)hipFree(s->off); (void)hipFree(s->len);
   (void)hipFree(s->mbr); (void)hipFree(s->ends); (void)hipFree(s->blk); (void)hipFree(s->sblk);
+  (void)hipFree(s->tiny);
   s->x = n.x; s->y = n.y; s->off = n.off; s->len = n.len;
-  s->mbr = n.mbr; s->ends = n.ends; s->blk = n.blk; s->sblk = n.sblk;
+  s->mbr = n.mbr; s->ends = n.ends; s->blk = n.blk; s->sblk = n.sblk; s->tiny = n.tiny;
   n.x = n.y = nullptr; n.off = nullptr; n.len = nullptr; n.mbr = n.ends = n.blk = n.sblk = nullptr;
+  n.tiny = nullptr;
   s->bytes = n.bytes;
   s->cap_points = points_cap;
   s->cap_count = count_cap;
@@ -241,7 +256,7 @@ ts_status set_load(ts_set* s, const ts_point* xy, const int64_t* offsets, int32_
   TS_HIP(hipMemcpyAsync(s->y, hy.data(), P * sizeof(float), hipMemcpyHostToDevice, st));
   TS_HIP(hipMemcpyAsync(s->off, doff.data(), (count + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
   TS_HIP(hipMemcpyAsync(s->len, s->h_len.data(), count * sizeof(int32_t), hipMemcpyHostToDevice, st));
-  TS_HIP(launch_summaries(st, s->view(), s->mbr, s->ends));
+  TS_HIP(launch_summaries(st, s->view(), s->mbr, s->ends, s->tiny));
   TS_HIP(launch_blocks(st, s->view(), P, s->blk, s->sblk));
   TS_HIP(hipStreamSynchronize(st));
   return TS_OK;
@@ -277,7 +292,7 @@ ts_status set_append(ts_set* s, const ts_point* xy, const int64_t* offsets, int3
   s->count = C;
   s->points = pts0 + offsets[count];
   s->dev_points = P;
-  TS_HIP(launch_summaries(nullptr, s->view(), s->mbr, s->ends, C0));
+  TS_HIP(launch_summaries(nullptr, s->view(), s->mbr, s->ends, s->tiny, C0));
   TS_HIP(launch_blocks(nullptr, s->view(), P, s->blk, s->sblk, C0, P0));
   TS_HIP(hipStreamSynchronize(nullptr));
   return TS_OK;
@@ -374,7 +389,19 @@ ts_status ts_set_destroy(ts_set* s) {
   if (!s) return TS_OK;
   (void)hipFree(s->x); (void)hipFree(s->y); (void)hipFree(s->off); (void)hipFree(s->len);
   (void)hipFree(s->mbr); (void)hipFree(s->ends); (void)hipFree(s->blk); (void)hipFree(s->sblk);
+  (void)hipFree(s->tiny);
   delete s;
+  return TS_OK;
+}
+
+ts_status ts_set_flags(const ts_set* s, int32_t* flags) {
+  if (!s || !flags) return fail(TS_ERR_INVALID, "ts_set_flags: null argument");
+  *flags = 0;
+  if (s->count <= 0) return TS_OK;
+  std::vector<uint8_t> t(s->count);
+  TS_HIP(hipDeviceSynchronize());
+  TS_HIP(hipMemcpy(t.data(), s->tiny, s->count * sizeof(uint8_t), hipMemcpyDeviceToHost));
+  if (std::find(t.begin(), t.end(), 1) != t.end()) *flags |= TS_SET_TINY_COORD;
   return TS_OK;
 }
 

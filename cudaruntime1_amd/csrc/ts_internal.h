@@ -56,7 +56,13 @@ struct SetView {
   const float4* blk;     // [points / 8] box of each 8-point block (blocks never straddle trajectories)
   const float4* sblk;    // [points / 8] box of blocks g..g+7 of the same trajectory (64 points)
   int32_t count;
+  const uint8_t* tiny;   // [count] 1: the trajectory has a tiny coordinate
 };
+
+// A tiny coordinate: 0 < |c| < 2^-25. Between two trajectories without one
+// every squared point distance is 0 or at least 2^-96 (coordinates of
+// magnitude 2^-25 or more are multiples of 2^-48, so are their nonzero
+// differences), the range where the DP's square root needs no rescaling.
 
 // Work item of the refine kernels: one (query, database) pair and the slot
 // its value is written to.
@@ -108,13 +114,14 @@ struct ts_set {
   float4* ends = nullptr;
   float4* blk = nullptr;
   float4* sblk = nullptr;
+  uint8_t* tiny = nullptr;  // [count]
   std::vector<int64_t> h_off;
   std::vector<int32_t> h_len;
   int64_t bytes = 0;
   // Capacity of the device arrays (sets reloaded in place by the server).
   int64_t cap_points = 0;
   int32_t cap_count = 0;
-  ts::SetView view() const { return {x, y, off, len, mbr, ends, blk, sblk, count}; }
+  ts::SetView view() const { return {x, y, off, len, mbr, ends, blk, sblk, count, tiny}; }
 };
 
 namespace ts {
@@ -123,9 +130,10 @@ namespace ts {
 // Each launcher enqueues its kernel(s) on `st` and returns the launch status.
 
 // set.hip
-// MBR and endpoints of trajectories [t0, count); 8-point block boxes from
-// point point0 on and superblock boxes of trajectories [t0, count).
-hipError_t launch_summaries(hipStream_t st, SetView s, float4* mbr, float4* ends, int32_t t0 = 0);
+// MBR, endpoints and tiny-coordinate flag of trajectories [t0, count);
+// 8-point block boxes from point point0 on and superblock boxes of
+// trajectories [t0, count).
+hipError_t launch_summaries(hipStream_t st, SetView s, float4* mbr, float4* ends, uint8_t* tiny, int32_t t0 = 0);
 hipError_t launch_blocks(hipStream_t st, SetView s, int64_t points, float4* blk, float4* sblk, int32_t t0 = 0,
                          int64_t point0 = 0);
 // (Re)load a set in place on `st`: grows the device arrays only when the new

@@ -76,6 +76,101 @@ def test_dtw_extreme_scales_bit_exact(eng, scale):
     check_pairs(eng, s, "dtw")
 
 
+def lattice(seed, lengths, below=False):
+    """Coordinates 0 or +-(2^-25 + m 2^-48), m < 6: no tiny coordinate
+    (0 < |c| < 2^-25), and coordinates 2^-48 apart, so squared distances of
+    exactly 2^-96. below: +-(2^-25 - m 2^-49), m in 1..5, all tiny, whose
+    squared distances to the first kind go down to 2^-98."""
+    rng = np.random.default_rng(seed)
+    off = ts.offsets_of(lengths)
+    n = int(off[-1])
+    sign = rng.choice([-1.0, 1.0], size=(n, 2))
+    if below:
+        xy = sign * (2.0 ** -25 - rng.integers(1, 6, size=(n, 2)) * 2.0 ** -49)
+    else:
+        xy = sign * (2.0 ** -25 + rng.integers(0, 6, size=(n, 2)) * 2.0 ** -48)
+        xy[rng.random((n, 2)) < 0.2] = 0.0
+    return xy.astype(np.float32), off
+
+
+def pair_case(q_xy, q_off, db_xy, db_off):
+    nq, nd = len(q_off) - 1, len(db_off) - 1
+    qi, di = np.meshgrid(np.arange(nq), np.arange(nd), indexing="ij")
+    return dict(db=ts.TrajSet(db_xy, db_off), qs=ts.TrajSet(q_xy, q_off), db_xy=db_xy, db_off=db_off, q_xy=q_xy,
+                q_off=q_off, qi=qi.ravel(), di=di.ravel())
+
+
+def min_sq_distance(a, b):
+    d = ((a[:, None, :].astype(np.float64) - b[None, :, :].astype(np.float64)) ** 2).sum(-1)
+    return d[d > 0].min()
+
+
+@pytest.mark.parametrize("band", [None, 3, -10])
+def test_dtw_at_the_tiny_threshold_bit_exact(eng, band):
+    """Sets without a tiny coordinate skip the DP's test for square roots
+    of values below 2^-96: on this lattice squared distances reach 2^-96
+    itself (and 0), the smallest values that path then has to round."""
+    q_xy, q_off = lattice(41, LENGTHS)
+    db_xy, db_off = lattice(42, LENGTHS[::-1])
+    assert min_sq_distance(q_xy, db_xy) == 2.0 ** -96
+    s = pair_case(q_xy, q_off, db_xy, db_off)
+    assert s["qs"].flags() == 0 and s["db"].flags() == 0
+    say("dtw lattice band", band)
+    check_pairs(eng, s, "dtw", band=band)
+    p = ts.params("dtw", mode=ts.EXACT, k=5, band=band)
+    prm = oracle.params(p.measure, p.mode, k=5, kc=p.kc, r=p.r, band=p.band)
+    got = eng.knn(s["db"], s["qs"], p)
+    want = oracle.knn(prm, db_xy, db_off, q_xy, q_off)
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1].view(np.int32), want[1].view(np.int32))
+
+
+@pytest.mark.parametrize("tiny_side", ["query", "database"])
+def test_dtw_one_set_with_tiny_coordinates_bit_exact(eng, tiny_side):
+    """Square roots of values below 2^-96 between a set without tiny
+    coordinates and one with them: either set's flag alone switches the
+    DP to the test."""
+    a_xy, a_off = lattice(43, LENGTHS)
+    b_xy, b_off = lattice(44, LENGTHS[::-1], below=True)
+    assert min_sq_distance(a_xy, b_xy) < 2.0 ** -96
+    if tiny_side == "query":
+        s = pair_case(b_xy, b_off, a_xy, a_off)
+        assert s["qs"].flags() == ts.SET_TINY_COORD and s["db"].flags() == 0
+    else:
+        s = pair_case(a_xy, a_off, b_xy, b_off)
+        assert s["qs"].flags() == 0 and s["db"].flags() == ts.SET_TINY_COORD
+    say("dtw tiny", tiny_side)
+    check_pairs(eng, s, "dtw")
+    check_pairs(eng, s, "dtw", band=-10)
+
+
+def test_dtw_long_pairs_with_tiny_coordinates_bit_exact(eng):
+    """The same on pairs long enough for the DP variants that read global
+    memory, and for banded pairs past the lean kernels' LDS budget."""
+    a_xy, a_off = lattice(47, [6000, 700])
+    b_xy, b_off = lattice(48, [6500, 800], below=True)
+    s = pair_case(b_xy, b_off, a_xy, a_off)
+    say("dtw tiny long")
+    check_pairs(eng, s, "dtw")
+    check_pairs(eng, s, "dtw", band=-10)
+
+
+def test_set_flags_follow_the_points(eng):
+    """The tiny-coordinate flag is found at load, kept by appends and
+    recomputed by a reload."""
+    xy, off = walks(45, LENGTHS, 2e-2)
+    xy = (xy + 2.0).astype(np.float32)  # every coordinate well above 2^-25
+    s = ts.TrajSet(xy, off)
+    say("set flags")
+    assert s.flags() == 0
+    s.append(xy[:off[2]], off[:3])
+    assert s.flags() == 0
+    t_xy, t_off = lattice(46, [5, 9], below=True)
+    s.append(t_xy, t_off)
+    assert s.flags() == ts.SET_TINY_COORD
+    s.reload(xy, off)
+    assert s.flags() == 0
+
+
 @pytest.mark.parametrize("band", [1, 3, 10, 40, -1, -10, -55])
 def test_dtw_band_bit_exact(eng, shapes, band):
     say("dtw band", band)
