@@ -42,6 +42,7 @@
 //    its own halo length, so short pairs do not pay for long rows' LDS.
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 #include "ts_internal.h"
 
@@ -518,10 +519,84 @@ __device__ __forceinline__ int next_event(int rem, int lg, int res_rem) {
   return ev;
 }
 
+// One step of k_dp_lean's loop for every lane of the wave; TINY: whether
+// the square roots test for inputs below 2^-96 (uniform).
+#define TS_LEAN_STEP(TINY)                                                                          \
+  do {                                                                                              \
+    if (wrapped) {                                                                                  \
+      ax = rp.x;                                                                                    \
+      ay = rp.y;                                                                                    \
+    }                                                                                               \
+                                                                                                    \
+    /* ---- U cells */                                                                              \
+    float up[U], on[U];                                                                             \
+    _Pragma("unroll")                                                                               \
+    for (int j = 0; j < U; ++j) up[j] = from_prev_lane<W>(o[j], hv[j], lg);                         \
+    const bool first = k == 0;                                                                      \
+    float d = first ? dfirst : up_last;                                                             \
+    float l = first ? BND : o[U - 1];                                                               \
+    float dist[U];                                                                                  \
+    if (MEAS == TS_DTW) {                                                                           \
+    _Pragma("unroll")                                                                               \
+      for (int j = 0; j < U; ++j) {                                                                 \
+        const float dx = ax - cx[j], dy = ay - cy[j];                                               \
+        dist[j] = dx * dx + dy * dy;                                                                \
+      }                                                                                             \
+      sqrt_cr<U>(dist, (TINY));                                                                     \
+    }                                                                                               \
+    _Pragma("unroll")                                                                               \
+    for (int j = 0; j < U; ++j) {                                                                   \
+      const float dx = ax - cx[j], dy = ay - cy[j];                                                 \
+      float v;                                                                                      \
+      if (MEAS == TS_DTW) {                                                                         \
+        v = dist[j] + fminf(fminf(up[j], l), d);                                                    \
+      } else if (MEAS == TS_FRECHET) {                                                              \
+        v = fmaxf(dx * dx + dy * dy, fminf(fminf(up[j], l), d));                                    \
+      } else {                                                                                      \
+        const bool match = fabsf(dx) < a.eps && fabsf(dy) < a.eps;                                  \
+        v = match ? d + 1.0f : fmaxf(up[j], l);                                                     \
+      }                                                                                             \
+      on[j] = v;                                                                                    \
+      d = up[j];                                                                                    \
+      l = v;                                                                                        \
+    }                                                                                               \
+    up_last = up[U - 1];                                                                            \
+    _Pragma("unroll")                                                                               \
+    for (int j = 0; j < U; ++j) o[j] = on[j];                                                       \
+                                                                                                    \
+    /* ---- operands of the next step, read into the registers the cells */                         \
+    /* are done with; the halo chunk (see above for its timing) before this */                      \
+    /* step's halo store */                                                                         \
+    const int kh = k;                                                                               \
+    wrapped = k + 1 == D;                                                                           \
+    k = wrapped ? 0 : k + 1;                                                                        \
+    row = wrapped ? row + W : row;                                                                  \
+    if (SR) {                                                                                       \
+      rp = rows[min(row, R - 1)];                                                                   \
+    } else {                                                                                        \
+      rp = make_float2(Ax[min(row, R - 1)], Ay[min(row, R - 1)]);                                   \
+    }                                                                                               \
+    if (SC) {                                                                                       \
+      load_pts<U>(cb + k * U, cx, cy);                                                              \
+    } else {                                                                                        \
+      /* idle chunks re-read the last one (sets pad trajectories to 8 points) */                    \
+      load_vec<U>(Bx + U * min(k, lastch), cx);                                                     \
+      load_vec<U>(By + U * min(k, lastch), cy);                                                     \
+    }                                                                                               \
+    const bool top = row == 0;                                                                      \
+    load_vec<U>(top ? brow : bnd + k * U, hv);                                                      \
+    dfirst = (MEAS == TS_LCSS || top) ? 0.0f : BND;                                                 \
+    if (lg == W - 1) store_vec<U>(bnd + kh * U, on);                                                \
+    --rem;                                                                                          \
+  } while (0)
+
 // SC / SR: columns / rows staged in LDS; otherwise read from global memory
 // (long trajectories: the launch then needs several waves per SIMD to hide
-// the reads, which the throughput regime has).
-template <int MEAS, int W, int U, bool SC, bool SR>
+// the reads, which the throughput regime has). SPLIT (DTW launches of 12 or
+// more waves per CU): the loop runs the steps between events in two copies,
+// with and without the tiny test, instead of testing the flag every step;
+// that is faster at those occupancies and slower below them.
+template <int MEAS, int W, int U, bool SC, bool SR, bool SPLIT>
 __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
   constexpr int G = 64 / W;
   constexpr float BND = (MEAS == TS_LCSS) ? 0.0f : kInf;
@@ -658,73 +733,28 @@ __global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
       tiny = __any(pair_tiny);
       if (__all(done)) break;
     }
-    if (wrapped) {
-      ax = rp.x;
-      ay = rp.y;
-    }
-
-    // ---- U cells
-    float up[U], on[U];
-#pragma unroll
-    for (int j = 0; j < U; ++j) up[j] = from_prev_lane<W>(o[j], hv[j], lg);
-    const bool first = k == 0;
-    float d = first ? dfirst : up_last;
-    float l = first ? BND : o[U - 1];
-    float dist[U];
-    if (MEAS == TS_DTW) {
-#pragma unroll
-      for (int j = 0; j < U; ++j) {
-        const float dx = ax - cx[j], dy = ay - cy[j];
-        dist[j] = dx * dx + dy * dy;
-      }
-      sqrt_cr<U>(dist, tiny);
-    }
-#pragma unroll
-    for (int j = 0; j < U; ++j) {
-      const float dx = ax - cx[j], dy = ay - cy[j];
-      float v;
-      if (MEAS == TS_DTW) {
-        v = dist[j] + fminf(fminf(up[j], l), d);
-      } else if (MEAS == TS_FRECHET) {
-        v = fmaxf(dx * dx + dy * dy, fminf(fminf(up[j], l), d));
+    if constexpr (SPLIT) {
+      // Steps until some lane's next event, with the test compiled in or
+      // out: the wave's tiny flag only changes at events. (Written as a
+      // lambda called twice, which measured faster than two plain loops.)
+      const auto steps = [&](auto tc) __attribute__((always_inline)) {
+        const bool tiny_now = tc;
+        do {
+          TS_LEAN_STEP(tiny_now);
+          __builtin_amdgcn_wave_barrier();
+        } while (!__any(rem == ev));
+      };
+      if (tiny) {
+        steps(std::true_type{});
       } else {
-        const bool match = fabsf(dx) < a.eps && fabsf(dy) < a.eps;
-        v = match ? d + 1.0f : fmaxf(up[j], l);
+        steps(std::false_type{});
       }
-      on[j] = v;
-      d = up[j];
-      l = v;
-    }
-    up_last = up[U - 1];
-#pragma unroll
-    for (int j = 0; j < U; ++j) o[j] = on[j];
-
-    // ---- operands of the next step, read into the registers the cells
-    // are done with; the halo chunk (see above for its timing) before this
-    // step's halo store
-    const int kh = k;
-    wrapped = k + 1 == D;
-    k = wrapped ? 0 : k + 1;
-    row = wrapped ? row + W : row;
-    if (SR) {
-      rp = rows[min(row, R - 1)];
     } else {
-      rp = make_float2(Ax[min(row, R - 1)], Ay[min(row, R - 1)]);
+      TS_LEAN_STEP(tiny);
     }
-    if (SC) {
-      load_pts<U>(cb + k * U, cx, cy);
-    } else {
-      // idle chunks re-read the last one (sets pad trajectories to 8 points)
-      load_vec<U>(Bx + U * min(k, lastch), cx);
-      load_vec<U>(By + U * min(k, lastch), cy);
-    }
-    const bool top = row == 0;
-    load_vec<U>(top ? brow : bnd + k * U, hv);
-    dfirst = (MEAS == TS_LCSS || top) ? 0.0f : BND;
-    if (lg == W - 1) store_vec<U>(bnd + kh * U, on);
-    --rem;
   }
 }
+#undef TS_LEAN_STEP
 
 // Banded DTW and windowed LCSS with the lean step. The strips' chunk ranges
 // come from a table the group writes at the fetch (the same strip_range as
@@ -1045,7 +1075,14 @@ static hipError_t launch_dp_t(hipStream_t st, const DpArgs& a, int grid_waves) {
     hipLaunchKernelGGL((k_dp_lean_r<MEAS, W, U, BAND, DELTA, LEAN == 4>), dim3(blocks), dim3(64 * wpb), lds, st, a);
   } else if constexpr (LEAN != 0) {
     static_assert(!BAND && !DELTA, "the lean kernel is unbanded");
-    hipLaunchKernelGGL((k_dp_lean<MEAS, W, U, LEAN != 3, LEAN == 1>), dim3(blocks), dim3(64 * wpb), lds, st, a);
+    // DTW launches of 12 or more waves per CU (the throughput regime) take
+    // the loop split by the tiny flag; fewer waves run faster without it.
+    if (MEAS == TS_DTW && grid_waves >= 12 * device_cu_count())
+      hipLaunchKernelGGL((k_dp_lean<MEAS, W, U, LEAN != 3, LEAN == 1, MEAS == TS_DTW>), dim3(blocks), dim3(64 * wpb),
+                         lds, st, a);
+    else
+      hipLaunchKernelGGL((k_dp_lean<MEAS, W, U, LEAN != 3, LEAN == 1, false>), dim3(blocks), dim3(64 * wpb), lds, st,
+                         a);
   } else {
     hipLaunchKernelGGL((k_dp<MEAS, W, U, BAND, DELTA>), dim3(blocks), dim3(64 * wpb), lds, st, a);
   }

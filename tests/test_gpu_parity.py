@@ -74,6 +74,18 @@ def test_dtw_extreme_scales_bit_exact(eng, scale):
     s = dict(db=t, qs=t, db_xy=xy, db_off=off, q_xy=xy, q_off=off, qi=qi.ravel(), di=di.ravel())
     say("dtw scale", scale)
     check_pairs(eng, s, "dtw")
+    check_pairs(dtw_engine_at(16), s, "dtw")
+
+
+def dtw_engine_at(waves):
+    """An engine whose unbanded DTW launches run `waves` waves per CU (each
+    bucket's tuned shape): at 12 or more they take the step loop split by
+    the tiny-coordinate flag."""
+    e = ts.Engine()
+    for b in range(len(ts.DP_BUCKET_CAPS)):
+        W, U, _ = e.get_dp_shape(0, b, table=1)
+        e.set_dp_shape(0, b, W, U, waves)
+    return e
 
 
 def lattice(seed, lengths, below=False):
@@ -117,6 +129,7 @@ def test_dtw_at_the_tiny_threshold_bit_exact(eng, band):
     assert s["qs"].flags() == 0 and s["db"].flags() == 0
     say("dtw lattice band", band)
     check_pairs(eng, s, "dtw", band=band)
+    check_pairs(dtw_engine_at(16), s, "dtw", band=band)
     p = ts.params("dtw", mode=ts.EXACT, k=5, band=band)
     prm = oracle.params(p.measure, p.mode, k=5, kc=p.kc, r=p.r, band=p.band)
     got = eng.knn(s["db"], s["qs"], p)
@@ -140,6 +153,7 @@ def test_dtw_one_set_with_tiny_coordinates_bit_exact(eng, tiny_side):
         assert s["qs"].flags() == 0 and s["db"].flags() == ts.SET_TINY_COORD
     say("dtw tiny", tiny_side)
     check_pairs(eng, s, "dtw")
+    check_pairs(dtw_engine_at(16), s, "dtw")
     check_pairs(eng, s, "dtw", band=-10)
 
 
@@ -186,14 +200,16 @@ def test_lcss_window_bit_exact(eng, shapes, delta):
 @pytest.mark.parametrize("variant,measure,kw", [(0, "dtw", {}), (1, "dtw", {"band": 3}), (1, "dtw", {"band": -20}),
                                                 (2, "frechet", {}), (3, "lcss", {"epsilon": 3e-2}),
                                                 (4, "lcss", {"epsilon": 3e-2, "delta": 5})])
-def test_every_dp_shape_bit_exact(shapes, variant, measure, kw):
+@pytest.mark.parametrize("waves", [4, 16])
+def test_every_dp_shape_bit_exact(shapes, variant, measure, kw, waves):
     """Each lane-group shape the autotuner can choose gives the oracle's
-    values bit for bit, in every bucket."""
+    values bit for bit, in every bucket; at 16 waves per CU unbanded DTW
+    runs the step loop split by the tiny-coordinate flag."""
     e = ts.Engine()
     for (W, U) in ts.dp_shapes():
-        say("dp shape", measure, kw, W, U)
+        say("dp shape", measure, kw, W, U, waves)
         for b in range(len(ts.DP_BUCKET_CAPS)):
-            e.set_dp_shape(variant, b, W, U, 4)
+            e.set_dp_shape(variant, b, W, U, waves)
         check_pairs(e, shapes, measure, **kw)
 
 
