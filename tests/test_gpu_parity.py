@@ -165,6 +165,7 @@ def test_dtw_long_pairs_with_tiny_coordinates_bit_exact(eng):
     s = pair_case(b_xy, b_off, a_xy, a_off)
     say("dtw tiny long")
     check_pairs(eng, s, "dtw")
+    check_pairs(dtw_engine_at(16), s, "dtw")
     check_pairs(eng, s, "dtw", band=-10)
 
 
@@ -241,6 +242,11 @@ def test_long_pairs_bit_exact(eng):
         got = eng.pair_values(db, qs, p, qi, di)
         want = oracle.pair_values(prm, db_xy, db_off, q_xy, q_off, qi, di)
         assert np.array_equal(got.view(np.int32), want.view(np.int32)), (measure, got, want)
+    # unbanded DTW at 16 waves per CU: the split step loop, trajectories read from global memory
+    p = ts.params("dtw")
+    got = dtw_engine_at(16).pair_values(db, qs, p, qi, di)
+    want = oracle.pair_values(oracle.params(p.measure), db_xy, db_off, q_xy, q_off, qi, di)
+    assert np.array_equal(got.view(np.int32), want.view(np.int32)), (got, want)
 
 
 @pytest.mark.parametrize("variant,measure,kw", [(1, "dtw", {"band": -10}), (1, "dtw", {"band": 205}),
