@@ -401,9 +401,14 @@ int orc_knn(const int32_t* params, int brute, const float* db_xy, const int64_t*
   Set db{(const Pt*)db_xy, db_off, N}, qs{(const Pt*)q_xy, q_off, Q};
   const int k = p.k;
   std::vector<Env> dbb(N);
+#pragma omp parallel for schedule(static)
   for (int j = 0; j < N; ++j) dbb[j] = mbr(db.t(j), db.len(j));
   int64_t refined = 0;
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : refined)
+  // Fewer queries than threads: queries run one after another and each
+  // query's bounds and refines run on every thread instead (a long query
+  // would otherwise hold one core while the others idle).
+  const bool par_queries = Q >= omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : refined) if (par_queries)
   for (int qi = 0; qi < Q; ++qi) {
     const Pt* q = qs.t(qi);
     int n = qs.len(qi);
@@ -414,7 +419,9 @@ int orc_knn(const int32_t* params, int brute, const float* db_xy, const int64_t*
       if (p.measure == DTW || p.measure == FRECHET || p.measure == LCSS) {
         std::vector<Env> env(n);
         envelope(q, n, p.r, env.data());
-        for (int j = 0; j < N; ++j) cand.push_back({ref_score(p, env.data(), n, db.t(j), db.len(j)), j});
+        cand.resize(N);
+#pragma omp parallel for schedule(static) if (!par_queries)
+        for (int j = 0; j < N; ++j) cand[j] = {ref_score(p, env.data(), n, db.t(j), db.len(j)), j};
       } else {
         Env qb = mbr(q, n);
         for (int j = 0; j < N; ++j)
@@ -435,22 +442,50 @@ int orc_knn(const int32_t* params, int brute, const float* db_xy, const int64_t*
     } else {
       // Best-first by bound; stop once the bound cannot beat the k-th best.
       Env qb = mbr(q, n);
-      std::vector<Cand> order;
+      std::vector<Cand> order(N);
+#pragma omp parallel for schedule(static) if (!par_queries)
       for (int j = 0; j < N; ++j)
-        if (eligible(j)) order.push_back({summary_bound(p, q, n, qb, db.t(j), db.len(j), dbb[j]), j});
+        order[j] = {eligible(j) ? summary_bound(p, q, n, qb, db.t(j), db.len(j), dbb[j]) : NAN, j};
+      order.erase(std::remove_if(order.begin(), order.end(), [](const Cand& c) { return std::isnan(c.v); }),
+                  order.end());
       bool mx = is_max_measure(p.measure);
       std::sort(order.begin(), order.end(), [&](const Cand& a, const Cand& b) { return better(p.measure, a, b); });
       std::vector<Cand> top;  // current best k, sorted
-      for (const Cand& o : order) {
-        if ((int)top.size() == k) {
-          const Cand& kth = top.back();
-          if (mx ? (o.v < kth.v) : (o.v > kth.v)) break;
+      // Candidates in bound order, a chunk at a time: the chunk's exact
+      // values are computed in parallel (those whose bound already cannot
+      // beat the k-th best at the chunk's start are skipped), then merged
+      // one by one with the sequential rule: stop at the first bound that
+      // cannot beat the current k-th best. Every merged value is the exact
+      // value, so the result equals the one-at-a-time search.
+      const size_t chunk = par_queries ? 1 : (size_t)4 * omp_get_max_threads();
+      std::vector<float> ex;
+      bool stop = false;
+      for (size_t c0 = 0; c0 < order.size() && !stop; c0 += chunk) {
+        const size_t c1 = std::min(order.size(), c0 + chunk);
+        const bool full = (int)top.size() == k;
+        const float kv = full ? top.back().v : 0.0f;
+        ex.assign(c1 - c0, 0.0f);
+#pragma omp parallel for schedule(dynamic, 1) if (!par_queries)
+        for (size_t t = c0; t < c1; ++t) {
+          const Cand& o = order[t];
+          const bool beyond = full && (mx ? (o.v < kv) : (o.v > kv));
+          if (!beyond && !(mx && o.v == 0.0f)) ex[t - c0] = exact_value(p, q, n, db.t(o.i), db.len(o.i));
         }
-        Cand c{(mx && o.v == 0.0f) ? 0.0f : exact_value(p, q, n, db.t(o.i), db.len(o.i)), o.i};
-        if (!(mx && o.v == 0.0f)) ++refined;
-        auto it = std::lower_bound(top.begin(), top.end(), c, [&](const Cand& a, const Cand& b) { return better(p.measure, a, b); });
-        top.insert(it, c);
-        if ((int)top.size() > k) top.pop_back();
+        for (size_t t = c0; t < c1; ++t) {
+          const Cand& o = order[t];
+          if ((int)top.size() == k) {
+            const Cand& kth = top.back();
+            if (mx ? (o.v < kth.v) : (o.v > kth.v)) {
+              stop = true;
+              break;
+            }
+          }
+          Cand c{(mx && o.v == 0.0f) ? 0.0f : ex[t - c0], o.i};
+          if (!(mx && o.v == 0.0f)) ++refined;
+          auto it = std::lower_bound(top.begin(), top.end(), c, [&](const Cand& a, const Cand& b) { return better(p.measure, a, b); });
+          top.insert(it, c);
+          if ((int)top.size() > k) top.pop_back();
+        }
       }
       result = top;
     }
