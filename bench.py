@@ -1,20 +1,32 @@
 """Benchmark of the trajectory k-NN hot path on one MI355X.
 
-Workload (the configuration the headline metric is quoted on): 64 query
-random walks against a resident database of 100,000 random walks, lengths
-uniform in [64, 256], start U[0,1)^2, Gaussian steps sigma=1e-3 (DB seed
-0xDB5EED, query seed 0x9E3779B9); DTW top-10 with the bounding-box/endpoint
-lower-bound filter (EXACT mode: the true top-10 over the whole DB).
+Default workload (the configuration the headline metric, batched vs plain
+pairs/s, is stated on): 4,096 query random walks with lengths log-uniform in
+[16, 2048] against a resident database of 1,000,000 random walks with
+lengths log-uniform in [16, 1024]; start U[0,1)^2, Gaussian steps
+sigma=1e-3 (DB seed 0xDB5EED, query seed 0x9E3779B9); DTW and discrete
+Frechet top-10 over the whole database (EXACT mode: bounding-box/endpoint
+lower bounds, tight point-to-box bounds, persistent length-bucketed DP).
 
-One step = one batched k-NN call (ts_knn) over the 64 queries. The line also
-carries Frechet, the plain one-launch-sequence-per-query method, per-query
-latency p50/p99, the dominant kernel's roofline (HIP events over the timed
-region) and the CPU oracle timed on a bounded sample of the same workload on
-this host's cores.
+One step = one batched k-NN call (ts_knn) over all 4,096 queries, sets
+already resident in HBM. The line carries:
+  value          DTW pairs/s of the batched call (4096 * 1e6 / seconds);
+  extra          Frechet, the plain method (one launch sequence per query) on
+                 a query sample, batched/plain ratios, cell-updates/s, the
+                 64 x 100k five-measure figures, the database-file leg;
+  parity         the batched rows of the longest, shortest and middle-length
+                 queries against the CPU oracle, bit for bit;
+  roofline       the dominant kernel (by time in the timed region) from HIP
+                 events on the stream it runs on;
+  cpu_baseline   the reference's pipeline (envelope filter, 10 candidates,
+                 exact refine), as restated by the oracle for variable
+                 lengths, on a bounded query sample over the full database
+                 on this host's cores.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1..c5]
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -34,13 +46,24 @@ FP32_PEAK_TFLOPS = 157.3
 # counting one operation per VALU lane-instruction: 256 CUs x 64 lanes x 2.4 GHz.
 VALU_ATTAINABLE_TOPS = 39.3
 HBM_PEAK_GBS = 8000.0
-# Source-level f32 operations per unit of work (DP cell or pair).
+# Source-level f32 operations per unit of work (DP cell).
 OPS = {"dtw_wavefront": 9.0, "frechet_wavefront": 9.0, "lcss_wavefront": 7.0, "dtw_wavefront_seed": 9.0,
-       "frechet_wavefront_seed": 9.0, "lcss_wavefront_seed": 7.0, "bound_seed": 24.0, "bound_compact": 24.0}
+       "frechet_wavefront_seed": 9.0, "lcss_wavefront_seed": 7.0}
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+_hip = None
+
+
+def device_sync():
+    """hipDeviceSynchronize: the timed region starts and ends with the device idle."""
+    global _hip
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+    assert _hip.hipDeviceSynchronize() == 0
 
 
 def build_workload(ts, n_db, n_q, lo, hi):
@@ -54,9 +77,11 @@ def build_workload(ts, n_db, n_q, lo, hi):
 def timed(fn, steps, warmup):
     for _ in range(warmup):
         fn()
+    device_sync()
     t0 = time.perf_counter()
     for _ in range(steps):
         out = fn()
+    device_sync()
     return (time.perf_counter() - t0) * 1e3 / steps, out
 
 
@@ -66,10 +91,15 @@ def ops_of(name):
     return OPS.get(name.split("/")[0])
 
 
-def roofline_of(kstats, pmc):
-    """Dominant kernel (one launch label: measure, bucket, shape) by total
-    time; achieved = algorithmic ops per launch / mean launch duration (HIP
-    events on the stream the kernel runs on)."""
+def load_pmc(name):
+    path = os.path.join(ROOT, "profiles", name)
+    return json.load(open(path)) if os.path.exists(path) else None
+
+
+def roofline_of(kstats, pmc, pmc_source):
+    """Dominant DP launch (one label: measure, phase, bucket, shape) by total
+    time; achieved = algorithmic ops per launch / mean launch duration, both
+    from the engine's HIP events on the stream the kernel runs on."""
     name = max((k for k in kstats if ops_of(k)), key=lambda k: kstats[k]["ms"])
     s = kstats[name]
     per_launch_units = s["units"] / s["launches"]
@@ -80,35 +110,21 @@ def roofline_of(kstats, pmc):
         traffic = pmc[name].get("hbm_bytes_per_launch")
     return {"bound": "valu", "kernel": name, "achieved": round(achieved, 4), "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
-            "units_per_launch": per_launch_units, "ms_per_launch": round(mean_ms, 4),
-            "ops_per_unit": ops_of(name),
+            "traffic_source": pmc_source if traffic is not None else None,
+            "units_per_launch": per_launch_units, "unit_name": "DP cell", "ms_per_launch": round(mean_ms, 4),
+            "launches": s["launches"], "ops_per_unit": ops_of(name),
             "attainable": VALU_ATTAINABLE_TOPS, "frac_of_attainable": round(achieved / VALU_ATTAINABLE_TOPS, 4),
+            "cell_updates_per_s": per_launch_units / (mean_ms * 1e-3),
             "peak_note": "peak: MI355X's published FP32 vector figure (FMA = 2 flops, packed FP32). The DP "
                          "is bound by VALU issue, neither by HBM nor by MFMA (a min-plus recurrence); built "
                          "without packed FP32 and at one op per lane-instruction its ceiling is 'attainable' "
                          "(256 CUs x 64 lanes x 2.4 GHz)"}
 
 
-def cpu_baseline(db_xy, db_off, q_xy, q_off, n_sample):
-    import oracle
-    prm = oracle.params(oracle.DTW, oracle.EXACT, k=10)
-    sub_off = q_off[:n_sample + 1]
-    sub_xy = q_xy[:sub_off[-1]]
-    t0 = time.perf_counter()
-    idx, val, refined = oracle.knn(prm, db_xy, db_off, sub_xy, sub_off)
-    dt = time.perf_counter() - t0
-    n_db = len(db_off) - 1
-    return {"value": round(n_sample * n_db / dt, 1), "unit": "pairs/s", "cores": oracle.num_threads(),
-            "kind": "port",
-            "sample": "%d of the 64 queries against the full %d-trajectory DB, EXACT DTW top-10 (oracle best-first "
-                      "search with the same bounds), %.2f s" % (n_sample, n_db, dt)}, idx, val
-
-
 def copy_calibration(nbytes=256 << 20, reps=3):
     """Plain device-to-device copies of known size through the HIP runtime's
     copy kernel: the reference point that scales rocprof's FETCH_SIZE and
     WRITE_SIZE counters to bytes."""
-    import ctypes
     hip = ctypes.CDLL("libamdhip64.so")
     a, b = ctypes.c_void_p(), ctypes.c_void_p()
     assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
@@ -122,53 +138,286 @@ def copy_calibration(nbytes=256 << 20, reps=3):
     log("copy calibration: %d x %d MiB device copies" % (reps, nbytes >> 20))
 
 
-def c2_other_measures(ts, eng, db, qs, args):
-    """The other measures of the 64 x 100k workload, batched EXACT top-10:
-    LCSS (eps 2e-3, window 16), Hausdorff, and lock-step Euclidean on a
-    separately generated DB of 100,000 walks of 128 points (Euclidean needs
-    equal lengths) with 64 queries of 128 points."""
+def sub_queries(q_xy, q_off, sel):
+    parts = [q_xy[q_off[i]:q_off[i + 1]] for i in sel]
+    return np.concatenate(parts), np.concatenate([[0], np.cumsum([len(p) for p in parts])]).astype(np.int64)
+
+
+# ------------------------------------------------------------------ c3 (default)
+def run_c3(ts, args):
+    """Uneven-length batching workload: 4,096 queries (log-uniform lengths
+    [16, 2048]) against 1,000,000 DB random walks (log-uniform [16, 1024]),
+    DTW and Frechet top-10. Batched = all queries in one ts_knn call; plain =
+    one launch sequence per query (ts_knn_plain) on a sample of queries, whose
+    top-10 must equal the batched rows."""
+    t0 = time.time()
+    db_len = ts.gen_lengths(DB_SEED, args.n_db, 16, 1024, True)
+    db_xy, db_off = ts.gen_walks(DB_SEED, db_len)
+    q_len = ts.gen_lengths(Q_SEED, args.n_q, 16, 2048, True)
+    q_xy, q_off = ts.gen_walks(Q_SEED, q_len)
+    db = ts.TrajSet(db_xy, db_off)
+    qs = ts.TrajSet(q_xy, q_off)
     Q, N = qs.count, db.count
-    out = {}
-    legs = [("lcss", ts.params("lcss", mode=ts.EXACT, k=10, epsilon=2e-3, delta=16), db, qs),
+    ns = min(args.plain_sample, Q)
+    sample = ts.TrajSet(q_xy[:q_off[ns]], q_off[:ns + 1])
+    dev = ts.device_name()
+    log("device", dev)
+    log("c3 workload ready in %.1fs: %d DB trajectories (%d points), %d queries" %
+        (time.time() - t0, N, len(db_xy), Q))
+    eng = ts.Engine(graphs=not args.no_graphs)
+    K, W = args.steps, args.warmup
+    pmc = load_pmc("pmc_traffic_c3.json")
+    out, rows, roof = {}, {}, None
+    for meas in ("dtw", "frechet"):
+        p = ts.params(meas, mode=ts.EXACT, k=10)
+        for _ in range(max(W, 1)):
+            eng.knn(db, qs, p)
+        # Timed region: K batched calls. Batches this large launch eagerly
+        # (no graph replay), and the engine brackets every launch with HIP
+        # events on the stream it runs on, for the per-kernel table.
+        eng.set_timing(True)
+        eng.reset_stats()
+        device_sync()
+        t1 = time.perf_counter()
+        for _ in range(K):
+            idx, val, st = eng.knn(db, qs, p)
+        device_sync()
+        tb = (time.perf_counter() - t1) / K
+        kst = eng.kernel_stats()
+        eng.set_timing(False)
+        rows[meas] = (idx, val)
+        t1 = time.perf_counter()
+        pi, pv, _ = eng.knn_plain(db, sample, p)
+        tp = time.perf_counter() - t1
+        same = bool(np.array_equal(pi, idx[:ns]) and np.array_equal(pv.view(np.int32), val[:ns].view(np.int32)))
+        out[meas] = {"batched_pairs_per_s": Q * N / tb, "batched_s": round(tb, 4), "graphs": eng.graph_info(),
+                     "batched_cell_updates_per_s": st["cells"] / tb, "cells_per_step": st["cells"],
+                     "refined_pairs_per_step": st["refined_pairs"], "launches_per_step": st["launches"],
+                     "plain_pairs_per_s": ns * N / tp, "plain_s_for_sample": round(tp, 4), "plain_sample": ns,
+                     "batched_vs_plain": round((Q * N / tb) / (ns * N / tp), 2), "plain_topk_equals_batched": same,
+                     "kernels": {k: {"launches": v["launches"], "ms_mean": round(v["ms"] / max(v["launches"], 1), 4)}
+                                 for k, v in sorted(kst.items(), key=lambda kv: -kv[1]["ms"])[:12]}}
+        if meas == "dtw":
+            roof = roofline_of(kst, pmc, "profiles/pmc_traffic_c3.json")
+        log(meas, {k: v for k, v in out[meas].items() if k != "kernels"})
+    out["db_file"] = db_file_leg(ts, eng, db, sample)
+    parity = None
+    if not args.no_oracle:
+        parity = c3_parity(ts, rows, db_xy, db_off, q_xy, q_off, q_len)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_ref_pipeline(db_xy, db_off, q_xy, q_off, min(args.cpu_sample, Q),
+                               "c3 (queries log-uniform [16,2048], DB 1M log-uniform [16,1024])")
+        out["cpu_other"] = cpu_other_c3(db_xy, db_off, q_xy, q_off)
+    sample.close()
+    qs.close()
+    db.close()
+    if not args.no_c2:
+        out["c2"] = c2_leg(ts, args)
+    line = {"metric": METRIC, "value": round(out["dtw"]["batched_pairs_per_s"], 1), "unit": "pairs/s", "n_gpus": 1,
+            "steps": K, "warmup": W, "ms_per_step": round(out["dtw"]["batched_s"] * 1e3, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "uneven-length batching: %d query random walks log-uniform [16,2048] vs %d-"
+                                   "trajectory DB log-uniform [16,1024], DTW top-10 (Frechet beside it), EXACT "
+                                   "bounding-box filter, batched one call (plain method on a %d-query sample)"
+                                   % (Q, N, ns),
+                       "queries": Q, "db_trajectories": N, "k": 10, "measure": "dtw", "mode": "exact",
+                       "device": dev},
+            "batched_vs_plain": {"dtw": out["dtw"]["batched_vs_plain"], "frechet": out["frechet"]["batched_vs_plain"]},
+            "parity": parity, "roofline": roof, "cpu_baseline": cpu, "extra": out}
+    print(json.dumps(line), flush=True)
+
+
+def c3_parity(ts, rows, db_xy, db_off, q_xy, q_off, q_len):
+    """The batched rows of the two longest, two shortest and two
+    middle-length queries against the oracle's EXACT top-10, bit for bit."""
+    import oracle
+    order = np.argsort(q_len, kind="stable")
+    sel = np.concatenate([order[-2:], order[:2], order[[len(order) // 3, 2 * len(order) // 3]]])
+    s_xy, s_off = sub_queries(q_xy, q_off, sel)
+    res = {"queries": sel.tolist(), "lengths": q_len[sel].tolist()}
+    t0 = time.perf_counter()
+    for meas, om in (("dtw", oracle.DTW), ("frechet", oracle.FRECHET)):
+        wi, wv, _ = oracle.knn(oracle.params(om, oracle.EXACT, k=10), db_xy, db_off, s_xy, s_off)
+        gi, gv = rows[meas]
+        res[meas + "_topk_equal"] = bool(np.array_equal(gi[sel], wi) and
+                                         np.array_equal(gv[sel].view(np.int32), wv.view(np.int32)))
+    res["seconds"] = round(time.perf_counter() - t0, 2)
+    res["rule"] = "indices and f32 values of the whole top-10 identical (ties by index)"
+    log("c3 parity", res)
+    return res
+
+
+def cpu_ref_pipeline(db_xy, db_off, q_xy, q_off, n_sample, what):
+    """The reference's CPU pipeline (dtw.cpp:14-113: envelope r=5, LB_DTW of
+    every DB trajectory, the 10 best, exact DTW, best first) as the oracle
+    restates it for variable lengths, on the first n_sample queries over the
+    whole database, parallel over queries on every core."""
+    import oracle
+    prm = oracle.params(oracle.DTW, oracle.REF, k=10)
+    s_off = q_off[:n_sample + 1]
+    t0 = time.perf_counter()
+    oracle.knn(prm, db_xy, db_off, q_xy[:s_off[-1]], s_off)
+    dt = time.perf_counter() - t0
+    n_db = len(db_off) - 1
+    cpu = {"value": round(n_sample * n_db / dt, 1), "unit": "pairs/s", "cores": oracle.num_threads(), "kind": "port",
+           "sample": "%s: the first %d queries against the whole %d-trajectory DB, the reference's DTW pipeline "
+                     "(envelope LB filter, 10 candidates, exact refine), %.2f s" % (what, n_sample, n_db, dt)}
+    log("cpu baseline", cpu)
+    return cpu
+
+
+def cpu_other_c3(db_xy, db_off, q_xy, q_off, n_exact=8, ref_num_t=2000, ref_len=256):
+    """More CPU figures: the oracle's EXACT search (the same answer as the
+    GPU's) on the first n_exact queries, and the reference's own compiled
+    launch_dtw_batch_cpu (oracle/_ref, dtw.cpp:14-113 built from the
+    reference sources) on its own fixed-length shape: ref_num_t queries
+    against ref_num_t DB walks of ref_len points (the workload's mean DB
+    length is 242), pairs/s = ref_num_t^2 / seconds."""
+    import oracle
+    n_db = len(db_off) - 1
+    prm = oracle.params(oracle.DTW, oracle.EXACT, k=10)
+    t0 = time.perf_counter()
+    oracle.knn(prm, db_xy, db_off, q_xy[:q_off[n_exact]], q_off[:n_exact + 1])
+    dt = time.perf_counter() - t0
+    res = {"oracle_exact_pairs_per_s": round(n_exact * n_db / dt, 1), "oracle_exact_sample": n_exact,
+           "oracle_exact_seconds": round(dt, 2), "cores": oracle.num_threads()}
+    if oracle.ref_available():
+        xy, _ = _walks_fixed(2 * ref_num_t, ref_len)
+        t1, t2 = xy[:ref_num_t * ref_len], xy[ref_num_t * ref_len:]
+        t0 = time.perf_counter()
+        _, ms = oracle.ref_batch(oracle.DTW, t1, t2, ref_num_t, ref_len, ref_len)
+        dt = time.perf_counter() - t0
+        res.update({"reference_build_pairs_per_s": round(ref_num_t * ref_num_t / dt, 1),
+                    "reference_build_ms": round(ms, 1),
+                    "reference_build_shape": "launch_dtw_batch_cpu, %d x %d walks of %d points (fixed length, "
+                                             "Q = N as the reference requires)" % (ref_num_t, ref_num_t, ref_len)})
+    log("cpu other", res)
+    return res
+
+
+def _walks_fixed(count, length):
+    import cudaruntime1_amd as ts
+    return ts.gen_walks(DB_SEED + 7, np.full(count, length, np.int32))
+
+
+def db_file_leg(ts, eng, db, sample):
+    """The DB as a tsdb file: save from HBM, load back (the file then sits in
+    the host's page cache: a warm restart), and the loaded set's DTW top-10
+    for the query sample equals the original's."""
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "db.tsdb")
+        t1 = time.perf_counter()
+        db.save(path)
+        t_save = time.perf_counter() - t1
+        nbytes = os.path.getsize(path)
+        t1 = time.perf_counter()
+        db2 = ts.TrajSet.load(path)
+        t_load = time.perf_counter() - t1
+    p = ts.params("dtw", mode=ts.EXACT, k=10)
+    a, b = eng.knn(db, sample, p), eng.knn(db2, sample, p)
+    same = bool(np.array_equal(a[0], b[0]) and np.array_equal(a[1].view(np.int32), b[1].view(np.int32)))
+    db2.close()
+    res = {"bytes": nbytes, "save_s": round(t_save, 3), "load_s": round(t_load, 3),
+           "load_GB_per_s": round(nbytes / t_load / 1e9, 2), "loaded_topk_equals": same,
+           "note": "load = map, check, copy points to HBM, recompute and compare summaries; file in the page cache"}
+    log("db file", res)
+    return res
+
+
+# ------------------------------------------------------------------ c2
+def c2_leg(ts, args, want_kernels=False):
+    """The five-measure workload: 64 queries vs 100,000 DB random walks,
+    lengths uniform [64, 256]; EXACT top-10 batched (graph replay), the plain
+    method and single-query latency for DTW; LCSS with eps 2e-3 and window
+    16; lock-step Euclidean on its own 100,000 x 128 DB."""
+    db_xy, db_off, q_xy, q_off = build_workload(ts, 100_000, 64, 64, 256)
+    db, qs = ts.TrajSet(db_xy, db_off), ts.TrajSet(q_xy, q_off)
+    Q, N = qs.count, db.count
+    eng = ts.Engine(graphs=not args.no_graphs)
+    steps = max(args.steps, 10)
+    res = {}
+    legs = [("dtw", ts.params("dtw", mode=ts.EXACT, k=10), db, qs),
+            ("frechet", ts.params("frechet", mode=ts.EXACT, k=10), db, qs),
+            ("lcss", ts.params("lcss", mode=ts.EXACT, k=10, epsilon=2e-3, delta=16), db, qs),
             ("hausdorff", ts.params("hausdorff", mode=ts.EXACT, k=10), db, qs)]
     e_db_xy, e_db_off = ts.gen_walks(DB_SEED + 1, np.full(N, 128, np.int32))
     e_q_xy, e_q_off = ts.gen_walks(Q_SEED + 1, np.full(Q, 128, np.int32))
     e_db, e_qs = ts.TrajSet(e_db_xy, e_db_off), ts.TrajSet(e_q_xy, e_q_off)
     legs.append(("euclidean", ts.params("euclidean", mode=ts.EXACT, k=10), e_db, e_qs))
+    dtw_rows = None
     for name, p, d, q in legs:
-        ms, (_, _, st) = timed(lambda: eng.knn(d, q, p), max(1, args.steps // 2), 2)
-        out[name] = {"pairs_per_s": q.count * d.count / (ms * 1e-3), "ms_per_step": round(ms, 4),
+        ms, (gi, gv, st) = timed(lambda: eng.knn(d, q, p), steps, 3)
+        res[name] = {"pairs_per_s": q.count * d.count / (ms * 1e-3), "ms_per_step": round(ms, 4),
                      "cells_per_step": st["cells"], "cell_updates_per_s": st["cells"] / (ms * 1e-3),
                      "refined_pairs_per_step": st["refined_pairs"]}
-        log("c2", name, out[name])
-    return out
+        if name == "dtw":
+            dtw_rows = (gi, gv)
+            res["graphs"] = eng.graph_info()
+    p = legs[0][1]
+    ms_plain, (pi, pv, _) = timed(lambda: eng.knn_plain(db, qs, p), max(1, steps // 3), 1)
+    res["dtw"].update({"plain_pairs_per_s": Q * N / (ms_plain * 1e-3), "plain_ms_per_step": round(ms_plain, 4),
+                       "batched_vs_plain": round(ms_plain / res["dtw"]["ms_per_step"], 2),
+                       "plain_equals_batched": bool(np.array_equal(pi, dtw_rows[0]) and
+                                                    np.array_equal(pv.view(np.int32), dtw_rows[1].view(np.int32)))})
+    lat = []
+    singles = [ts.TrajSet(q_xy[q_off[i]:q_off[i + 1]], np.array([0, q_off[i + 1] - q_off[i]], np.int64))
+               for i in range(Q)]
+    for s in singles[:4]:
+        eng.knn(db, s, p)
+    for s in singles:
+        t1 = time.perf_counter()
+        eng.knn(db, s, p)
+        lat.append((time.perf_counter() - t1) * 1e3)
+    res["dtw"].update({"single_query_ms_p50": round(float(np.percentile(lat, 50)), 4),
+                       "single_query_ms_p99": round(float(np.percentile(lat, 99)), 4)})
+    for s in singles:
+        s.close()
+    kst = None
+    if want_kernels:
+        eng.set_timing(True)
+        eng.reset_stats()
+        for _ in range(steps):
+            ti, tv, _ = eng.knn(db, qs, p)
+        kst = eng.kernel_stats()
+        eng.set_timing(False)
+        assert np.array_equal(ti, dtw_rows[0]) and np.array_equal(tv.view(np.int32), dtw_rows[1].view(np.int32))
+    for s in (db, qs, e_db, e_qs):
+        s.close()
+    eng.close()
+    log("c2", {k: (v.get("ms_per_step") if isinstance(v, dict) else v) for k, v in res.items()})
+    return (res, kst, (db_xy, db_off, q_xy, q_off, dtw_rows)) if want_kernels else res
 
 
-def cpu_extra(db_xy, db_off, q_xy, q_off, n_one=4):
-    """More CPU figures on the same workload: the oracle with one thread
-    (the reference's CPU build ran single-threaded) on n_one queries, and the
-    REF pipeline (envelope filter, top-10 candidates, exact refine) over all
-    queries on every core."""
-    import oracle
-    n_db = len(db_off) - 1
-    cores = oracle.num_threads()
-    prm = oracle.params(oracle.DTW, oracle.EXACT, k=10)
-    oracle.set_threads(1)
-    try:
-        t0 = time.perf_counter()
-        oracle.knn(prm, db_xy, db_off, q_xy[:q_off[n_one]], q_off[:n_one + 1])
-        dt1 = time.perf_counter() - t0
-    finally:
-        oracle.set_threads(cores)
-    ref = oracle.params(oracle.DTW, oracle.REF, k=10)
-    t0 = time.perf_counter()
-    oracle.knn(ref, db_xy, db_off, q_xy, q_off)
-    dtr = time.perf_counter() - t0
-    nq = len(q_off) - 1
-    return {"exact_1_thread_pairs_per_s": round(n_one * n_db / dt1, 1), "exact_1_thread_sample": n_one,
-            "ref_pairs_per_s": round(nq * n_db / dtr, 1), "ref_cores": cores, "ref_seconds": round(dtr, 3)}
+def run_c2(ts, args):
+    """The 64 x 100k five-measure workload as its own line (value: batched
+    EXACT DTW pairs/s), with the DTW roofline of its dominant launch and the
+    reference pipeline on the host beside it."""
+    if args.copy_calibration:
+        copy_calibration()
+    res, kst, (db_xy, db_off, q_xy, q_off, dtw_rows) = c2_leg(ts, args, want_kernels=True)
+    roof = roofline_of(kst, load_pmc("pmc_traffic_c2.json"), "profiles/pmc_traffic_c2.json")
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        import oracle
+        cpu = cpu_ref_pipeline(db_xy, db_off, q_xy, q_off, 64, "64 x 100k, lengths U[64,256]")
+        wi, wv, _ = oracle.knn(oracle.params(oracle.DTW, oracle.EXACT, k=10), db_xy, db_off, q_xy, q_off)
+        parity = {"queries": 64, "dtw_topk_equal": bool(np.array_equal(wi, dtw_rows[0]) and
+                                                        np.array_equal(wv.view(np.int32), dtw_rows[1].view(np.int32)))}
+    d = res["dtw"]
+    line = {"metric": METRIC, "value": round(d["pairs_per_s"], 1), "unit": "pairs/s", "n_gpus": 1,
+            "steps": max(args.steps, 10), "warmup": 3, "ms_per_step": d["ms_per_step"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "DTW top-10 k-NN, 64 query random walks vs 100000-trajectory DB (lengths "
+                                   "U[64,256], sigma 1e-3), bounding-box/endpoint lower-bound filter, batched",
+                       "queries": 64, "db_trajectories": 100_000, "k": 10, "measure": "dtw", "mode": "exact"},
+            "parity": parity, "roofline": roof, "cpu_baseline": cpu, "extra": res}
+    print(json.dumps(line), flush=True)
 
 
+# ------------------------------------------------------------------ c1
 def run_c1(ts, args):
     """The reference's CPU-sized case: 1 query against 2,000 DB random walks,
     lengths uniform [32, 128]; DTW REF (r=5, 10 candidates) and EXACT top-10;
@@ -198,98 +447,15 @@ def run_c1(ts, args):
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": "1 query vs 2,000 DB random walks, lengths U[32,128], DTW top-10 (REF and EXACT)",
                        "queries": 1, "db_trajectories": N, "k": 10},
-            "cpu_baseline": {"value": round(out["exact"]["cpu_pairs_per_s"], 1), "unit": "pairs/s",
+            "cpu_baseline": {"value": round(out["ref"]["cpu_pairs_per_s"], 1), "unit": "pairs/s",
                              "cores": 1, "kind": "port",
                              "sample": "the whole workload (one query: the oracle's loop over queries runs on one "
-                                       "thread), EXACT DTW top-10, mean of 20 runs"},
+                                       "thread), the reference's DTW pipeline (REF), mean of 20 runs"},
             "extra": out}
     print(json.dumps(line), flush=True)
 
 
-def run_c3(ts, args):
-    """Uneven-length batching workload: 4,096 queries (log-uniform lengths
-    [16, 2048]) against 1,000,000 DB random walks (log-uniform [16, 1024]),
-    DTW and Frechet top-10. Batched = all queries in one ts_knn call; plain =
-    one launch sequence per query (ts_knn_plain) on a sample of queries, whose
-    top-10 must equal the batched ones."""
-    t0 = time.time()
-    db_len = ts.gen_lengths(DB_SEED, args.n_db, 16, 1024, True)
-    db_xy, db_off = ts.gen_walks(DB_SEED, db_len)
-    q_len = ts.gen_lengths(Q_SEED, args.n_q, 16, 2048, True)
-    q_xy, q_off = ts.gen_walks(Q_SEED, q_len)
-    db = ts.TrajSet(db_xy, db_off)
-    qs = ts.TrajSet(q_xy, q_off)
-    ns = min(args.plain_sample, args.n_q)
-    sample = ts.TrajSet(q_xy[:q_off[ns]], q_off[:ns + 1])
-    log("c3 workload ready in %.1fs: %d DB trajectories (%d points), %d queries" %
-        (time.time() - t0, db.count, len(db_xy), qs.count))
-    eng = ts.Engine(graphs=not args.no_graphs)
-    Q, N = qs.count, db.count
-    out = {}
-    for meas in ("dtw", "frechet"):
-        p = ts.params(meas, mode=ts.EXACT, k=10)
-        eng.knn(db, sample, p)  # warm-up (workspace growth); batches this large launch eagerly
-        t1 = time.perf_counter()
-        idx, val, st = eng.knn(db, qs, p)
-        tb = time.perf_counter() - t1
-        t1 = time.perf_counter()
-        pi, pv, pst = eng.knn_plain(db, sample, p)
-        tp = time.perf_counter() - t1
-        same = bool(np.array_equal(pi, idx[:ns]) and np.array_equal(pv, val[:ns]))
-        out[meas] = {"batched_pairs_per_s": Q * N / tb, "batched_s": round(tb, 4), "graphs": eng.graph_info(),
-                     "batched_cell_updates_per_s": st["cells"] / tb, "cells": st["cells"],
-                     "plain_pairs_per_s": ns * N / tp, "plain_s_for_sample": round(tp, 4), "plain_sample": ns,
-                     "batched_vs_plain": round((Q * N / tb) / (ns * N / tp), 2), "plain_topk_equals_batched": same,
-                     "overflow_retries": st["overflow_retries"]}
-        log(meas, out[meas])
-    out["db_file"] = db_file_leg(ts, eng, db, sample)
-    cpu = None
-    if not args.no_cpu:
-        import oracle
-        nc = min(16, Q)
-        prm = oracle.params(oracle.DTW, oracle.EXACT, k=10)
-        t1 = time.perf_counter()
-        oracle.knn(prm, db_xy, db_off, q_xy[:q_off[nc]], q_off[:nc + 1])
-        dt = time.perf_counter() - t1
-        cpu = {"value": round(nc * N / dt, 1), "unit": "pairs/s", "cores": min(nc, oracle.num_threads()),
-               "kind": "port", "sample": "%d of the %d queries against the full DB, EXACT DTW top-10 (oracle "
-                                         "best-first search, one thread per query), %.2f s" % (nc, Q, dt)}
-        log("c3 cpu", cpu)
-    line = {"metric": METRIC, "value": round(out["dtw"]["batched_pairs_per_s"], 1), "unit": "pairs/s", "n_gpus": 1,
-            "steps": 1, "warmup": 1, "ms_per_step": round(out["dtw"]["batched_s"] * 1e3, 3), "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": "uneven-length batching: %d queries log-uniform [16,2048] vs %d DB random walks "
-                                   "log-uniform [16,1024], DTW/Frechet top-10, EXACT bounding-box filter" % (Q, N),
-                       "queries": Q, "db_trajectories": N, "k": 10},
-            "cpu_baseline": cpu, "extra": out}
-    print(json.dumps(line), flush=True)
-
-
-def db_file_leg(ts, eng, db, sample):
-    """The DB as a tsdb file: save from HBM, load back (the file then sits in
-    the host's page cache: a warm restart), and the loaded set's DTW top-10
-    for the query sample equals the original's."""
-    import tempfile
-    with tempfile.TemporaryDirectory() as td:
-        path = os.path.join(td, "db.tsdb")
-        t1 = time.perf_counter()
-        db.save(path)
-        t_save = time.perf_counter() - t1
-        nbytes = os.path.getsize(path)
-        t1 = time.perf_counter()
-        db2 = ts.TrajSet.load(path)
-        t_load = time.perf_counter() - t1
-    p = ts.params("dtw", mode=ts.EXACT, k=10)
-    a, b = eng.knn(db, sample, p), eng.knn(db2, sample, p)
-    same = bool(np.array_equal(a[0], b[0]) and np.array_equal(a[1].view(np.int32), b[1].view(np.int32)))
-    db2.close()
-    res = {"bytes": nbytes, "save_s": round(t_save, 3), "load_s": round(t_load, 3),
-           "load_GB_per_s": round(nbytes / t_load / 1e9, 2), "loaded_topk_equals": same,
-           "note": "load = map, check, copy points to HBM, recompute and compare summaries; file in the page cache"}
-    log("db file", res)
-    return res
-
-
+# ------------------------------------------------------------------ c4
 def run_c4(ts, args):
     """Long trajectories: 256 queries against 20,000 DB random walks, lengths
     uniform in [2048, 8192]; DTW with a Sakoe-Chiba band of
@@ -312,15 +478,20 @@ def run_c4(ts, args):
     out = {}
     legs = (("dtw", ts.params("dtw", mode=ts.EXACT, k=10, band=-10)),
             ("lcss", ts.params("lcss", mode=ts.EXACT, k=10, epsilon=2e-3, delta=64)))
+    roof = None
     for meas, p in legs:
         eng.knn(db, sample, p)  # warm-up (workspace growth)
         eng.set_timing(True)
         eng.reset_stats()
+        device_sync()
         t1 = time.perf_counter()
         idx, val, st = eng.knn(db, qs, p)
+        device_sync()
         tb = time.perf_counter() - t1
         kst = eng.kernel_stats()
         eng.set_timing(False)
+        if meas == "dtw":
+            roof = roofline_of(kst, load_pmc("pmc_traffic_c4.json"), "profiles/pmc_traffic_c4.json")
         wave = {k: v for k, v in kst.items() if "wavefront" in k}
         dp_ms = sum(v["ms"] for v in wave.values())
         t1 = time.perf_counter()
@@ -340,7 +511,8 @@ def run_c4(ts, args):
             t1 = time.perf_counter()
             oi, ov, _ = oracle.knn(prm, db_xy, db_off, q_xy[:q_off[nq]], q_off[:nq + 1])
             r["oracle_check"] = {"queries": nq, "seconds": round(time.perf_counter() - t1, 2),
-                                 "topk_equal": bool(np.array_equal(oi, idx[:nq]) and np.array_equal(ov, val[:nq]))}
+                                 "topk_equal": bool(np.array_equal(oi, idx[:nq]) and
+                                                    np.array_equal(ov.view(np.int32), val[:nq].view(np.int32)))}
         out[meas] = r
         log("c4", meas, {k: v for k, v in r.items() if k != "kernels"})
     cpu = None
@@ -369,7 +541,7 @@ def run_c4(ts, args):
             "config": {"workload": "long trajectories: %d queries vs %d DB random walks, lengths U[2048,8192], "
                                    "DTW band ceil(0.1 max(n,m)) and LCSS eps 2e-3 delta 64, EXACT top-10" % (Q, N),
                        "queries": Q, "db_trajectories": N, "k": 10},
-            "cpu_baseline": cpu, "extra": out}
+            "roofline": roof, "cpu_baseline": cpu, "extra": out}
     print(json.dumps(line), flush=True)
 
 
@@ -395,12 +567,15 @@ def band_cells(p, n, m):
     return tot
 
 
+# ------------------------------------------------------------------ c5
 def run_c5(ts, args):
     """Query serving: a 2,000,000-trajectory DB (log-uniform lengths [16, 512])
-    resident in HBM; Poisson arrivals at each stated QPS; query lengths
-    log-uniform [16, 1024]; measure mix DTW 40% / LCSS 20% / Frechet 20% /
-    Hausdorff 20%; k uniform in {1, 10, 100}; EXACT top-k. Latency is the
-    server's own clock from submit to result ready."""
+    resident in HBM; Poisson arrivals at each stated QPS for args.duration
+    seconds; query lengths log-uniform [16, 1024]; measure mix DTW 40% / LCSS
+    20% / Frechet 20% / Hausdorff 20%; k uniform in {1, 10, 100}; EXACT top-k.
+    Latency is the server's own clock from submit to result ready. The first
+    args.check_requests results of the first QPS point are checked against the
+    oracle."""
     import threading
     t0 = time.time()
     db_len = ts.gen_lengths(DB_SEED, args.n_db, 16, 512, True)
@@ -416,13 +591,15 @@ def run_c5(ts, args):
     names = ["dtw", "lcss", "frechet", "hausdorff"]
     prm = {(m, k): ts.params(names[m], mode=ts.EXACT, k=int(k), epsilon=2e-3) for m in range(4) for k in (1, 10, 100)}
     results = []
-    for qps in args.qps:
-        srv = ts.Server(db, streams=4, max_batch=256, batch_wait_us=500)
+    checked = {}
+    for qi_point, qps in enumerate(args.qps):
+        srv = ts.Server(db, streams=args.streams, max_batch=256, batch_wait_us=500)
         n = int(qps * args.duration)
         arrivals = np.cumsum(rng.exponential(1.0 / qps, size=n))
         tickets = []
         refused = 0
         lat = []
+        keep = {}
         lock = threading.Lock()
         done = threading.Event()
 
@@ -438,15 +615,18 @@ def run_c5(ts, args):
                 if item is None:
                     time.sleep(0.0005)
                     continue
-                t, k = item
-                _, _, ms = srv.wait(t, k, timeout_ms=120000)
+                i, t, k = item
+                gi, gv, ms = srv.wait(t, k, timeout_ms=120000)
                 with lock:
                     lat.append(ms)
+                    if qi_point == 0 and i < args.check_requests:
+                        keep[i] = (gi, gv)
 
         ws = [threading.Thread(target=waiter) for _ in range(4)]
         for w in ws:
             w.start()
         start = time.perf_counter()
+        last_log = start
         for i in range(n):
             while time.perf_counter() - start < arrivals[i]:
                 pass
@@ -454,11 +634,14 @@ def run_c5(ts, args):
             try:
                 t = srv.submit(q_xy[q_off[i]:q_off[i + 1]], p)
                 with lock:
-                    tickets.append((t, p.k))
+                    tickets.append((i, t, p.k))
             except ts.TrajsimError as e:
                 if e.status != 6:
                     raise
                 refused += 1
+            if time.perf_counter() - last_log > 5.0:
+                last_log = time.perf_counter()
+                log("c5 %d QPS: %d of %d submitted" % (qps, i + 1, n))
         submit_s = time.perf_counter() - start
         done.set()
         for w in ws:
@@ -474,6 +657,27 @@ def run_c5(ts, args):
              "mean_batch": round(len(lat) / max(1, c["batches"]), 2)}
         log("c5", r)
         results.append(r)
+        if qi_point == 0:
+            checked = keep
+    parity = None
+    if not args.no_oracle and checked:
+        import oracle
+        om = {0: oracle.DTW, 1: oracle.LCSS, 2: oracle.FRECHET, 3: oracle.HAUSDORFF}
+        bad = 0
+        t1 = time.perf_counter()
+        for (m, k), _ in prm.items():
+            sel = sorted(i for i in checked if mix[i] == m and ks[i] == k)
+            if not sel:
+                continue
+            s_xy, s_off = sub_queries(q_xy, q_off, sel)
+            wi, wv, _ = oracle.knn(oracle.params(om[m], oracle.EXACT, k=int(k), epsilon=2e-3), db_xy, db_off,
+                                   s_xy, s_off)
+            for r_, i in enumerate(sel):
+                gi, gv = checked[i]
+                bad += not (np.array_equal(gi, wi[r_]) and np.array_equal(gv.view(np.int32), wv[r_].view(np.int32)))
+        parity = {"requests": len(checked), "mismatches": bad, "seconds": round(time.perf_counter() - t1, 2),
+                  "rule": "each served top-k (indices, f32 values) equal to the oracle's EXACT top-k"}
+        log("c5 parity", parity)
     cpu = None
     if not args.no_cpu:
         # CPU QPS: the oracle answers the first 200 requests of the stream,
@@ -485,11 +689,10 @@ def run_c5(ts, args):
             sel = [i for i in range(ncpu) if mix[i] == m and ks[i] == k]
             if not sel:
                 continue
-            sub = [q_xy[q_off[i]:q_off[i + 1]] for i in sel]
-            so = ts.offsets_of([len(t) for t in sub])
+            s_xy, s_off = sub_queries(q_xy, q_off, sel)
             meas = {0: oracle.DTW, 1: oracle.LCSS, 2: oracle.FRECHET, 3: oracle.HAUSDORFF}[m]
             op = oracle.params(meas, oracle.EXACT, k=int(k), epsilon=2e-3)
-            oracle.knn(op, db_xy, db_off, np.concatenate(sub), so)
+            oracle.knn(op, db_xy, db_off, s_xy, s_off)
         dt = time.perf_counter() - t1
         cpu = {"value": round(ncpu / dt, 2), "unit": "queries/s", "cores": oracle.num_threads(), "kind": "port",
                "sample": "the first %d requests of the stream (same mix and k), oracle EXACT k-NN grouped by "
@@ -501,29 +704,33 @@ def run_c5(ts, args):
             "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": "serving: %d-trajectory DB (log-uniform [16,512]) resident, Poisson arrivals, "
                                    "query lengths log-uniform [16,1024], mix DTW 40/LCSS 20/Frechet 20/Hausdorff 20, "
-                                   "k in {1,10,100}, EXACT top-k, continuous batching over 4 streams" % db.count,
+                                   "k in {1,10,100}, EXACT top-k, continuous batching over %d streams"
+                                   % (db.count, args.streams),
                        "db_trajectories": db.count, "duration_s": args.duration},
-            "cpu_baseline": cpu, "extra": {"sweep": results}}
+            "parity": parity, "cpu_baseline": cpu, "extra": {"sweep": results}}
     print(json.dumps(line), flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--qps", type=lambda v: [int(x) for x in v.split(",")], default=[2000, 5000, 10000, 20000])
-    ap.add_argument("--duration", type=float, default=2.0)
+    ap.add_argument("--duration", type=float, default=10.0)
+    ap.add_argument("--streams", type=int, default=4)
+    ap.add_argument("--check-requests", type=int, default=64, help="c5: served results checked against the oracle")
     ap.add_argument("--n-db", type=int, default=None)
     ap.add_argument("--n-q", type=int, default=None)
     ap.add_argument("--plain-sample", type=int, default=256)
-    ap.add_argument("--cpu-sample", type=int, default=64)
+    ap.add_argument("--cpu-sample", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-graphs", action="store_true", help="c2: launch eagerly instead of replaying HIP graphs")
-    ap.add_argument("--oracle-queries", type=int, default=0,
-                    help="c4: also check the top-10 of this many queries against the CPU oracle (about a minute each)")
-    ap.add_argument("--quick", action="store_true", help="skip plain/latency/frechet legs (profiling runs)")
+    ap.add_argument("--no-oracle", action="store_true", help="skip the parity checks against the CPU oracle")
+    ap.add_argument("--no-c2", action="store_true", help="c3: skip the 64 x 100k five-measure figures")
+    ap.add_argument("--no-graphs", action="store_true", help="launch eagerly instead of replaying HIP graphs")
+    ap.add_argument("--oracle-queries", type=int, default=2,
+                    help="c4: check the top-10 of this many queries against the CPU oracle")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (HBM byte-counter calibration for rocprof --pmc)")
     args = ap.parse_args()
@@ -536,10 +743,8 @@ def main():
     ts.init(0)
     if args.config == "c1":
         return run_c1(ts, args)
-    if args.config == "c3":
-        args.n_db = args.n_db or 1_000_000
-        args.n_q = args.n_q or 4096
-        return run_c3(ts, args)
+    if args.config == "c2":
+        return run_c2(ts, args)
     if args.config == "c4":
         args.n_db = args.n_db or 20_000
         args.n_q = args.n_q or 256
@@ -548,101 +753,11 @@ def main():
     if args.config == "c5":
         args.n_db = args.n_db or 2_000_000
         return run_c5(ts, args)
-    args.n_db = args.n_db or 100_000
-    args.n_q = args.n_q or 64
+    args.n_db = args.n_db or 1_000_000
+    args.n_q = args.n_q or 4096
     if args.copy_calibration:
         copy_calibration()
-    dev = ts.device_name()
-    log("device", dev)
-    t0 = time.time()
-    db_xy, db_off, q_xy, q_off = build_workload(ts, args.n_db, args.n_q, 64, 256)
-    db = ts.TrajSet(db_xy, db_off)
-    qs = ts.TrajSet(q_xy, q_off)
-    log("workload ready in %.1fs: %d DB trajectories (%d points), %d queries" %
-        (time.time() - t0, db.count, len(db_xy), qs.count))
-    Q, N = qs.count, db.count
-    eng = ts.Engine(graphs=not args.no_graphs)
-    p_dtw = ts.params("dtw", mode=ts.EXACT, k=10)
-    p_fr = ts.params("frechet", mode=ts.EXACT, k=10)
-
-    # Timed region: K batched DTW steps, replayed from a HIP graph (the
-    # warm-up records it; --no-graphs launches eagerly).
-    for _ in range(max(args.warmup, 2)):
-        eng.knn(db, qs, p_dtw)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        idx, val, st = eng.knn(db, qs, p_dtw)
-    ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
-    value = Q * N / (ms_step * 1e-3)
-    cells = st["cells"]
-    ginfo = eng.graph_info()
-    log("DTW batched: %.3f ms/step, %.3e pairs/s, %.3e cells/step, graphs %s" % (ms_step, value, cells, ginfo))
-    # Per-kernel durations (HIP events around every launch, so eager): the
-    # same K steps again, for the roofline and the kernel table.
-    eng.set_timing(True)
-    eng.reset_stats()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ti, tv, _ = eng.knn(db, qs, p_dtw)
-    ms_eager = (time.perf_counter() - t0) * 1e3 / args.steps
-    kstats = eng.kernel_stats()
-    eng.set_timing(False)
-    assert np.array_equal(ti, idx) and np.array_equal(tv, val)
-    log("DTW batched eager with kernel events: %.3f ms/step" % ms_eager)
-
-    pmc = None
-    pmc_path = os.path.join(ROOT, "profiles", "pmc_traffic_c2.json")
-    if os.path.exists(pmc_path):
-        pmc = json.load(open(pmc_path))
-    roof = roofline_of(kstats, pmc)
-    extra = {"device": dev, "dtw_ms_per_step": round(ms_step, 4), "dtw_cells_per_step": cells,
-             "graphs": ginfo, "dtw_ms_per_step_eager_timed": round(ms_eager, 4),
-             "dtw_cell_updates_per_s": cells / (ms_step * 1e-3),
-             "dtw_device_ms": round(st["ms_device"], 4), "launches_per_step": st["launches"],
-             "kernels": {k: {"launches": v["launches"], "ms_mean": round(v["ms"] / max(v["launches"], 1), 5)}
-                         for k, v in kstats.items()}}
-
-    if not args.quick:
-        ms_fr, (fi, fv, fst) = timed(lambda: eng.knn(db, qs, p_fr), args.steps, 2)
-        extra.update({"frechet_pairs_per_s": Q * N / (ms_fr * 1e-3), "frechet_ms_per_step": round(ms_fr, 4),
-                      "frechet_cell_updates_per_s": fst["cells"] / (ms_fr * 1e-3)})
-        ms_plain, (pi, pv, _) = timed(lambda: eng.knn_plain(db, qs, p_dtw), max(1, args.steps // 3), 1)
-        same = bool(np.array_equal(pi, idx) and np.array_equal(pv, val))
-        extra.update({"dtw_plain_pairs_per_s": Q * N / (ms_plain * 1e-3), "dtw_plain_ms_per_step": round(ms_plain, 4),
-                      "batched_vs_plain": round(ms_plain / ms_step, 3), "plain_equals_batched": same})
-        # Per-query latency of a single-query request on the resident DB.
-        lat = []
-        singles = [ts.TrajSet(q_xy[q_off[i]:q_off[i + 1]], np.array([0, q_off[i + 1] - q_off[i]], np.int64))
-                   for i in range(Q)]
-        for s in singles[:4]:
-            eng.knn(db, s, p_dtw)
-        for s in singles:
-            t1 = time.perf_counter()
-            eng.knn(db, s, p_dtw)
-            lat.append((time.perf_counter() - t1) * 1e3)
-        extra.update({"query_latency_ms_p50": round(float(np.percentile(lat, 50)), 4),
-                      "query_latency_ms_p99": round(float(np.percentile(lat, 99)), 4),
-                      "latency_note": "single-query DTW top-10 calls, serial, DB resident"})
-        log("frechet %.3f ms, plain %.3f ms, p50 %.3f ms" % (ms_fr, ms_plain, extra["query_latency_ms_p50"]))
-        extra["measures"] = c2_other_measures(ts, eng, db, qs, args)
-
-    cpu = None
-    if not args.no_cpu:
-        cpu, ci, cv = cpu_baseline(db_xy, db_off, q_xy, q_off, args.cpu_sample)
-        extra["cpu_matches_gpu"] = bool(np.array_equal(ci, idx[:args.cpu_sample]) and
-                                        np.array_equal(cv, val[:args.cpu_sample]))
-        log("cpu baseline", cpu)
-        extra["cpu_other"] = cpu_extra(db_xy, db_off, q_xy, q_off)
-        log("cpu other", extra["cpu_other"])
-
-    line = {"metric": METRIC, "value": round(value, 1), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
-            "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": "DTW top-10 k-NN, %d query random walks vs %d-trajectory DB (lengths U[64,256], "
-                                   "sigma 1e-3), bounding-box/endpoint lower-bound filter, batched" % (Q, N),
-                       "queries": Q, "db_trajectories": N, "k": 10, "measure": "dtw", "mode": "exact"},
-            "roofline": roof, "cpu_baseline": cpu, "extra": extra}
-    print(json.dumps(line), flush=True)
+    return run_c3(ts, args)
 
 
 if __name__ == "__main__":

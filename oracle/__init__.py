@@ -20,7 +20,7 @@ REF, EXACT = 0, 1
 if not os.path.exists(ORACLE_PATH):
     raise ImportError("oracle/liboracle.so is not built (make -C oracle)")
 _o = ctypes.CDLL(ORACLE_PATH)
-_ref = ctypes.CDLL(REF_PATH) if os.path.exists(REF_PATH) else None
+_ref = None  # the reference's own build, loaded on first use (_load_ref)
 
 _P = ctypes.c_void_p
 _i, _f = ctypes.c_int, ctypes.c_float
@@ -50,12 +50,22 @@ for name, res, args in [
     fn = getattr(_o, name)
     fn.restype = res
     fn.argtypes = args
-if _ref is not None:
-    for name in ("ref_dtw_cpu", "ref_frechet_cpu"):
-        getattr(_ref, name).restype = _f
-        getattr(_ref, name).argtypes = [_P, _P, _P, _i, _i, _i]
-    _ref.ref_lcss_cpu.restype = _f
-    _ref.ref_lcss_cpu.argtypes = [_P, _P, _P, _i, _i, _i, _f]
+
+
+def _load_ref():
+    """oracle/_ref/libref.so, loaded only by the calls that run it, so a
+    process that never times or checks against the reference's own build
+    never maps it."""
+    global _ref
+    if _ref is None and os.path.exists(REF_PATH):
+        lib = ctypes.CDLL(REF_PATH)
+        for name in ("ref_dtw_cpu", "ref_frechet_cpu"):
+            getattr(lib, name).restype = _f
+            getattr(lib, name).argtypes = [_P, _P, _P, _i, _i, _i]
+        lib.ref_lcss_cpu.restype = _f
+        lib.ref_lcss_cpu.argtypes = [_P, _P, _P, _i, _i, _i, _f]
+        _ref = lib
+    return _ref
 
 
 def _p(a):
@@ -184,13 +194,13 @@ def set_threads(n):
 
 # ------------------------------------------------- the reference's own build
 def ref_available():
-    return _ref is not None
+    return os.path.exists(REF_PATH)
 
 
 def ref_batch(measure, t1, t2, num_t, n, m, eps=0.5):
     """Run the reference's launch_<measure>_batch_cpu (dtw.cpp / frechet.cpp /
     lcss.cpp compiled from the reference sources) -> (h_results, ms)."""
-    if _ref is None:
+    if _load_ref() is None:
         raise RuntimeError("oracle/_ref/libref.so not built")
     t1, t2 = _pts(t1), _pts(t2)
     out = np.empty(num_t, np.float32)
