@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cstdio>
 #include <cmath>
 #include <cstdlib>
@@ -1257,6 +1258,8 @@ ts_status ts_pair_values(ts_engine* e, const ts_set* db, const ts_set* qs, const
   if (!e || !qidx || !didx || !out || npairs < 0) return fail(TS_ERR_INVALID, "null argument");
   TS_TRY(check_params(p, db, qs));
   if (npairs == 0) return TS_OK;
+  // a pair's value slot is a 32-bit index
+  if (npairs > INT32_MAX) return fail(TS_ERR_UNSUPPORTED, "ts_pair_values: more than 2^31 - 1 pairs in one call");
   std::vector<Item> h(npairs);
   for (int64_t i = 0; i < npairs; ++i) {
     if (qidx[i] < 0 || qidx[i] >= qs->count || didx[i] < 0 || didx[i] >= db->count)

@@ -487,3 +487,17 @@ def test_tuning_table_lines_fill_their_tables(tmp_path):
             e.get_dp_shape(0, 2, 2)
     finally:
         e.close()
+
+
+def test_pair_values_refuses_more_than_int32_pairs(eng, shapes):
+    """A pair's value slot is a 32-bit index: 2^31 pairs in one call are
+    refused before any array is read (the arrays here hold 4 entries)."""
+    lib = ctypes.CDLL(ts.LIB_PATH)
+    qi = np.zeros(4, np.int32)
+    out = np.zeros(4, np.float32)
+    p = ts.params("dtw")
+    st = lib.ts_pair_values(ctypes.c_void_p(eng._h.value), ctypes.c_void_p(shapes["db"].handle.value),
+                            ctypes.c_void_p(shapes["qs"].handle.value), ctypes.byref(p),
+                            qi.ctypes.data_as(ctypes.c_void_p), qi.ctypes.data_as(ctypes.c_void_p),
+                            ctypes.c_int64(2 ** 31), out.ctypes.data_as(ctypes.c_void_p))
+    assert st == 4, st  # TS_ERR_UNSUPPORTED
