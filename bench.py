@@ -331,7 +331,9 @@ def c2_leg(ts, args, want_kernels=False):
     """The five-measure workload: 64 queries vs 100,000 DB random walks,
     lengths uniform [64, 256]; EXACT top-10 batched (graph replay), the plain
     method and single-query latency for DTW; LCSS with eps 2e-3 and window
-    16; lock-step Euclidean on its own 100,000 x 128 DB."""
+    16; lock-step Euclidean on its own 100,000 x 128 DB; and "dtw_ref", the
+    reference's own DTW pipeline (envelope filter over every DB point, 10
+    candidates, exact refine)."""
     db_xy, db_off, q_xy, q_off = build_workload(ts, 100_000, 64, 64, 256)
     db, qs = ts.TrajSet(db_xy, db_off), ts.TrajSet(q_xy, q_off)
     Q, N = qs.count, db.count
@@ -339,6 +341,7 @@ def c2_leg(ts, args, want_kernels=False):
     steps = max(args.steps, 10)
     res = {}
     legs = [("dtw", ts.params("dtw", mode=ts.EXACT, k=10), db, qs),
+            ("dtw_ref", ts.params("dtw", mode=ts.REF, k=10), db, qs),
             ("frechet", ts.params("frechet", mode=ts.EXACT, k=10), db, qs),
             ("lcss", ts.params("lcss", mode=ts.EXACT, k=10, epsilon=2e-3, delta=16), db, qs),
             ("hausdorff", ts.params("hausdorff", mode=ts.EXACT, k=10), db, qs)]
