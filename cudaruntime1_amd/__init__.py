@@ -21,6 +21,7 @@ LIB_PATH = os.path.join(_HERE, "libtrajsim.so")
 DTW, LCSS, FRECHET, HAUSDORFF, EUCLIDEAN = 0, 1, 2, 3, 4
 MEASURES = {"dtw": DTW, "lcss": LCSS, "frechet": FRECHET, "hausdorff": HAUSDORFF, "euclidean": EUCLIDEAN}
 REF, EXACT = 0, 1
+NOALLOC_MAX_PAIRS = 16384  # include/trajsim.h TS_NOALLOC_MAX_PAIRS
 SET_TINY_COORD = 1  # TrajSet.flags(): a coordinate c with 0 < |c| < 2^-25
 
 if not os.path.exists(LIB_PATH):
@@ -90,6 +91,7 @@ _sig = {
     "ts_knn_plain": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(Params), _P, _P, ctypes.POINTER(Stats)]),
     "ts_filter_ref": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(Params), _P]),
     "ts_pair_values": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(Params), _P, _P, ctypes.c_int64, _P]),
+    "ts_pair_values_noalloc": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(Params), _P, _P, ctypes.c_int64, _P]),
     "ts_topk_rows": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P, _P]),
     "ts_server_config_default": (None, [ctypes.POINTER(ServerConfig)]),
     "ts_server_create": (ctypes.c_int, [_P, ctypes.POINTER(ServerConfig), ctypes.POINTER(_P)]),
@@ -366,6 +368,15 @@ class Engine:
         out = np.empty(len(qidx), np.float32)
         _check(_lib.ts_pair_values(self._h, db.handle, queries.handle, ctypes.byref(p), _ptr(qidx), _ptr(didx),
                                    len(qidx), _ptr(out)), "ts_pair_values")
+        return out
+
+    def pair_values_noalloc(self, db, queries, p, qidx, didx):
+        """pair_values with no device allocation (at most NOALLOC_MAX_PAIRS pairs)."""
+        qidx = np.ascontiguousarray(qidx, np.int32)
+        didx = np.ascontiguousarray(didx, np.int32)
+        out = np.empty(len(qidx), np.float32)
+        _check(_lib.ts_pair_values_noalloc(self._h, db.handle, queries.handle, ctypes.byref(p), _ptr(qidx),
+                                           _ptr(didx), len(qidx), _ptr(out)), "ts_pair_values_noalloc")
         return out
 
     def topk_rows(self, scores, k, descending=False):

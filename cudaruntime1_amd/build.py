@@ -20,7 +20,7 @@ OUT = os.path.join(HERE, "libtrajsim.so")
 OBJ = os.path.join(HERE, "_build")
 
 SOURCES = ["set.hip", "filter.hip", "select.hip", "dp.hip", "reduce.hip", "engine.hip", "server.hip",
-           "compat.cpp", "dbfile.hip"]
+           "compat.cpp", "dbfile.hip", "noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TS_OFFLOAD_ARCH", "gfx950")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fopenmp", "-Wno-unused-result"]

@@ -272,10 +272,6 @@ struct ts_engine {
 namespace ts {
 namespace {
 
-// Cell / pair counters: kCellSlots of each, then the candidate total of
-// device-sized EXACT batches.
-constexpr int kCellSlots = 1024;
-constexpr int kCellBuf = 2 * kCellSlots + 1;
 
 hipError_t zero_cells(hipStream_t st, unsigned long long* cells) {
   hipLaunchKernelGGL(k_zero_u64, dim3((kCellBuf + 255) / 256), dim3(256), 0, st, cells, kCellBuf);
@@ -441,14 +437,17 @@ struct KeyItems {
 // refine's own set is zeroed by its cell count.
 ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_set* qs, Item* items,
                  const int32_t* nitems_dev, int64_t max_items, float* vals, bool seed = false,
-                 int32_t* ctr_zeroed = nullptr, const KeyItems* from_keys = nullptr) {
+                 int32_t* ctr_zeroed = nullptr, const KeyItems* from_keys = nullptr,
+                 const RefineBufs* fixed = nullptr) {
   if (max_items == 0) return TS_OK;
   if (from_keys && !ctr_zeroed) return fail(TS_ERR_INVALID, "items from keys need zeroed counters");
+  if (fixed && max_items > fixed->max_items) return fail(TS_ERR_INVALID, "refine larger than its fixed buffers");
   const SetView qv = qs->view(), dv = db->view();
   const MeasureArgs ma = margs(p);
   const int cus = device_cu_count();
   unsigned long long* cells = nullptr;
-  TS_TRY(ensure(e, B_CELLS, kCellBuf, &cells));
+  if (fixed) cells = fixed->cells;
+  else TS_TRY(ensure(e, B_CELLS, kCellBuf, &cells));
   // kDpBuckets counter slots for this refine (the last group is shared once
   // the slots run out; totals stay right, per-launch credit blurs)
   int slot = e->cells_used;
@@ -458,7 +457,10 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   // DP counters: [0, B) work counters of the bucket launches, [B, 2B) and
   // [2B, 3B) the heads and tails of the bucket lists
   int32_t* ctr = ctr_zeroed;
-  if (!ctr) TS_TRY(ensure(e, B_CTR, 32, &ctr));
+  if (!ctr) {
+    if (fixed) ctr = fixed->ctr;
+    else TS_TRY(ensure(e, B_CTR, 32, &ctr));
+  }
   static_assert(3 * kDpBuckets <= 32, "DP counters");
   // Buckets of the row width the DP visits: one persistent launch per bucket
   // that can occur for these two sets (the width grows with both lengths).
@@ -474,7 +476,8 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   Item* itemsb = nullptr;
   BucketCells long_cells{};
   if (lists) {
-    TS_TRY(ensure(e, B_ITEMS3, (int64_t)kDpBuckets * max_items, &itemsb));
+    if (fixed) itemsb = fixed->itemsb;
+    else TS_TRY(ensure(e, B_ITEMS3, (int64_t)kDpBuckets * max_items, &itemsb));
     for (int b = 0; b < kDpBuckets; ++b) {
       const int64_t cb = dp_ranged(ma) ? hi : std::min<int64_t>(kDpBucketCap[b], hi);
       long_cells.t[b] = cb * rmax / 2;
@@ -526,8 +529,9 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   if (b_lo == b_hi) {
     int sh = tune[b_lo].shape;
     const DpHalo halo = shape_halo(e, sh, b_lo, ma, hi, whi);
-    const std::string label = dp_label(nm, b_lo, sh);
-    return run(e, label.c_str(), 0, 0, [&] {
+    // launch labels name the per-kernel statistics, kept only while timing
+    const std::string label = e->timing ? dp_label(nm, b_lo, sh) : std::string();
+    return run(e, e->timing ? label.c_str() : nm, 0, 0, [&] {
       return launch_dp(e->st, ma, sh, qv, dv, items, nitems_dev, nullptr, max_items, ctr, vals, halo, rmax, hi,
                        cus * tune[b_lo].waves_per_cu);
     }, slot + b_lo);
@@ -538,9 +542,9 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   for (int b = b_lo; b <= b_hi; ++b) {
     int sh = tune[b].shape;
     const DpHalo halo = shape_halo(e, sh, b, ma, hi, whi);
-    const std::string label = dp_label(nm, b, sh);
+    const std::string label = e->timing ? dp_label(nm, b, sh) : std::string();
     TS_HIP(hipStreamWaitEvent(e->aux[b], e->fork, 0));
-    TS_TRY(run_on(e, e->aux[b], label.c_str(), 0, 0, [&] {
+    TS_TRY(run_on(e, e->aux[b], e->timing ? label.c_str() : nm, 0, 0, [&] {
       return launch_dp(e->aux[b], ma, sh, qv, dv, itemsb + (int64_t)b * max_items, ctr + kDpBuckets + b,
                        ctr + 2 * kDpBuckets + b, max_items, ctr + b, vals, halo, rmax, hi,
                        cus * tune[b].waves_per_cu);
@@ -1005,6 +1009,23 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
 }
 
 }  // namespace
+
+ts_status refine_fixed(ts_engine* e, const ts_params& p, const ts_set* db, const ts_set* qs, Item* items,
+                       int64_t n, float* vals, const RefineBufs& bufs) {
+  TS_TRY(check_params(&p, db, qs));
+  const bool timing = e->timing;
+  e->timing = false;
+  e->cells_used = 0;
+  const ts_status s = refine(e, p, db, qs, items, nullptr, n, vals, false, nullptr, nullptr, &bufs);
+  e->timing = timing;
+  return s;
+}
+
+ts_status validate_params(const ts_params* p, const ts_set* db, const ts_set* qs) { return check_params(p, db, qs); }
+
+hipStream_t engine_stream(const ts_engine* e) { return e->st; }
+int engine_device(const ts_engine* e) { return e->device; }
+
 }  // namespace ts
 
 extern "C" {

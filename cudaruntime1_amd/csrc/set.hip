@@ -341,6 +341,8 @@ ts_status ts_init(int32_t device) {
   if (device < 0 || device >= n) return fail(TS_ERR_INVALID, "device index out of range");
   TS_HIP(hipSetDevice(device));
   TS_HIP(hipFree(nullptr));
+  // the allocation-free entry point's scratch array on this device
+  TS_TRY(noalloc_setup(device));
   g_device = device;
   return TS_OK;
 }

@@ -295,4 +295,31 @@ hipError_t launch_euclidean(hipStream_t st, SetView q, SetView db, const Item* i
 
 int device_cu_count();
 
+// engine.hip -- cell / pair counters of a k-NN call or refine: kCellSlots
+// of each, then the candidate total of device-sized EXACT batches.
+constexpr int kCellSlots = 1024;
+constexpr int kCellBuf = 2 * kCellSlots + 1;
+// Device buffers a refine takes instead of growing the engine's workspace:
+// cells[kCellBuf] (zero on entry), ctr[32] DP counters, and
+// itemsb[kDpBuckets * max_items] bucket work lists.
+struct RefineBufs {
+  unsigned long long* cells;
+  int32_t* ctr;
+  Item* itemsb;
+  int64_t max_items;
+};
+// Exact values of items[0, n) into vals[slot] on the engine's stream,
+// every device buffer from `bufs`: no device allocation (per-kernel timing
+// is off for the call). Does not wait for the stream.
+ts_status refine_fixed(ts_engine* e, const ts_params& p, const ts_set* db, const ts_set* qs, Item* items,
+                       int64_t n, float* vals, const RefineBufs& bufs);
+hipStream_t engine_stream(const ts_engine* e);
+int engine_device(const ts_engine* e);
+// The argument checks of the k-NN and pair-value entry points.
+ts_status validate_params(const ts_params* p, const ts_set* db, const ts_set* qs);
+
+// noalloc.hip -- looks up the device address of the allocation-free entry
+// point's scratch array on `device`, which must be current (once per device).
+ts_status noalloc_setup(int device);
+
 }  // namespace ts

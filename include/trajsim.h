@@ -218,6 +218,19 @@ ts_status ts_filter_ref(ts_engine* e, const ts_set* db, const ts_set* queries, c
 /* Exact value of npairs given (query, db) index pairs into host out[npairs]. */
 ts_status ts_pair_values(ts_engine* e, const ts_set* db, const ts_set* queries, const ts_params* p,
                          const int32_t* qidx, const int32_t* didx, int64_t npairs, float* out);
+/* ts_pair_values without any device allocation. No counterpart in the
+ * reference: for callers that cannot allocate device memory (serving loops
+ * that keep a fixed device footprint, harnesses that forbid hipMalloc after
+ * start-up). Same arguments and results as ts_pair_values, bit for bit;
+ * at most TS_NOALLOC_MAX_PAIRS pairs per call (more: TS_ERR_INVALID). Every
+ * device buffer of the call lies in a static scratch array of about 2 MB
+ * that the library reserves on each GPU at ts_init (or at the first call on
+ * a device that is current then) and holds until the process ends. Calls on
+ * one GPU share that array; overlapping calls would race on it, so the
+ * library runs them one at a time per GPU. Synchronous. */
+#define TS_NOALLOC_MAX_PAIRS 16384
+ts_status ts_pair_values_noalloc(ts_engine* e, const ts_set* db, const ts_set* queries, const ts_params* p,
+                                 const int32_t* qidx, const int32_t* didx, int64_t npairs, float* out);
 /* Per-row top-k of a host score matrix [rows][cols] by (value, column)
  * ascending, or value descending for descending != 0; used for parity tests
  * of the device selection. */
