@@ -114,37 +114,6 @@ __device__ __forceinline__ void store_vec(float* p, const float (&v)[U]) {
   }
 }
 
-// Correctly rounded sqrtf of U values. v_sqrt_f32 is within 1 ulp; for
-// x >= 2^-96, x = 0 and x = +inf, testing its two neighbours with fma
-// residuals (exact in that range) selects the correctly rounded root. That
-// is the expansion the compiler emits for sqrtf minus its rescaling of tiny
-// inputs and its 0/inf select, which these inputs do not need (0 and +inf
-// come out of the tests unchanged). Inputs in (0, 2^-96) need a trajectory
-// with a tiny coordinate (0 < |c| < 2^-25); `tiny` says whether a pair of the
-// wave's lanes has one (uniform), and then a wave holding any such x takes
-// sqrtf itself.
-template <int U>
-__device__ __forceinline__ void sqrt_cr(float (&x)[U], bool tiny) {
-  if (__builtin_expect(tiny, 0)) {
-    uint32_t mb = __float_as_uint(x[0]) - 1u;  // 0 maps to the top
-#pragma unroll
-    for (int j = 1; j < U; ++j) mb = min(mb, __float_as_uint(x[j]) - 1u);
-    if (__builtin_expect(__any(mb < 0x0f7fffffu), 0)) {
-#pragma unroll
-      for (int j = 0; j < U; ++j) x[j] = sqrtf(x[j]);
-      return;
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < U; ++j) {
-    const float s = __builtin_amdgcn_sqrtf(x[j]);
-    const float sm = __uint_as_float(__float_as_uint(s) - 1u);
-    const float sp = __uint_as_float(__float_as_uint(s) + 1u);
-    const float t = __builtin_fmaf(-sm, s, x[j]) <= 0.0f ? sm : s;
-    x[j] = __builtin_fmaf(-sp, s, x[j]) > 0.0f ? sp : t;
-  }
-}
-
 // Band half-width in rows of a pair (R rows, C columns): band cells, or
 // -band percent of R when negative; a width of C or more admits every cell,
 // so it is clamped there (w*R then stays below 2^31).

@@ -578,7 +578,7 @@ ts_status topk(ts_engine* e, const uint64_t* in, int64_t stride, int32_t len, co
 
 // REF pipeline for queries [q0, q0+nq); results into device top keys [nq][k].
 ts_status knn_ref_batch(ts_engine* e, const ts_params& p, const ts_set* db, const ts_set* qs, int32_t q0, int32_t nq,
-                        const float4* env, uint64_t* top) {
+                        const float4* env, const TileView* tiles, uint64_t* top) {
   const int32_t N = db->count;
   const int32_t kc = p.kc;
   const bool desc = desc_measure(p.measure);
@@ -596,8 +596,10 @@ ts_status knn_ref_batch(ts_engine* e, const ts_params& p, const ts_set* db, cons
   TS_TRY(ensure(e, B_KEYS2, (int64_t)nq * kc, &keys2));
   const SetView qv = qs->view(), dv = db->view();
   const double dbpts = (double)db->points;
-  TS_TRY(run(e, "filter_ref", nq * (dbpts * 8.0), nq * dbpts, [&] {
-    return launch_filter_ref(e->st, p.measure, qv, q0, nq, env, dv, p.epsilon, scores);
+  // algorithmic bytes: the DB's points once per pass of 8 queries (the
+  // tiled filter's query group); units: point evaluations
+  TS_TRY(run(e, "filter_ref", ((nq + 7) / 8) * (dbpts * 8.0), nq * dbpts, [&] {
+    return launch_filter_ref(e->st, p.measure, qv, q0, nq, env, dv, tiles, p.epsilon, scores);
   }));
   TS_TRY(run(e, "keys", 0, 0, [&] { return launch_keys_from_rows(e->st, scores, nq, N, desc, keys); }));
   TS_TRY(topk(e, keys, N, N, nullptr, nq, kc, cand));
@@ -856,8 +858,13 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
   const int64_t l0 = e->launches;
   e->cells_used = 0;
   float4* env = nullptr;
-  if (p.mode == TS_MODE_REF && (p.measure == TS_DTW || p.measure == TS_FRECHET || p.measure == TS_LCSS)) {
+  TileView tiles{};
+  const bool point_filter =
+      p.mode == TS_MODE_REF && (p.measure == TS_DTW || p.measure == TS_FRECHET || p.measure == TS_LCSS);
+  if (point_filter) {
     TS_TRY(ensure(e, B_ENV, qs->dev_points, &env));
+    // the DB's length-bucketed tiles, (re)built here, outside any recording
+    TS_TRY(set_tiles(db, e->st, &tiles));
   }
   // Results block, laid out as the host staging buffer: indices, values
   // (padded to 16 bytes), then a copy of the cell / pair counters and the
@@ -912,7 +919,7 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
   }
   auto batch = [&](int32_t q0, int32_t nq) -> ts_status {
     uint64_t* topq = top + (int64_t)q0 * k;
-    if (p.mode == TS_MODE_REF) return knn_ref_batch(e, p, db, qs, q0, nq, env, topq);
+    if (p.mode == TS_MODE_REF) return knn_ref_batch(e, p, db, qs, q0, nq, env, &tiles, topq);
     if (all_pairs) return knn_all_batch(e, p, db, qs, q0, nq, topq);
     return knn_exact_batch(e, p, db, qs, q0, nq, topq);
   };
@@ -938,7 +945,8 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
     int32_t eps_bits;
     std::memcpy(&eps_bits, &p.epsilon, 4);
     key.insert(key.end(), {p.measure, p.mode, p.k, p.kc, p.r, p.band, p.delta, eps_bits, q0, nq,
-                           (int64_t)(intptr_t)env, (int64_t)(intptr_t)(top + (int64_t)q0 * k)});
+                           (int64_t)(intptr_t)env, (int64_t)(intptr_t)(top + (int64_t)q0 * k),
+                           (int64_t)(intptr_t)tiles.x, (int64_t)(intptr_t)tiles.off, tiles.ntiles});
     return key;
   };
   TS_HIP(hipEventRecord(e->call_a, e->st));
@@ -1262,12 +1270,15 @@ ts_status ts_filter_ref(ts_engine* e, const ts_set* db, const ts_set* qs, const 
   TS_TRY(check_params(p, db, qs));
   float4* env = nullptr;
   float* d = nullptr;
+  TileView tiles{};
   TS_TRY(ensure(e, B_ENV, qs->dev_points, &env));
   TS_TRY(ensure(e, B_SCORES, (int64_t)qs->count * db->count, &d));
-  if (p->measure == TS_DTW || p->measure == TS_FRECHET || p->measure == TS_LCSS)
+  if (p->measure == TS_DTW || p->measure == TS_FRECHET || p->measure == TS_LCSS) {
+    TS_TRY(set_tiles(db, e->st, &tiles));
     TS_TRY(run(e, "envelope", 0, 0, [&] { return launch_envelope(e->st, qs->view(), qs->max_len, p->r, env); }));
+  }
   TS_TRY(run(e, "filter_ref", 0, 0, [&] {
-    return launch_filter_ref(e->st, p->measure, qs->view(), 0, qs->count, env, db->view(), p->epsilon, d);
+    return launch_filter_ref(e->st, p->measure, qs->view(), 0, qs->count, env, db->view(), &tiles, p->epsilon, d);
   }));
   TS_HIP(hipMemcpyAsync(scores, d, sizeof(float) * qs->count * db->count, hipMemcpyDeviceToHost, e->st));
   TS_HIP(hipStreamSynchronize(e->st));

@@ -48,62 +48,134 @@ hipError_t launch_envelope(hipStream_t st, SetView q, int32_t max_len, int32_t r
   return hipGetLastError();
 }
 
-// REF filter, one lane per (query, DB trajectory).
+// REF filter of the MBR measures, one lane per (query, DB trajectory): the
+// squared MBR gap, the R-tree's comparable box distance (hausdorff.cpp:162,
+// euclidean.cpp:93-96); Euclidean pairs of unequal length are never
+// candidates.
 template <int MEAS>
 __global__ void __launch_bounds__(256) k_filter_ref(SetView q, int32_t q0, const float4* env, SetView db,
                                                     float eps, float* scores) {
+  static_assert(MEAS == TS_HAUSDORFF || MEAS == TS_EUCLIDEAN, "point measures run k_filter_ref_tiled");
   const int j = blockIdx.x * 256 + threadIdx.x;
   if (j >= db.count) return;
   const int qi = q0 + blockIdx.y;
   const int n = q.len[qi];
   const int m = db.len[j];
-  const int64_t co = db.off[j];
   float out;
-  if (MEAS == TS_HAUSDORFF || MEAS == TS_EUCLIDEAN) {
-    // Squared MBR gap: the R-tree's comparable box distance
-    // (hausdorff.cpp:162, euclidean.cpp:93-96).
-    if (MEAS == TS_EUCLIDEAN && m != n) {
-      out = __int_as_float(0x7f800000);  // not comparable: never a candidate
-    } else {
-      float4 a = q.mbr[qi], b = db.mbr[j];
-      float gx = fmaxf(0.0f, fmaxf(b.x - a.y, a.x - b.y));
-      float gy = fmaxf(0.0f, fmaxf(b.z - a.w, a.z - b.w));
-      out = gx * gx + gy * gy;
-    }
+  if (MEAS == TS_EUCLIDEAN && m != n) {
+    out = __int_as_float(0x7f800000);
   } else {
-    const float4* e = env + q.off[qi];
-    const int qd = n / m, rm = n - qd * m;
-    int idx = 0, acc = 0;
-    float s = 0.0f;
-    int cnt = 0;
-    for (int v = 0; v < m; ++v) {
-      const float4 E = e[idx];
-      const float px = db.x[co + v], py = db.y[co + v];
-      if (MEAS == TS_LCSS) {
-        if (px > E.x - eps && px < E.y + eps && py > E.z - eps && py < E.w + eps) ++cnt;
-      } else {
-        float dx = fmaxf(0.0f, fmaxf(E.x - px, px - E.y));
-        float dy = fmaxf(0.0f, fmaxf(E.z - py, py - E.w));
-        float d = sqrtf(dx * dx + dy * dy);
-        if (MEAS == TS_DTW) s += d;
-        else s = fmaxf(s, d);
-      }
-      idx += qd;
-      acc += rm;
-      if (acc >= m) { acc -= m; ++idx; }
-    }
-    out = (MEAS == TS_LCSS) ? (float)cnt / (float)min(n, m) : s;
+    float4 a = q.mbr[qi], b = db.mbr[j];
+    float gx = fmaxf(0.0f, fmaxf(b.x - a.y, a.x - b.y));
+    float gy = fmaxf(0.0f, fmaxf(b.z - a.w, a.z - b.w));
+    out = gx * gx + gy * gy;
   }
   scores[(int64_t)blockIdx.y * db.count + j] = out;
 }
 
+// REF filter of the point measures over the DB's length-bucketed tiles: a
+// wave takes one tile (64 DB trajectories of nearly equal length, lane l
+// the trajectory in slot l) against QT queries of the batch. Each step reads
+// point v of all 64 trajectories with one coalesced load per coordinate and
+// folds it into the QT pairs of every lane, in order of v per pair (the
+// reference's left fold, dtw.cpp:45-58), so each DB point is read once per
+// QT queries instead of once per query, with no scattered per-lane streams.
+// Envelope reads are per query and lane (one broadcast line when the tile's
+// lengths are equal). Square roots are sqrtf's correctly rounded value
+// (sqrt_cr), the slow path only for pairs with a tiny coordinate.
+template <int MEAS, int QT>
+__global__ void __launch_bounds__(256) k_filter_ref_tiled(SetView q, int32_t q0, int32_t nq, const float4* env,
+                                                          SetView db, TileView tv, float eps, float* scores) {
+  const int lane = threadIdx.x & (kTile - 1);
+  const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (tile >= tv.ntiles) return;  // wave-uniform
+  const int qb = blockIdx.y * QT;
+  const int id = tv.id[(int64_t)tile * kTile + lane];
+  const int m = id >= 0 ? db.len[id] : 0;
+  const int L = tv.len[tile];
+  const float* tx = tv.x + tv.off[tile] + lane;
+  const float* ty = tv.y + tv.off[tile] + lane;
+  int n[QT], idx[QT], acc[QT], qd[QT], rm[QT];
+  float s[QT];
+  const float4* e[QT];
+  bool tiny = id >= 0 && MEAS != TS_LCSS && db.tiny[id] != 0;
+#pragma unroll
+  for (int j = 0; j < QT; ++j) {
+    const int qi = q0 + min(qb + j, nq - 1);  // past the batch: repeats its last query, never stored
+    n[j] = q.len[qi];
+    e[j] = env + q.off[qi];
+    const int mm = max(m, 1);
+    qd[j] = n[j] / mm;
+    rm[j] = n[j] - qd[j] * mm;
+    idx[j] = 0;
+    acc[j] = 0;
+    s[j] = 0.0f;
+    if (MEAS != TS_LCSS) tiny = tiny || q.tiny[qi] != 0;
+  }
+  const bool wave_tiny = __any(tiny);
+  for (int v = 0; v < L; ++v) {
+    const float px = tx[(int64_t)v * kTile], py = ty[(int64_t)v * kTile];
+    const bool live = v < m, step = v + 1 < m;
+#pragma unroll
+    for (int j = 0; j < QT; ++j) {
+      const float4 E = e[j][idx[j]];
+      if (MEAS == TS_LCSS) {
+        const bool in = px > E.x - eps && px < E.y + eps && py > E.z - eps && py < E.w + eps;
+        s[j] += (live && in) ? 1.0f : 0.0f;  // an exact count below 2^24
+      } else {
+        const float dx = fmaxf(0.0f, fmaxf(E.x - px, px - E.y));
+        const float dy = fmaxf(0.0f, fmaxf(E.z - py, py - E.w));
+        float d[1] = {dx * dx + dy * dy};
+        sqrt_cr<1>(d, wave_tiny);
+        const float dv = live ? d[0] : 0.0f;  // adding or maxing 0 leaves s unchanged (s >= 0)
+        if (MEAS == TS_DTW) s[j] += dv;
+        else s[j] = fmaxf(s[j], dv);
+      }
+      // index map v * n / m, stepped by quotient and remainder up to the
+      // trajectory's last point (the envelope read above stays inside the
+      // query's n entries on every lane)
+      int ni = idx[j] + qd[j], na = acc[j] + rm[j];
+      if (na >= m) { na -= m; ++ni; }
+      idx[j] = step ? ni : idx[j];
+      acc[j] = step ? na : acc[j];
+    }
+  }
+  if (id < 0) return;
+  const int64_t N = db.count;
+#pragma unroll
+  for (int j = 0; j < QT; ++j) {
+    if (qb + j < nq) {
+      const float out = (MEAS == TS_LCSS) ? s[j] / (float)min(n[j], m) : s[j];
+      scores[(int64_t)(qb + j) * N + id] = out;
+    }
+  }
+}
+
+constexpr int kRefQT = 8;
+
 hipError_t launch_filter_ref(hipStream_t st, int32_t measure, SetView q, int32_t q0, int32_t nq,
-                             const float4* env, SetView db, float eps, float* scores) {
+                             const float4* env, SetView db, const TileView* tiles, float eps, float* scores) {
+  if (measure == TS_DTW || measure == TS_FRECHET || measure == TS_LCSS) {
+    if (!tiles) return hipErrorInvalidValue;
+    if (tiles->ntiles == 0 || nq == 0) return hipSuccess;
+    const dim3 grid((tiles->ntiles + 3) / 4, (nq + kRefQT - 1) / kRefQT);
+    switch (measure) {
+      case TS_DTW:
+        hipLaunchKernelGGL((k_filter_ref_tiled<TS_DTW, kRefQT>), grid, dim3(256), 0, st, q, q0, nq, env, db, *tiles,
+                           eps, scores);
+        break;
+      case TS_FRECHET:
+        hipLaunchKernelGGL((k_filter_ref_tiled<TS_FRECHET, kRefQT>), grid, dim3(256), 0, st, q, q0, nq, env, db,
+                           *tiles, eps, scores);
+        break;
+      default:
+        hipLaunchKernelGGL((k_filter_ref_tiled<TS_LCSS, kRefQT>), grid, dim3(256), 0, st, q, q0, nq, env, db, *tiles,
+                           eps, scores);
+    }
+    return hipGetLastError();
+  }
   dim3 grid((db.count + 255) / 256, nq);
   switch (measure) {
-    case TS_DTW: hipLaunchKernelGGL(k_filter_ref<TS_DTW>, grid, dim3(256), 0, st, q, q0, env, db, eps, scores); break;
-    case TS_FRECHET: hipLaunchKernelGGL(k_filter_ref<TS_FRECHET>, grid, dim3(256), 0, st, q, q0, env, db, eps, scores); break;
-    case TS_LCSS: hipLaunchKernelGGL(k_filter_ref<TS_LCSS>, grid, dim3(256), 0, st, q, q0, env, db, eps, scores); break;
     case TS_HAUSDORFF: hipLaunchKernelGGL(k_filter_ref<TS_HAUSDORFF>, grid, dim3(256), 0, st, q, q0, env, db, eps, scores); break;
     case TS_EUCLIDEAN: hipLaunchKernelGGL(k_filter_ref<TS_EUCLIDEAN>, grid, dim3(256), 0, st, q, q0, env, db, eps, scores); break;
     default: return hipErrorInvalidValue;

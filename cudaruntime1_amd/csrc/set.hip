@@ -238,6 +238,85 @@ ts_status set_reserve(ts_set* s, int64_t points_cap, int32_t count_cap, bool kee
   return TS_OK;
 }
 
+// Points of tile t (blockIdx.x): slot l of row v takes point min(v, len-1)
+// of its trajectory (the last one for v past its end), -1 slots point 0 of
+// trajectory 0 (never read as a result).
+__global__ void __launch_bounds__(256) k_tile_points(SetView s, const int64_t* toff, const int32_t* tlen,
+                                                     const int32_t* tid, float* tx, float* ty) {
+  const int t = blockIdx.x;
+  const int lane = threadIdx.x & (kTile - 1);
+  const int id = tid[(int64_t)t * kTile + lane];
+  const int m = id >= 0 ? s.len[id] : 1;
+  const int64_t src = id >= 0 ? s.off[id] : 0;
+  const int64_t base = toff[t];
+  const int L = tlen[t];
+  for (int v = threadIdx.x / kTile; v < L; v += 256 / kTile) {
+    const int64_t o = src + min(v, m - 1);
+    tx[base + (int64_t)v * kTile + lane] = s.x[o];
+    ty[base + (int64_t)v * kTile + lane] = s.y[o];
+  }
+}
+
+void free_tiles(const ts_set* s) {
+  ts_set::Tiles& t = s->tiles;
+  (void)hipFree(t.x); (void)hipFree(t.y); (void)hipFree(t.off); (void)hipFree(t.len); (void)hipFree(t.id);
+  t = ts_set::Tiles{};
+}
+
+ts_status set_tiles(const ts_set* s, hipStream_t st, TileView* out) {
+  std::lock_guard<std::mutex> lk(s->tiles_mu);
+  ts_set::Tiles& t = s->tiles;
+  if (t.gen != s->gen) {
+    // (length, index) order by a counting sort over the lengths
+    const int32_t C = s->count;
+    const int32_t maxl = s->max_len;
+    std::vector<int64_t> start((size_t)maxl + 2, 0);
+    for (int32_t i = 0; i < C; ++i) ++start[s->h_len[i] + 1];
+    for (int32_t l = 1; l <= maxl + 1; ++l) start[l] += start[l - 1];
+    const int32_t nt = (C + kTile - 1) / kTile;
+    std::vector<int32_t> ids((size_t)nt * kTile, -1), tlen(nt, 1);
+    for (int32_t i = 0; i < C; ++i) ids[start[s->h_len[i]]++] = i;
+    std::vector<int64_t> toff(nt);
+    int64_t elems = 0;
+    for (int32_t k = 0; k < nt; ++k) {
+      const int32_t last = ids[(size_t)std::min<int64_t>((int64_t)k * kTile + kTile, C) - 1];
+      tlen[k] = s->h_len[last];
+      toff[k] = elems;
+      elems += (int64_t)tlen[k] * kTile;
+    }
+    if (elems > t.cap_elems || nt > t.cap_tiles) {
+      free_tiles(s);
+      const int64_t ce = std::max<int64_t>(elems, 1);
+      const int32_t ct = std::max(nt, 1);
+      hipError_t e = hipMalloc((void**)&t.x, ce * sizeof(float));
+      if (e == hipSuccess) e = hipMalloc((void**)&t.y, ce * sizeof(float));
+      if (e == hipSuccess) e = hipMalloc((void**)&t.off, ct * sizeof(int64_t));
+      if (e == hipSuccess) e = hipMalloc((void**)&t.len, ct * sizeof(int32_t));
+      if (e == hipSuccess) e = hipMalloc((void**)&t.id, (size_t)ct * kTile * sizeof(int32_t));
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        free_tiles(s);
+        return fail(TS_ERR_OOM, "tile view allocation of " + std::to_string(2 * ce * 4) + " bytes failed");
+      }
+      t.cap_elems = ce;
+      t.cap_tiles = ct;
+      t.bytes = 2 * ce * 4 + (int64_t)ct * (8 + 4 + 4 * kTile);
+    }
+    t.ntiles = nt;
+    TS_HIP(hipMemcpyAsync(t.off, toff.data(), nt * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    TS_HIP(hipMemcpyAsync(t.len, tlen.data(), nt * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    TS_HIP(hipMemcpyAsync(t.id, ids.data(), (size_t)nt * kTile * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    if (nt > 0) {
+      hipLaunchKernelGGL(k_tile_points, dim3(nt), dim3(256), 0, st, s->view(), t.off, t.len, t.id, t.x, t.y);
+      TS_HIP(hipGetLastError());
+    }
+    TS_HIP(hipStreamSynchronize(st));
+    t.gen = s->gen;
+  }
+  *out = TileView{t.x, t.y, t.off, t.len, t.id, t.ntiles};
+  return TS_OK;
+}
+
 ts_status set_load(ts_set* s, const ts_point* xy, const int64_t* offsets, int32_t count, hipStream_t st) {
   std::vector<int32_t> lens;
   std::vector<int64_t> doff;
@@ -252,6 +331,7 @@ ts_status set_load(ts_set* s, const ts_point* xy, const int64_t* offsets, int32_
   s->max_len = *std::max_element(lens.begin(), lens.end());
   s->h_off.assign(offsets, offsets + count + 1);
   s->h_len = lens;
+  ++s->gen;
   TS_HIP(hipMemcpyAsync(s->x, hx.data(), P * sizeof(float), hipMemcpyHostToDevice, st));
   TS_HIP(hipMemcpyAsync(s->y, hy.data(), P * sizeof(float), hipMemcpyHostToDevice, st));
   TS_HIP(hipMemcpyAsync(s->off, doff.data(), (count + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
@@ -292,6 +372,7 @@ ts_status set_append(ts_set* s, const ts_point* xy, const int64_t* offsets, int3
   s->count = C;
   s->points = pts0 + offsets[count];
   s->dev_points = P;
+  ++s->gen;
   TS_HIP(launch_summaries(nullptr, s->view(), s->mbr, s->ends, s->tiny, C0));
   TS_HIP(launch_blocks(nullptr, s->view(), P, s->blk, s->sblk, C0, P0));
   TS_HIP(hipStreamSynchronize(nullptr));
@@ -392,6 +473,7 @@ ts_status ts_set_destroy(ts_set* s) {
   (void)hipFree(s->x); (void)hipFree(s->y); (void)hipFree(s->off); (void)hipFree(s->len);
   (void)hipFree(s->mbr); (void)hipFree(s->ends); (void)hipFree(s->blk); (void)hipFree(s->sblk);
   (void)hipFree(s->tiny);
+  free_tiles(s);
   delete s;
   return TS_OK;
 }
@@ -418,7 +500,8 @@ ts_status ts_set_info(const ts_set* s, int32_t* count, int64_t* points, int32_t*
 
 ts_status ts_set_bytes(const ts_set* s, int64_t* bytes) {
   if (!s || !bytes) return fail(TS_ERR_INVALID, "null argument");
-  *bytes = s->bytes;
+  std::lock_guard<std::mutex> lk(s->tiles_mu);
+  *bytes = s->bytes + s->tiles.bytes;
   return TS_OK;
 }
 

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -20,6 +21,37 @@ constexpr float kInf = 1e20f;
 // the correctly rounded sqrtf's sixteen.
 __device__ __forceinline__ float sqrt_lo(float x) {
   return x < 0x1p-96f ? 0.0f : __builtin_amdgcn_sqrtf(x) * (1.0f - 0x1p-21f);
+}
+
+// Correctly rounded sqrtf of U values. v_sqrt_f32 is within 1 ulp; for
+// x >= 2^-96, x = 0 and x = +inf, testing its two neighbours with fma
+// residuals (exact in that range) selects the correctly rounded root. That
+// is the expansion the compiler emits for sqrtf minus its rescaling of tiny
+// inputs and its 0/inf select, which these inputs do not need (0 and +inf
+// come out of the tests unchanged). Inputs in (0, 2^-96) need a trajectory
+// with a tiny coordinate (0 < |c| < 2^-25); `tiny` says whether a pair of the
+// wave's lanes has one (uniform), and then a wave holding any such x takes
+// sqrtf itself.
+template <int U>
+__device__ __forceinline__ void sqrt_cr(float (&x)[U], bool tiny) {
+  if (__builtin_expect(tiny, 0)) {
+    uint32_t mb = __float_as_uint(x[0]) - 1u;  // 0 maps to the top
+#pragma unroll
+    for (int j = 1; j < U; ++j) mb = min(mb, __float_as_uint(x[j]) - 1u);
+    if (__builtin_expect(__any(mb < 0x0f7fffffu), 0)) {
+#pragma unroll
+      for (int j = 0; j < U; ++j) x[j] = sqrtf(x[j]);
+      return;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    const float s = __builtin_amdgcn_sqrtf(x[j]);
+    const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+    const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+    const float t = __builtin_fmaf(-sm, s, x[j]) <= 0.0f ? sm : s;
+    x[j] = __builtin_fmaf(-sp, s, x[j]) > 0.0f ? sp : t;
+  }
 }
 
 // Thread-local error message behind ts_last_error().
@@ -58,6 +90,23 @@ struct SetView {
   int32_t count;
   const uint8_t* tiny;   // [count] 1: the trajectory has a tiny coordinate
 };
+
+// Length-bucketed tiles of a set: its trajectories sorted by (length, index)
+// and cut into tiles of 64, each tile's points interleaved, so that lane l of
+// a wave reading point v of the tile's trajectory l reads
+// x[off[t] + v * 64 + l]: one coalesced 256-byte load per coordinate for 64
+// trajectories. len[t] is the tile's longest length (its last trajectory's),
+// id[t * 64 + l] the trajectory in slot l (-1 past the last trajectory);
+// slots shorter than len[t] repeat their last point.
+struct TileView {
+  const float* x;
+  const float* y;
+  const int64_t* off;  // [ntiles]
+  const int32_t* len;  // [ntiles]
+  const int32_t* id;   // [ntiles * 64]
+  int32_t ntiles;
+};
+constexpr int kTile = 64;
 
 // A tiny coordinate: 0 < |c| < 2^-25. Between two trajectories without one
 // every squared point distance is 0 or at least 2^-96 (coordinates of
@@ -121,6 +170,25 @@ struct ts_set {
   // Capacity of the device arrays (sets reloaded in place by the server).
   int64_t cap_points = 0;
   int32_t cap_count = 0;
+  // Contents generation: bumped by every load or append, so derived views
+  // (the tiles) know when they are stale.
+  uint64_t gen = 0;
+  // Length-bucketed tile view (TileView), built on first use by set_tiles;
+  // device arrays grow only.
+  struct Tiles {
+    float* x = nullptr;
+    float* y = nullptr;
+    int64_t* off = nullptr;
+    int32_t* len = nullptr;
+    int32_t* id = nullptr;
+    int32_t ntiles = 0;
+    int64_t cap_elems = 0;
+    int32_t cap_tiles = 0;
+    uint64_t gen = ~0ull;  // contents generation the tiles were built from
+    int64_t bytes = 0;
+  };
+  mutable Tiles tiles;
+  mutable std::mutex tiles_mu;
   ts::SetView view() const { return {x, y, off, len, mbr, ends, blk, sblk, count, tiny}; }
 };
 
@@ -136,6 +204,10 @@ namespace ts {
 hipError_t launch_summaries(hipStream_t st, SetView s, float4* mbr, float4* ends, uint8_t* tiny, int32_t t0 = 0);
 hipError_t launch_blocks(hipStream_t st, SetView s, int64_t points, float4* blk, float4* sblk, int32_t t0 = 0,
                          int64_t point0 = 0);
+// The set's tile view, built on `st` (and waited for) when the set's
+// contents changed since the last build; thread-safe.
+ts_status set_tiles(const ts_set* s, hipStream_t st, TileView* out);
+void free_tiles(const ts_set* s);
 // (Re)load a set in place on `st`: grows the device arrays only when the new
 // contents do not fit, so a warmed-up set reloads without hipMalloc/hipFree.
 ts_status set_load(ts_set* s, const ts_point* xy, const int64_t* offsets, int32_t count, hipStream_t st);
@@ -154,9 +226,10 @@ ts_status pad_layout(const ts_point* xy, const int64_t* offsets, int32_t count, 
 
 // filter.hip
 hipError_t launch_envelope(hipStream_t st, SetView q, int32_t max_len, int32_t r, float4* env);
-// REF filter scores[q * N + j] for queries [q0, q0+nq).
+// REF filter scores[q * N + j] for queries [q0, q0+nq); the point measures
+// (DTW, Frechet, LCSS) read the DB through its tile view.
 hipError_t launch_filter_ref(hipStream_t st, int32_t measure, SetView q, int32_t q0, int32_t nq,
-                             const float4* env, SetView db, float eps, float* scores);
+                             const float4* env, SetView db, const TileView* tiles, float eps, float* scores);
 // Admissible summary bounds (EXACT mode) of queries [q0,q0+nq) against the
 // whole DB, reduced per DB chunk of bound_chunk_items(s) trajectories to the
 // best `s` keys per query: keys[(q * nchunks + chunk) * s + t].
