@@ -294,14 +294,19 @@ __device__ __forceinline__ float pair_bound_m(const MeasureArgs& ma, const QSum&
 constexpr int kSeedPerThread = 16, kSeedPerThreadWhole = 8;
 int32_t bound_chunk_items(int32_t s) { return 256 * (s <= 256 ? kSeedPerThread : kSeedPerThreadWhole); }
 
-__device__ void bitonic_sort_lds(uint64_t* s, int n, int tid, int nthreads) {
-  for (int k = 2; k <= n; k <<= 1) {
+// Bitonic sort of `nseg` consecutive segments of `seg` keys each (seg a
+// power of two), every segment ascending: compare partners and directions
+// below the last stage depend only on an index's low bits, so all segments
+// sort together; the last stage merges each segment ascending.
+__device__ void bitonic_sort_segments(uint64_t* s, int seg, int nseg, int tid, int nthreads) {
+  const int n = seg * nseg;
+  for (int k = 2; k <= seg; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
       for (int i = tid; i < n; i += nthreads) {
-        int ixj = i ^ j;
+        const int ixj = i ^ j;
         if (ixj > i) {
-          uint64_t a = s[i], b = s[ixj];
-          bool asc = (i & k) == 0;
+          const uint64_t a = s[i], b = s[ixj];
+          const bool asc = k == seg || (i & k) == 0;
           if ((a > b) == asc) { s[i] = b; s[ixj] = a; }
         }
       }
@@ -314,44 +319,70 @@ __device__ void bitonic_sort_lds(uint64_t* s, int n, int tid, int nthreads) {
 // trajectories). Any real pair's exact value bounds the k-th best from above,
 // so seeds need not be the exact top-s by bound: each thread keeps the best
 // key of its kSeedPerThread trajectories and the block keeps the best `s` of
-// those (s <= 256). Larger s sorts the whole chunk.
-template <bool WHOLE, int MEAS>
-__global__ void __launch_bounds__(256) k_bound_chunks(MeasureArgs ma, SetView q, int32_t q0, SetView db,
+// those (s <= 256), for QB queries at once: every DB summary (36 B) is read
+// once per QB queries. Larger s sorts the whole chunk, one query per block.
+template <bool WHOLE, int MEAS, int QB>
+__global__ void __launch_bounds__(256) k_bound_chunks(MeasureArgs ma, SetView q, int32_t q0, int32_t nq, SetView db,
                                                       int32_t s, uint64_t* keys, float* bstore, int32_t* zero,
                                                       int32_t nzero) {
+  static_assert(!WHOLE || QB == 1, "whole-chunk sorts take one query per block");
   constexpr int PER = WHOLE ? kSeedPerThreadWhole : kSeedPerThread;
   constexpr int NS = WHOLE ? 256 * PER : 256;
-  __shared__ uint64_t sk[NS];
+  __shared__ uint64_t sk[NS * QB];
   // counters the batch's later kernels expect to start at zero
   if (blockIdx.x == 0 && blockIdx.y == 0 && (int)threadIdx.x < nzero) zero[threadIdx.x] = 0;
-  const int qi = q0 + blockIdx.y;
-  QSum qs{q.mbr[qi], q.ends[qi], q.len[qi], q.off[qi]};
+  const int qr0 = blockIdx.y * QB;  // first query of the block, relative to q0
+  QSum qs[QB];
+#pragma unroll
+  for (int j = 0; j < QB; ++j) {
+    const int qi = q0 + min(qr0 + j, nq - 1);  // past the batch: repeats its last query, never stored
+    qs[j] = QSum{q.mbr[qi], q.ends[qi], q.len[qi], q.off[qi]};
+  }
   const bool desc = ma.measure == TS_LCSS;
   const int base = blockIdx.x * 256 * PER;
-  uint64_t best = ~0ull;
+  uint64_t best[QB];
+#pragma unroll
+  for (int j = 0; j < QB; ++j) best[j] = ~0ull;
   // eight trajectories' summaries in flight at a time
 #pragma unroll 1
   for (int t0 = 0; t0 < PER; t0 += 8) {
 #pragma unroll
     for (int t = t0; t < t0 + 8; ++t) {
       const int i = t * 256 + threadIdx.x;
-      const int j = base + i;
-      uint64_t key = ~0ull;
-      if (j < db.count) {
-        const float b = pair_bound_m<MEAS>(ma, qs, q, db.mbr[j], db.ends[j], db.len[j]);
-        if (bstore) bstore[(int64_t)blockIdx.y * db.count + j] = b;  // for the compaction's count pass
-        key = make_key(b, j, desc);
+      const int jd = base + i;
+      const bool ok = jd < db.count;
+      const float4 cm = ok ? db.mbr[jd] : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float4 ce = ok ? db.ends[jd] : make_float4(0.f, 0.f, 0.f, 0.f);
+      const int m = ok ? db.len[jd] : 1;
+#pragma unroll
+      for (int j = 0; j < QB; ++j) {
+        uint64_t key = ~0ull;
+        if (ok) {
+          const float b = pair_bound_m<MEAS>(ma, qs[j], q, cm, ce, m);
+          if (bstore && qr0 + j < nq) bstore[(int64_t)(qr0 + j) * db.count + jd] = b;  // for the count pass
+          key = make_key(b, jd, desc);
+        }
+        if (WHOLE) sk[i] = key;
+        else best[j] = key < best[j] ? key : best[j];
       }
-      if (WHOLE) sk[i] = key;
-      else best = key < best ? key : best;
     }
   }
-  if (!WHOLE) sk[threadIdx.x] = best;
+  if (!WHOLE) {
+#pragma unroll
+    for (int j = 0; j < QB; ++j) sk[j * NS + threadIdx.x] = best[j];
+  }
   __syncthreads();
-  bitonic_sort_lds(sk, NS, threadIdx.x, 256);
-  uint64_t* out = keys + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * s;
-  for (int t = threadIdx.x; t < s; t += 256) out[t] = t < NS ? sk[t] : ~0ull;
+  bitonic_sort_segments(sk, NS, QB, threadIdx.x, 256);
+#pragma unroll
+  for (int j = 0; j < QB; ++j) {
+    if (qr0 + j >= nq) break;
+    uint64_t* out = keys + ((int64_t)(qr0 + j) * gridDim.x + blockIdx.x) * s;
+    for (int t = threadIdx.x; t < s; t += 256) out[t] = t < NS ? sk[j * NS + t] : ~0ull;
+  }
 }
+
+// Queries per seeding block (the keep-the-best form).
+constexpr int kSeedQB = 8;
 
 hipError_t launch_bound_chunks(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, int32_t nq,
                                SetView db, int32_t s, int32_t chunk, uint64_t* keys, float* bstore, int32_t* zero,
@@ -359,13 +390,14 @@ hipError_t launch_bound_chunks(hipStream_t st, MeasureArgs m, int32_t k, SetView
   (void)k;
   (void)chunk;
   const int32_t items = bound_chunk_items(s);
-  dim3 grid((db.count + items - 1) / items, nq);
-#define TS_CHUNKS(MEAS)                                                                              \
-  if (s <= 256)                                                                                     \
-    hipLaunchKernelGGL((k_bound_chunks<false, MEAS>), grid, dim3(256), 0, st, m, q, q0, db, s, keys, bstore, zero, \
-                       nzero);                                                                          \
-  else hipLaunchKernelGGL((k_bound_chunks<true, MEAS>), grid, dim3(256), 0, st, m, q, q0, db, s, keys, bstore,   \
-                          zero, nzero);
+  const int32_t gx = (db.count + items - 1) / items;
+#define TS_CHUNKS(MEAS)                                                                                         \
+  if (s <= 256)                                                                                                 \
+    hipLaunchKernelGGL((k_bound_chunks<false, MEAS, kSeedQB>), dim3(gx, (nq + kSeedQB - 1) / kSeedQB), dim3(256), \
+                       0, st, m, q, q0, nq, db, s, keys, bstore, zero, nzero);                                   \
+  else                                                                                                          \
+    hipLaunchKernelGGL((k_bound_chunks<true, MEAS, 1>), dim3(gx, nq), dim3(256), 0, st, m, q, q0, nq, db, s, keys, \
+                       bstore, zero, nzero);
   switch (m.measure) {
     case TS_DTW: TS_CHUNKS(TS_DTW) break;
     case TS_FRECHET: TS_CHUNKS(TS_FRECHET) break;
