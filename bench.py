@@ -596,7 +596,7 @@ def run_c5(ts, args):
     results = []
     checked = {}
     for qi_point, qps in enumerate(args.qps):
-        srv = ts.Server(db, streams=args.streams, max_batch=256, batch_wait_us=500)
+        srv = ts.Server(db, streams=args.streams, max_batch=args.max_batch, batch_wait_us=500)
         n = int(qps * args.duration)
         arrivals = np.cumsum(rng.exponential(1.0 / qps, size=n))
         tickets = []
@@ -707,8 +707,8 @@ def run_c5(ts, args):
             "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": "serving: %d-trajectory DB (log-uniform [16,512]) resident, Poisson arrivals, "
                                    "query lengths log-uniform [16,1024], mix DTW 40/LCSS 20/Frechet 20/Hausdorff 20, "
-                                   "k in {1,10,100}, EXACT top-k, continuous batching over %d streams"
-                                   % (db.count, args.streams),
+                                   "k in {1,10,100}, EXACT top-k, continuous batching over %d streams, batches of "
+                                   "up to %d queries" % (db.count, args.streams, args.max_batch),
                        "db_trajectories": db.count, "duration_s": args.duration},
             "parity": parity, "cpu_baseline": cpu, "extra": {"sweep": results}}
     print(json.dumps(line), flush=True)
@@ -722,7 +722,8 @@ def main():
     ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--qps", type=lambda v: [int(x) for x in v.split(",")], default=[2000, 5000, 10000, 20000])
     ap.add_argument("--duration", type=float, default=10.0)
-    ap.add_argument("--streams", type=int, default=4)
+    ap.add_argument("--streams", type=int, default=4, help="c5: serving lanes")
+    ap.add_argument("--max-batch", type=int, default=256, help="c5: queries per serving batch")
     ap.add_argument("--check-requests", type=int, default=64, help="c5: served results checked against the oracle")
     ap.add_argument("--n-db", type=int, default=None)
     ap.add_argument("--n-q", type=int, default=None)

@@ -89,6 +89,7 @@ _sig = {
     "ts_engine_reset_stats": (ctypes.c_int, [_P]),
     "ts_knn": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(Params), _P, _P, ctypes.POINTER(Stats)]),
     "ts_knn_plain": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(Params), _P, _P, ctypes.POINTER(Stats)]),
+    "ts_knn_multi_k": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(Params), _P, _P, _P, ctypes.POINTER(Stats)]),
     "ts_filter_ref": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(Params), _P]),
     "ts_pair_values": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(Params), _P, _P, ctypes.c_int64, _P]),
     "ts_pair_values_noalloc": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(Params), _P, _P, ctypes.c_int64, _P]),
@@ -352,6 +353,19 @@ class Engine:
     def knn(self, db, queries, p):
         """Batched k-NN -> (idx [Q,k] int32, val [Q,k] float32, stats)."""
         return self._knn(_lib.ts_knn, db, queries, p)
+
+    def knn_multi_k(self, db, queries, p, k_per_query):
+        """Batched k-NN with a k per query (each <= p.k) -> (idx [Q,p.k], val, stats);
+        row i holds its first k_per_query[i] results, then idx -1."""
+        kq = np.ascontiguousarray(k_per_query, np.int32)
+        assert len(kq) == queries.count
+        Q, k = queries.count, p.k
+        idx = np.empty((Q, k), np.int32)
+        val = np.empty((Q, k), np.float32)
+        st = Stats()
+        _check(_lib.ts_knn_multi_k(self._h, db.handle, queries.handle, ctypes.byref(p), _ptr(kq), _ptr(idx),
+                                   _ptr(val), ctypes.byref(st)), "ts_knn_multi_k")
+        return idx, val, st.as_dict()
 
     def knn_plain(self, db, queries, p):
         """Plain k-NN (one launch sequence per query) -> (idx, val, stats)."""

@@ -173,7 +173,7 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 enum BufId {
   B_ENV, B_SCORES, B_KEYS, B_KEYS2, B_SCA, B_SCB, B_CAND, B_CANDIDX, B_ITEMS, B_ITEMS3, B_VALS, B_REGION,
   B_RVALS, B_COUNTS, B_CTR, B_TAU, B_TOP, B_OIDX, B_OVAL, B_CELLS, B_HOSTIN, B_ITEMS2, B_RITEMS, B_SEGC, B_NEED, B_BCNT,
-  B_BITS, B_BOUNDS, B_NBUF
+  B_BITS, B_BOUNDS, B_KQ, B_NBUF
 };
 
 struct Buf {
@@ -642,7 +642,7 @@ constexpr int64_t kStoredBoundsMax = 1ll << 26;
 
 // EXACT pipeline for queries [q0, q0+nq).
 ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, const ts_set* qs, int32_t q0,
-                          int32_t nq, uint64_t* top) {
+                          int32_t nq, uint64_t* top, const int32_t* kq_dev) {
   const int32_t N = db->count;
   const int32_t k = p.k;
   const bool desc = desc_measure(p.measure);
@@ -680,7 +680,9 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   const KeyItems seed_items{seeds, s, q0, seed_idx};
   TS_TRY(refine(e, p, db, qs, items, nullptr, (int64_t)nq * s, vals, true, ctrs, &seed_items));
   // 2. threshold: k-th best exact value among the seeds
-  TS_TRY(run(e, "threshold", 0, 0, [&] { return launch_seed_tau(e->st, vals, seed_idx, nq, s, k, desc, tau); }));
+  TS_TRY(run(e, "threshold", 0, 0, [&] {
+    return launch_seed_tau(e->st, vals, seed_idx, nq, s, k, desc, tau, kq_dev ? kq_dev + q0 : nullptr);
+  }));
   // 3. compaction of every pair that can still reach tau: count, size each
   // query's region, write. Eager calls size the regions exactly from the
   // counts read back to the host; graph-captured calls cannot read back, so
@@ -846,8 +848,13 @@ ts_status wait_stream(hipStream_t st) {
   }
 }
 
+// kq (host, optional): per-query k, each in [1, pp->k]. Every stage then
+// runs at k = pp->k except the EXACT threshold, which takes query q's
+// kq[q]-th best seed: the first kq[q] results of query q are its exact top
+// kq[q] (the top-k_max of a candidate set that holds every pair better than
+// the kq[q]-th begins with them); its later slots hold idx -1.
 ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_params* pp, int32_t* out_idx,
-                   float* out_val, ts_stats* stats, bool plain, int retry = 0) {
+                   float* out_val, ts_stats* stats, bool plain, int retry = 0, const int32_t* kq = nullptr) {
   auto t0 = std::chrono::steady_clock::now();
   if (!e) return fail(TS_ERR_INVALID, "null engine");
   TS_TRY(check_params(pp, db, qs));
@@ -903,7 +910,14 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
   // at most kGraphMaxPairs pairs. Above that (measured on 4096 x 1M DTW:
   // replays 1.28-1.48 s against 1.25 s eager, with the DP buckets' parallel
   // branches) batches launch eagerly.
-  const bool use_graphs = e->graphs && !e->timing && !plain && (int64_t)bq * N <= kGraphMaxPairs;
+  const bool use_graphs = e->graphs && !e->timing && !plain && !kq && (int64_t)bq * N <= kGraphMaxPairs;
+  int32_t* kq_dev = nullptr;
+  if (kq) {
+    for (int32_t i = 0; i < Q; ++i)
+      if (kq[i] < 1 || kq[i] > k) return fail(TS_ERR_INVALID, "per-query k must be in [1, k]");
+    TS_TRY(ensure(e, B_KQ, Q, &kq_dev));
+    TS_HIP(hipMemcpyAsync(kq_dev, kq, sizeof(int32_t) * Q, hipMemcpyHostToDevice, e->st));
+  }
   // Results, the capacity check and the cell counters come back through the
   // pinned staging buffer in one copy behind the last kernel.
   const bool check_need = use_graphs && p.mode == TS_MODE_EXACT && !all_pairs;
@@ -921,7 +935,7 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
     uint64_t* topq = top + (int64_t)q0 * k;
     if (p.mode == TS_MODE_REF) return knn_ref_batch(e, p, db, qs, q0, nq, env, &tiles, topq);
     if (all_pairs) return knn_all_batch(e, p, db, qs, q0, nq, topq);
-    return knn_exact_batch(e, p, db, qs, q0, nq, topq);
+    return knn_exact_batch(e, p, db, qs, q0, nq, topq, kq_dev);
   };
   auto prologue = [&]() -> ts_status {
     TS_TRY(run(e, "fill", 0, 0, [&] { return zero_cells(e->st, cells); }));
@@ -981,6 +995,13 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
   TS_TRY(wait_stream(e->st));
   std::memcpy(out_idx, h_idx, sizeof(int32_t) * Q * k);
   std::memcpy(out_val, h_val, sizeof(float) * Q * k);
+  if (kq) {
+    for (int32_t i = 0; i < Q; ++i)
+      for (int32_t t = kq[i]; t < k; ++t) {
+        out_idx[(int64_t)i * k + t] = -1;
+        out_val[(int64_t)i * k + t] = 0.0f;
+      }
+  }
   if (check_need) {
     // A replayed batch sized its candidates on the device: if any total
     // exceeded the capacity its results are incomplete; grow and repeat.
@@ -989,7 +1010,7 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
       if (retry > 0) return fail(TS_ERR_HIP, "candidate capacity still exceeded after growth");
       e->exact_cap = std::min<int64_t>((int64_t)need + (int64_t)need / 4 + 4096, (1ll << 31) - 1);
       ++e->overflow_retries;
-      return knn_impl(e, db, qs, pp, out_idx, out_val, stats, plain, retry + 1);
+      return knn_impl(e, db, qs, pp, out_idx, out_val, stats, plain, retry + 1, kq);
     }
   }
   if (stats) {
@@ -1258,6 +1279,12 @@ ts_status ts_engine_reset_stats(ts_engine* e) {
 ts_status ts_knn(ts_engine* e, const ts_set* db, const ts_set* queries, const ts_params* p, int32_t* out_idx,
                  float* out_val, ts_stats* stats) {
   return knn_impl(e, db, queries, p, out_idx, out_val, stats, false);
+}
+
+ts_status ts_knn_multi_k(ts_engine* e, const ts_set* db, const ts_set* queries, const ts_params* p,
+                         const int32_t* k_per_query, int32_t* out_idx, float* out_val, ts_stats* stats) {
+  if (!k_per_query) return fail(TS_ERR_INVALID, "null k_per_query");
+  return knn_impl(e, db, queries, p, out_idx, out_val, stats, false, 0, k_per_query);
 }
 
 ts_status ts_knn_plain(ts_engine* e, const ts_set* db, const ts_set* queries, const ts_params* p, int32_t* out_idx,

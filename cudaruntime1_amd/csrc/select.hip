@@ -189,7 +189,7 @@ __global__ void __launch_bounds__(256) k_topk_stage(const uint64_t* in, int64_t 
 // values (index -1 or a non-finite value: empty), sorted in LDS; tau is the
 // k-th best value, or +-inf when fewer than k seeds have one.
 __global__ void __launch_bounds__(256) k_seed_tau(const float* vals, const int32_t* seed_idx, int32_t s, int32_t k,
-                                                  int32_t cs, int32_t desc, float* tau) {
+                                                  const int32_t* kq, int32_t cs, int32_t desc, float* tau) {
   __shared__ uint64_t sk[kSelChunk];
   const int q = blockIdx.x;
   for (int i = threadIdx.x; i < cs; i += 256)
@@ -209,7 +209,7 @@ __global__ void __launch_bounds__(256) k_seed_tau(const float* vals, const int32
     }
   }
   if (threadIdx.x == 0) {
-    const uint64_t key = sk[k - 1];
+    const uint64_t key = sk[(kq ? kq[q] : k) - 1];
     const float inf = __int_as_float(0x7f800000);
     tau[q] = key == ~0ull ? (desc ? -inf : inf) : value_of(key, desc != 0);
   }
@@ -298,11 +298,11 @@ __global__ void k_kth(const uint64_t* keys, int32_t nseg, int32_t s, int32_t k, 
 }
 
 hipError_t launch_seed_tau(hipStream_t st, const float* vals, const int32_t* seed_idx, int32_t nq, int32_t s,
-                           int32_t k, int32_t descending, float* tau) {
+                           int32_t k, int32_t descending, float* tau, const int32_t* kq) {
   if (nq == 0) return hipSuccess;
   if (s > kSelChunk || k < 1 || k > s) return hipErrorInvalidValue;
   const int cs = chunk_for(s);
-  hipLaunchKernelGGL(k_seed_tau, dim3(nq), dim3(256), 0, st, vals, seed_idx, s, k, cs, descending, tau);
+  hipLaunchKernelGGL(k_seed_tau, dim3(nq), dim3(256), 0, st, vals, seed_idx, s, k, kq, cs, descending, tau);
   return hipGetLastError();
 }
 

@@ -4,10 +4,11 @@
 // measure, length and k are queued by ts_server_submit. `streams` worker
 // lanes each own an engine (its own HIP stream and workspace) and a reusable
 // device query set; a free lane takes every compatible request that is
-// waiting (same parameters, up to max_batch), uploads them into its set and
-// runs one batched k-NN. Requests that arrive while other lanes' batches are
-// on the GPU are taken by the next free lane, so their work overlaps on the
-// device. Admission is by free device memory: a batch starts only when its
+// waiting (same parameters apart from k, up to max_batch), uploads them into
+// its set and runs one batched k-NN at the batch's largest k, each query
+// keeping the exact answer for its own k. Requests that arrive while other
+// lanes' batches are on the GPU are taken by the next free lane, so their
+// work overlaps on the device. Admission is by free device memory: a batch starts only when its
 // workspace estimate fits into the free HBM (hipMemGetInfo, or a fixed value
 // injected by tests) minus the reserve and minus what running batches hold;
 // a request that could never fit is refused, and a full queue refuses with
@@ -41,7 +42,14 @@ struct Request {
   std::vector<float> val;
 };
 
-bool same_params(const ts_params& a, const ts_params& b) { return std::memcmp(&a, &b, sizeof(ts_params)) == 0; }
+// Requests that differ only in k share a batch: it runs at the largest k
+// and each query keeps its own first k results (ts_knn_multi_k gives every
+// query the exact answer for its own k).
+bool batchable(const ts_params& a, const ts_params& b) {
+  ts_params x = a, y = b;
+  x.k = y.k = 0;
+  return std::memcmp(&x, &y, sizeof(ts_params)) == 0;
+}
 
 }  // namespace
 
@@ -63,20 +71,21 @@ struct ts_server {
 
 namespace {
 
-// Device workspace a request needs in a batch (query points, candidate
-// regions / score rows and keys), a deliberate over-estimate.
-int64_t estimate_bytes(const ts_server* s, int32_t len, const ts_params& p) {
+// Device workspace of a batch of n queries with sum_len points in all, run
+// at k (query points, candidate regions / score rows and keys, seeds sized
+// by k), a deliberate over-estimate.
+int64_t estimate_bytes(const ts_server* s, int64_t sum_len, int64_t n, const ts_params& p, int32_t k) {
   const int64_t N = s->db->count;
-  int64_t b = (int64_t)len * 16 + 256;
+  int64_t per = 256;
   if (p.mode == TS_MODE_REF) {
-    b += N * 16 + (int64_t)p.kc * 64;
+    per += N * 16 + (int64_t)p.kc * 64;
   } else {
-    const int64_t sd = std::max<int64_t>(2 * p.k, 16);
+    const int64_t sd = std::max<int64_t>(2 * (int64_t)k, 16);
     const int64_t nch = (N + ts::bound_chunk_items((int32_t)std::min<int64_t>(sd, 1024)) - 1) /
                         ts::bound_chunk_items((int32_t)std::min<int64_t>(sd, 1024));
-    b += nch * sd * 8 * 2 + std::min<int64_t>(N, 4096) * 48 + sd * 64;
+    per += nch * sd * 8 * 2 + std::min<int64_t>(N, 4096) * 48 + sd * 64;
   }
-  return b;
+  return sum_len * 16 + n * per;
 }
 
 int64_t free_bytes(ts_server* s) {
@@ -104,17 +113,27 @@ void lane_main(ts_server* s, int lane) {
         if (s->stop) break;
         if (!s->queue.empty()) {
           // Continuous batching: take the oldest request and every waiting
-          // request with the same parameters, while the memory budget allows.
+          // request with the same parameters but possibly another k, while
+          // the memory budget allows.
           const int64_t budget = free_bytes(s) - s->cfg.mem_reserve - s->held_bytes;
           const ts_params p0 = s->queue.front()->p;
+          int64_t sum_len = 0;
+          int32_t kmax = 0;
           for (auto it = s->queue.begin(); it != s->queue.end() && (int)batch.size() < s->cfg.max_batch;) {
-            if (same_params((*it)->p, p0) && batch_bytes + (*it)->est_bytes <= budget) {
-              batch_bytes += (*it)->est_bytes;
-              batch.push_back(*it);
-              it = s->queue.erase(it);
-            } else {
-              ++it;
+            if (batchable((*it)->p, p0)) {
+              const int64_t sl = sum_len + (int64_t)(*it)->pts.size();
+              const int32_t km = std::max(kmax, (*it)->p.k);
+              const int64_t est = estimate_bytes(s, sl, (int64_t)batch.size() + 1, p0, km);
+              if (est <= budget) {
+                sum_len = sl;
+                kmax = km;
+                batch_bytes = est;
+                batch.push_back(*it);
+                it = s->queue.erase(it);
+                continue;
+              }
             }
+            ++it;
           }
           if (!batch.empty()) break;
         }
@@ -132,12 +151,17 @@ void lane_main(ts_server* s, int lane) {
       xy.insert(xy.end(), r->pts.begin(), r->pts.end());
       off.push_back((int64_t)xy.size());
     }
-    const ts_params p = batch.front()->p;
+    ts_params p = batch.front()->p;
     const int32_t B = (int32_t)batch.size();
+    std::vector<int32_t> kq(B);
+    for (int32_t i = 0; i < B; ++i) {
+      kq[i] = batch[i]->p.k;
+      p.k = std::max(p.k, kq[i]);
+    }
     std::vector<int32_t> idx((size_t)B * p.k);
     std::vector<float> val((size_t)B * p.k);
     ts_status stt = ts::set_load(qset, xy.data(), off.data(), B, st);
-    if (stt == TS_OK) stt = ts_knn(eng, s->db, qset, &p, idx.data(), val.data(), nullptr);
+    if (stt == TS_OK) stt = ts_knn_multi_k(eng, s->db, qset, &p, kq.data(), idx.data(), val.data(), nullptr);
     const auto t_done = Clock::now();
     {
       std::lock_guard<std::mutex> lk(s->mu);
@@ -148,8 +172,8 @@ void lane_main(ts_server* s, int lane) {
         if (r->state == Request::CANCELLED) continue;
         r->status = stt;
         if (stt == TS_OK) {
-          r->idx.assign(idx.begin() + (size_t)i * p.k, idx.begin() + (size_t)(i + 1) * p.k);
-          r->val.assign(val.begin() + (size_t)i * p.k, val.begin() + (size_t)(i + 1) * p.k);
+          r->idx.assign(idx.begin() + (size_t)i * p.k, idx.begin() + (size_t)i * p.k + kq[i]);
+          r->val.assign(val.begin() + (size_t)i * p.k, val.begin() + (size_t)i * p.k + kq[i]);
           r->state = Request::DONE;
           s->counters[2] += 1;
         } else {
@@ -215,7 +239,7 @@ ts_status ts_server_submit(ts_server* s, const ts_point* q, int32_t len, const t
   auto r = std::make_shared<Request>();
   r->pts.assign(q, q + len);
   r->p = *p;
-  r->est_bytes = estimate_bytes(s, len, *p);
+  r->est_bytes = estimate_bytes(s, len, 1, *p, p->k);
   r->t_submit = Clock::now();
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->stop) return fail(TS_ERR_CANCELLED, "server is shutting down");

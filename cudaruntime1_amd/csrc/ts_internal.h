@@ -289,8 +289,9 @@ hipError_t launch_decode_keys(hipStream_t st, const uint64_t* keys, int64_t n, i
 // if the segment has fewer keys.
 // tau[q] = k-th best of query q's s refined seed values (vals[q*s + i],
 // seed_idx the DB index or -1); one sort per query.
+// With kq, query q's threshold is its kq[q]-th best (each kq[q] in [1, k]).
 hipError_t launch_seed_tau(hipStream_t st, const float* vals, const int32_t* seed_idx, int32_t nq, int32_t s,
-                           int32_t k, int32_t descending, float* tau);
+                           int32_t k, int32_t descending, float* tau, const int32_t* kq = nullptr);
 hipError_t launch_kth_threshold(hipStream_t st, const uint64_t* keys, int32_t nseg, int32_t s,
                                 int32_t k, int32_t descending, float* tau);
 // Build Items (q0+row, d=decoded idx, slot=row*s+t) from per-row sorted keys;

@@ -204,6 +204,15 @@ ts_status ts_engine_reset_stats(ts_engine* e);
 ts_status ts_knn(ts_engine* e, const ts_set* db, const ts_set* queries, const ts_params* p,
                  int32_t* out_idx, float* out_val, ts_stats* stats);
 
+/* Batched k-NN where query i asks for its own k: k_per_query[i] in [1, p->k].
+ * out_idx/out_val are n_queries * p->k; query i's first k_per_query[i]
+ * entries equal what ts_knn with k = k_per_query[i] returns, its later slots
+ * hold idx -1 (EXACT: each query's pruning threshold comes from its own k;
+ * REF: the best of its p->kc candidates). No counterpart in the reference;
+ * the serving loop batches requests that differ only in k through it. */
+ts_status ts_knn_multi_k(ts_engine* e, const ts_set* db, const ts_set* queries, const ts_params* p,
+                         const int32_t* k_per_query, int32_t* out_idx, float* out_val, ts_stats* stats);
+
 /* Plain k-NN: the same result, one launch sequence per query (the drop-in
  * execution model of the reference entry points, with the database resident). */
 ts_status ts_knn_plain(ts_engine* e, const ts_set* db, const ts_set* queries, const ts_params* p,
