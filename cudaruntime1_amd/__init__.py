@@ -7,7 +7,7 @@ silently run without the native library.
 
 Measures and modes mirror the reference's entry points
 (1li2yang3/CudaRuntime1: dtw.cuh, frechet.cuh, lcss.cuh, hausdorff.cuh,
-euclidean.cuh); the 13 original C++ signatures themselves are exported by
+euclidean.cuh); the 12 original C++ signatures themselves are exported by
 libtrajsim.so (include/trajsim_compat.hpp).
 """
 import ctypes
@@ -292,13 +292,14 @@ class Engine:
         if timing:
             self.set_timing(True)
         if graphs:
-            self.set_graphs(True)
+            self.set_graphs(graphs)
 
     def set_timing(self, on):
         _check(_lib.ts_engine_set_timing(self._h, int(on)), "ts_engine_set_timing")
 
     def set_graphs(self, on):
-        """Replay batched k-NN calls of a repeated shape from HIP graphs."""
+        """Replay batched k-NN calls of a repeated shape from HIP graphs: 0 off,
+        1 (True) batches up to 2^26 (query, DB) pairs, 2 batches of every size."""
         _check(_lib.ts_engine_set_graphs(self._h, int(on)), "ts_engine_set_graphs")
 
     def load_tuning(self, path):

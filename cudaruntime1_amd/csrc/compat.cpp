@@ -1,4 +1,4 @@
-// compat.cpp -- the reference's 13 entry points over the C ABI.
+// compat.cpp -- the reference's 12 entry points over the C ABI.
 //
 // Each call uploads its two fixed-length batches into device sets, runs the
 // REF pipeline (the reference's filter -> K candidates -> exact refine ->

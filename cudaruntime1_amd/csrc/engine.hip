@@ -250,6 +250,7 @@ struct ts_engine {
   // hipGraph replay of batches (off by default; never with timing or for
   // the plain method).
   bool graphs = false;
+  bool graphs_all = false;  // also batches above kGraphMaxPairs
   bool capturing = false;       // launches are being recorded, not run
   bool capture_broken = false;  // the workspace had to grow while recording
   uint64_t ws_gen = 0;          // bumped whenever a workspace buffer moves
@@ -910,7 +911,8 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
   // at most kGraphMaxPairs pairs. Above that (measured on 4096 x 1M DTW:
   // replays 1.28-1.48 s against 1.25 s eager, with the DP buckets' parallel
   // branches) batches launch eagerly.
-  const bool use_graphs = e->graphs && !e->timing && !plain && !kq && (int64_t)bq * N <= kGraphMaxPairs;
+  const bool use_graphs =
+      e->graphs && !e->timing && !plain && !kq && (e->graphs_all || (int64_t)bq * N <= kGraphMaxPairs);
   int32_t* kq_dev = nullptr;
   if (kq) {
     for (int32_t i = 0; i < Q; ++i)
@@ -1088,7 +1090,7 @@ int shape_of(int W, int U) {
 ts_status load_tuning(ts_engine* e, const char* path) {
   FILE* f = std::fopen(path, "r");
   if (!f) return fail(TS_ERR_INVALID, std::string("cannot open tuning table ") + path);
-  char line[256];
+  char line[1024];
   int lineno = 0;
   ts_status st = TS_OK;
   bool set[kTables][kDpVariants][kDpBuckets] = {};
@@ -1246,7 +1248,9 @@ ts_status ts_dp_shape(int32_t i, int32_t* lanes, int32_t* cells) {
 
 ts_status ts_engine_set_graphs(ts_engine* e, int32_t on) {
   if (!e) return fail(TS_ERR_INVALID, "null engine");
+  if (on < 0 || on > 2) return fail(TS_ERR_INVALID, "ts_engine_set_graphs: on must be 0, 1 or 2");
   e->graphs = on != 0;
+  e->graphs_all = on == 2;
   return TS_OK;
 }
 

@@ -4,7 +4,7 @@ grid (waves per CU) with the highest measured cell rate, and for small
 refines (one round of pairs, up to 4096) and mid-size refines (4096 pairs)
 the ones with the lowest time.
 
-    python -m cudaruntime1_amd.tune [--out PATH] [--quick]
+    python -m cudaruntime1_amd.tune [--out PATH] [--quick] [--variants dtw,frechet]
 
 For every (variant, bucket) it builds a set of random-walk pairs whose DP
 row width lies in the bucket, runs the DP over them once per candidate
@@ -65,14 +65,17 @@ def pair_lengths(v, lo, hi, rng, n):
         C = rng.integers(4 * hi, 8 * hi + 1, n)
         R = (C * rng.uniform(1.0, 1.5, n)).astype(np.int64)
     else:
-        C = rng.integers(lo, hi + 1, n)
-        R = (C * rng.uniform(1.0, 2.0, n)).astype(np.int64)
+        # widths log-uniform in the bucket and the longer side 1-4 times the
+        # shorter, log-uniform: the shape of pairs drawn from log-uniform
+        # length distributions, where short widths dominate a bucket
+        C = np.clip(np.exp(rng.uniform(np.log(lo), np.log(hi + 1), n)).astype(np.int64), lo, hi)
+        R = (C * np.exp(rng.uniform(0.0, np.log(4.0), n))).astype(np.int64)
     swap = rng.random(n) < 0.5
     a, b = np.where(swap, C, R), np.where(swap, R, C)
     return a.astype(np.int32), b.astype(np.int32)
 
 
-def tune(quick=False, log=print):
+def tune(quick=False, log=print, only=None):
     ts.init(0)
     shapes = ts.dp_shapes()
     rng = np.random.default_rng(0x7E5)
@@ -82,6 +85,8 @@ def tune(quick=False, log=print):
     variants = ts.DP_VARIANTS
     waves = [8] if quick else WAVES
     for vi, v in enumerate(variants):
+        if only and v not in only:
+            continue
         for b, (lo, hi) in enumerate(BUCKET_WIDTHS):
             width = (lo + hi) // 2
             p = variant_params(v, width)
@@ -169,18 +174,52 @@ def dp_width(v, p, n, m):
     return c
 
 
+def line_key(line):
+    """(variant, bucket, table) of a table line, None for comments."""
+    body = line.split("#")[0].split()
+    if len(body) < 5:
+        return None
+    return body[0], int(body[1]), body[5] if len(body) > 5 else ""
+
+
+def merge(old_lines, new_lines):
+    """The old table with the new lines in place of the lines of the same
+    (variant, bucket, table); an old line whose comment says it was set by
+    hand stays."""
+    new = {line_key(l): l for l in new_lines}
+    out, seen = [], set()
+    for l in old_lines:
+        k = line_key(l)
+        if k in new and "set by hand" not in l:
+            out.append(new[k])
+            seen.add(k)
+        else:
+            out.append(l)
+            seen.add(k)
+    out += [l for k, l in new.items() if k not in seen]
+    return out
+
+
 def main():
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "dp_tuning.txt")
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "dp_tuning.txt"))
+    ap.add_argument("--out", default=here)
     ap.add_argument("--quick", action="store_true", help="waves per CU fixed at 8")
+    ap.add_argument("--variants", default="", help="comma-separated variants to tune; the table beside the library "
+                                                   "keeps its other lines")
     args = ap.parse_args()
+    only = [v for v in args.variants.split(",") if v]
     t0 = time.time()
-    lines, _ = tune(args.quick, log=lambda *a: print(*a, file=sys.stderr, flush=True))
+    lines, _ = tune(args.quick, log=lambda *a: print(*a, file=sys.stderr, flush=True), only=only)
+    head = ["# DP tuning table written by cudaruntime1_amd/tune.py on %s (%.0f s%s)" %
+            (ts.device_name().strip(), time.time() - t0, ", variants " + ",".join(only) if only else ""),
+            "# variant bucket lanes cells waves_per_cu"]
+    if only and os.path.exists(here):
+        old = [l.rstrip("\n") for l in open(here) if l.strip()]
+        lines = merge([l for l in old if not l.startswith("# variant")], lines)
+        head = head[:1] + ["# variant bucket lanes cells waves_per_cu"]
     with open(args.out, "w") as f:
-        f.write("# DP tuning table written by cudaruntime1_amd/tune.py on %s (%.0f s)\n" %
-                (ts.device_name().strip(), time.time() - t0))
-        f.write("# variant bucket lanes cells waves_per_cu\n")
-        f.write("\n".join(lines) + "\n")
+        f.write("\n".join(head + lines) + "\n")
     print("wrote", args.out)
 
 

@@ -7,7 +7,7 @@
  * host, refine on the GPU and return the best value per query. Here the
  * database and the query batch are uploaded once into device-resident sets
  * and a k-NN call runs filter -> candidate selection -> exact refine -> top-k
- * on the device. The 13 reference signatures themselves are re-exported with
+ * on the device. The 12 reference signatures themselves are re-exported with
  * C++ linkage by trajsim_compat.hpp.
  *
  * Conventions: every call returns ts_status (TS_OK == 0); a failing call
@@ -158,8 +158,9 @@ ts_status ts_engine_set_timing(ts_engine* e, int32_t on);
  * Recorded EXACT batches size their candidate lists on the device within a
  * capacity; a call whose candidates exceed it grows the capacity and runs
  * again (ts_stats.overflow_retries). Results are identical either way.
- * Batches of more than 2^26 (query, DB) pairs always launch eagerly: there
- * launch overhead is negligible and replays measured slower. */
+ * With on == 1, batches of more than 2^26 (query, DB) pairs launch eagerly
+ * (there launch overhead is a negligible share); on == 2 records and
+ * replays batches of every size. */
 ts_status ts_engine_set_graphs(ts_engine* e, int32_t on);
 /* Graphs recorded and replayed so far, and graphs currently cached. */
 ts_status ts_engine_graph_info(ts_engine* e, int64_t* captures, int64_t* replays, int64_t* cached);

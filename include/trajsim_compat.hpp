@@ -1,4 +1,4 @@
-// trajsim_compat.hpp -- the reference's 13 entry points, re-exported with
+// trajsim_compat.hpp -- the reference's 12 entry points, re-exported with
 // their original C++ signatures so a harness written against
 // 1li2yang3/CudaRuntime1 (main.cu) relinks against libtrajsim.so unchanged.
 //

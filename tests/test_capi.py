@@ -1,5 +1,5 @@
 """CPU-side checks of the C ABI: the library loads, exports every function
-that include/trajsim.h declares and the 13 reference signatures of
+that include/trajsim.h declares and the 12 reference signatures of
 include/trajsim_compat.hpp, and the synthetic generator is deterministic.
 No compute call touches a GPU here."""
 import ctypes
@@ -72,3 +72,26 @@ def test_init_reports_status():
     # no GPU in the build container: a clean status, never a crash
     rc = ts._lib.ts_init(0)
     assert rc in (0, 5)
+
+
+def test_committed_tuning_table_is_well_formed():
+    """Every line of the DP tuning table beside the library parses as the
+    engine's loader reads it: "<variant> <bucket> <lanes> <cells>
+    <waves_per_cu> [lat|mid]" with a known variant, bucket and shape, waves
+    per CU in 1..64, '#' comments, lines shorter than the loader's buffer."""
+    path = os.path.join(os.path.dirname(ts.LIB_PATH), "dp_tuning.txt")
+    shapes = {(16, 1), (16, 2), (16, 4), (32, 2), (32, 4), (64, 2), (64, 4), (32, 8), (64, 8)}
+    seen = set()
+    for n, line in enumerate(open(path), 1):
+        assert len(line) < 1024, n
+        body = line.split("#")[0].split()
+        if not body:
+            continue
+        assert len(body) in (5, 6), (n, line)
+        v, b, W, U, g = body[0], int(body[1]), int(body[2]), int(body[3]), int(body[4])
+        assert v in ts.DP_VARIANTS and 0 <= b < len(ts.DP_BUCKET_CAPS), (n, line)
+        assert (W, U) in shapes and 1 <= g <= 64, (n, line)
+        table = body[5] if len(body) == 6 else ""
+        assert table in ("", "lat", "mid"), (n, line)
+        seen.add((v, b, table))
+    assert len(seen) == len(ts.DP_VARIANTS) * len(ts.DP_BUCKET_CAPS) * 3
