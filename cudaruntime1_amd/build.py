@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libtrajsim.so")
 OBJ = os.path.join(HERE, "_build")
 
-SOURCES = ["set.hip", "filter.hip", "select.hip", "dp.hip", "reduce.hip", "engine.hip", "server.hip",
+SOURCES = ["set.hip", "filter.hip", "select.hip", "dp.hip", "dp_mw.hip", "reduce.hip", "engine.hip", "server.hip",
            "compat.cpp", "dbfile.hip", "noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TS_OFFLOAD_ARCH", "gfx950")

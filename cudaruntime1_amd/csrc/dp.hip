@@ -1007,6 +1007,12 @@ __global__ void __launch_bounds__(256) k_dp_lean_r(DpArgs a) {
 // before the strip starts.
 DpHalo dp_halo(int shape, int b, const MeasureArgs& m, int32_t cmax_c, int32_t wmax_c) {
   const DpShape& k = kDpShapeList[shape];
+  if (k.W > 64) {
+    // bucket b bounds the shorter length of unbanded pairs, and the visited
+    // width of banded and windowed ones (whose shorter length it does not bound)
+    const int32_t c = dp_ranged(m) ? cmax_c : std::min<int32_t>(cmax_c, kDpBucketCap[b]);
+    return DpHalo{dp_mw_hcap(k.U, m, c, std::min<int32_t>(wmax_c, kDpBucketCap[b])), -1};
+  }
   if (!dp_ranged(m)) {
     const int32_t c = std::min<int32_t>(cmax_c, kDpBucketCap[b]);
     const int32_t chunks = std::max((c + k.U - 1) / k.U, k.W + 1);
@@ -1061,7 +1067,8 @@ static hipError_t launch_dp_t(hipStream_t st, const DpArgs& a, int grid_waves) {
 // Whether a launch of `shape` with this halo fits the 64 KiB of LDS a block
 // may use (one wave per block at least; staging is only chosen when it fits
 // in 20 KiB per wave).
-bool dp_fits(int shape, DpHalo halo) {
+bool dp_fits(int shape, DpHalo halo, int32_t rmax) {
+  if (kDpShapeList[shape].W > 64) return dp_mw_lds_bytes(rmax, halo.len) <= (size_t)dp_mw_lds_max();
   const int G = 64 / kDpShapeList[shape].W;
   return (size_t)G * dp_group_floats(halo.len) * sizeof(float) + G * sizeof(int) + 32 + 64 <= 65536;
 }
@@ -1116,6 +1123,9 @@ hipError_t launch_dp(hipStream_t st, MeasureArgs m, int shape, SetView q, SetVie
   // column buffer (ranged): the longest shorter trajectory, likewise
   const int32_t rmax = ((std::max(rmax_c, 1) + 3) / 4) * 4;
   const int32_t ccap = ((std::max(ccap_c, 1) + 3) / 4) * 4;
+  if (kDpShapeList[shape].W > 64)
+    return launch_dp_mw(st, m, kDpShapeList[shape].W / 64, kDpShapeList[shape].U, q, db, items, nitems, ntail,
+                        max_items, next, out, halo.len, rmax_c, grid_waves);
   DpArgs a{q, db, items, nitems, ntail, max_items, next, out, halo.len, halo.mask, rmax, ccap, 0, m.band, m.delta,
            m.eps};
   switch (m.measure) {

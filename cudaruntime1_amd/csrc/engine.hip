@@ -377,9 +377,9 @@ ts_status resolve_timing(ts_engine* e) {
 
 // Halo of the tuned shape; a shape whose halo does not fit a block's LDS
 // (a long band on a narrow group) falls back to the bucket's default shape.
-DpHalo shape_halo(ts_engine* e, int& sh, int b, const MeasureArgs& ma, int32_t hi, int32_t whi) {
+DpHalo shape_halo(ts_engine* e, int& sh, int b, const MeasureArgs& ma, int32_t hi, int32_t whi, int32_t rmax) {
   DpHalo h = dp_halo(sh, b, ma, hi, whi);
-  if (!dp_fits(sh, h) && sh != kDpDefaultShape[b]) {
+  if (!dp_fits(sh, h, rmax) && sh != kDpDefaultShape[b]) {
     sh = kDpDefaultShape[b];
     h = dp_halo(sh, b, ma, hi, whi);
     ++e->dp_fallbacks;
@@ -529,7 +529,7 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   const DpTune* tune = e->tune[dp_table_of(max_items)][dp_variant(p)];
   if (b_lo == b_hi) {
     int sh = tune[b_lo].shape;
-    const DpHalo halo = shape_halo(e, sh, b_lo, ma, hi, whi);
+    const DpHalo halo = shape_halo(e, sh, b_lo, ma, hi, whi, rmax);
     // launch labels name the per-kernel statistics, kept only while timing
     const std::string label = e->timing ? dp_label(nm, b_lo, sh) : std::string();
     return run(e, e->timing ? label.c_str() : nm, 0, 0, [&] {
@@ -542,7 +542,7 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
   TS_HIP(hipEventRecord(e->fork, e->st));
   for (int b = b_lo; b <= b_hi; ++b) {
     int sh = tune[b].shape;
-    const DpHalo halo = shape_halo(e, sh, b, ma, hi, whi);
+    const DpHalo halo = shape_halo(e, sh, b, ma, hi, whi, rmax);
     const std::string label = e->timing ? dp_label(nm, b, sh) : std::string();
     TS_HIP(hipStreamWaitEvent(e->aux[b], e->fork, 0));
     TS_TRY(run_on(e, e->aux[b], e->timing ? label.c_str() : nm, 0, 0, [&] {

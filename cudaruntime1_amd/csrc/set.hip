@@ -424,6 +424,8 @@ ts_status ts_init(int32_t device) {
   TS_HIP(hipFree(nullptr));
   // the allocation-free entry point's scratch array on this device
   TS_TRY(noalloc_setup(device));
+  // the multi-wave DP's LDS limit (two halo rows of the longest trajectories)
+  TS_TRY(dp_mw_setup(device));
   g_device = device;
   return TS_OK;
 }

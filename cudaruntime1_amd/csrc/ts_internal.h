@@ -335,12 +335,13 @@ constexpr int kDpBuckets = 6;
 struct BucketCells {
   int64_t t[kDpBuckets];
 };
-constexpr int kDpShapes = 9;
+// W > 64: a pair spans W / 64 waves of a block (dp_mw.hip).
+constexpr int kDpShapes = 13;
 struct DpShape {
   int W, U;
 };
-constexpr DpShape kDpShapeList[kDpShapes] = {{16, 1}, {16, 2}, {16, 4}, {32, 2}, {32, 4},
-                                            {64, 2}, {64, 4}, {32, 8}, {64, 8}};
+constexpr DpShape kDpShapeList[kDpShapes] = {{16, 1}, {16, 2}, {16, 4}, {32, 2}, {32, 4}, {64, 2}, {64, 4},
+                                            {32, 8}, {64, 8}, {128, 4}, {128, 8}, {256, 4}, {256, 8}};
 // Largest row width of each bucket.
 constexpr int32_t kDpBucketCap[kDpBuckets] = {31, 63, 127, 255, 1023, 1 << 24};
 // Shape used when no tuning table says otherwise.
@@ -356,10 +357,24 @@ struct DpHalo {
   int32_t len, mask;
 };
 DpHalo dp_halo(int shape, int b, const MeasureArgs& m, int32_t cmax_c, int32_t wmax_c);
-bool dp_fits(int shape, DpHalo halo);
+// Whether a launch of `shape` with this halo fits a block's LDS (multi-wave
+// shapes: with the strip table of pairs whose longer length is at most rmax).
+bool dp_fits(int shape, DpHalo halo, int32_t rmax);
 hipError_t launch_dp(hipStream_t st, MeasureArgs m, int shape, SetView q, SetView db, const Item* items,
                      const int32_t* nitems, const int32_t* ntail, int64_t max_items, int32_t* next, float* out,
                      DpHalo halo, int32_t rmax_c, int32_t ccap_c, int grid_waves);
+// dp_mw.hip -- the multi-wave DP (nw waves per pair, u cells per lane step):
+// halo floats per buffer and LDS bytes of a launch, and the launch itself
+// (hcap from dp_mw_hcap, a grid of grid_waves waves).
+int32_t dp_mw_hcap(int u, const MeasureArgs& m, int32_t cmax_c, int32_t wmax_c);
+size_t dp_mw_lds_bytes(int32_t rmax_c, int32_t hcap);
+// Lets the multi-wave kernels take the device's LDS per block (up to 160 KiB
+// on gfx950); called by ts_init for each device. dp_mw_lds_max: the limit.
+ts_status dp_mw_setup(int device);
+int32_t dp_mw_lds_max();
+hipError_t launch_dp_mw(hipStream_t st, MeasureArgs m, int nw, int u, SetView q, SetView db, const Item* items,
+                        const int32_t* nitems, const int32_t* ntail, int64_t max_items, int32_t* next, float* out,
+                        int32_t hcap, int32_t rmax_c, int grid_waves);
 
 // reduce.hip -- Hausdorff (wave per pair) and lock-step Euclidean (lane per pair).
 hipError_t launch_hausdorff(hipStream_t st, SetView q, SetView db, const Item* items, const int32_t* nitems,

@@ -80,7 +80,8 @@ def test_committed_tuning_table_is_well_formed():
     <waves_per_cu> [lat|mid]" with a known variant, bucket and shape, waves
     per CU in 1..64, '#' comments, lines shorter than the loader's buffer."""
     path = os.path.join(os.path.dirname(ts.LIB_PATH), "dp_tuning.txt")
-    shapes = {(16, 1), (16, 2), (16, 4), (32, 2), (32, 4), (64, 2), (64, 4), (32, 8), (64, 8)}
+    shapes = {(16, 1), (16, 2), (16, 4), (32, 2), (32, 4), (64, 2), (64, 4), (32, 8), (64, 8), (128, 4), (128, 8),
+              (256, 4), (256, 8)}
     seen = set()
     for n, line in enumerate(open(path), 1):
         assert len(line) < 1024, n

@@ -38,6 +38,24 @@ def test_lean_ranged_dp_schedule_model_matches_oracle(tmp_path):
     assert r.returncode != 0, r.stdout[-2000:]
 
 
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_multi_wave_dp_schedule_model_matches_oracle(tmp_path):
+    """k_dp_mw (a pair over 2 or 4 waves, barriers every 8 steps, two LDS
+    halo buffers): every halo read finds the right strip and chunk written
+    before the barrier that precedes it, with the waves of a superstep run
+    in either order; values equal the oracle's bit for bit. A lag of 70
+    steps between strips (the kernel uses 72, the least safe is 71) is
+    caught."""
+    exe = str(tmp_path / "dp_mw_schedule")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fopenmp", "-o", exe,
+                    os.path.join(HERE, "emu", "dp_mw_schedule.cpp"), os.path.join(ROOT, "oracle", "oracle.cpp")],
+                   check=True, timeout=300)
+    r = subprocess.run([exe, "12"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+    r = subprocess.run([exe, "3", "70"], capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0, r.stdout[-2000:]
+
+
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
 def test_sqrt_neighbour_fixup_is_correctly_rounded(tmp_path):
     exe = str(tmp_path / "sqrt_fixup")
