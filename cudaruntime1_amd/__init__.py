@@ -307,7 +307,8 @@ class Engine:
 
     def set_dp_shape(self, variant, bucket, lanes, cells, waves_per_cu, table=2):
         """Force the DP launch shape of one (variant, bucket); see DP_VARIANTS.
-        table 0: throughput, 1: small refines, 3: mid-size refines, 2: all."""
+        table 0: throughput, 1: small refines, 3: mid-size refines, 4: refines
+        of at most 256 pairs, 2: all."""
         _check(_lib.ts_engine_set_dp_shape(self._h, table, variant, bucket, lanes, cells, waves_per_cu),
                "ts_engine_set_dp_shape")
 

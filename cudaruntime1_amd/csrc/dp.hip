@@ -489,53 +489,32 @@ __device__ __forceinline__ int next_event(int rem, int lg, int res_rem) {
 }
 
 // One step of k_dp_lean's loop for every lane of the wave; TINY: whether
-// the square roots test for inputs below 2^-96 (uniform).
+// the square roots test for inputs below 2^-96 (uniform). The column chunk
+// and row point only feed the cells' point terms (squared distances, LCSS
+// matches), which come first; the next step's chunk and row point are then
+// read into the same registers, so the reads have the rest of the step (the
+// square roots, the recurrence, the halo traffic) to arrive in. Scheduling
+// barriers keep the reads below the point terms and above the rest.
 #define TS_LEAN_STEP(TINY)                                                                          \
   do {                                                                                              \
     if (wrapped) {                                                                                  \
       ax = rp.x;                                                                                    \
       ay = rp.y;                                                                                    \
     }                                                                                               \
-                                                                                                    \
-    /* ---- U cells */                                                                              \
-    float up[U], on[U];                                                                             \
-    _Pragma("unroll")                                                                               \
-    for (int j = 0; j < U; ++j) up[j] = from_prev_lane<W>(o[j], hv[j], lg);                         \
-    const bool first = k == 0;                                                                      \
-    float d = first ? dfirst : up_last;                                                             \
-    float l = first ? BND : o[U - 1];                                                               \
-    float dist[U];                                                                                  \
-    if (MEAS == TS_DTW) {                                                                           \
-    _Pragma("unroll")                                                                               \
-      for (int j = 0; j < U; ++j) {                                                                 \
-        const float dx = ax - cx[j], dy = ay - cy[j];                                               \
-        dist[j] = dx * dx + dy * dy;                                                                \
-      }                                                                                             \
-      sqrt_cr<U>(dist, (TINY));                                                                     \
-    }                                                                                               \
+    /* ---- point terms */                                                                          \
+    float pt[U];                                                                                    \
     _Pragma("unroll")                                                                               \
     for (int j = 0; j < U; ++j) {                                                                   \
       const float dx = ax - cx[j], dy = ay - cy[j];                                                 \
-      float v;                                                                                      \
-      if (MEAS == TS_DTW) {                                                                         \
-        v = dist[j] + fminf(fminf(up[j], l), d);                                                    \
-      } else if (MEAS == TS_FRECHET) {                                                              \
-        v = fmaxf(dx * dx + dy * dy, fminf(fminf(up[j], l), d));                                    \
+      if (MEAS == TS_LCSS) {                                                                        \
+        pt[j] = (fabsf(dx) < a.eps && fabsf(dy) < a.eps) ? 1.0f : 0.0f;                             \
       } else {                                                                                      \
-        const bool match = fabsf(dx) < a.eps && fabsf(dy) < a.eps;                                  \
-        v = match ? d + 1.0f : fmaxf(up[j], l);                                                     \
+        pt[j] = dx * dx + dy * dy;                                                                  \
       }                                                                                             \
-      on[j] = v;                                                                                    \
-      d = up[j];                                                                                    \
-      l = v;                                                                                        \
     }                                                                                               \
-    up_last = up[U - 1];                                                                            \
-    _Pragma("unroll")                                                                               \
-    for (int j = 0; j < U; ++j) o[j] = on[j];                                                       \
-                                                                                                    \
-    /* ---- operands of the next step, read into the registers the cells */                         \
-    /* are done with; the halo chunk (see above for its timing) before this */                      \
-    /* step's halo store */                                                                         \
+    /* ---- the next step's chunk, row and operands (idle chunks re-read */                         \
+    /* the last one: sets pad trajectories to 8 points) */                                          \
+    __builtin_amdgcn_sched_barrier(0);                                                              \
     const int kh = k;                                                                               \
     wrapped = k + 1 == D;                                                                           \
     k = wrapped ? 0 : k + 1;                                                                        \
@@ -548,10 +527,39 @@ __device__ __forceinline__ int next_event(int rem, int lg, int res_rem) {
     if (SC) {                                                                                       \
       load_pts<U>(cb + k * U, cx, cy);                                                              \
     } else {                                                                                        \
-      /* idle chunks re-read the last one (sets pad trajectories to 8 points) */                    \
       load_vec<U>(Bx + U * min(k, lastch), cx);                                                     \
       load_vec<U>(By + U * min(k, lastch), cy);                                                     \
     }                                                                                               \
+    __builtin_amdgcn_sched_barrier(0);                                                              \
+                                                                                                    \
+    /* ---- U cells */                                                                              \
+    float up[U], on[U];                                                                             \
+    _Pragma("unroll")                                                                               \
+    for (int j = 0; j < U; ++j) up[j] = from_prev_lane<W>(o[j], hv[j], lg);                         \
+    const bool first = kh == 0;                                                                     \
+    float d = first ? dfirst : up_last;                                                             \
+    float l = first ? BND : o[U - 1];                                                               \
+    if (MEAS == TS_DTW) sqrt_cr<U>(pt, (TINY));                                                     \
+    _Pragma("unroll")                                                                               \
+    for (int j = 0; j < U; ++j) {                                                                   \
+      float v;                                                                                      \
+      if (MEAS == TS_DTW) {                                                                         \
+        v = pt[j] + fminf(fminf(up[j], l), d);                                                      \
+      } else if (MEAS == TS_FRECHET) {                                                              \
+        v = fmaxf(pt[j], fminf(fminf(up[j], l), d));                                                \
+      } else {                                                                                      \
+        v = pt[j] != 0.0f ? d + 1.0f : fmaxf(up[j], l);                                             \
+      }                                                                                             \
+      on[j] = v;                                                                                    \
+      d = up[j];                                                                                    \
+      l = v;                                                                                        \
+    }                                                                                               \
+    up_last = up[U - 1];                                                                            \
+    _Pragma("unroll")                                                                               \
+    for (int j = 0; j < U; ++j) o[j] = on[j];                                                       \
+                                                                                                    \
+    /* ---- the next step's halo chunk (see above for its timing), before */                        \
+    /* this step's halo store */                                                                    \
     const bool top = row == 0;                                                                      \
     load_vec<U>(top ? brow : bnd + k * U, hv);                                                      \
     dfirst = (MEAS == TS_LCSS || top) ? 0.0f : BND;                                                 \
@@ -565,8 +573,10 @@ __device__ __forceinline__ int next_event(int rem, int lg, int res_rem) {
 // more waves per CU): the loop runs the steps between events in two copies,
 // with and without the tiny test, instead of testing the flag every step;
 // that is faster at those occupancies and slower below them.
+// Register budget: 128 VGPRs, so 4 waves fit a SIMD (launches of up to 16
+// waves per CU).
 template <int MEAS, int W, int U, bool SC, bool SR, bool SPLIT>
-__global__ void __launch_bounds__(256) k_dp_lean(DpArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_dp_lean(DpArgs a) {
   constexpr int G = 64 / W;
   constexpr float BND = (MEAS == TS_LCSS) ? 0.0f : kInf;
   bool tiny = false;  // a lane's pair has a trajectory with a tiny coordinate (uniform)

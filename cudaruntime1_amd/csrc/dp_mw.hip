@@ -87,13 +87,36 @@ __device__ __forceinline__ void mw_range(int s, int R, int C, int wb, int delta,
   chi = hi / U;
 }
 
+// The U column points of chunk kc: one or two aligned 16-byte loads per
+// coordinate (sets pad trajectories with their last point to 8).
+template <int U>
+__device__ __forceinline__ void load_chunk(const float* Bx, const float* By, int kc, float (&cx)[U], float (&cy)[U]) {
+#pragma unroll
+  for (int j = 0; j < U; j += 4) {
+    const float4 x4 = *(const float4*)(Bx + kc * U + j), y4 = *(const float4*)(By + kc * U + j);
+    cx[j] = x4.x; cx[j + 1] = x4.y; cx[j + 2] = x4.z; cx[j + 3] = x4.w;
+    cy[j] = y4.x; cy[j + 1] = y4.y; cy[j + 2] = y4.z; cy[j + 3] = y4.w;
+  }
+}
+
+// U consecutive floats from LDS (16-byte aligned).
+template <int U>
+__device__ __forceinline__ void load_lds(const float* p, float (&v)[U]) {
+#pragma unroll
+  for (int j = 0; j < U; j += 4) {
+    const float4 t = *(const float4*)(p + j);
+    v[j] = t.x; v[j + 1] = t.y; v[j + 2] = t.z; v[j + 3] = t.w;
+  }
+}
+
 __device__ __forceinline__ int mw_band_half_width(int band, int R, int C) {
   const int w = band > 0 ? band : (-band * R + 99) / 100;
   return min(w, C);
 }
 
-// LDS: [0, 64) bytes header (item index, step total), the strip table,
-// then the two halo buffers.
+// LDS: [0, 64) bytes header (item index, step total; bytes 32..63 a
+// border chunk of 8 values outside the table), the strip table, then the
+// two halo buffers.
 __host__ __device__ constexpr size_t mw_lds_bytes(int smax, int hcap) {
   return 64 + (size_t)smax * sizeof(MwStrip) + 2 * (size_t)hcap * sizeof(float);
 }
@@ -105,6 +128,7 @@ __global__ void __launch_bounds__(256) k_dp_mw(MwArgs a) {
   const int wv = threadIdx.x >> 6;
   extern __shared__ float4 lds_mw4[];
   int* hdr = (int*)lds_mw4;                       // [0] item, [1] step total
+  float* brow = (float*)lds_mw4 + 8;              // [8] the border value
   MwStrip* tbl = (MwStrip*)((char*)lds_mw4 + 64);
   float* halo = (float*)(tbl + a.smax);           // [2][hcap]
   const int hcap = a.hcap, delta = a.delta;
@@ -114,6 +138,7 @@ __global__ void __launch_bounds__(256) k_dp_mw(MwArgs a) {
   const int64_t n_head = n_items;
   if (a.ntail) n_items = min<int64_t>(a.max_items, n_head + (int64_t)*a.ntail);
 
+  if (threadIdx.x < 8) brow[threadIdx.x] = BND;
   for (;;) {
     __syncthreads();  // the previous pair's table and halo are no longer read
     if (threadIdx.x == 0) hdr[0] = atomicAdd(a.next, 1);
@@ -159,15 +184,25 @@ __global__ void __launch_bounds__(256) k_dp_mw(MwArgs a) {
     __syncthreads();
     const int total = hdr[1];
 
-    // ---- per-lane state: the strip this lane is in (or will be in next)
+    // ---- per-lane state: the strip this lane is in (or will be in next),
+    // its chunk at the current step, its row point and the chunk's points
     int s = wv;
     MwStrip st = s < S ? tbl[s] : MwStrip{INT32_MAX / 2, 0, 0, 0};
+    // the previous strip's chunk range (where this strip's halo comes from)
+    int pcs = 0, pce = 0;
+    if (s > 0 && s < S) {
+      pcs = tbl[s - 1].cs;
+      pce = tbl[s - 1].ce;
+    }
     int row = s * 64 + lane;
     float ax = 0.0f, ay = 0.0f;
     if (s < S) {
       ax = Ax[min(row, R - 1)];
       ay = Ay[min(row, R - 1)];
     }
+    int k = st.cs - st.t0 - lane;  // step 0 (at most cs: no switch yet)
+    float cx[U], cy[U];
+    load_chunk<U>(Bx, By, min(max(k, 0), lastch), cx, cy);
     float o[U], hv[U];
 #pragma unroll
     for (int j = 0; j < U; ++j) { o[j] = BND; hv[j] = BND; }
@@ -177,8 +212,31 @@ __global__ void __launch_bounds__(256) k_dp_mw(MwArgs a) {
     // step before the strip starts (T_s >= kMwLag > 0).
 
     for (int t = 0; t < total; ++t) {
-      // ---- this lane's chunk at step t
-      int k = t - st.t0 - lane + st.cs;
+      // ---- this step: strip, chunk, row (the next step's replace them below)
+      const int sc = s, kk = k, rowc = row, cs_c = st.cs;
+      const bool active = sc < S && kk >= cs_c && kk < st.ce;
+      const int kc = min(max(kk, 0), lastch);  // idle lanes read a valid chunk
+      // ---- point terms: squared distances (DTW, Frechet) or matches (LCSS)
+      float pt[U];
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const float dx = ax - cx[j], dy = ay - cy[j];
+        if (MEAS == TS_LCSS) {
+          pt[j] = (fabsf(dx) < a.eps && fabsf(dy) < a.eps) ? 1.0f : 0.0f;
+        } else {
+          pt[j] = dx * dx + dy * dy;
+        }
+      }
+      // ---- the next step's strip, chunk and operands, read now into the
+      // registers the point terms are done with, so the reads have the rest
+      // of the step to arrive in. The empty asm takes the point terms and
+      // orders memory: no compiler pass moves their arithmetic below the
+      // reads (which would keep both chunks live and copy one into the other
+      // every step), nor the reads above it.
+#pragma unroll
+      for (int j = 0; j < U; ++j) asm volatile("" : : "v"(pt[j]) : "memory");
+      __builtin_amdgcn_sched_barrier(0);
+      k = kk + 1;
       if (k >= st.ce) {
         // past this strip: the wave's next one (at most one switch per step)
         s += NW;
@@ -187,62 +245,43 @@ __global__ void __launch_bounds__(256) k_dp_mw(MwArgs a) {
         if (s < S) {
           ax = Ax[min(row, R - 1)];
           ay = Ay[min(row, R - 1)];
+          pcs = tbl[s - 1].cs;
+          pce = tbl[s - 1].ce;
         }
-        k = t - st.t0 - lane + st.cs;
+        k = t + 1 - st.t0 - lane + st.cs;
       }
-      const bool active = s < S && k >= st.cs && k < st.ce;
-      // the chunk's U column points: one or two aligned 16-byte loads per
-      // coordinate (sets pad trajectories with their last point to 8);
-      // idle lanes read a valid chunk
-      const int kc = min(max(k, 0), lastch);
-      float cx[U], cy[U];
-#pragma unroll
-      for (int j = 0; j < U; j += 4) {
-        const float4 x4 = *(const float4*)(Bx + kc * U + j), y4 = *(const float4*)(By + kc * U + j);
-        cx[j] = x4.x; cx[j + 1] = x4.y; cx[j + 2] = x4.z; cx[j + 3] = x4.w;
-        cy[j] = y4.x; cy[j + 1] = y4.y; cy[j + 2] = y4.z; cy[j + 3] = y4.w;
-      }
+      load_chunk<U>(Bx, By, min(max(k, 0), lastch), cx, cy);
+      __builtin_amdgcn_sched_barrier(0);
       // ---- U cells (as in dp.hip's k_dp_lean_r)
       float up[U], on[U];
 #pragma unroll
       for (int j = 0; j < U; ++j) {
-        const float prev = __int_as_float(
-            __builtin_amdgcn_update_dpp(__float_as_int(hv[j]), __float_as_int(o[j]), 0x138 /* wave_shr:1 */, 0xf,
-                                        0xf, false));
-        up[j] = prev;
+        up[j] = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(hv[j]), __float_as_int(o[j]),
+                                                           0x138 /* wave_shr:1 */, 0xf, 0xf, false));
       }
-      const bool first = k == st.cs;
-      float d = first ? ((row == 0) ? (st.cs == 0 ? 0.0f : BND) : (lane == 0 ? hdiag : BND)) : up_last;
-      if (MEAS == TS_LCSS && first && st.cs == 0) d = 0.0f;
+      const bool first = kk == cs_c;
+      float d = first ? ((rowc == 0) ? (cs_c == 0 ? 0.0f : BND) : (lane == 0 ? hdiag : BND)) : up_last;
+      if (MEAS == TS_LCSS && first && cs_c == 0) d = 0.0f;
       float l = first ? BND : o[U - 1];
-      float dist[U];
-      if (MEAS == TS_DTW) {
-#pragma unroll
-        for (int j = 0; j < U; ++j) {
-          const float dx = ax - cx[j], dy = ay - cy[j];
-          dist[j] = dx * dx + dy * dy;
-        }
-        sqrt_cr<U>(dist, tiny);
-      }
+      if (MEAS == TS_DTW) sqrt_cr<U>(pt, tiny);
       int bd = 0;
-      if (BAND) bd = row * C - kc * U * R;  // kc == k on active lanes
+      if (BAND) bd = rowc * C - kc * U * R;  // kc == kk on active lanes
 #pragma unroll
       for (int j = 0; j < U; ++j) {
-        const float dx = ax - cx[j], dy = ay - cy[j];
         float v;
         if (MEAS == TS_DTW) {
-          v = dist[j] + fminf(fminf(up[j], l), d);
+          v = pt[j] + fminf(fminf(up[j], l), d);
           if (BAND) {
             const int bj = bd - j * R;
             if ((bj < 0 ? -bj : bj) > bw) v = kInf;
           }
         } else if (MEAS == TS_FRECHET) {
-          v = fmaxf(dx * dx + dy * dy, fminf(fminf(up[j], l), d));
+          v = fmaxf(pt[j], fminf(fminf(up[j], l), d));
         } else {
-          bool match = fabsf(dx) < a.eps && fabsf(dy) < a.eps;
+          bool match = pt[j] != 0.0f;
           float uu = up[j], ll = l;
           if (DELTA) {
-            const int dd = row - (kc * U + j);
+            const int dd = rowc - (kc * U + j);
             match = match && (dd < 0 ? -dd : dd) <= delta;
             if (dd - 1 > delta || 1 - dd > delta) uu = d;  // (row-1, col) outside the window
             if (dd + 1 > delta || -1 - dd > delta) ll = d;  // (row, col-1) outside the window
@@ -258,11 +297,11 @@ __global__ void __launch_bounds__(256) k_dp_mw(MwArgs a) {
       for (int j = 0; j < U; ++j) o[j] = on[j];
       // ---- the row between strips, and the pair's result
       if (lane == 63 && active) {
-        float* hb = halo + (s & 1) * hcap + (k - st.cs) * U;
+        float* hb = halo + (sc & 1) * hcap + (kk - cs_c) * U;
 #pragma unroll
         for (int j = 0; j < U; ++j) hb[j] = on[j];
       }
-      if (active && row == res_row && k == lastch) {
+      if (active && rowc == res_row && kk == lastch) {
         const int rc = (C - 1) - lastch * U;
         float v = on[0];
 #pragma unroll
@@ -273,39 +312,18 @@ __global__ void __launch_bounds__(256) k_dp_mw(MwArgs a) {
         a.out[it.slot] = v;
       }
       if ((t + 1) % kMwB == 0) __syncthreads();
-      // ---- lane 0: the halo chunk (and at a strip's start the diagonal)
-      // of step t+1, from the previous strip's buffer
-      if (lane == 0) {
-        int s1 = s, k1 = t + 1 - st.t0 + st.cs;
-        MwStrip st1 = st;
-        if (k1 >= st.ce) {
-          s1 = s + NW;
-          st1 = s1 < S ? tbl[s1] : MwStrip{INT32_MAX / 2, 0, 0, 0};
-          k1 = t + 1 - st1.t0 + st1.cs;
-        }
-        if (s1 < S && k1 >= st1.cs && k1 < st1.ce) {
-          if (s1 == 0) {
-#pragma unroll
-            for (int j = 0; j < U; ++j) hv[j] = BND;  // above row 0: the border
-            hdiag = BND;
-          } else {
-            const MwStrip pv = tbl[s1 - 1];
-            const float* hb = halo + ((s1 - 1) & 1) * hcap;
-            if (k1 < pv.ce) {
-#pragma unroll
-              for (int j = 0; j < U; ++j) hv[j] = hb[(k1 - pv.cs) * U + j];
-            } else {
-#pragma unroll
-              for (int j = 0; j < U; ++j) hv[j] = BND;  // above the previous strip's range: outside the band
-            }
-            if (k1 == st1.cs) {
-              // the diagonal of the strip's first cell: column cs*U-1 of the
-              // row above, when the previous strip computed it
-              const int dc = st1.cs * U - 1;
-              hdiag = (dc >= 0 && dc >= pv.cs * U && dc < pv.ce * U && dc < C) ? hb[dc - pv.cs * U] : BND;
-            }
-          }
-        }
+      // ---- the halo chunk of step t+1 (strip s, chunk k): the previous
+      // strip's row there, else the border (above row 0, or above the
+      // previous strip's range: outside the band). Every lane reads it, so
+      // it dies at the shift above; only lane 0's is used.
+      const bool from_halo = s > 0 && s < S && k >= pcs && k < pce;
+      const float* hb = halo + ((s - 1) & 1) * hcap;
+      load_lds<U>(from_halo ? hb + (k - pcs) * U : brow, hv);
+      // lane 0 at a strip's start: the diagonal of its first cell, column
+      // cs*U-1 of the row above, when the previous strip computed it
+      if (lane == 0 && s < S && k == st.cs) {
+        const int dc = st.cs * U - 1;
+        hdiag = (s > 0 && dc >= 0 && dc >= pcs * U && dc < pce * U && dc < C) ? hb[dc - pcs * U] : BND;
       }
     }
   }

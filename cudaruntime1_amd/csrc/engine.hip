@@ -193,13 +193,19 @@ struct DpTune {
   int waves_per_cu;
 };
 constexpr int kDpVariants = 5;  // DTW, DTW banded, Frechet, LCSS, LCSS windowed
-// DP shape tables by refine size (pairs the refine can hold): small refines
-// (one round of the grid or less: a pair's latency decides), mid-size ones
-// (a few rounds; c2's main refine), and throughput.
+// DP shape tables by refine size (pairs the refine can hold): few pairs (at
+// most one per CU: a CU's waves may share a pair, the multi-wave shapes),
+// small refines (one round of the grid or less: a pair's latency decides),
+// mid-size ones (a few rounds: the main refine of 64 queries against 100k
+// trajectories), and throughput.
+constexpr int64_t kFewPairs = 256;
 constexpr int64_t kLatencyPairs = 4096;
 constexpr int64_t kMidPairs = 65536;
-constexpr int kTables = 3;  // 0: throughput, 1: small, 2: mid-size
-int dp_table_of(int64_t max_items) { return max_items <= kLatencyPairs ? 1 : (max_items <= kMidPairs ? 2 : 0); }
+constexpr int kTables = 4;  // 0: throughput, 1: small, 2: mid-size, 3: few
+int dp_table_of(int64_t max_items) {
+  if (max_items <= kFewPairs) return 3;
+  return max_items <= kLatencyPairs ? 1 : (max_items <= kMidPairs ? 2 : 0);
+}
 
 struct Pending {
   int kid;
@@ -1066,6 +1072,8 @@ namespace {
 
 const char* const kVariantNames[kDpVariants] = {"dtw", "dtw_band", "frechet", "lcss", "lcss_window"};
 
+int shape_of(int W, int U);
+
 void tune_defaults(ts_engine* e) {
   const char* v = std::getenv("TS_DP_WAVES_PER_CU");
   int w = v ? std::atoi(v) : 0;
@@ -1073,7 +1081,14 @@ void tune_defaults(ts_engine* e) {
   for (int m = 0; m < kTables; ++m)
     for (int var = 0; var < kDpVariants; ++var)
       for (int b = 0; b < kDpBuckets; ++b) e->tune[m][var][b] = DpTune{kDpDefaultShape[b], w};
+  // few pairs of rows wider than 1023: four waves share each (one pair of
+  // 8192 x 8192 DTW 60 -> 20 ms); LCSS windows stay on one wave
+  for (int var = 0; var < kDpVariants; ++var)
+    if (var != 4) e->tune[3][var][kDpBuckets - 1] = DpTune{shape_of(256, var == 1 ? 4 : 8), 4};
 }
+
+// The API's code of internal table m (2 means every table there).
+int table_code(int m) { return m == 2 ? 3 : (m == 3 ? 4 : m); }
 
 int shape_of(int W, int U) {
   for (int i = 0; i < kDpShapes; ++i)
@@ -1082,11 +1097,12 @@ int shape_of(int W, int U) {
 }
 
 // Tuning table: text, one line per (variant, bucket[, table]),
-//   <variant> <bucket> <W> <U> <waves_per_cu> [lat|mid]
+//   <variant> <bucket> <W> <U> <waves_per_cu> [lat|mid|few]
 // variant one of dtw, dtw_band, frechet, lcss, lcss_window; a trailing
-// "lat" fills the small-refine table, "mid" the mid-size one, nothing the
-// throughput table (and the other two where no line of theirs came
-// first); '#' comments. Lines not listed keep their current values.
+// "lat" fills the small-refine table, "mid" the mid-size one, "few" the
+// few-pair one, nothing the throughput table (and the others where no line
+// of theirs came first); '#' comments. Lines not listed keep their current
+// values.
 ts_status load_tuning(ts_engine* e, const char* path) {
   FILE* f = std::fopen(path, "r");
   if (!f) return fail(TS_ERR_INVALID, std::string("cannot open tuning table ") + path);
@@ -1108,13 +1124,14 @@ ts_status load_tuning(ts_engine* e, const char* path) {
     const int sh = shape_of(W, U);
     const bool lat = n == 6 && std::strcmp(mode, "lat") == 0;
     const bool mid = n == 6 && std::strcmp(mode, "mid") == 0;
-    if (n < 5 || (n == 6 && !lat && !mid) || var < 0 || b < 0 || b >= kDpBuckets || sh < 0 || waves < 1 ||
-        waves > 64) {
+    const bool few = n == 6 && std::strcmp(mode, "few") == 0;
+    if (n < 5 || (n == 6 && !lat && !mid && !few) || var < 0 || b < 0 || b >= kDpBuckets || sh < 0 ||
+        waves < 1 || waves > 64) {
       st = fail(TS_ERR_INVALID, std::string(path) + ":" + std::to_string(lineno) + ": bad tuning line");
       break;
     }
-    if (lat || mid) {
-      const int t = lat ? 1 : 2;
+    if (lat || mid || few) {
+      const int t = lat ? 1 : (mid ? 2 : 3);
       e->tune[t][var][b] = DpTune{sh, waves};
       set[t][var][b] = true;
     } else {
@@ -1210,21 +1227,21 @@ ts_status ts_engine_set_dp_shape(ts_engine* e, int32_t table, int32_t variant, i
                                  int32_t cells, int32_t waves_per_cu) {
   if (!e) return fail(TS_ERR_INVALID, "null engine");
   const int sh = shape_of(lanes, cells);
-  if (table < 0 || table > 3 || variant < 0 || variant >= kDpVariants || bucket < 0 || bucket >= kDpBuckets ||
+  if (table < 0 || table > 4 || variant < 0 || variant >= kDpVariants || bucket < 0 || bucket >= kDpBuckets ||
       sh < 0 || waves_per_cu < 1 || waves_per_cu > 64)
     return fail(TS_ERR_INVALID, "ts_engine_set_dp_shape arguments");
-  // table 0: throughput, 1: small, 2: every table, 3: mid-size
+  // table 0: throughput, 1: small, 2: every table, 3: mid-size, 4: few
   for (int m = 0; m < kTables; ++m)
-    if (table == 2 || table == (m == 2 ? 3 : m)) e->tune[m][variant][bucket] = DpTune{sh, waves_per_cu};
+    if (table == 2 || table == table_code(m)) e->tune[m][variant][bucket] = DpTune{sh, waves_per_cu};
   return TS_OK;
 }
 
 ts_status ts_engine_get_dp_shape(ts_engine* e, int32_t table, int32_t variant, int32_t bucket, int32_t* lanes,
                                  int32_t* cells, int32_t* waves_per_cu) {
-  if (!e || table < 0 || table > 3 || table == 2 || variant < 0 || variant >= kDpVariants || bucket < 0 ||
+  if (!e || table < 0 || table > 4 || table == 2 || variant < 0 || variant >= kDpVariants || bucket < 0 ||
       bucket >= kDpBuckets)
     return fail(TS_ERR_INVALID, "ts_engine_get_dp_shape arguments");
-  const DpTune& t = e->tune[table == 3 ? 2 : table][variant][bucket];
+  const DpTune& t = e->tune[table == 3 ? 2 : (table == 4 ? 3 : table)][variant][bucket];
   if (lanes) *lanes = kDpShapeList[t.shape].W;
   if (cells) *cells = kDpShapeList[t.shape].U;
   if (waves_per_cu) *waves_per_cu = t.waves_per_cu;

@@ -1,10 +1,11 @@
 """Autotuner for the wavefront DP: per DP variant and length bucket, the
 lane-group shape (lanes per pair, cells per lane step) and the persistent
-grid (waves per CU) with the highest measured cell rate, and for small
-refines (one round of pairs, up to 4096) and mid-size refines (4096 pairs)
-the ones with the lowest time.
+grid (waves per CU) with the highest measured cell rate, and for refines of
+few pairs (at most one per CU: 128 pairs), small refines (one round of
+pairs, up to 4096) and mid-size refines (4096 pairs) the ones with the
+lowest time.
 
-    python -m cudaruntime1_amd.tune [--out PATH] [--quick] [--variants dtw,frechet]
+    python -m cudaruntime1_amd.tune [--out PATH] [--quick] [--variants dtw,frechet] [--few-only]
 
 For every (variant, bucket) it builds a set of random-walk pairs whose DP
 row width lies in the bucket, runs the DP over them once per candidate
@@ -36,6 +37,9 @@ NLAT = 1024
 # pairs of the mid-size measurement (a few rounds of the grid: the main refine
 # of a batch of tens of queries)
 NMID = 4096
+# pairs of the few-pair measurement (at most one per CU: the seeds of one
+# query, a pair-value call on a handful of long pairs)
+NFEW = 128
 
 
 def variant_params(v, width):
@@ -75,7 +79,7 @@ def pair_lengths(v, lo, hi, rng, n):
     return a.astype(np.int32), b.astype(np.int32)
 
 
-def tune(quick=False, log=print, only=None):
+def tune(quick=False, log=print, only=None, few_only=False):
     ts.init(0)
     shapes = ts.dp_shapes()
     rng = np.random.default_rng(0x7E5)
@@ -110,10 +114,12 @@ def tune(quick=False, log=print, only=None):
             default = eng.get_dp_shape(vi, b, 0)
             default_lat = eng.get_dp_shape(vi, b, 1)
             default_mid = eng.get_dp_shape(vi, b, 3)
+            default_few = eng.get_dp_shape(vi, b, 4)
             ref = eng.pair_values(db, qs, p, idx, idx)
             lat_idx = idx[:NLAT]
             mid_idx = idx[:NMID]
-            best, best_lat, best_mid = None, None, None
+            few_idx = idx[:NFEW]
+            best, best_lat, best_mid, best_few = None, None, None, None
 
             def dp_time(sel):
                 eng.reset_stats()
@@ -135,6 +141,11 @@ def tune(quick=False, log=print, only=None):
                     eng.pair_values(db, qs, p, idx[:256], idx[:256])  # warm-up
                     if eng.dp_fallbacks() != f0:
                         break  # this shape's halo does not fit the LDS for these pairs
+                    fms, _ = dp_time(few_idx)  # a few pairs: the longest one's latency decides
+                    if best_few is None or fms < best_few[0]:
+                        best_few = (fms, W, U, g)
+                    if few_only:
+                        continue
                     ms, cells = dp_time(idx)
                     rate = cells / (ms * 1e-3)
                     if best is None or rate > best[0]:
@@ -148,6 +159,13 @@ def tune(quick=False, log=print, only=None):
             eng.set_dp_shape(vi, b, *default, table=0)
             eng.set_dp_shape(vi, b, *default_lat, table=1)
             eng.set_dp_shape(vi, b, *default_mid, table=3)
+            eng.set_dp_shape(vi, b, *default_few, table=4)
+            fms, W, U, g = best_few
+            log("%-12s bucket %d: few pairs (%d): lanes %3d cells %d waves/CU %2d  %.3f ms" %
+                (v, b, NFEW, W, U, g, fms))
+            lines.append("%s %d %d %d %d few  # %d pairs in %.3f ms" % (v, b, W, U, g, NFEW, fms))
+            if few_only:
+                continue
             rate, W, U, g, ms = best
             results[(v, b)] = best
             log("%-12s bucket %d: lanes %2d cells %d waves/CU %2d  %.1f G cells/s (%d pairs, %.3f ms)%s" %
@@ -207,14 +225,20 @@ def main():
     ap.add_argument("--quick", action="store_true", help="waves per CU fixed at 8")
     ap.add_argument("--variants", default="", help="comma-separated variants to tune; the table beside the library "
                                                    "keeps its other lines")
+    ap.add_argument("--few-only", action="store_true", help="tune only the few-pair table; the table beside the "
+                                                           "library keeps its other lines")
     args = ap.parse_args()
     only = [v for v in args.variants.split(",") if v]
     t0 = time.time()
-    lines, _ = tune(args.quick, log=lambda *a: print(*a, file=sys.stderr, flush=True), only=only)
+    lines, _ = tune(args.quick, log=lambda *a: print(*a, file=sys.stderr, flush=True), only=only,
+                    few_only=args.few_only)
+    what = ", variants " + ",".join(only) if only else ""
+    if args.few_only:
+        what += ", few-pair lines"
     head = ["# DP tuning table written by cudaruntime1_amd/tune.py on %s (%.0f s%s)" %
-            (ts.device_name().strip(), time.time() - t0, ", variants " + ",".join(only) if only else ""),
+            (ts.device_name().strip(), time.time() - t0, what),
             "# variant bucket lanes cells waves_per_cu"]
-    if only and os.path.exists(here):
+    if (only or args.few_only) and os.path.exists(here):
         old = [l.rstrip("\n") for l in open(here) if l.strip()]
         lines = merge([l for l in old if not l.startswith("# variant")], lines)
         head = head[:1] + ["# variant bucket lanes cells waves_per_cu"]

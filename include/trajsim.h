@@ -166,22 +166,25 @@ ts_status ts_engine_set_graphs(ts_engine* e, int32_t on);
 ts_status ts_engine_graph_info(ts_engine* e, int64_t* captures, int64_t* replays, int64_t* cached);
 
 /* DP launch shapes. The wavefront DP runs a (query, DB) pair on a group of
- * `lanes` lanes (16, 32 or 64: a quarter, half or whole wave) that updates
- * `cells` DP cells per lane step; pairs are split into 6 buckets by the row
- * width the DP visits (shorter length, or band width): <= 31, 63, 127, 255,
- * 1023, larger. Variants: 0 DTW, 1 DTW with a band, 2 Frechet, 3 LCSS,
- * 4 LCSS with a window. Each (variant, bucket) has a shape and a persistent
- * grid of waves_per_cu waves per CU, in three tables chosen by how many
- * pairs a refine can hold: table 1 up to 4096 (one pair's latency decides),
- * table 3 up to 65536 (a few rounds of the grid), table 0 above (throughput).
- * Engines start from built-in defaults, then read dp_tuning.txt beside the
- * library (or the file TS_DP_TUNING names), written by the autotuner
+ * `lanes` lanes (16, 32 or 64: a quarter, half or whole wave; 128 or 256:
+ * the 2 or 4 waves of a block, the rows between their strips passed in LDS)
+ * that updates `cells` DP cells per lane step; pairs are split into 6
+ * buckets by the row width the DP visits (shorter length, or band width):
+ * <= 31, 63, 127, 255, 1023, larger. Variants: 0 DTW, 1 DTW with a band,
+ * 2 Frechet, 3 LCSS, 4 LCSS with a window. Each (variant, bucket) has a
+ * shape and a persistent grid of waves_per_cu waves per CU, in four tables
+ * chosen by how many pairs a refine can hold: table 4 up to 256 (at most a
+ * pair per CU), table 1 up to 4096 (one pair's latency decides), table 3 up
+ * to 65536 (a few rounds of the grid), table 0 above (throughput). Engines
+ * start from built-in defaults, then read dp_tuning.txt beside the library
+ * (or the file TS_DP_TUNING names), written by the autotuner
  * (cudaruntime1_amd/tune.py): lines "<variant> <bucket> <lanes> <cells>
- * <waves_per_cu> [lat|mid]" with variant one of dtw, dtw_band, frechet,
- * lcss, lcss_window ("lat" lines fill table 1, "mid" lines table 3, others
- * table 0). ts_engine_set_dp_shape sets table 0, 1 or 3, or all of them (2);
- * ts_engine_get_dp_shape reads table 0, 1 or 3. Results never depend on the
- * shape. */
+ * <waves_per_cu> [lat|mid|few]" with variant one of dtw, dtw_band, frechet,
+ * lcss, lcss_window ("lat" lines fill table 1, "mid" lines table 3, "few"
+ * lines table 4, others table 0 and the tables no line of theirs filled
+ * before). ts_engine_set_dp_shape sets table 0, 1, 3 or 4, or all of them
+ * (2); ts_engine_get_dp_shape reads table 0, 1, 3 or 4. Results never
+ * depend on the shape. */
 int32_t ts_dp_shape_count(void);
 ts_status ts_dp_shape(int32_t i, int32_t* lanes, int32_t* cells);
 ts_status ts_engine_load_tuning(ts_engine* e, const char* path);

@@ -77,8 +77,9 @@ def test_init_reports_status():
 def test_committed_tuning_table_is_well_formed():
     """Every line of the DP tuning table beside the library parses as the
     engine's loader reads it: "<variant> <bucket> <lanes> <cells>
-    <waves_per_cu> [lat|mid]" with a known variant, bucket and shape, waves
-    per CU in 1..64, '#' comments, lines shorter than the loader's buffer."""
+    <waves_per_cu> [lat|mid|few]" with a known variant, bucket and shape,
+    waves per CU in 1..64, '#' comments, lines shorter than the loader's
+    buffer; every (variant, bucket) has a line for each of the four tables."""
     path = os.path.join(os.path.dirname(ts.LIB_PATH), "dp_tuning.txt")
     shapes = {(16, 1), (16, 2), (16, 4), (32, 2), (32, 4), (64, 2), (64, 4), (32, 8), (64, 8), (128, 4), (128, 8),
               (256, 4), (256, 8)}
@@ -93,6 +94,6 @@ def test_committed_tuning_table_is_well_formed():
         assert v in ts.DP_VARIANTS and 0 <= b < len(ts.DP_BUCKET_CAPS), (n, line)
         assert (W, U) in shapes and 1 <= g <= 64, (n, line)
         table = body[5] if len(body) == 6 else ""
-        assert table in ("", "lat", "mid"), (n, line)
+        assert table in ("", "lat", "mid", "few"), (n, line)
         seen.add((v, b, table))
-    assert len(seen) == len(ts.DP_VARIANTS) * len(ts.DP_BUCKET_CAPS) * 3
+    assert len(seen) == len(ts.DP_VARIANTS) * len(ts.DP_BUCKET_CAPS) * 4

@@ -461,24 +461,28 @@ def test_compat_reproduces_reference_fixtures(eng):
 
 
 def test_tuning_table_lines_fill_their_tables(tmp_path):
-    """Plain lines fill every table not yet given its own line; "lat" and
-    "mid" lines fill the small- and mid-size tables; bad lines are refused."""
+    """Plain lines fill every table not yet given its own line; "lat", "mid"
+    and "few" lines fill the small-, mid-size and few-pair tables; bad lines
+    are refused."""
     e = ts.Engine()
     try:
         t = tmp_path / "table.txt"
         t.write_text("dtw 3 32 4 8\n"
                      "dtw 2 16 2 6 lat\n"
                      "dtw 2 64 4 12 mid\n"
-                     "dtw 2 32 8 16  # throughput after lat and mid\n")
+                     "dtw 2 128 4 4 few\n"
+                     "dtw 2 32 8 16  # throughput after lat, mid and few\n")
         e.load_tuning(str(t))
-        assert e.get_dp_shape(0, 3, 0) == (32, 4, 8)
-        assert e.get_dp_shape(0, 3, 1) == (32, 4, 8)
-        assert e.get_dp_shape(0, 3, 3) == (32, 4, 8)
+        for table in (0, 1, 3, 4):
+            assert e.get_dp_shape(0, 3, table) == (32, 4, 8)
         assert e.get_dp_shape(0, 2, 0) == (32, 8, 16)
         assert e.get_dp_shape(0, 2, 1) == (16, 2, 6)
         assert e.get_dp_shape(0, 2, 3) == (64, 4, 12)
+        assert e.get_dp_shape(0, 2, 4) == (128, 4, 4)
         e.set_dp_shape(0, 2, 64, 2, 4, table=3)
         assert e.get_dp_shape(0, 2, 3) == (64, 2, 4) and e.get_dp_shape(0, 2, 1) == (16, 2, 6)
+        e.set_dp_shape(0, 2, 256, 8, 4, table=4)
+        assert e.get_dp_shape(0, 2, 4) == (256, 8, 4) and e.get_dp_shape(0, 2, 3) == (64, 2, 4)
         bad = tmp_path / "bad.txt"
         bad.write_text("dtw 2 64 4 12 fast\n")
         with pytest.raises(ts.TrajsimError):
