@@ -643,6 +643,13 @@ ts_status knn_all_batch(ts_engine* e, const ts_params& p, const ts_set* db, cons
   return topk(e, keys, N, N, nullptr, nq, p.k, top);
 }
 
+// DP work scale of an EXACT batch of nq queries (for exact_seeds).
+double exact_work(const ts_set* qs, const ts_set* db, int64_t nq) {
+  const double ql = qs->count ? (double)qs->points / qs->count : 0.0;
+  const double dl = db->count ? (double)db->points / db->count : 0.0;
+  return (double)nq * db->count * ql * dl;
+}
+
 // Largest batch (queries x DB trajectories) whose seed-pass bounds are kept
 // for the compaction (4 bytes a pair).
 constexpr int64_t kStoredBoundsMax = 1ll << 26;
@@ -653,7 +660,7 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   const int32_t N = db->count;
   const int32_t k = p.k;
   const bool desc = desc_measure(p.measure);
-  const int32_t s = std::min<int32_t>(std::max<int32_t>(2 * k, 16), 1024);
+  const int32_t s = exact_seeds(k, exact_work(qs, db, nq));
   const int32_t nch = (N + bound_chunk_items(s) - 1) / bound_chunk_items(s);
   const SetView qv = qs->view(), dv = db->view();
   const MeasureArgs ma = margs(p);
@@ -759,7 +766,8 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   }));
   // 3b. tighter O(n+m) bound on the survivors
   int32_t* nitems2 = counts + nq;
-  TS_TRY(run(e, "tight_filter", 0, 0, [&] {
+  // units: the pairs it tests
+  TS_TRY(run(e, "tight_filter", 0, (double)total, [&] {
     return launch_tight_filter(e->st, ma, k, qv, q0, dv, tau, citems, ntot, total, items2, nitems2, rvals,
                                device_cu_count() * 16);
   }));
@@ -906,7 +914,7 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
   } else if (p.mode == TS_MODE_REF) {
     bq = (int32_t)std::max<int64_t>(1, std::min<int64_t>(Q, (int64_t)(256ll << 20) / (8ll * std::max(1, N))));
   } else {
-    int64_t s = std::min<int64_t>(std::max<int64_t>(2 * k, 16), 1024);
+    int64_t s = exact_seeds(k, exact_work(qs, db, Q));
     int64_t nch = (N + bound_chunk_items((int32_t)s) - 1) / bound_chunk_items((int32_t)s);
     bq = (int32_t)std::max<int64_t>(1, std::min<int64_t>({(int64_t)Q, 4096, (int64_t)(512ll << 20) / (8 * nch * s)}));
   }

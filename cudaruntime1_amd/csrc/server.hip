@@ -80,9 +80,8 @@ int64_t estimate_bytes(const ts_server* s, int64_t sum_len, int64_t n, const ts_
   if (p.mode == TS_MODE_REF) {
     per += N * 16 + (int64_t)p.kc * 64;
   } else {
-    const int64_t sd = std::max<int64_t>(2 * (int64_t)k, 16);
-    const int64_t nch = (N + ts::bound_chunk_items((int32_t)std::min<int64_t>(sd, 1024)) - 1) /
-                        ts::bound_chunk_items((int32_t)std::min<int64_t>(sd, 1024));
+    const int64_t sd = ts::exact_seeds(k, 1e300);  // the most seeds a batch may take
+    const int64_t nch = (N + ts::bound_chunk_items((int32_t)sd) - 1) / ts::bound_chunk_items((int32_t)sd);
     per += nch * sd * 8 * 2 + std::min<int64_t>(N, 4096) * 48 + sd * 64;
   }
   return sum_len * 16 + n * per;
