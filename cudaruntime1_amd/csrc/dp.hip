@@ -64,6 +64,13 @@ struct DpArgs {
   int32_t band;
   int32_t delta;
   float eps;
+  // Early abandoning (k_dp_lean, DTW and Frechet): a pair stops once a
+  // finished row's minimum exceeds its query's threshold tau[q - tau_q0]
+  // (null: never); its value slot then holds NaN and `cells` (the launch's
+  // cell counter) loses the cells of the rows it did not finish.
+  const float* tau;
+  int32_t tau_q0;
+  unsigned long long* cells;
 };
 
 // v of the previous lane of the group; the group's lane 0 gets first0.
@@ -515,7 +522,7 @@ __device__ __forceinline__ int next_event(int rem, int lg, int res_rem) {
     /* ---- the next step's chunk, row and operands (idle chunks re-read */                         \
     /* the last one: sets pad trajectories to 8 points) */                                          \
     __builtin_amdgcn_sched_barrier(0);                                                              \
-    const int kh = k;                                                                               \
+    const int kh = k, rowh = row;                                                                   \
     wrapped = k + 1 == D;                                                                           \
     k = wrapped ? 0 : k + 1;                                                                        \
     row = wrapped ? row + W : row;                                                                  \
@@ -564,6 +571,35 @@ __device__ __forceinline__ int next_event(int rem, int lg, int res_rem) {
     load_vec<U>(top ? brow : bnd + k * U, hv);                                                      \
     dfirst = (MEAS == TS_LCSS || top) ? 0.0f : BND;                                                 \
     if (lg == W - 1) store_vec<U>(bnd + kh * U, on);                                                \
+    /* ---- early abandoning: row minima never fall from one row to the */                          \
+    /* next (a cell is its point cost, >= 0, plus or max'ed with a */                               \
+    /* neighbour's value, and f32 rounding is monotone), and the pair's */                          \
+    /* value is at least each of them; when lane W-1's row, not the last, */                        \
+    /* ends above tau at its last real chunk, the group drops the pair (NaN */                      \
+    /* in its slot) and fetches the next one as if the pair had ended. The */                       \
+    /* test comes before the row's idle chunks: during those a lane of the */                       \
+    /* next strip may already hold the pair's result (its chunk lastch comes */                     \
+    /* D > W-1 steps later than lane W-1's, so not before). */                                      \
+    if (MEAS != TS_LCSS) {                                                                          \
+      float mn = on[0];                                                                             \
+      _Pragma("unroll")                                                                             \
+      for (int j = 1; j < U; ++j) mn = fminf(mn, on[j]);                                            \
+      rowmin = fminf(kh == 0 ? kNoTau : rowmin, mn);                                                \
+      bool stop = false;                                                                            \
+      if (lg == W - 1 && kh == lastch && rowh + 1 < R)                                              \
+        stop = (MEAS == TS_DTW ? rowmin : sqrtf(rowmin)) > tau_l;                                   \
+      const uint64_t ab = __ballot(stop);                                                           \
+      if (__builtin_expect(ab != 0, 0) && ((ab >> (g * W + W - 1)) & 1ull)) {                       \
+        if (lg == W - 1) {                                                                          \
+          a.out[slot] = __int_as_float(0x7fc00000);                                                 \
+          atomicAdd(a.cells, (unsigned long long)(-((int64_t)(R - 1 - rowh) * C)));                 \
+        }                                                                                           \
+        rem = lg + 1;                                                                               \
+        ev = lg;                                                                                    \
+        res_rem = kNoResult;                                                                        \
+        tau_l = kNoTau; /* no second drop before the next pair starts */                            \
+      }                                                                                             \
+    }                                                                                               \
     --rem;                                                                                          \
   } while (0)
 
@@ -615,6 +651,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   const float *fAx = a.q.x, *fAy = a.q.y, *fBx = a.q.x, *fBy = a.q.y;
   bool f_ok = false, f_live = false;
   bool f_tiny = false, pair_tiny = false;  // fetched / current pair has a tiny coordinate
+  // fetched / current pair's abandoning threshold, and the running minimum of
+  // the lane's row
+  const float kNoTau = __int_as_float(0x7f800000);
+  float f_tau = kNoTau, tau_l = kNoTau, rowmin = kNoTau;
   float o[U], cx[U], cy[U], hv[U];
 #pragma unroll
   for (int j = 0; j < U; ++j) { o[j] = 0.0f; hv[j] = BND; cx[j] = 0.0f; cy[j] = 0.0f; }
@@ -659,6 +699,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             }
             f_slot = it.slot;
             f_tiny = MEAS == TS_DTW && (a.q.tiny[it.q] | a.db.tiny[it.d]) != 0;
+            f_tau = (MEAS != TS_LCSS && a.tau) ? a.tau[it.q - a.tau_q0] : kNoTau;
             if (SC) stage_pts<W>(cols + gpar * cmax, fBx, fBy, fC, lg);
             if (SR) stage_pts<W>(rows, fAx, fAy, fR, lg);
           }
@@ -671,8 +712,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
           done = true;
           rem = -2;  // no further events
           res_rem = kNoResult;
+          tau_l = kNoTau;
         } else {
           pair_tiny = f_live && f_tiny;
+          tau_l = f_live ? f_tau : kNoTau;
           R = f_live ? fR : 1;
           C = f_live ? fC : 1;
           slot = f_slot;
@@ -1128,7 +1171,8 @@ static hipError_t launch_dp_b(hipStream_t st, const DpArgs& a, int shape, int gr
 
 hipError_t launch_dp(hipStream_t st, MeasureArgs m, int shape, SetView q, SetView db, const Item* items,
                      const int32_t* nitems, const int32_t* ntail, int64_t max_items, int32_t* next, float* out,
-                     DpHalo halo, int32_t rmax_c, int32_t ccap_c, int grid_waves) {
+                     DpHalo halo, int32_t rmax_c, int32_t ccap_c, int grid_waves, const float* tau, int32_t tau_q0,
+                     unsigned long long* cells) {
   // row buffer: the longest trajectory of either set, in whole float4s;
   // column buffer (ranged): the longest shorter trajectory, likewise
   const int32_t rmax = ((std::max(rmax_c, 1) + 3) / 4) * 4;
@@ -1136,8 +1180,8 @@ hipError_t launch_dp(hipStream_t st, MeasureArgs m, int shape, SetView q, SetVie
   if (kDpShapeList[shape].W > 64)
     return launch_dp_mw(st, m, kDpShapeList[shape].W / 64, kDpShapeList[shape].U, q, db, items, nitems, ntail,
                         max_items, next, out, halo.len, rmax_c, grid_waves);
-  DpArgs a{q, db, items, nitems, ntail, max_items, next, out, halo.len, halo.mask, rmax, ccap, 0, m.band, m.delta,
-           m.eps};
+  DpArgs a{q,          db,      items,      nitems,  ntail,   max_items, next, out,   halo.len, halo.mask, rmax,
+           ccap,       0,       m.band,     m.delta, m.eps,   tau,       tau_q0, cells};
   switch (m.measure) {
     case TS_DTW:
       return m.band != 0 ? launch_dp_b<TS_DTW, true, false>(st, a, shape, grid_waves)

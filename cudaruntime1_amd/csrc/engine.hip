@@ -445,7 +445,7 @@ struct KeyItems {
 ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_set* qs, Item* items,
                  const int32_t* nitems_dev, int64_t max_items, float* vals, bool seed = false,
                  int32_t* ctr_zeroed = nullptr, const KeyItems* from_keys = nullptr,
-                 const RefineBufs* fixed = nullptr) {
+                 const RefineBufs* fixed = nullptr, const float* tau = nullptr, int32_t tau_q0 = 0) {
   if (max_items == 0) return TS_OK;
   if (from_keys && !ctr_zeroed) return fail(TS_ERR_INVALID, "items from keys need zeroed counters");
   if (fixed && max_items > fixed->max_items) return fail(TS_ERR_INVALID, "refine larger than its fixed buffers");
@@ -540,7 +540,7 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
     const std::string label = e->timing ? dp_label(nm, b_lo, sh) : std::string();
     return run(e, e->timing ? label.c_str() : nm, 0, 0, [&] {
       return launch_dp(e->st, ma, sh, qv, dv, items, nitems_dev, nullptr, max_items, ctr, vals, halo, rmax, hi,
-                       cus * tune[b_lo].waves_per_cu);
+                       cus * tune[b_lo].waves_per_cu, tau, tau_q0, cells + slot + b_lo);
     }, slot + b_lo);
   }
   // Buckets run concurrently on their own streams (fork/join by events), so
@@ -554,7 +554,7 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
     TS_TRY(run_on(e, e->aux[b], e->timing ? label.c_str() : nm, 0, 0, [&] {
       return launch_dp(e->aux[b], ma, sh, qv, dv, itemsb + (int64_t)b * max_items, ctr + kDpBuckets + b,
                        ctr + 2 * kDpBuckets + b, max_items, ctr + b, vals, halo, rmax, hi,
-                       cus * tune[b].waves_per_cu);
+                       cus * tune[b].waves_per_cu, tau, tau_q0, cells + slot + b);
     }, slot + b));
     TS_HIP(hipEventRecord(e->join[b], e->aux[b]));
     TS_HIP(hipStreamWaitEvent(e->st, e->join[b], 0));
@@ -771,7 +771,8 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
     return launch_tight_filter(e->st, ma, k, qv, q0, dv, tau, citems, ntot, total, items2, nitems2, rvals,
                                device_cu_count() * 16);
   }));
-  TS_TRY(refine(e, p, db, qs, items2, nitems2, cap, rvals, false, ctrs + 32));
+  // the main refine abandons pairs whose value must exceed the threshold
+  TS_TRY(refine(e, p, db, qs, items2, nitems2, cap, rvals, false, ctrs + 32, nullptr, nullptr, tau, q0));
   // 4. top-k over each query's refined candidates (slot = position in citems)
   TS_TRY(run(e, "keys", 0, 0, [&] { return launch_keys_from_items(e->st, rvals, citems, cap, desc, rkeys); }));
   // Region lengths: clamped to the capacity (device-sized), or the counted

@@ -377,9 +377,13 @@ DpHalo dp_halo(int shape, int b, const MeasureArgs& m, int32_t cmax_c, int32_t w
 // Whether a launch of `shape` with this halo fits a block's LDS (multi-wave
 // shapes: with the strip table of pairs whose longer length is at most rmax).
 bool dp_fits(int shape, DpHalo halo, int32_t rmax);
+// tau (with tau_q0), cells: early abandoning of unbanded DTW and Frechet
+// pairs whose value must exceed tau[q - tau_q0]; their slot gets NaN and
+// `cells` loses their unfinished rows' cells. tau null: none.
 hipError_t launch_dp(hipStream_t st, MeasureArgs m, int shape, SetView q, SetView db, const Item* items,
                      const int32_t* nitems, const int32_t* ntail, int64_t max_items, int32_t* next, float* out,
-                     DpHalo halo, int32_t rmax_c, int32_t ccap_c, int grid_waves);
+                     DpHalo halo, int32_t rmax_c, int32_t ccap_c, int grid_waves, const float* tau = nullptr,
+                     int32_t tau_q0 = 0, unsigned long long* cells = nullptr);
 // dp_mw.hip -- the multi-wave DP (nw waves per pair, u cells per lane step):
 // halo floats per buffer and LDS bytes of a launch, and the launch itself
 // (hcap from dp_mw_hcap, a grid of grid_waves waves).

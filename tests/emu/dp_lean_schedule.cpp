@@ -8,7 +8,15 @@
 // chunks, rows rounded up to 4 points), every staged operand carries the
 // pair and index that wrote it and is checked where a real cell uses it, and
 // results are compared bit for bit with the oracle's DTW, Frechet and LCSS.
-// Usage: dp_lean_schedule [trials per shape]; exit status 0 when all match.
+// Early abandoning as the kernel does it: every DTW / Frechet pair has a
+// threshold (below, at, just above or far above its oracle value, or none);
+// lane W-1's finished row minimum above it drops the pair for the group,
+// which fetches the next one at once. A pair whose value is at most its
+// threshold must never be dropped, a dropped one must exceed it, and the
+// pairs fetched after a drop pass every staging and halo check.
+// Usage: dp_lean_schedule [trials per shape] [late]; exit status 0 when all
+// match. "late" tests the row minimum at the row's last chunk including its
+// idle ones, a step too late: the model must then fail.
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -35,6 +43,8 @@ enum { DTW = 0, LCSS = 1, FRECHET = 2 };
 
 // The kernels' next event (group fetch at rem == lg, pair start at rem == 0,
 // result at rem + 1 == res_rem): the largest trigger below rem, INT_MIN if none.
+static bool g_late = false;  // test rows at their padded end (a planted mistake)
+
 static int next_event(int rem, int lg, int res_rem) {
   int ev = INT_MIN;
   if (0 < rem) ev = 0;
@@ -43,17 +53,20 @@ static int next_event(int rem, int lg, int res_rem) {
   return ev;
 }
 
-struct Pair { std::vector<float> a, b; bool dead; };  // a: query (xy), b: db (xy)
+struct Pair { std::vector<float> a, b; bool dead; float tau; };  // a: query (xy), b: db (xy)
 
 struct Tagged { float v; int pair; int idx; };
 
 template <int MEAS, int W, int U>
-std::vector<float> emulate(const std::vector<Pair>& pairs, float eps, int cmax, int rmax, int nwaves, long* steps) {
+std::vector<float> emulate(const std::vector<Pair>& pairs, float eps, int cmax, int rmax, int nwaves, long* steps,
+                           std::vector<int>* dropped) {
   constexpr int G = 64 / W;
   constexpr float BND = MEAS == LCSS ? 0.0f : kInf;
   const int n_items = pairs.size();
   std::vector<float> out(n_items, NAN);
   std::vector<int> written(n_items, 0);
+  dropped->assign(n_items, 0);
+  const float kNoTau = INFINITY;
   int next = 0;
   struct Lane {
     int R = 1, C = 1, lastch = 0, D = W + 1, k = 0, row, rem, res_rem = INT_MIN, slot = 0, par = 0, gpar = 0;
@@ -67,6 +80,7 @@ std::vector<float> emulate(const std::vector<Pair>& pairs, float eps, int cmax, 
     int hvpair = -2, hvidx = -1, hvrow = -1;
     float up_last = 0, dfirst = 0;
     int lastres_row = -1, lastres_k = -1;
+    float f_tau = INFINITY, tau = INFINITY, rowmin = INFINITY;
   };
   struct Group {
     std::vector<Tagged> bnd, sbx, sby, sax, say;
@@ -154,6 +168,7 @@ std::vector<float> emulate(const std::vector<Pair>& pairs, float eps, int cmax, 
                 ln.fC = std::min(n, m);
                 ln.f_slot = it;
                 ln.f_pair = it;
+                ln.f_tau = MEAS == LCSS ? kNoTau : p.tau;
                 for (int c = lg; c < ((ln.fC + 3) >> 2); c += W)
                   for (int e = 0; e < 4; ++e) {
                     const int i = 4 * c + e, src = std::min(i, ln.fC - 1);
@@ -183,9 +198,11 @@ std::vector<float> emulate(const std::vector<Pair>& pairs, float eps, int cmax, 
             ln.done = true;
             ln.rem = -2;
             ln.res_rem = INT_MIN;
+            ln.tau = kNoTau;
             continue;
           }
           ln.live = ln.f_live;
+          ln.tau = ln.f_live ? ln.f_tau : kNoTau;
           ln.pair = ln.f_live ? ln.f_pair : -1;
           ln.R = ln.f_live ? ln.fR : 1;
           ln.C = ln.f_live ? ln.fC : 1;
@@ -305,6 +322,40 @@ std::vector<float> emulate(const std::vector<Pair>& pairs, float eps, int cmax, 
           gr.bnd_row[ln.k * U + j] = ln.row;
         }
       }
+      // early abandoning (DTW, Frechet): lane W-1's row, not the pair's last,
+      // at its last real chunk (later, in the row's idle chunks, a lane of the
+      // next strip may already hold the pair's result) with a minimum above
+      // the threshold drops the group's pair
+      bool stop_g[G] = {};
+      for (int l = 0; l < 64; ++l) {
+        Lane& ln = L[l];
+        float mn = on[l][0];
+        for (int j = 1; j < U; ++j) mn = fminf(mn, on[l][j]);
+        ln.rowmin = fminf(ln.k == 0 ? kNoTau : ln.rowmin, mn);
+        if (MEAS != LCSS && lgof(l) == W - 1 && ln.k == (g_late ? ln.D - 1 : ln.lastch) && ln.row + 1 < ln.R) {
+          const float v = MEAS == DTW ? ln.rowmin : sqrtf(ln.rowmin);
+          if (v > ln.tau) stop_g[l / W] = true;
+        }
+      }
+      for (int g = 0; g < G; ++g) {
+        if (!stop_g[g]) continue;
+        Lane& lw = L[g * W + W - 1];
+        CHK(lw.live && !lw.done && lw.slot >= 0 && lw.slot < n_items && !written[lw.slot]);
+        written[lw.slot] = 1;
+        (*dropped)[lw.slot] = 1;
+        out[lw.slot] = NAN;
+        for (int lg = 0; lg < W; ++lg) {
+          Lane& ln = L[g * W + lg];
+          ln.rem = lg + 1;
+          ln.ev = lg;
+          ln.res_rem = INT_MIN;
+          ln.tau = kNoTau;  // no second drop before the next pair starts
+          // the dropped pair's remaining cells are garbage nothing reads (the
+          // next pair's rows may already be staged under them)
+          ln.live = false;
+          ln.pair = -1;
+        }
+      }
       // advance
       for (int l = 0; l < 64; ++l) {
         Lane& ln = L[l];
@@ -327,7 +378,7 @@ std::vector<float> emulate(const std::vector<Pair>& pairs, float eps, int cmax, 
 
 template <int MEAS, int W, int U>
 int run(std::mt19937& rng, int trials) {
-  int bad = 0;
+  int bad = 0, drops = 0;
   for (int t = 0; t < trials; ++t) {
     const int maxlen = std::uniform_int_distribution<int>(1, 300)(rng);
     const int npairs = std::uniform_int_distribution<int>(1, 40)(rng);
@@ -351,22 +402,39 @@ int run(std::mt19937& rng, int trials) {
     long steps = 0;
     const float eps = 0.03f;
     const int nwaves = 1 + t % 3;
-    std::vector<float> got = emulate<MEAS, W, U>(pairs, eps, cmax, rmax, nwaves, &steps);
+    std::vector<float> wants(npairs, 0.0f);
+    for (int i = 0; i < npairs; ++i) {
+      Pair& p = pairs[i];
+      const int n = p.a.size() / 2, m = p.b.size() / 2;
+      if (MEAS == DTW) wants[i] = orc_dtw(p.a.data(), n, p.b.data(), m, 0);
+      else if (MEAS == FRECHET) wants[i] = orc_frechet(p.a.data(), n, p.b.data(), m);
+      else wants[i] = (float)orc_lcss(p.a.data(), n, p.b.data(), m, eps, -1) / (float)std::min(n, m);
+      static const float kFactor[7] = {0.25f, 0.8f, 1.0f, 1.0f, 1.0001f, 3.0f, INFINITY};
+      const int f = std::uniform_int_distribution<int>(0, 6)(rng);
+      p.tau = f == 3 ? nextafterf(wants[i], 0.0f) : wants[i] * kFactor[f];
+    }
+    std::vector<int> dropped;
+    std::vector<float> got = emulate<MEAS, W, U>(pairs, eps, cmax, rmax, nwaves, &steps, &dropped);
     for (int i = 0; i < npairs; ++i) {
       if (pairs[i].dead) continue;
       const Pair& p = pairs[i];
       const int n = p.a.size() / 2, m = p.b.size() / 2;
-      float want;
-      if (MEAS == DTW) want = orc_dtw(p.a.data(), n, p.b.data(), m, 0);
-      else if (MEAS == FRECHET) want = orc_frechet(p.a.data(), n, p.b.data(), m);
-      else want = (float)orc_lcss(p.a.data(), n, p.b.data(), m, eps, -1) / (float)std::min(n, m);
+      const float want = wants[i];
+      if (dropped[i]) {
+        ++drops;
+        if (!(want > p.tau)) {
+          if (bad < 5) fprintf(stderr, "MEAS %d W %d U %d trial %d pair %d n %d m %d: dropped at value %.9g, tau %.9g\n", MEAS, W, U, t, i, n, m, want, p.tau);
+          ++bad;
+        }
+        continue;
+      }
       if (memcmp(&want, &got[i], 4) != 0) {
         if (bad < 5) fprintf(stderr, "MEAS %d W %d U %d trial %d pair %d n %d m %d: got %.9g want %.9g\n", MEAS, W, U, t, i, n, m, got[i], want);
         ++bad;
       }
     }
   }
-  printf("MEAS %d W %2d U %d: %d trials, %d mismatches\n", MEAS, W, U, trials, bad);
+  printf("MEAS %d W %2d U %d: %d trials, %d pairs dropped early, %d mismatches\n", MEAS, W, U, trials, drops, bad);
   return bad;
 }
 
@@ -374,6 +442,7 @@ int main(int argc, char** argv) {
   std::mt19937 rng(12345);
   int bad = 0;
   const int T = argc > 1 ? std::atoi(argv[1]) : 60;
+  g_late = argc > 2 && std::strcmp(argv[2], "late") == 0;
   bad += run<DTW, 16, 1>(rng, T); bad += run<DTW, 16, 2>(rng, T); bad += run<DTW, 16, 4>(rng, T);
   bad += run<DTW, 32, 2>(rng, T); bad += run<DTW, 32, 4>(rng, T); bad += run<DTW, 64, 2>(rng, T);
   bad += run<DTW, 64, 4>(rng, T); bad += run<DTW, 32, 8>(rng, T); bad += run<DTW, 64, 8>(rng, T);

@@ -17,12 +17,22 @@ ROOT = os.path.dirname(HERE)
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
 def test_lean_dp_schedule_model_matches_oracle(tmp_path):
+    """k_dp_lean's schedule, with early abandoning of DTW and Frechet pairs
+    against thresholds around their values: every kept value equals the
+    oracle's, no pair at or below its threshold is dropped, and pairs are
+    dropped. Testing the row minimum at the row's padded end instead of its
+    last real chunk (where a lane of the next strip may already have written
+    the pair's result) is caught."""
     exe = str(tmp_path / "dp_lean_schedule")
     subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fopenmp", "-o", exe,
                     os.path.join(HERE, "emu", "dp_lean_schedule.cpp"), os.path.join(ROOT, "oracle", "oracle.cpp")],
                    check=True, timeout=300)
     r = subprocess.run([exe, "12"], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+    drops = [int(line.split(",")[1].split()[0]) for line in r.stdout.splitlines() if "dropped early" in line]
+    assert sum(drops) > 0
+    r = subprocess.run([exe, "12", "late"], capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0
 
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
