@@ -592,7 +592,7 @@ __device__ __forceinline__ int next_event(int rem, int lg, int res_rem) {
       if (__builtin_expect(ab != 0, 0) && ((ab >> (g * W + W - 1)) & 1ull)) {                       \
         if (lg == W - 1) {                                                                          \
           a.out[slot] = __int_as_float(0x7fc00000);                                                 \
-          atomicAdd(a.cells, (unsigned long long)(-((int64_t)(R - 1 - rowh) * C)));                 \
+          atomicAdd(a.cells, (unsigned long long)(0ull - (uint64_t)(R - 1 - rowh) * (uint64_t)C));  \
         }                                                                                           \
         rem = lg + 1;                                                                               \
         ev = lg;                                                                                    \
@@ -789,6 +789,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 // the halo chunk of a step is its chunk clamped to the row's last one.
 // ST: both trajectories staged in LDS (short pairs), else both read from
 // global memory.
+// Last chunk (of U columns) holding an in-band cell of row r of a banded
+// pair: column min(C-1, floor(r*C/R) + wb) (r*C < 2^31, as in strip_range).
+template <int U>
+__device__ __forceinline__ int band_last_chunk(int r, int R, int C, int wb) {
+  return min(C - 1, (int)(((uint32_t)r * (uint32_t)C) / (uint32_t)R) + wb) / U;
+}
+
 __host__ __device__ constexpr int lean_r_group_floats(int cmax, int ccap, int rmax, int smax, bool st) {
   return cmax + (st ? 4 * ccap + 2 * rmax : 0) + ((2 * smax + 3) & ~3);  // whole float4s per group
 }
@@ -830,6 +837,13 @@ __global__ void __launch_bounds__(256) k_dp_lean_r(DpArgs a) {
   int strip = 0, k = 0, cs = 0, ce = W + 1, pcs = 0, pce = 0, row = lg;
   int bw = 0, bd = 0;
   int2 nxt = make_int2(0, W + 1);  // the next strip's (cstart, cend)
+  // early abandoning (banded DTW): the fetched / current pair's threshold,
+  // the band's half-width, the last chunk holding an in-band cell of the
+  // lane's row, and the running minimum of the row
+  constexpr bool kAbandon = MEAS == TS_DTW && BAND;
+  const float kNoTau = __int_as_float(0x7f800000);
+  float f_tau = kNoTau, tau_l = kNoTau, rowmin = kNoTau;
+  int wbl = 0, rlast = 0;
   int gpar = 0;
   const float2* cb = cols;
   const float *Ax = a.q.x, *Ay = a.q.y, *Bx = a.q.x, *By = a.q.y;
@@ -879,6 +893,7 @@ __global__ void __launch_bounds__(256) k_dp_lean_r(DpArgs a) {
             }
             f_slot = it.slot;
             f_tiny = MEAS == TS_DTW && (a.q.tiny[it.q] | a.db.tiny[it.d]) != 0;
+            f_tau = (kAbandon && a.tau) ? a.tau[it.q - a.tau_q0] : kNoTau;
             if (ST) {
               stage_pts<W>(cols + gpar * ccap, fBx, fBy, fC, lg);
               stage_pts<W>(rows, fAx, fAy, fR, lg);
@@ -920,13 +935,19 @@ __global__ void __launch_bounds__(256) k_dp_lean_r(DpArgs a) {
           done = true;
           rem = -2;
           res_rem = kNoResult;
+          tau_l = kNoTau;
         } else {
           pair_tiny = f_live && f_tiny;
           R = f_live ? fR : 1;
           C = f_live ? fC : 1;
           slot = f_slot;
           lastch = (C - 1) / U;
-          if (BAND) bw = band_half_width(a.band, R, C) * R;
+          if (BAND) {
+            wbl = band_half_width(a.band, R, C);
+            bw = wbl * R;
+          }
+          tau_l = f_live ? f_tau : kNoTau;
+          if (kAbandon) rlast = band_last_chunk<U>(row, R, C, wbl);
           S = DELTA ? min(R - 1, C - 1 + delta) / W + 1 : (R + W - 1) / W;
           rem = f_total;
           res_rem = lg == f_lgres ? f_total - f_ires : kNoResult;
@@ -1015,12 +1036,43 @@ __global__ void __launch_bounds__(256) k_dp_lean_r(DpArgs a) {
 #pragma unroll
     for (int j = 0; j < U; ++j) o[j] = on[j];
 
+    // ---- early abandoning (banded DTW), as in k_dp_lean: row minima never
+    // fall from row to row (cells outside the band hold 1e20f, more than
+    // any in-band cell they could stand for); lane W-1's row, not the last,
+    // is tested at its last in-band chunk, at least two steps before a lane
+    // of a later strip can compute the result cell (a strip lasts at least
+    // W + 1 + its successor's shift steps); the group drops the pair (NaN in
+    // its slot) and fetches the next one as if the pair had ended.
+    if (kAbandon) {
+      float mn = on[0];
+#pragma unroll
+      for (int j = 1; j < U; ++j) mn = fminf(mn, on[j]);
+      rowmin = fminf(first ? kNoTau : rowmin, mn);
+      const bool stop = lg == W - 1 && k == rlast && row + 1 < R && rowmin > tau_l;
+      const uint64_t ab = __ballot(stop);
+      if (__builtin_expect(ab != 0, 0) && ((ab >> (g * W + W - 1)) & 1ull)) {
+        if (lg == W - 1) {
+          a.out[slot] = __int_as_float(0x7fc00000);
+          // the unfinished rows' in-band cells, at most min(2w+1, C) a row:
+          // the launch's cell count errs low, never high (integer arithmetic
+          // throughout: the device min template is floating point)
+          const uint64_t per_row = (uint64_t)(2 * wbl + 1 < C ? 2 * wbl + 1 : C);
+          atomicAdd(a.cells, (unsigned long long)(0ull - (uint64_t)(R - 1 - row) * per_row));
+        }
+        rem = lg + 1;
+        ev = lg;
+        res_rem = kNoResult;
+        tau_l = kNoTau;  // no second drop before the next pair starts
+      }
+    }
+
     // ---- advance; the next step's operands (before this step's halo store)
     const int kh = k;
     wrapped = k + 1 == ce;
     if (wrapped) {
       ++strip;
       row += W;
+      if (kAbandon) rlast = band_last_chunk<U>(min(row, R - 1), R, C, wbl);
       pcs = cs;
       pce = ce;
       cs = nxt.x;
@@ -1179,7 +1231,7 @@ hipError_t launch_dp(hipStream_t st, MeasureArgs m, int shape, SetView q, SetVie
   const int32_t ccap = ((std::max(ccap_c, 1) + 3) / 4) * 4;
   if (kDpShapeList[shape].W > 64)
     return launch_dp_mw(st, m, kDpShapeList[shape].W / 64, kDpShapeList[shape].U, q, db, items, nitems, ntail,
-                        max_items, next, out, halo.len, rmax_c, grid_waves);
+                        max_items, next, out, halo.len, rmax_c, grid_waves, tau, tau_q0, cells);
   DpArgs a{q,          db,      items,      nitems,  ntail,   max_items, next, out,   halo.len, halo.mask, rmax,
            ccap,       0,       m.band,     m.delta, m.eps,   tau,       tau_q0, cells};
   switch (m.measure) {

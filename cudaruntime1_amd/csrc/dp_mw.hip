@@ -58,6 +58,10 @@ struct MwArgs {
   int32_t smax;  // strip table entries
   int32_t band, delta;
   float eps;
+  // early abandoning (DTW, banded DTW, Frechet): as in dp.hip's DpArgs
+  const float* tau;
+  int32_t tau_q0;
+  unsigned long long* cells;
 };
 
 struct MwStrip {
@@ -109,14 +113,20 @@ __device__ __forceinline__ void load_lds(const float* p, float (&v)[U]) {
   }
 }
 
+// Last chunk holding an in-band cell of row r (r*C < 2^31 for banded pairs).
+template <int U>
+__device__ __forceinline__ int mw_band_last_chunk(int r, int R, int C, int wb) {
+  return min(C - 1, (int)(((uint32_t)r * (uint32_t)C) / (uint32_t)R) + wb) / U;
+}
+
 __device__ __forceinline__ int mw_band_half_width(int band, int R, int C) {
   const int w = band > 0 ? band : (-band * R + 99) / 100;
   return min(w, C);
 }
 
-// LDS: [0, 64) bytes header (item index, step total; bytes 32..63 a
-// border chunk of 8 values outside the table), the strip table, then the
-// two halo buffers.
+// LDS: [0, 64) bytes header (item index, step total, the pair's drop flag;
+// bytes 32..63 a border chunk of 8 values outside the table), the strip
+// table, then the two halo buffers.
 __host__ __device__ constexpr size_t mw_lds_bytes(int smax, int hcap) {
   return 64 + (size_t)smax * sizeof(MwStrip) + 2 * (size_t)hcap * sizeof(float);
 }
@@ -124,10 +134,12 @@ __host__ __device__ constexpr size_t mw_lds_bytes(int smax, int hcap) {
 template <int MEAS, int U, int NW, bool BAND, bool DELTA>
 __global__ void __launch_bounds__(256) k_dp_mw(MwArgs a) {
   constexpr float BND = (MEAS == TS_LCSS) ? 0.0f : kInf;
+  constexpr bool kAbandon = MEAS != TS_LCSS;
+  const float kNoTau = __int_as_float(0x7f800000);
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   extern __shared__ float4 lds_mw4[];
-  int* hdr = (int*)lds_mw4;                       // [0] item, [1] step total
+  int* hdr = (int*)lds_mw4;                       // [0] item, [1] step total, [2] dropped
   float* brow = (float*)lds_mw4 + 8;              // [8] the border value
   MwStrip* tbl = (MwStrip*)((char*)lds_mw4 + 64);
   float* halo = (float*)(tbl + a.smax);           // [2][hcap]
@@ -180,9 +192,11 @@ __global__ void __launch_bounds__(256) k_dp_mw(MwArgs a) {
         total = max(total, t0 + 63 + (tbl[s].ce - tbl[s].cs));
       }
       hdr[1] = total;
+      hdr[2] = 0;
     }
     __syncthreads();
     const int total = hdr[1];
+    const float tau_p = (kAbandon && a.tau) ? a.tau[it.q - a.tau_q0] : kNoTau;
 
     // ---- per-lane state: the strip this lane is in (or will be in next),
     // its chunk at the current step, its row point and the chunk's points
@@ -200,6 +214,10 @@ __global__ void __launch_bounds__(256) k_dp_mw(MwArgs a) {
       ax = Ax[min(row, R - 1)];
       ay = Ay[min(row, R - 1)];
     }
+    // early abandoning: the last chunk holding a (band) cell of the lane's
+    // row, and the running minimum of the row
+    int rlast = BAND ? mw_band_last_chunk<U>(min(row, R - 1), R, C, wb) : lastch;
+    float rowmin = kNoTau;
     int k = st.cs - st.t0 - lane;  // step 0 (at most cs: no switch yet)
     float cx[U], cy[U];
     load_chunk<U>(Bx, By, min(max(k, 0), lastch), cx, cy);
@@ -213,7 +231,7 @@ __global__ void __launch_bounds__(256) k_dp_mw(MwArgs a) {
 
     for (int t = 0; t < total; ++t) {
       // ---- this step: strip, chunk, row (the next step's replace them below)
-      const int sc = s, kk = k, rowc = row, cs_c = st.cs;
+      const int sc = s, kk = k, rowc = row, cs_c = st.cs, rlc = rlast;
       const bool active = sc < S && kk >= cs_c && kk < st.ce;
       const int kc = min(max(kk, 0), lastch);  // idle lanes read a valid chunk
       // ---- point terms: squared distances (DTW, Frechet) or matches (LCSS)
@@ -247,6 +265,7 @@ __global__ void __launch_bounds__(256) k_dp_mw(MwArgs a) {
           ay = Ay[min(row, R - 1)];
           pcs = tbl[s - 1].cs;
           pce = tbl[s - 1].ce;
+          if (BAND) rlast = mw_band_last_chunk<U>(min(row, R - 1), R, C, wb);
         }
         k = t + 1 - st.t0 - lane + st.cs;
       }
@@ -301,7 +320,27 @@ __global__ void __launch_bounds__(256) k_dp_mw(MwArgs a) {
 #pragma unroll
         for (int j = 0; j < U; ++j) hb[j] = on[j];
       }
-      if (active && rowc == res_row && kk == lastch) {
+      // ---- early abandoning: row minima never fall from row to row (see
+      // dp.hip); a strip's last row, not the pair's last, whose minimum at
+      // its last (band) chunk exceeds tau drops the pair: the flag is seen
+      // at the next barrier by every thread, at most kMwB - 1 steps later,
+      // and the result cell of a later strip comes at least kMwLag - 63 =
+      // kMwB + 1 steps later, so the result is never written for a dropped
+      // pair (the write also tests the flag)
+      if (kAbandon) {
+        float mn = on[0];
+#pragma unroll
+        for (int j = 1; j < U; ++j) mn = fminf(mn, on[j]);
+        rowmin = fminf(kk == cs_c ? kNoTau : rowmin, mn);
+        if (lane == 63 && active && kk == rlc && rowc + 1 < R &&
+            (MEAS == TS_FRECHET ? sqrtf(rowmin) : rowmin) > tau_p && atomicExch(&hdr[2], 1) == 0) {
+          a.out[it.slot] = __int_as_float(0x7fc00000);
+          // (integer arithmetic: the device min template is floating point)
+          const uint64_t per_row = (uint64_t)(BAND && 2 * wb + 1 < C ? 2 * wb + 1 : C);
+          atomicAdd(a.cells, (unsigned long long)(0ull - (uint64_t)(R - 1 - rowc) * per_row));
+        }
+      }
+      if (active && rowc == res_row && kk == lastch && !(kAbandon && *(volatile int*)&hdr[2])) {
         const int rc = (C - 1) - lastch * U;
         float v = on[0];
 #pragma unroll
@@ -311,7 +350,10 @@ __global__ void __launch_bounds__(256) k_dp_mw(MwArgs a) {
         if (MEAS == TS_LCSS) v = v / (float)C;
         a.out[it.slot] = v;
       }
-      if ((t + 1) % kMwB == 0) __syncthreads();
+      if ((t + 1) % kMwB == 0) {
+        __syncthreads();
+        if (kAbandon && *(volatile int*)&hdr[2]) break;  // dropped: the same on every thread
+      }
       // ---- the halo chunk of step t+1 (strip s, chunk k): the previous
       // strip's row there, else the border (above row 0, or above the
       // previous strip's range: outside the band). Every lane reads it, so
@@ -404,9 +446,11 @@ size_t dp_mw_lds_bytes(int32_t rmax_c, int32_t hcap) { return mw_lds_bytes((rmax
 
 hipError_t launch_dp_mw(hipStream_t st, MeasureArgs m, int nw, int u, SetView q, SetView db, const Item* items,
                         const int32_t* nitems, const int32_t* ntail, int64_t max_items, int32_t* next, float* out,
-                        int32_t hcap, int32_t rmax_c, int grid_waves) {
+                        int32_t hcap, int32_t rmax_c, int grid_waves, const float* tau, int32_t tau_q0,
+                        unsigned long long* cells) {
   if ((nw != 2 && nw != 4) || (u != 4 && u != 8)) return hipErrorInvalidValue;
-  MwArgs a{q, db, items, nitems, ntail, max_items, next, out, hcap, (rmax_c + 63) / 64 + 1, m.band, m.delta, m.eps};
+  MwArgs a{q,       db,      items, nitems, ntail, max_items, next,   out,  hcap, (rmax_c + 63) / 64 + 1,
+           m.band,  m.delta, m.eps, tau,    tau_q0, cells};
   switch (m.measure) {
     case TS_DTW:
       return m.band != 0 ? launch_mw_m<TS_DTW, true, false>(st, a, nw, u, grid_waves)

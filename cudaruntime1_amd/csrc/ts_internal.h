@@ -395,7 +395,8 @@ ts_status dp_mw_setup(int device);
 int32_t dp_mw_lds_max();
 hipError_t launch_dp_mw(hipStream_t st, MeasureArgs m, int nw, int u, SetView q, SetView db, const Item* items,
                         const int32_t* nitems, const int32_t* ntail, int64_t max_items, int32_t* next, float* out,
-                        int32_t hcap, int32_t rmax_c, int grid_waves);
+                        int32_t hcap, int32_t rmax_c, int grid_waves, const float* tau, int32_t tau_q0,
+                        unsigned long long* cells);
 
 // reduce.hip -- Hausdorff (wave per pair) and lock-step Euclidean (lane per pair).
 hipError_t launch_hausdorff(hipStream_t st, SetView q, SetView db, const Item* items, const int32_t* nitems,

@@ -9,8 +9,14 @@
 // the size dp_halo gives the kernel. Every read that a real cell uses is
 // checked to come from the right pair, row and chunk; results are compared
 // bit for bit with the oracle's banded DTW and windowed LCSS.
-// Usage: dp_lean_ranged_schedule [trials per shape] [ring divisor]; exit 0
-// when all match (a ring divisor above 1 shrinks the ring to show a hazard).
+// Early abandoning of banded DTW pairs as the kernel does it (thresholds
+// below, at, just above or far above each pair's oracle value, or none): a
+// pair at or below its threshold is never dropped, a dropped one exceeds it,
+// and the pairs fetched after a drop pass every check.
+// Usage: dp_lean_ranged_schedule [trials per shape] [ring divisor] [late];
+// exit 0 when all match (a ring divisor above 1 shrinks the ring to show a
+// hazard; "late" tests a row's minimum at its strip's padded end, where the
+// result may already be out: the model must then fail).
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -45,7 +51,8 @@ static int next_event(int rem, int lg, int res_rem) {
   return ev;
 }
 
-struct Pair { std::vector<float> a, b; bool dead; };
+struct Pair { std::vector<float> a, b; bool dead; float tau; };
+static bool g_late = false;
 struct Tag { float v; int pair, row, chunk; };
 
 static int band_half_width(int band, int R, int C) {
@@ -72,13 +79,19 @@ static void strip_range(int s, int R, int C, int wb, int delta, int lastch, int&
 
 template <int MEAS, int W, int U>
 static std::vector<float> emulate(const std::vector<Pair>& pairs, int band, int delta, float eps, int hlen,
-                                  int hmask, int ccap, int rmax, int smax, int nwaves) {
+                                  int hmask, int ccap, int rmax, int smax, int nwaves, std::vector<int>* dropped) {
   constexpr bool BAND = MEAS == DTW, DELTA = MEAS == LCSS;
   constexpr int G = 64 / W;
   constexpr float BND = MEAS == LCSS ? 0.0f : kInf;
   const int n_items = pairs.size();
   std::vector<float> out(n_items, NAN);
   std::vector<int> written(n_items, 0);
+  dropped->assign(n_items, 0);
+  const float kNoTau = INFINITY;
+  // last chunk holding an in-band cell of row r (the kernel's band_last_chunk)
+  auto last_chunk = [](int r, int R, int C, int wb) {
+    return std::min(C - 1, (int)(((uint32_t)r * (uint32_t)C) / (uint32_t)R) + wb) / U;
+  };
   int next = 0;
   struct Lane {
     int R = 1, C = 1, lastch = 0, S = 1, rem, res_rem = INT_MIN, slot = 0, pair = -1;
@@ -96,6 +109,8 @@ static std::vector<float> emulate(const std::vector<Pair>& pairs, int band, int 
     float up_last = 0;
     Tag hdiag{0, -2, -1, -1};
     int last_row = -1, last_k = -1;
+    float f_tau = INFINITY, tau = INFINITY, rowmin = INFINITY;
+    int wbl = 0, rlast = 0;
   };
   struct Group {
     std::vector<Tag> bnd, colx, coly, rowx, rowy;
@@ -173,6 +188,7 @@ static std::vector<float> emulate(const std::vector<Pair>& pairs, int band, int 
                 ln.fC = std::min(n, m);
                 ln.f_slot = it;
                 ln.f_pair = it;
+                ln.f_tau = MEAS == DTW ? p.tau : kNoTau;
               }
             }
           }
@@ -234,6 +250,7 @@ static std::vector<float> emulate(const std::vector<Pair>& pairs, int band, int 
             ln.done = true;
             ln.rem = -2;
             ln.res_rem = INT_MIN;
+            ln.tau = kNoTau;
             continue;
           }
           ln.live = ln.f_live;
@@ -242,7 +259,12 @@ static std::vector<float> emulate(const std::vector<Pair>& pairs, int band, int 
           ln.C = ln.f_live ? ln.fC : 1;
           ln.slot = ln.f_slot;
           ln.lastch = (ln.C - 1) / U;
-          if (BAND) ln.bw = band_half_width(band, ln.R, ln.C) * ln.R;
+          if (BAND) {
+            ln.wbl = band_half_width(band, ln.R, ln.C);
+            ln.bw = ln.wbl * ln.R;
+          }
+          ln.tau = ln.f_live ? ln.f_tau : kNoTau;
+          if (BAND) ln.rlast = last_chunk(ln.row, ln.R, ln.C, ln.wbl);
           ln.S = DELTA ? std::min(ln.R - 1, ln.C - 1 + delta) / W + 1 : (ln.R + W - 1) / W;
           ln.rem = ln.f_total;
           ln.res_rem = lg == ln.f_lgres ? ln.f_total - ln.f_ires : INT_MIN;
@@ -329,6 +351,36 @@ static std::vector<float> emulate(const std::vector<Pair>& pairs, int band, int 
         ln.last_k = ln.k;
       }
       for (int l = 0; l < 64; ++l) for (int j = 0; j < U; ++j) L[l].o[j] = on[l][j];
+      // early abandoning (banded DTW): lane W-1's row, not the pair's last,
+      // at its last in-band chunk, with a minimum above the threshold
+      if (MEAS == DTW) {
+        bool stop_g[G] = {};
+        for (int l = 0; l < 64; ++l) {
+          Lane& ln = L[l];
+          float mn = on[l][0];
+          for (int j = 1; j < U; ++j) mn = fminf(mn, on[l][j]);
+          ln.rowmin = fminf(ln.k == ln.cs ? kNoTau : ln.rowmin, mn);
+          const int at = g_late ? ln.ce - 1 : ln.rlast;
+          if (lgof(l) == W - 1 && ln.k == at && ln.row + 1 < ln.R && ln.rowmin > ln.tau) stop_g[l / W] = true;
+        }
+        for (int g = 0; g < G; ++g) {
+          if (!stop_g[g]) continue;
+          Lane& lw = L[g * W + W - 1];
+          CHK(lw.live && !lw.done && lw.slot >= 0 && lw.slot < n_items && !written[lw.slot]);
+          written[lw.slot] = 1;
+          (*dropped)[lw.slot] = 1;
+          out[lw.slot] = NAN;
+          for (int lg = 0; lg < W; ++lg) {
+            Lane& ln = L[g * W + lg];
+            ln.rem = lg + 1;
+            ln.ev = lg;
+            ln.res_rem = INT_MIN;
+            ln.tau = kNoTau;
+            ln.live = false;  // the rest of the dropped pair is garbage nothing reads
+            ln.pair = -1;
+          }
+        }
+      }
       // advance and the next step's reads (before the halo store)
       int kh[64];
       for (int l = 0; l < 64; ++l) {
@@ -339,6 +391,7 @@ static std::vector<float> emulate(const std::vector<Pair>& pairs, int band, int 
         if (ln.wrapped) {
           ++ln.strip;
           ln.row += W;
+          if (BAND) ln.rlast = last_chunk(std::min(ln.row, ln.R - 1), ln.R, ln.C, ln.wbl);
           ln.pcs = ln.cs;
           ln.pce = ln.ce;
           if (!ln.done && ln.live && ln.strip < ln.S) CHK(ln.nx_pair == ln.pair);
@@ -420,7 +473,7 @@ static void halo_of(int W, int U, int cmax_c, int wmax_c, int& hlen, int& hmask,
 
 template <int MEAS, int W, int U>
 static int run(std::mt19937& rng, int trials, int div) {
-  int bad = 0;
+  int bad = 0, drops = 0;
   for (int t = 0; t < trials; ++t) {
     const int maxlen = std::uniform_int_distribution<int>(1, 400)(rng);
     const int npairs = std::uniform_int_distribution<int>(1, 30)(rng);
@@ -452,14 +505,32 @@ static int run(std::mt19937& rng, int trials, int div) {
     const float eps = 0.03f;
     int ccap = ((cmax_c + 3) / 4) * 4;
     ccap = ((ccap + U - 1) / U) * U;  // launch_dp_s: whole chunks
-    std::vector<float> got = emulate<MEAS, W, U>(pairs, band, delta, eps, hlen, hmask, ccap, rmax, smax, 1 + t % 3);
+    std::vector<float> wants(npairs, 0.0f);
+    for (int i = 0; i < npairs; ++i) {
+      Pair& p = pairs[i];
+      const int n = p.a.size() / 2, m = p.b.size() / 2;
+      if (MEAS == DTW) wants[i] = orc_dtw(p.a.data(), n, p.b.data(), m, band);
+      else wants[i] = (float)orc_lcss(p.a.data(), n, p.b.data(), m, eps, delta) / (float)std::min(n, m);
+      static const float kFactor[7] = {0.25f, 0.8f, 1.0f, 1.0f, 1.0001f, 3.0f, INFINITY};
+      const int f = std::uniform_int_distribution<int>(0, 6)(rng);
+      p.tau = f == 3 ? nextafterf(wants[i], 0.0f) : wants[i] * kFactor[f];
+    }
+    std::vector<int> dropped;
+    std::vector<float> got = emulate<MEAS, W, U>(pairs, band, delta, eps, hlen, hmask, ccap, rmax, smax, 1 + t % 3,
+                                                 &dropped);
     for (int i = 0; i < npairs; ++i) {
       if (pairs[i].dead) continue;
       const Pair& p = pairs[i];
       const int n = p.a.size() / 2, m = p.b.size() / 2;
-      float want;
-      if (MEAS == DTW) want = orc_dtw(p.a.data(), n, p.b.data(), m, band);
-      else want = (float)orc_lcss(p.a.data(), n, p.b.data(), m, eps, delta) / (float)std::min(n, m);
+      const float want = wants[i];
+      if (dropped[i]) {
+        ++drops;
+        if (!(want > p.tau)) {
+          if (bad < 5) fprintf(stderr, "MEAS %d W %d U %d trial %d pair %d: dropped at value %.9g, tau %.9g\n", MEAS, W, U, t, i, want, p.tau);
+          ++bad;
+        }
+        continue;
+      }
       if (memcmp(&want, &got[i], 4) != 0) {
         if (bad < 5) fprintf(stderr, "MEAS %d W %d U %d trial %d pair %d n %d m %d band %d delta %d: got %.9g want %.9g\n",
                              MEAS, W, U, t, i, n, m, band, delta, got[i], want);
@@ -467,13 +538,14 @@ static int run(std::mt19937& rng, int trials, int div) {
       }
     }
   }
-  printf("MEAS %d W %2d U %d: %d trials, %d mismatches\n", MEAS, W, U, trials, bad);
+  printf("MEAS %d W %2d U %d: %d trials, %d pairs dropped early, %d mismatches\n", MEAS, W, U, trials, drops, bad);
   return bad;
 }
 
 int main(int argc, char** argv) {
   const int T = argc > 1 ? std::atoi(argv[1]) : 40;
   const int div = argc > 2 ? std::atoi(argv[2]) : 1;
+  g_late = argc > 3 && std::strcmp(argv[3], "late") == 0;
   std::mt19937 rng(4242);
   int bad = 0;
   bad += run<DTW, 16, 1>(rng, T, div); bad += run<DTW, 16, 4>(rng, T, div); bad += run<DTW, 32, 2>(rng, T, div);

@@ -37,14 +37,23 @@ def test_lean_dp_schedule_model_matches_oracle(tmp_path):
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
 def test_lean_ranged_dp_schedule_model_matches_oracle(tmp_path):
+    """k_dp_lean_r's schedule for banded DTW (with early abandoning against
+    thresholds around the pairs' values) and windowed LCSS: values equal the
+    oracle's, no pair at or below its threshold is dropped, pairs are
+    dropped. A halo ring half the launch's size, and a row minimum tested at
+    the strip's padded end instead of the row's last in-band chunk, are
+    caught."""
     exe = str(tmp_path / "dp_lean_ranged_schedule")
     subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fopenmp", "-o", exe,
                     os.path.join(HERE, "emu", "dp_lean_ranged_schedule.cpp"),
                     os.path.join(ROOT, "oracle", "oracle.cpp")], check=True, timeout=300)
     r = subprocess.run([exe, "40"], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
-    # a ring half the size the launch gives is caught
+    drops = [int(line.split(",")[1].split()[0]) for line in r.stdout.splitlines() if "dropped early" in line]
+    assert sum(drops) > 0
     r = subprocess.run([exe, "20", "2"], capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0, r.stdout[-2000:]
+    r = subprocess.run([exe, "20", "1", "late"], capture_output=True, text=True, timeout=300)
     assert r.returncode != 0, r.stdout[-2000:]
 
 
@@ -53,8 +62,10 @@ def test_multi_wave_dp_schedule_model_matches_oracle(tmp_path):
     """k_dp_mw (a pair over 2 or 4 waves, barriers every 8 steps, two LDS
     halo buffers): every halo read finds the right strip and chunk written
     before the barrier that precedes it, with the waves of a superstep run
-    in either order; values equal the oracle's bit for bit. A lag of 70
-    steps between strips (the kernel uses 72, the least safe is 71) is
+    in either order; values equal the oracle's bit for bit; DTW and Frechet
+    pairs dropped early (against thresholds around their values) exceed
+    their threshold, and are dropped before their result is written. A lag
+    of 70 steps between strips (the kernel uses 72, the least safe is 71) is
     caught."""
     exe = str(tmp_path / "dp_mw_schedule")
     subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fopenmp", "-o", exe,
@@ -62,6 +73,7 @@ def test_multi_wave_dp_schedule_model_matches_oracle(tmp_path):
                    check=True, timeout=300)
     r = subprocess.run([exe, "12"], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+    assert int(r.stdout.split("failures, ")[1].split()[0]) > 0  # pairs dropped early
     r = subprocess.run([exe, "3", "70"], capture_output=True, text=True, timeout=300)
     assert r.returncode != 0, r.stdout[-2000:]
 
