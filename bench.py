@@ -778,6 +778,8 @@ def main():
         args.n_db = args.n_db or 20_000
         args.n_q = args.n_q or 256
         args.plain_sample = min(args.plain_sample, 16)
+        if args.copy_calibration:
+            copy_calibration()
         return run_c4(ts, args)
     if args.config == "c5":
         args.n_db = args.n_db or 2_000_000
