@@ -579,7 +579,10 @@ __device__ __forceinline__ int next_event(int rem, int lg, int res_rem) {
     /* in its slot) and fetches the next one as if the pair had ended. The */                       \
     /* test comes before the row's idle chunks: during those a lane of the */                       \
     /* next strip may already hold the pair's result (its chunk lastch comes */                     \
-    /* D > W-1 steps later than lane W-1's, so not before). */                                      \
+    /* D > W-1 steps later than lane W-1's, so not before). Lane W-1 alone */                       \
+    /* tests: it starts a pair last (no lane still owes the previous pair's */                      \
+    /* result) and never tests in the pair's last strip (where lane 0 may */                        \
+    /* already run the next pair). */                                                               \
     if (MEAS != TS_LCSS) {                                                                          \
       float mn = on[0];                                                                             \
       _Pragma("unroll")                                                                             \
