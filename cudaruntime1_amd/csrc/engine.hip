@@ -769,7 +769,7 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   // units: the pairs it tests
   TS_TRY(run(e, "tight_filter", 0, (double)total, [&] {
     return launch_tight_filter(e->st, ma, k, qv, q0, dv, tau, citems, ntot, total, items2, nitems2, rvals,
-                               device_cu_count() * 16);
+                               device_cu_count() * 32);
   }));
   // the main refine abandons pairs whose value must exceed the threshold
   TS_TRY(refine(e, p, db, qs, items2, nitems2, cap, rvals, false, ctrs + 32, nullptr, nullptr, tau, q0));

@@ -573,14 +573,14 @@ hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetVie
 // monotone). Hence, with d(p, B) the point-to-box distance:
 //   DTW       >= max( d(q0,c0) + sum_{0<i<n-1} d(q_i, B_c) + d(q_n-1,c_m-1),
 //                     the same over c's points against B_q ),
-//             discounted by (L+72)*2^-23 for the f32 sums (tree order here,
-//             running order in the DP); with a band, B_c is the box of c's
-//             points inside row i's band (band_side_sum), and likewise;
+//             discounted for the f32 sums (dtw_disc: tree order here,
+//             running order in the DP); without a band, B_c is the nearest
+//             of the boxes that tile c (tile_pass2), with a band the box of
+//             c's points inside row i's band (band_side_sum), and likewise;
 //   Frechet   >= max( max_i d(q_i,B_c), max_j d(c_j,B_q), endpoint costs );
 //   Hausdorff >= max( max_i d(q_i,B_c), max_j d(c_j,B_q) );
 //             and then, with B_c the nearest of the 64-point tiles of c
-//             instead of its MBR, the same forms again (tile_side_*; DTW
-//             without a band likewise by tile_side_sum / tile_side_sums2);
+//             instead of its MBR, the same forms again (tile_side_*);
 //   Euclidean >= sqrt(sum_i d(q_i,B_c)^2) (double, discounted 1e-6);
 //   LCSS      <= min(#q points in c's eps-box, #c points in q's eps-box,
 //                    min(n,m)) / min(n,m); with a window delta, the box of
@@ -666,31 +666,6 @@ __device__ float band_side_sum(const float* ax, const float* ay, int na, const f
   return total;
 }
 
-// Unbanded DTW form of one side for long pairs: row i of `a` has a path cell
-// somewhere in b, so it costs at least the distance from a_i to the nearest
-// of the boxes that tile b (64-point superblocks at blocks 0, 8, 16, ...).
-// Sum over the points of a except the first and last, 64 rows at a time,
-// returning early once (total + base) * disc exceeds limit.
-__device__ float tile_side_sum(const float* ax, const float* ay, int na, const float4* bsblk, int nb, float base,
-                               float disc, float limit) {
-  const int lane = threadIdx.x & 63;
-  const int nblk = (nb + 7) >> 3;
-  float total = 0.0f;
-  for (int i0 = 1; i0 < na - 1; i0 += 64) {
-    const int i = i0 + lane;
-    float acc = 0.0f;
-    if (i < na - 1) {
-      const float px = ax[i], py = ay[i];
-      float best = pt_box(px, py, bsblk[0]);
-      for (int kb = 8; kb < nblk; kb += 8) best = fminf(best, pt_box(px, py, bsblk[kb]));
-      acc = sqrt_lo(best);
-    }
-    total += wave_sum(acc);
-    if ((total + base) * disc > limit) break;  // uniform: total is the same in every lane
-  }
-  return total;
-}
-
 // Sum over the wave (all lanes active) by DPP: prefix sums within rows of 16,
 // rows combined by row_bcast 15 and 31; lane 63's total returned to every lane.
 template <int CTRL, int ROWS>
@@ -707,44 +682,71 @@ __device__ __forceinline__ float wave_sum_dpp(float v) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
-// The same form for short pairs, both sides in one pass: the interior points
-// of a (all but the first and last) against b's superblock boxes into sa,
-// those of b against a's into sb. The
-// nearest superblock box lies inside the MBR, so each side also bounds the
-// MBR form from above. Loads of both trajectories and of up to 8 superblock
-// boxes of each go out together; one reduction per side. The f32 sums are in
-// a different order than a serial sum; the caller's factor disc covers any
-// order.
-__device__ void tile_side_sums2(const float* ax, const float* ay, int na, const float4* asb, const float* bx,
-                                const float* by, int nb, const float4* bsb, float& sa, float& sb) {
+// Boxes that tile a trajectory of n points, into the wave's LDS slice `dst`
+// (one per lane, at most 64): its 64-point superblocks (every 8th entry of a
+// set's sblk), or for trajectories of more than 4096 points the unions of
+// g = 2, 4, ... consecutive superblocks (a coarser tiling; every box still
+// lies inside the MBR). Returns the number of boxes.
+constexpr int kTileBoxes = 64;
+__device__ int stage_tiles(const float4* sblk, int n, float4* dst) {
   const int lane = threadIdx.x & 63;
-  const int nsa = (((na + 7) >> 3) + 7) >> 3, nsbx = (((nb + 7) >> 3) + 7) >> 3;  // superblocks
+  const int ns = (((n + 7) >> 3) + 7) >> 3;
+  int g = 1;
+  while (ns > g * kTileBoxes) g <<= 1;
+  const int nb = (ns + g - 1) / g;
+  if (lane < nb) {
+    float4 b = sblk[8 * lane * g];
+    for (int u = lane * g + 1; u < min(ns, (lane + 1) * g); ++u) {
+      const float4 t = sblk[8 * u];
+      b.x = fminf(b.x, t.x); b.y = fmaxf(b.y, t.y); b.z = fminf(b.z, t.z); b.w = fmaxf(b.w, t.w);
+    }
+    dst[lane] = b;
+  }
+  return nb;
+}
+
+// Tiled form of both sides of an unbanded DTW pair in one pass: the interior
+// points of a (all but the first and last) against the nearest of the boxes
+// that tile b, summed into sa, and those of b against a's into sb (row i of a
+// has a path cell somewhere in b, in a row of its own). The nearest tile box
+// lies inside the MBR, so each side also bounds the MBR form from above. The
+// boxes sit in the wave's LDS slice (abox, bbox:
+// nsa and nsb of them), read by all lanes at once; both trajectories' next
+// 64-point chunk is loaded before this chunk's boxes are scanned (indices
+// clamped to the last point, so every load stays inside the trajectory).
+// After each chunk both partial sums are reduced; the pair fails as soon as
+// the larger, plus the endpoint costs e and scaled by disc, exceeds t: the
+// partial sums of non-negative terms never exceed the full sums. The f32
+// sums are in a different order than a serial sum; disc covers it.
+__device__ bool tile_pass2(const float* ax, const float* ay, int na, const float* bx, const float* by, int nb,
+                           const float4* abox, int nsa, const float4* bbox, int nsb, float e, float disc, float t) {
+  const int lane = threadIdx.x & 63;
   const int L = max(na, nb);
-  float a = 0.0f, b = 0.0f;
+  float sa = 0.0f, sb = 0.0f;
+  float px = ax[min(lane, na - 1)], py = ay[min(lane, na - 1)];
+  float rx = bx[min(lane, nb - 1)], ry = by[min(lane, nb - 1)];
   for (int i0 = 0; i0 < L; i0 += 64) {
     const int i = i0 + lane;
-    const bool ia = i >= 1 && i < na - 1, ib = i >= 1 && i < nb - 1;
-    const float px = ia ? ax[i] : 0.0f, py = ia ? ay[i] : 0.0f;
-    const float rx = ib ? bx[i] : 0.0f, ry = ib ? by[i] : 0.0f;
-    float ba = 3.4e38f, bbest = 3.4e38f;
-    for (int s0 = 0; s0 < max(nsa, nsbx); s0 += 8) {
-      float4 tb[8], ta[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        tb[u] = s0 + u < nsbx ? bsb[8 * (s0 + u)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        ta[u] = s0 + u < nsa ? asb[8 * (s0 + u)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        if (s0 + u < nsbx) ba = fminf(ba, pt_box(px, py, tb[u]));
-        if (s0 + u < nsa) bbest = fminf(bbest, pt_box(rx, ry, ta[u]));
-      }
+    const int ja = min(i + 64, na - 1), jb = min(i + 64, nb - 1);
+    const float npx = ax[ja], npy = ay[ja], nrx = bx[jb], nry = by[jb];
+    float da = 3.4e38f, db = 3.4e38f;
+    if (i0 < na - 1) {
+#pragma unroll 4
+      for (int s = 0; s < nsb; ++s) da = fminf(da, pt_box(px, py, bbox[s]));
     }
-    if (ia) a += sqrt_lo(ba);
-    if (ib) b += sqrt_lo(bbest);
+    if (i0 < nb - 1) {
+#pragma unroll 4
+      for (int s = 0; s < nsa; ++s) db = fminf(db, pt_box(rx, ry, abox[s]));
+    }
+    sa += wave_sum_dpp(i >= 1 && i < na - 1 ? sqrt_lo(da) : 0.0f);
+    sb += wave_sum_dpp(i >= 1 && i < nb - 1 ? sqrt_lo(db) : 0.0f);
+    if (!((fmaxf(sa, sb) + e) * disc <= t)) return false;  // uniform: the sums are the same in every lane
+    px = npx;
+    py = npy;
+    rx = nrx;
+    ry = nry;
   }
-  sa = wave_sum_dpp(a);
-  sb = wave_sum_dpp(b);
+  return true;
 }
 
 // Tiled max form (Frechet, Hausdorff): every point of a is matched to some
@@ -819,15 +821,26 @@ __device__ int window_side_count(const float* ax, const float* ay, int na, const
 // one atomic per block at the end (a single hot counter per pair serialises
 // in L2); a full buffer sends the rest directly.
 constexpr int kTightBuf = 256;
-// Unbanded DTW pairs up to this length skip the whole-MBR form.
-constexpr int kTiledDirect = 256;
 
-template <int MEAS>
+// The factor that discounts a DTW bound of pairs up to L points per side for
+// f32 rounding: the DP's running sum along a path of at most 2L-1 cells can
+// fall short of the exact sum by (L-1)*2^-23 of it, the bound's sums (64-lane
+// reductions chained over L/64 chunks, then the endpoint costs) can exceed
+// theirs by (L/64 + 9)*2^-24, and the final product rounds once more.
+__device__ __forceinline__ float dtw_disc(int L) {
+  return 1.0f - (float)(L + (L >> 6) + 72) * 1.1920929e-7f;
+}
+
+// TILED: unbanded DTW, every pair by the one-pass tiled form.
+template <int MEAS, bool TILED>
 __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k, SetView q, int32_t q0, SetView db,
                                                       const float* tau, const Item* items, const int32_t* nitems,
                                                       int64_t max_items, Item* out, int32_t* nout, float* rej) {
+  static_assert(!TILED || MEAS == TS_DTW, "the tiled pass is DTW's");
   __shared__ Item sbuf[kTightBuf];
   __shared__ int scnt, sbase;
+  // each wave's slice: the query's tile boxes, then the DB trajectory's
+  __shared__ float4 sbox[TILED ? 4 : 1][2 * kTileBoxes];
   if (threadIdx.x == 0) scnt = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -885,16 +898,19 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
       side_bound<MEAS>(q.x + qo, q.y + qo, n, cb, ma.eps, false, f, s, c);
       s = wave_sum_d(s);
       pass = (float)(sqrt(s) * (1.0 - 1e-6)) <= t;
-    } else if (MEAS == TS_DTW && ma.band == 0 && n >= 3 && m >= 3 && max(n, m) <= kTiledDirect) {
-      // short pairs: the tiled form of both sides directly (it implies the
-      // MBR form, and a point meets at most 4 tiles)
+    } else if (TILED) {
+      // the tiled form of both sides (it implies the MBR form). Rows 0 and
+      // n-1 hold the cells (0,0) and (n-1,m-1), distinct unless n = m = 1.
+      float4* wbox = sbox[threadIdx.x >> 6];
+      const int nsa = stage_tiles(q.sblk + (qo >> 3), n, wbox);
+      const int nsc = stage_tiles(db.sblk + (co >> 3), m, wbox + kTileBoxes);
+      __builtin_amdgcn_wave_barrier();
       const float dx0 = qe.x - ce.x, dy0 = qe.y - ce.y, dx1 = qe.z - ce.z, dy1 = qe.w - ce.w;
-      const float e = sqrt_lo(dx0 * dx0 + dy0 * dy0) + sqrt_lo(dx1 * dx1 + dy1 * dy1);
-      const float disc = 1.0f - (float)(max(n, m) + 72) * 1.1920929e-7f;
-      float b1, b2;
-      tile_side_sums2(q.x + qo, q.y + qo, n, q.sblk + (qo >> 3), db.x + co, db.y + co, m, db.sblk + (co >> 3), b1,
-                      b2);
-      pass = fmaxf(b1 + e, b2 + e) * disc <= t;
+      const float e0 = sqrt_lo(dx0 * dx0 + dy0 * dy0);
+      const float e = n >= 2 || m >= 2 ? e0 + sqrt_lo(dx1 * dx1 + dy1 * dy1) : e0;
+      pass = tile_pass2(q.x + qo, q.y + qo, n, db.x + co, db.y + co, m, wbox, nsa, wbox + kTileBoxes, nsc, e,
+                        dtw_disc(max(n, m)), t);
+      __builtin_amdgcn_wave_barrier();  // every lane's box reads before the next pair's writes
     } else if (MEAS == TS_DTW) {
       float a1 = 0.0f, a2 = 0.0f;
       double fd = 0.0;
@@ -908,23 +924,13 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
       // Row form: rows 0 and n-1 contain (0,0) and (n-1,m-1); with n == 1 the
       // single row holds both when m >= 2. Column form likewise.
       const int L = max(n, m);
-      const float disc = 1.0f - (float)(L + 72) * 1.1920929e-7f;
+      const float disc = dtw_disc(L);
       auto bound = [&](float s1, float s2) {
         const float r1 = n >= 2 ? s1 + e0 + e1 : (m >= 2 ? e0 + e1 : e0);
         const float r2 = m >= 2 ? s2 + e0 + e1 : (n >= 2 ? e0 + e1 : e0);
         return fmaxf(r1, r2) * disc;
       };
       pass = bound(a1, a2) <= t;
-      if (pass && ma.band == 0 && n >= 3 && m >= 3) {
-        // The whole-MBR form passed: the tiled form (nearest superblock box)
-        // is tighter and costs O(m / 64) per point.
-        const float b1 = tile_side_sum(q.x + qo, q.y + qo, n, db.sblk + (co >> 3), m, e0 + e1, disc, t);
-        pass = (b1 + e0 + e1) * disc <= t;
-        if (pass) {
-          const float b2 = tile_side_sum(db.x + co, db.y + co, m, q.sblk + (qo >> 3), n, e0 + e1, disc, t);
-          pass = bound(b1, b2) <= t;
-        }
-      }
       if (pass && ma.band != 0 && n >= 3 && m >= 3) {
         // The whole-MBR form passed: the banded form (row windows of the
         // band) is tighter and costs O(window / 64) per point. Each side alone
@@ -979,16 +985,37 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
   for (int i = threadIdx.x; i < n; i += blockDim.x) out[sbase + i] = sbuf[i];
 }
 
+// At most grid_waves waves, and no more blocks than fit on the device at once
+// (each block's share of the pairs is fixed at launch).
+template <int MEAS, bool TILED>
+void launch_tight(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, SetView db, const float* tau,
+                  const Item* items, const int32_t* nitems, int64_t max_items, Item* out, int32_t* nout, float* rej,
+                  int grid_waves) {
+  static int per_cu = 0;
+  if (!per_cu && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tight_filter<MEAS, TILED>, 256, 0) !=
+                      hipSuccess ||
+                  per_cu < 1))
+    per_cu = 1;
+  const dim3 grid(std::max(1, std::min((grid_waves + 3) / 4, per_cu * device_cu_count())));
+  hipLaunchKernelGGL((k_tight_filter<MEAS, TILED>), grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems,
+                     max_items, out, nout, rej);
+}
+
 hipError_t launch_tight_filter(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, SetView db,
                                const float* tau, const Item* items, const int32_t* nitems, int64_t max_items,
                                Item* out, int32_t* nout, float* rej, int grid_waves) {
-  const dim3 grid((grid_waves + 3) / 4);
   switch (m.measure) {
-    case TS_DTW: hipLaunchKernelGGL(k_tight_filter<TS_DTW>, grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout, rej); break;
-    case TS_FRECHET: hipLaunchKernelGGL(k_tight_filter<TS_FRECHET>, grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout, rej); break;
-    case TS_LCSS: hipLaunchKernelGGL(k_tight_filter<TS_LCSS>, grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout, rej); break;
-    case TS_HAUSDORFF: hipLaunchKernelGGL(k_tight_filter<TS_HAUSDORFF>, grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout, rej); break;
-    case TS_EUCLIDEAN: hipLaunchKernelGGL(k_tight_filter<TS_EUCLIDEAN>, grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout, rej); break;
+#define TS_TIGHT(MEAS, TILED) \
+  launch_tight<MEAS, TILED>(st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout, rej, grid_waves)
+    case TS_DTW:
+      if (m.band == 0) TS_TIGHT(TS_DTW, true);
+      else TS_TIGHT(TS_DTW, false);
+      break;
+    case TS_FRECHET: TS_TIGHT(TS_FRECHET, false); break;
+    case TS_LCSS: TS_TIGHT(TS_LCSS, false); break;
+    case TS_HAUSDORFF: TS_TIGHT(TS_HAUSDORFF, false); break;
+    case TS_EUCLIDEAN: TS_TIGHT(TS_EUCLIDEAN, false); break;
+#undef TS_TIGHT
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
