@@ -23,6 +23,7 @@ MEASURES = {"dtw": DTW, "lcss": LCSS, "frechet": FRECHET, "hausdorff": HAUSDORFF
 REF, EXACT = 0, 1
 NOALLOC_MAX_PAIRS = 16384  # include/trajsim.h TS_NOALLOC_MAX_PAIRS
 SET_TINY_COORD = 1  # TrajSet.flags(): a coordinate c with 0 < |c| < 2^-25
+SET_HUGE_COORD = 2  # TrajSet.flags(): a coordinate c with |c| >= 2^62, or not finite
 
 if not os.path.exists(LIB_PATH):
     raise ImportError("libtrajsim.so is not built (run `python cudaruntime1_amd/build.py` or "
@@ -266,7 +267,7 @@ class TrajSet:
 
     def flags(self):
         """Bits found in the points at load: SET_TINY_COORD (a coordinate c
-        with 0 < |c| < 2^-25)."""
+        with 0 < |c| < 2^-25), SET_HUGE_COORD (|c| >= 2^62 or not finite)."""
         f = ctypes.c_int32()
         _check(_lib.ts_set_flags(self._h, ctypes.byref(f)), "ts_set_flags")
         return f.value

@@ -71,6 +71,9 @@ struct DpArgs {
   const float* tau;
   int32_t tau_q0;
   unsigned long long* cells;
+  // k_dp_lean, DTW: entries of each lane group's suffix-bound table (a
+  // multiple of 4; 0: no table), see lean_suffix_table.
+  int32_t sfx;
 };
 
 // v of the previous lane of the group; the group's lane 0 gets first0.
@@ -477,9 +480,62 @@ __device__ __forceinline__ void stage_pts(float2* dst, const float* gx, const fl
 }
 
 // LDS per lane group of k_dp_lean, in floats: the halo row, with SC the two
-// column buffers of (x, y) points, with SR the row buffer.
-__host__ __device__ constexpr int lean_group_floats(int cmax, int rmax, bool sc, bool sr) {
-  return cmax + (sc ? 4 * cmax : 0) + (sr ? 2 * rmax : 0);
+// column buffers of (x, y) points, with SR the row buffer, then the suffix
+// table (sfx entries).
+__host__ __device__ constexpr int lean_group_floats(int cmax, int rmax, bool sc, bool sr, int sfx) {
+  return cmax + (sc ? 4 * cmax : 0) + (sr ? 2 * rmax : 0) + sfx;
+}
+
+// Suffix bounds of an unbanded DTW pair for early abandoning, built by the
+// group at the pair's fetch. Every warping path visits each row i of A (the
+// R-point row trajectory) at some cell, whose point cost is at least
+// lb_i = sqrt_lo of the squared distance from a_i to the nearest 64-point
+// box of B (the column trajectory; the set's superblock boxes, staged here
+// into the group's halo row, which no lane reads during a fetch: the pair
+// lane W-1 still runs is in its last strip, whose halo writes are never
+// read, and the new pair's first strip reads the border row). Entry s of
+// the table is the sum of lb_i over the rows of strips s+1, s+2, ...: the
+// path's cost after row sW+W-1 is at least that, so the pair's value is at
+// least the row's minimum plus it. Lane lg sums the W rows of strips lg,
+// lg+W, ... by itself, and a suffix scan over the group's lanes, a round of
+// W strips at a time from the last, turns the strip sums into entries: no
+// cross-lane step per strip. All in f32 over non-negative terms, so every
+// entry is at most the exact sum times (1 + 2^-24)^(R-1) whatever the order;
+// the test's discount covers that and the DP's own rounding. S - 1 entries
+// for S strips.
+template <int W>
+__device__ __forceinline__ void lean_suffix_table(float* tbl, float4* boxes, const float4* bsblk, int C,
+                                                  const float* ax, const float* ay, int R, int lg) {
+  const int nbox = (((C + 7) >> 3) + 7) >> 3;
+  for (int t = lg; t < nbox; t += W) boxes[t] = bsblk[8 * t];
+  __builtin_amdgcn_wave_barrier();
+  const int S = (R + W - 1) / W;
+  const int K = (S + W - 1) / W;  // rounds of W strips
+  float carry = 0.0f;             // strips of the rounds after this one
+  for (int k = K - 1; k >= 0; --k) {
+    const int s = k * W + lg;  // this lane's strip
+    float ts = 0.0f;
+    if (s >= 1 && s < S) {
+      const int i0 = s * W, i1 = min(i0 + W, R);
+      for (int i = i0; i < i1; ++i) {
+        const float px = ax[i], py = ay[i];
+        float m = pt_box(px, py, boxes[0]);
+        for (int t = 1; t < nbox; ++t) m = fminf(m, pt_box(px, py, boxes[t]));
+        ts += sqrt_lo(m);
+      }
+    }
+    // suffix scan: v = sum over strips s..(k+1)W-1 of this round, plus carry
+    float v = ts;
+#pragma unroll
+    for (int d = 1; d < W; d <<= 1) {
+      const float u = __shfl_down(v, d, W);
+      v += lg + d < W ? u : 0.0f;
+    }
+    v += carry;
+    if (s >= 1 && s < S) tbl[s - 1] = v;
+    carry = __shfl(v, 0, W);
+  }
+  __builtin_amdgcn_wave_barrier();
 }
 
 // Next event of a lane of the lean kernels, tested at the top of its loop:
@@ -529,13 +585,15 @@ __device__ __forceinline__ int next_event(int rem, int lg, int res_rem) {
     if (SR) {                                                                                       \
       rp = rows[min(row, R - 1)];                                                                   \
     } else {                                                                                        \
-      rp = make_float2(Ax[min(row, R - 1)], Ay[min(row, R - 1)]);                                   \
+      const uint32_t ri = (uint32_t)min(row, R - 1);                                                \
+      rp = make_float2(Ax[ri], Ay[ri]);                                                             \
     }                                                                                               \
     if (SC) {                                                                                       \
       load_pts<U>(cb + k * U, cx, cy);                                                              \
     } else {                                                                                        \
-      load_vec<U>(Bx + U * min(k, lastch), cx);                                                     \
-      load_vec<U>(By + U * min(k, lastch), cy);                                                     \
+      const uint32_t ci = (uint32_t)(U * min(k, lastch));                                           \
+      load_vec<U>(Bx + ci, cx);                                                                     \
+      load_vec<U>(By + ci, cy);                                                                     \
     }                                                                                               \
     __builtin_amdgcn_sched_barrier(0);                                                              \
                                                                                                     \
@@ -589,8 +647,12 @@ __device__ __forceinline__ int next_event(int rem, int lg, int res_rem) {
       for (int j = 1; j < U; ++j) mn = fminf(mn, on[j]);                                            \
       rowmin = fminf(kh == 0 ? kNoTau : rowmin, mn);                                                \
       bool stop = false;                                                                            \
-      if (lg == W - 1 && kh == lastch && rowh + 1 < R)                                              \
+      if (lg == W - 1 && kh == lastch && rowh + 1 < R) {                                            \
         stop = (MEAS == TS_DTW ? rowmin : sqrtf(rowmin)) > tau_l;                                   \
+        /* DTW: the row's minimum plus the rows below it (suffix table) */                          \
+        if (MEAS == TS_DTW && disc_l > 0.0f)                                                        \
+          stop = stop || (rowmin + sfxt[(uint32_t)rowh / W]) * disc_l > tau_l;                               \
+      }                                                                                             \
       const uint64_t ab = __ballot(stop);                                                           \
       if (__builtin_expect(ab != 0, 0) && ((ab >> (g * W + W - 1)) & 1ull)) {                       \
         if (lg == W - 1) {                                                                          \
@@ -629,7 +691,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   const int hdr = ((nwaves * G + 3) >> 2) << 2;
   int* slot_item = (int*)lds_dyn4 + wv * G + g;
   float* brow = (float*)lds_dyn4 + hdr + wv * 8;
-  float* gbase = (float*)lds_dyn4 + hdr + nwaves * 8 + (size_t)(wv * G + g) * lean_group_floats(cmax, rmax, SC, SR);
+  float* gbase =
+      (float*)lds_dyn4 + hdr + nwaves * 8 + (size_t)(wv * G + g) * lean_group_floats(cmax, rmax, SC, SR, a.sfx);
+  float* sfxt = gbase + lean_group_floats(cmax, rmax, SC, SR, 0);  // [a.sfx] suffix table
   if (lane < 8) brow[lane] = BND;
   float* bnd = gbase;                                         // [cmax] halo row
   float2* cols = (float2*)(gbase + cmax);                     // SC: [2][cmax] column points
@@ -658,6 +722,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   // the lane's row
   const float kNoTau = __int_as_float(0x7f800000);
   float f_tau = kNoTau, tau_l = kNoTau, rowmin = kNoTau;
+  // fetched / current pair's suffix-bound discount (DTW; 0: no suffix test)
+  float f_disc = 0.0f, disc_l = 0.0f;
   float o[U], cx[U], cy[U], hv[U];
 #pragma unroll
   for (int j = 0; j < U; ++j) { o[j] = 0.0f; hv[j] = BND; cx[j] = 0.0f; cy[j] = 0.0f; }
@@ -705,6 +771,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             f_tau = (MEAS != TS_LCSS && a.tau) ? a.tau[it.q - a.tau_q0] : kNoTau;
             if (SC) stage_pts<W>(cols + gpar * cmax, fBx, fBy, fC, lg);
             if (SR) stage_pts<W>(rows, fAx, fAy, fR, lg);
+            // Suffix bounds (DTW pairs with a finite threshold, coordinates
+            // in the range where the bound's rounding argument holds, more
+            // than one strip and a table that holds them; uniform in the
+            // group). A row's minimum plus its table entry, scaled by
+            // 1 - (2R + C + 64) 2^-23, exceeding the threshold proves the
+            // pair's value does: the value is at least (row minimum + exact
+            // suffix) (1 - 2^-24)^(R+C) (the DP's f32 additions after the
+            // row), the table entry at most the exact suffix times
+            // (1 + 2^-24)^(R-1), and the test's own sum and product round
+            // twice more.
+            f_disc = 0.0f;
+            const int S = (fR + W - 1) / W;
+            if (MEAS == TS_DTW && a.sfx > 0 && !f_tiny && f_tau < kNoTau && S >= 2 && S - 1 <= a.sfx &&
+                4 * ((((fC + 7) >> 3) + 7) >> 3) <= cmax) {
+              const float4* bsblk = n >= m ? a.db.sblk + (dof >> 3) : a.q.sblk + (qo >> 3);
+              lean_suffix_table<W>(sfxt, (float4*)bnd, bsblk, fC, fAx, fAy, fR, lg);
+              f_disc = 1.0f - (float)(2 * fR + fC + 64) * 0x1p-23f;
+            }
           }
         }
         gpar ^= 1;
@@ -719,6 +803,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         } else {
           pair_tiny = f_live && f_tiny;
           tau_l = f_live ? f_tau : kNoTau;
+          disc_l = f_live ? f_disc : 0.0f;
           R = f_live ? fR : 1;
           C = f_live ? fC : 1;
           slot = f_slot;
@@ -1145,7 +1230,7 @@ static hipError_t launch_dp_t(hipStream_t st, const DpArgs& a, int grid_waves) {
   constexpr int G = 64 / W;
   const size_t gf = LEAN == 0   ? (size_t)dp_group_floats(a.cmax)
                     : LEAN >= 4 ? (size_t)lean_r_group_floats(a.cmax, a.ccap, a.rmax, a.smax, LEAN == 4)
-                                : (size_t)lean_group_floats(a.cmax, a.rmax, LEAN != 3, LEAN == 1);
+                                : (size_t)lean_group_floats(a.cmax, a.rmax, LEAN != 3, LEAN == 1, a.sfx);
   const size_t per_wave = (size_t)G * gf * sizeof(float) + G * sizeof(int) + 32;
   int wpb = 4;
   while (wpb > 1 && per_wave * wpb + 64 > 65536) --wpb;
@@ -1199,10 +1284,13 @@ static hipError_t launch_dp_s(hipStream_t st, DpArgs a, int grid_waves) {
       return launch_dp_t<MEAS, W, U, BAND, DELTA, 5>(st, a, grid_waves);
     return launch_dp_t<MEAS, W, U, BAND, DELTA, 0>(st, a, grid_waves);
   } else {
-    if ((size_t)G * lean_group_floats(a.cmax, a.rmax, true, true) * sizeof(float) <= 20480)
+    // DTW with thresholds: a suffix table of one entry per strip of the
+    // longest row trajectory, up to 256 (longer pairs test row minima only)
+    a.sfx = (MEAS == TS_DTW && a.tau) ? std::min(((a.rmax + W - 1) / W + 3) / 4 * 4, 256) : 0;
+    if ((size_t)G * lean_group_floats(a.cmax, a.rmax, true, true, a.sfx) * sizeof(float) <= 20480)
       return launch_dp_t<MEAS, W, U, BAND, DELTA, 1>(st, a, grid_waves);
     a.rmax = 0;
-    if ((size_t)G * lean_group_floats(a.cmax, 0, true, false) * sizeof(float) <= 20480)
+    if ((size_t)G * lean_group_floats(a.cmax, 0, true, false, a.sfx) * sizeof(float) <= 20480)
       return launch_dp_t<MEAS, W, U, BAND, DELTA, 2>(st, a, grid_waves);
     return launch_dp_t<MEAS, W, U, BAND, DELTA, 3>(st, a, grid_waves);
   }
@@ -1236,7 +1324,7 @@ hipError_t launch_dp(hipStream_t st, MeasureArgs m, int shape, SetView q, SetVie
     return launch_dp_mw(st, m, kDpShapeList[shape].W / 64, kDpShapeList[shape].U, q, db, items, nitems, ntail,
                         max_items, next, out, halo.len, rmax_c, grid_waves, tau, tau_q0, cells);
   DpArgs a{q,          db,      items,      nitems,  ntail,   max_items, next, out,   halo.len, halo.mask, rmax,
-           ccap,       0,       m.band,     m.delta, m.eps,   tau,       tau_q0, cells};
+           ccap,       0,       m.band,     m.delta, m.eps,   tau,       tau_q0, cells, 0};
   switch (m.measure) {
     case TS_DTW:
       return m.band != 0 ? launch_dp_b<TS_DTW, true, false>(st, a, shape, grid_waves)

@@ -587,12 +587,6 @@ hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetVie
 //                    the other trajectory's points within delta of the
 //                    point's index (window_side_count).
 // min/max do not round, so the Frechet/Hausdorff/LCSS forms are exact.
-__device__ __forceinline__ float pt_box(float px, float py, float4 b) {
-  const float gx = fmaxf(0.0f, fmaxf(b.x - px, px - b.y));
-  const float gy = fmaxf(0.0f, fmaxf(b.z - py, py - b.w));
-  return gx * gx + gy * gy;  // squared
-}
-
 __device__ __forceinline__ float wave_sum(float v) {
   for (int sh = 32; sh > 0; sh >>= 1) v += __shfl_xor(v, sh);
   return v;

@@ -50,9 +50,13 @@ int device_cu_count() {
 
 // Nonzero and below 2^-25 in magnitude.
 __device__ __forceinline__ bool tiny_coord(float c) { return c != 0.0f && fabsf(c) < 0x1p-25f; }
+// 2^62 or more in magnitude, or not finite: the squared distance to another
+// point may be +inf or NaN.
+__device__ __forceinline__ bool huge_coord(float c) { return !(fabsf(c) < 0x1p62f); }
 
 // One wave per trajectory: MBR by wave min/max, endpoints from the first and
-// last point, whether a point has a tiny coordinate.
+// last point, whether a point has a tiny (flag bit 1) or huge (bit 2)
+// coordinate.
 __global__ void __launch_bounds__(256) k_summaries(SetView s, int32_t t0, float4* mbr, float4* ends,
                                                    uint8_t* tiny) {
   const int lane = threadIdx.x & 63;
@@ -61,14 +65,15 @@ __global__ void __launch_bounds__(256) k_summaries(SetView s, int32_t t0, float4
   const int64_t o = s.off[t];
   const int n = s.len[t];
   float mnx = s.x[o], mxx = mnx, mny = s.y[o], mxy = mny;
-  bool tc = false;
+  bool tc = false, hc = false;
   for (int i = lane; i < n; i += 64) {
     float px = s.x[o + i], py = s.y[o + i];
     mnx = fminf(mnx, px); mxx = fmaxf(mxx, px);
     mny = fminf(mny, py); mxy = fmaxf(mxy, py);
     tc |= tiny_coord(px) || tiny_coord(py);
+    hc |= huge_coord(px) || huge_coord(py);
   }
-  const bool any_tc = __any(tc);
+  const bool any_tc = __any(tc), any_hc = __any(hc);
   for (int sh = 32; sh > 0; sh >>= 1) {
     mnx = fminf(mnx, __shfl_xor(mnx, sh));
     mxx = fmaxf(mxx, __shfl_xor(mxx, sh));
@@ -78,7 +83,7 @@ __global__ void __launch_bounds__(256) k_summaries(SetView s, int32_t t0, float4
   if (lane == 0) {
     mbr[t] = make_float4(mnx, mxx, mny, mxy);
     ends[t] = make_float4(s.x[o], s.y[o], s.x[o + n - 1], s.y[o + n - 1]);
-    tiny[t] = any_tc ? 1 : 0;
+    tiny[t] = (any_tc ? TS_SET_TINY_COORD : 0) | (any_hc ? TS_SET_HUGE_COORD : 0);
   }
 }
 
@@ -487,7 +492,7 @@ ts_status ts_set_flags(const ts_set* s, int32_t* flags) {
   std::vector<uint8_t> t(s->count);
   TS_HIP(hipDeviceSynchronize());
   TS_HIP(hipMemcpy(t.data(), s->tiny, s->count * sizeof(uint8_t), hipMemcpyDeviceToHost));
-  if (std::find(t.begin(), t.end(), 1) != t.end()) *flags |= TS_SET_TINY_COORD;
+  for (const uint8_t f : t) *flags |= f;
   return TS_OK;
 }
 

@@ -24,22 +24,48 @@ __device__ __forceinline__ float sqrt_lo(float x) {
   return x < 0x1p-96f ? 0.0f : __builtin_amdgcn_sqrtf(x) * (1.0f - 0x1p-21f);
 }
 
-// Correctly rounded sqrtf of U values. v_sqrt_f32 is within 1 ulp; for
-// x >= 2^-96, x = 0 and x = +inf, testing its two neighbours with fma
-// residuals (exact in that range) selects the correctly rounded root. That
-// is the expansion the compiler emits for sqrtf minus its rescaling of tiny
-// inputs and its 0/inf select, which these inputs do not need (0 and +inf
-// come out of the tests unchanged). Inputs in (0, 2^-96) need a trajectory
-// with a tiny coordinate (0 < |c| < 2^-25); `tiny` says whether a pair of the
-// wave's lanes has one (uniform), and then a wave holding any such x takes
-// sqrtf itself.
+// Squared distance from a point to a box (minx, maxx, miny, maxy). For any
+// point b of the box, the f32 squared distance from p to b is at least this
+// value: f32 subtraction, squaring of magnitudes and addition are monotone.
+__device__ __forceinline__ float pt_box(float px, float py, float4 b) {
+  const float gx = fmaxf(0.0f, fmaxf(b.x - px, px - b.y));
+  const float gy = fmaxf(0.0f, fmaxf(b.z - py, py - b.w));
+  return gx * gx + gy * gy;
+}
+
+// Correctly rounded sqrtf of U values. v_sqrt_f32 returns s within 1 ulp
+// of the root; the root rounds to s - 1 ulp, s or s + 1 ulp, and fma
+// residuals against the two midpoint products decide which: with
+// sm = s - 1 ulp (0 for s = 0) and sp = s + 1 ulp, the result is
+// sm + [x - sm*s > 0] + [x - sp*s > 0] in ulps. Each bracket is the residual's
+// bit pattern clamped to [0, 1] as an integer (v_med3_i32: +0 and negative
+// values give 0), and the sum one v_add3_u32, so no compare masks are
+// needed. The residuals are exact for x = 0 and x in [2^-96, 2^127), checked
+// on the CPU for every such float and every start within 1 ulp. Inputs in
+// (0, 2^-96) need a tiny coordinate (0 < |c| < 2^-25), and +inf or NaN a huge
+// or non-finite one (|c| >= 2^62); sets flag such trajectories, `tiny` says
+// whether a pair of the wave's lanes has one (uniform), and then a wave
+// holding any such x takes sqrtf itself.
+// v clamped to [0, 1] as a signed integer, in one v_med3_i32 (written out:
+// the compiler turns min/max into a compare and a carry, which costs a
+// lane-mask hazard per use).
+__device__ __forceinline__ uint32_t med3_01(int32_t v) {
+  uint32_t r;
+  asm("v_med3_i32 %0, %1, 0, 1" : "=v"(r) : "v"(v));
+  return r;
+}
+
 template <int U>
 __device__ __forceinline__ void sqrt_cr(float (&x)[U], bool tiny) {
   if (__builtin_expect(tiny, 0)) {
     uint32_t mb = __float_as_uint(x[0]) - 1u;  // 0 maps to the top
+    uint32_t mx = __float_as_uint(x[0]);
 #pragma unroll
-    for (int j = 1; j < U; ++j) mb = min(mb, __float_as_uint(x[j]) - 1u);
-    if (__builtin_expect(__any(mb < 0x0f7fffffu), 0)) {
+    for (int j = 1; j < U; ++j) {
+      mb = min(mb, __float_as_uint(x[j]) - 1u);
+      mx = max(mx, __float_as_uint(x[j]));
+    }
+    if (__builtin_expect(__any(mb < 0x0f7fffffu || mx >= 0x7f800000u), 0)) {
 #pragma unroll
       for (int j = 0; j < U; ++j) x[j] = sqrtf(x[j]);
       return;
@@ -48,10 +74,11 @@ __device__ __forceinline__ void sqrt_cr(float (&x)[U], bool tiny) {
 #pragma unroll
   for (int j = 0; j < U; ++j) {
     const float s = __builtin_amdgcn_sqrtf(x[j]);
-    const float sm = __uint_as_float(__float_as_uint(s) - 1u);
-    const float sp = __uint_as_float(__float_as_uint(s) + 1u);
-    const float t = __builtin_fmaf(-sm, s, x[j]) <= 0.0f ? sm : s;
-    x[j] = __builtin_fmaf(-sp, s, x[j]) > 0.0f ? sp : t;
+    const uint32_t sb = __float_as_uint(s);
+    const uint32_t smb = __builtin_elementwise_sub_sat(sb, 1u);
+    const float fm = __builtin_fmaf(-__uint_as_float(smb), s, x[j]);
+    const float fp = __builtin_fmaf(-__uint_as_float(sb + 1u), s, x[j]);
+    x[j] = __uint_as_float(smb + med3_01(__float_as_int(fm)) + med3_01(__float_as_int(fp)));
   }
 }
 

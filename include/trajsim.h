@@ -115,11 +115,13 @@ ts_status ts_set_bytes(const ts_set* s, int64_t* bytes);
 ts_status ts_set_lengths(const ts_set* s, int32_t* out_len);
 /* Properties found in the points when they were loaded (no counterpart in the
  * reference). Bit TS_SET_TINY_COORD: some trajectory has a coordinate c with
- * 0 < |c| < 2^-25. Between two trajectories without one, DTW's squared point
- * distances are 0 or at least 2^-96, and the DP kernels skip their test for
- * smaller ones; the kernels keep that flag per trajectory. Waits for the
- * set's pending uploads. */
+ * 0 < |c| < 2^-25. Bit TS_SET_HUGE_COORD: some trajectory has a coordinate
+ * with |c| >= 2^62, or one that is not finite. Between two trajectories
+ * without either, DTW's squared point distances are 0 or in [2^-96, 2^127),
+ * and the DP kernels skip their test for other values; the kernels keep
+ * both flags per trajectory. Waits for the set's pending uploads. */
 #define TS_SET_TINY_COORD 1
+#define TS_SET_HUGE_COORD 2
 ts_status ts_set_flags(const ts_set* s, int32_t* flags);
 /* Append trajectories (offsets[0] == 0, as for ts_set_create) behind the
  * set's current ones; they get the next indices. Device arrays grow when

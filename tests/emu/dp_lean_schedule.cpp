@@ -14,10 +14,17 @@
 // which fetches the next one at once. A pair whose value is at most its
 // threshold must never be dropped, a dropped one must exceed it, and the
 // pairs fetched after a drop pass every staging and halo check.
-// Usage: dp_lean_schedule [trials per shape] [late]; exit status 0 when all
-// match. "late" tests the row minimum at the row's last chunk including its
-// idle ones, a step too late: the model must then fail.
+// DTW pairs also test the row minimum plus the suffix table the group builds
+// at the fetch (the sum of each later row's bound to the nearest 64-point box
+// of the column trajectory), scaled by the kernel's discount: the boxes are
+// staged over the group's halo row, which the model then fills with tags no
+// real cell may read, and the bounds take v_sqrt_f32's largest result.
+// Usage: dp_lean_schedule [trials per shape] [late|nodisc]; exit status 0
+// when all match. "late" tests the row minimum at the row's last chunk
+// including its idle ones, a step too late, and "nodisc" drops the suffix
+// test's discount: the model must then fail.
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -43,7 +50,20 @@ enum { DTW = 0, LCSS = 1, FRECHET = 2 };
 
 // The kernels' next event (group fetch at rem == lg, pair start at rem == 0,
 // result at rem + 1 == res_rem): the largest trigger below rem, INT_MIN if none.
-static bool g_late = false;  // test rows at their padded end (a planted mistake)
+static bool g_late = false;    // test rows at their padded end (a planted mistake)
+static bool g_nodisc = false;  // suffix test without its discount (a planted mistake)
+static bool g_sfx1 = false;    // suffix table one strip short: it counts the tested strip's rows (a planted mistake)
+static long g_sfx_drops = 0;   // drops the suffix test made (with or without the row test)
+
+// sqrt_lo with v_sqrt_f32's largest result (1 ulp above the root)
+static float sqrt_lo_max(float x) {
+  return x < 0x1p-96f ? 0.0f : nextafterf(sqrtf(x), INFINITY) * (1.0f - 0x1p-21f);
+}
+static float pt_box(float px, float py, const float b[4]) {
+  const float gx = std::max(0.0f, std::max(b[0] - px, px - b[1]));
+  const float gy = std::max(0.0f, std::max(b[2] - py, py - b[3]));
+  return gx * gx + gy * gy;
+}
 
 static int next_event(int rem, int lg, int res_rem) {
   int ev = INT_MIN;
@@ -59,7 +79,7 @@ struct Tagged { float v; int pair; int idx; };
 
 template <int MEAS, int W, int U>
 std::vector<float> emulate(const std::vector<Pair>& pairs, float eps, int cmax, int rmax, int nwaves, long* steps,
-                           std::vector<int>* dropped) {
+                           std::vector<int>* dropped, int sfx) {
   constexpr int G = 64 / W;
   constexpr float BND = MEAS == LCSS ? 0.0f : kInf;
   const int n_items = pairs.size();
@@ -81,10 +101,12 @@ std::vector<float> emulate(const std::vector<Pair>& pairs, float eps, int cmax, 
     float up_last = 0, dfirst = 0;
     int lastres_row = -1, lastres_k = -1;
     float f_tau = INFINITY, tau = INFINITY, rowmin = INFINITY;
+    float f_disc = 0, disc = 0;
   };
   struct Group {
     std::vector<Tagged> bnd, sbx, sby, sax, say;
     int slot_item = 0;
+    std::vector<float> sfxt;  // suffix table
     std::vector<int> bnd_row;  // row whose values each halo slot holds
   };
   struct Wave { Lane L[64]; Group Gr[G]; bool fin = false; };
@@ -105,6 +127,7 @@ std::vector<float> emulate(const std::vector<Pair>& pairs, float eps, int cmax, 
       g.sby.assign(2 * cmax, {0, -1, -1});
       g.sax.assign(rmax, {0, -1, -1});
       g.say.assign(rmax, {0, -1, -1});
+      g.sfxt.assign(sfx, NAN);
     }
   }
   long st = 0;
@@ -187,6 +210,62 @@ std::vector<float> emulate(const std::vector<Pair>& pairs, float eps, int cmax, 
             }
             ln.gpar ^= 1;
           }
+          // suffix table (lean_suffix_table), group-uniform
+          Lane& lz = L[l0];
+          for (int lg = 0; lg < W; ++lg) L[l0 + lg].f_disc = 0.0f;
+          const int S = (lz.fR + W - 1) / W;
+          const int nbox = (((lz.fC + 7) >> 3) + 7) >> 3;
+          if (MEAS == DTW && sfx > 0 && lz.f_ok && lz.f_live && lz.f_tau < kNoTau && S >= 2 && S - 1 <= sfx &&
+              4 * nbox <= cmax) {
+            const Pair& p = pairs[it];
+            const int n = p.a.size() / 2, m = p.b.size() / 2;
+            const std::vector<float>& A = n >= m ? p.a : p.b;
+            const std::vector<float>& B = n >= m ? p.b : p.a;
+            const int R = lz.fR, C = lz.fC;
+            std::vector<std::array<float, 4>> box(nbox);
+            for (int t = 0; t < nbox; ++t) {
+              box[t] = {B[2 * 64 * t], B[2 * 64 * t], B[2 * 64 * t + 1], B[2 * 64 * t + 1]};
+              for (int i = 64 * t; i < std::min(C, 64 * t + 64); ++i) {
+                box[t][0] = std::min(box[t][0], B[2 * i]); box[t][1] = std::max(box[t][1], B[2 * i]);
+                box[t][2] = std::min(box[t][2], B[2 * i + 1]); box[t][3] = std::max(box[t][3], B[2 * i + 1]);
+              }
+            }
+            // the boxes overwrite the halo row: tag it so no real cell reads it
+            for (int i = 0; i < 4 * nbox; ++i) { gr.bnd[i] = {NAN, -3, -3}; gr.bnd_row[i] = -3; }
+            // lane lg sums the rows of strips lg, lg+W, ... in order; a
+            // suffix scan over the lanes per round of W strips, from the last
+            const int K = (S + W - 1) / W;
+            float carry = 0.0f;
+            for (int k = K - 1; k >= 0; --k) {
+              float v[W];
+              for (int lg = 0; lg < W; ++lg) {
+                const int s = k * W + lg;
+                float ts = 0.0f;
+                if (s >= 1 && s < S)
+                  for (int i = s * W; i < std::min(s * W + W, R); ++i) {
+                    float mb = pt_box(A[2 * i], A[2 * i + 1], box[0].data());
+                    for (int t = 1; t < nbox; ++t) mb = std::min(mb, pt_box(A[2 * i], A[2 * i + 1], box[t].data()));
+                    ts += sqrt_lo_max(mb);
+                  }
+                v[lg] = ts;
+              }
+              for (int d = 1; d < W; d <<= 1) {
+                float nv[W];
+                for (int lg = 0; lg < W; ++lg) nv[lg] = v[lg] + (lg + d < W ? v[lg + d] : 0.0f);
+                for (int lg = 0; lg < W; ++lg) v[lg] = nv[lg];
+              }
+              for (int lg = 0; lg < W; ++lg) {
+                v[lg] += carry;
+                const int s = k * W + lg;
+                if (s >= 1 && s < S) gr.sfxt[s - 1] = v[lg];
+              }
+              carry = v[0];
+            }
+            if (g_sfx1)  // entry s also counts strip s
+              for (int s = S - 2; s >= 1; --s) gr.sfxt[s] = gr.sfxt[s - 1];
+            const float disc = g_nodisc ? 1.0f : 1.0f - (float)(2 * R + C + 64) * 0x1p-23f;
+            for (int lg = 0; lg < W; ++lg) L[l0 + lg].f_disc = disc;
+          }
         }
         // start
         for (int l = 0; l < 64; ++l) {
@@ -203,6 +282,7 @@ std::vector<float> emulate(const std::vector<Pair>& pairs, float eps, int cmax, 
           }
           ln.live = ln.f_live;
           ln.tau = ln.f_live ? ln.f_tau : kNoTau;
+          ln.disc = ln.f_live ? ln.f_disc : 0.0f;
           ln.pair = ln.f_live ? ln.f_pair : -1;
           ln.R = ln.f_live ? ln.fR : 1;
           ln.C = ln.f_live ? ln.fC : 1;
@@ -335,6 +415,13 @@ std::vector<float> emulate(const std::vector<Pair>& pairs, float eps, int cmax, 
         if (MEAS != LCSS && lgof(l) == W - 1 && ln.k == (g_late ? ln.D - 1 : ln.lastch) && ln.row + 1 < ln.R) {
           const float v = MEAS == DTW ? ln.rowmin : sqrtf(ln.rowmin);
           if (v > ln.tau) stop_g[l / W] = true;
+          if (MEAS == DTW && ln.disc > 0.0f) {
+            const int s = ln.row / W;
+            CHK(s >= 0 && s < sfx && s < (ln.R + W - 1) / W - 1);
+            const float tv = w.Gr[l / W].sfxt[s];
+            CHK(tv == tv);  // written for this pair
+            if ((ln.rowmin + tv) * ln.disc > ln.tau) stop_g[l / W] = true, ++g_sfx_drops;
+          }
         }
       }
       for (int g = 0; g < G; ++g) {
@@ -385,7 +472,10 @@ int run(std::mt19937& rng, int trials) {
     std::vector<Pair> pairs(npairs);
     int cmaxlen = 1, rmaxlen = 1;
     for (auto& p : pairs) {
-      const int n = std::uniform_int_distribution<int>(1, maxlen)(rng), m = std::uniform_int_distribution<int>(1, maxlen)(rng);
+      int n = std::uniform_int_distribution<int>(1, maxlen)(rng), m = std::uniform_int_distribution<int>(1, maxlen)(rng);
+      // one-point trajectories: DTW is the sum of the other's distances to
+      // the point, and the suffix bound is tight up to its rounding
+      if (std::uniform_int_distribution<int>(0, 7)(rng) == 0) (n < m ? n : m) = 1;
       p.dead = std::uniform_int_distribution<int>(0, 9)(rng) == 0;
       std::normal_distribution<float> nd(0.0f, 0.03f);
       float x = 0, y = 0;
@@ -414,7 +504,9 @@ int run(std::mt19937& rng, int trials) {
       p.tau = f == 3 ? nextafterf(wants[i], 0.0f) : wants[i] * kFactor[f];
     }
     std::vector<int> dropped;
-    std::vector<float> got = emulate<MEAS, W, U>(pairs, eps, cmax, rmax, nwaves, &steps, &dropped);
+    // launch_dp_s: one entry per strip of the longest row trajectory, up to 256
+    const int sfx = MEAS == DTW ? std::min(((rmax + W - 1) / W + 3) / 4 * 4, 256) : 0;
+    std::vector<float> got = emulate<MEAS, W, U>(pairs, eps, cmax, rmax, nwaves, &steps, &dropped, sfx);
     for (int i = 0; i < npairs; ++i) {
       if (pairs[i].dead) continue;
       const Pair& p = pairs[i];
@@ -443,11 +535,14 @@ int main(int argc, char** argv) {
   int bad = 0;
   const int T = argc > 1 ? std::atoi(argv[1]) : 60;
   g_late = argc > 2 && std::strcmp(argv[2], "late") == 0;
+  g_nodisc = argc > 2 && std::strcmp(argv[2], "nodisc") == 0;
+  g_sfx1 = argc > 2 && std::strcmp(argv[2], "sfx1") == 0;
   bad += run<DTW, 16, 1>(rng, T); bad += run<DTW, 16, 2>(rng, T); bad += run<DTW, 16, 4>(rng, T);
   bad += run<DTW, 32, 2>(rng, T); bad += run<DTW, 32, 4>(rng, T); bad += run<DTW, 64, 2>(rng, T);
   bad += run<DTW, 64, 4>(rng, T); bad += run<DTW, 32, 8>(rng, T); bad += run<DTW, 64, 8>(rng, T);
   bad += run<FRECHET, 16, 4>(rng, T); bad += run<FRECHET, 64, 8>(rng, T); bad += run<FRECHET, 32, 2>(rng, T);
   bad += run<LCSS, 16, 1>(rng, T); bad += run<LCSS, 32, 4>(rng, T); bad += run<LCSS, 64, 8>(rng, T);
+  printf("suffix-bound drops: %ld\n", g_sfx_drops);
   printf(bad ? "FAIL\n" : "ALL OK\n");
   return bad != 0;
 }

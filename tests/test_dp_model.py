@@ -18,11 +18,13 @@ ROOT = os.path.dirname(HERE)
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
 def test_lean_dp_schedule_model_matches_oracle(tmp_path):
     """k_dp_lean's schedule, with early abandoning of DTW and Frechet pairs
-    against thresholds around their values: every kept value equals the
+    against thresholds around their values (DTW also by the row minimum plus
+    the suffix table of later rows' box bounds): every kept value equals the
     oracle's, no pair at or below its threshold is dropped, and pairs are
     dropped. Testing the row minimum at the row's padded end instead of its
     last real chunk (where a lane of the next strip may already have written
-    the pair's result) is caught."""
+    the pair's result) is caught, and so are a suffix test without its
+    rounding discount and a table entry that counts the tested strip."""
     exe = str(tmp_path / "dp_lean_schedule")
     subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fopenmp", "-o", exe,
                     os.path.join(HERE, "emu", "dp_lean_schedule.cpp"), os.path.join(ROOT, "oracle", "oracle.cpp")],
@@ -31,8 +33,15 @@ def test_lean_dp_schedule_model_matches_oracle(tmp_path):
     assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
     drops = [int(line.split(",")[1].split()[0]) for line in r.stdout.splitlines() if "dropped early" in line]
     assert sum(drops) > 0
+    sfx = [int(line.split(":")[1]) for line in r.stdout.splitlines() if line.startswith("suffix-bound drops")]
+    assert sfx and sfx[0] > 0
     r = subprocess.run([exe, "12", "late"], capture_output=True, text=True, timeout=300)
     assert r.returncode != 0
+    # the suffix test without its rounding discount, and a table that also
+    # counts the tested strip's rows, drop pairs at their own value
+    for plant in ("nodisc", "sfx1"):
+        r = subprocess.run([exe, "12", plant], capture_output=True, text=True, timeout=300)
+        assert r.returncode != 0, plant
 
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
@@ -80,8 +89,12 @@ def test_multi_wave_dp_schedule_model_matches_oracle(tmp_path):
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
 def test_sqrt_neighbour_fixup_is_correctly_rounded(tmp_path):
+    """The DP's square root (v_sqrt_f32's result moved by the clamped signs
+    of two fma residuals) equals sqrtf for every float x in [2^-96, 2^127)
+    and every start within 1 ulp of the root, and for x = 0."""
     exe = str(tmp_path / "sqrt_fixup")
-    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-o", exe, os.path.join(HERE, "emu", "sqrt_fixup.c"), "-lm"],
-                   check=True, timeout=120)
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fopenmp", "-o", exe,
+                    os.path.join(HERE, "emu", "sqrt_fixup.c"), "-lm"], check=True, timeout=120)
+    r = subprocess.run([exe, "all"], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0 and " 0 mismatches" in r.stdout, r.stdout + r.stderr
+    assert int(r.stdout.split()[0]) == 3 * ((0x7f0 - 0x0f8) << 20)

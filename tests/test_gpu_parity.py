@@ -77,6 +77,25 @@ def test_dtw_extreme_scales_bit_exact(eng, scale):
     check_pairs(dtw_engine_at(16), s, "dtw")
 
 
+@pytest.mark.parametrize("scale", [3e18, 1e20])
+def test_dtw_huge_coordinates_bit_exact(eng, scale):
+    """Coordinates near 2^62 (3e18: unflagged, squared distances finite,
+    DP sums that may overflow) and above it (1e20: flagged, squared
+    distances of +inf, which the DP's fast square root does not take; the
+    flagged path's plain sqrtf does)."""
+    xy, off = walks(32, LENGTHS, 2e-2)
+    xy = ((xy.astype(np.float64) - 0.5) * scale).astype(np.float32)
+    t = ts.TrajSet(xy, off)
+    assert t.flags() == (ts.SET_HUGE_COORD if np.abs(xy).max() >= 2.0 ** 62 else 0)
+    n = len(LENGTHS)
+    qi, di = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    s = dict(db=t, qs=t, db_xy=xy, db_off=off, q_xy=xy, q_off=off, qi=qi.ravel(), di=di.ravel())
+    say("dtw huge scale", scale)
+    check_pairs(eng, s, "dtw")
+    check_pairs(dtw_engine_at(16), s, "dtw")
+    check_pairs(eng, s, "dtw", band=-10)
+
+
 def dtw_engine_at(waves):
     """An engine whose unbanded DTW launches run `waves` waves per CU (each
     bucket's tuned shape): at 12 or more they take the step loop split by
