@@ -486,6 +486,27 @@ __host__ __device__ constexpr int lean_group_floats(int cmax, int rmax, bool sc,
   return cmax + (sc ? 4 * cmax : 0) + (sr ? 2 * rmax : 0) + sfx;
 }
 
+// Sum of v over the W lanes of a lane group, in its lane W-1 (other lanes
+// hold partial sums): Hillis-Steele steps within each 16-lane row by DPP
+// row_shr 1, 2, 4, 8, then rows combined by row_bcast 15 (and 31 for W = 64)
+// into the rows that hold a group's last lane. Only the group's own lanes
+// need be active: the broadcasts write only rows whose source row is in the
+// same group.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float add_dpp_rows(float v) {
+  return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xf, false));
+}
+template <int W>
+__device__ __forceinline__ float group_sum_last(float v) {
+  v = add_dpp_rows<0x111, 0xf>(v);
+  v = add_dpp_rows<0x112, 0xf>(v);
+  v = add_dpp_rows<0x114, 0xf>(v);
+  v = add_dpp_rows<0x118, 0xf>(v);
+  if constexpr (W >= 32) v = add_dpp_rows<0x142, 0xa>(v);
+  if constexpr (W == 64) v = add_dpp_rows<0x143, 0xc>(v);
+  return v;
+}
+
 // Suffix bounds of an unbanded DTW pair for early abandoning, built by the
 // group at the pair's fetch. Every warping path visits each row i of A (the
 // R-point row trajectory) at some cell, whose point cost is at least
@@ -496,13 +517,14 @@ __host__ __device__ constexpr int lean_group_floats(int cmax, int rmax, bool sc,
 // read, and the new pair's first strip reads the border row). Entry s of
 // the table is the sum of lb_i over the rows of strips s+1, s+2, ...: the
 // path's cost after row sW+W-1 is at least that, so the pair's value is at
-// least the row's minimum plus it. Lane lg sums the W rows of strips lg,
-// lg+W, ... by itself, and a suffix scan over the group's lanes, a round of
-// W strips at a time from the last, turns the strip sums into entries: no
-// cross-lane step per strip. All in f32 over non-negative terms, so every
-// entry is at most the exact sum times (1 + 2^-24)^(R-1) whatever the order;
-// the test's discount covers that and the DP's own rounding. S - 1 entries
-// for S strips.
+// least the row's minimum plus it. Strips are summed from the last, each by
+// a DPP reduction into lane W-1, in f32 over non-negative terms, so every
+// entry is at most the exact sum times (1 + 2^-24)^(R-1) whatever the
+// order; the test's discount covers that and the DP's own rounding. S - 1
+// entries for S strips; each lane loads its row point of the next strip
+// before scanning the boxes for this one. (8-point block boxes, found
+// through the superblocks, cut the refine's cells another 10% on the
+// 4096 x 1M DTW call but cost more to build than they saved.)
 template <int W>
 __device__ __forceinline__ void lean_suffix_table(float* tbl, float4* boxes, const float4* bsblk, int C,
                                                   const float* ax, const float* ay, int R, int lg) {
@@ -510,30 +532,19 @@ __device__ __forceinline__ void lean_suffix_table(float* tbl, float4* boxes, con
   for (int t = lg; t < nbox; t += W) boxes[t] = bsblk[8 * t];
   __builtin_amdgcn_wave_barrier();
   const int S = (R + W - 1) / W;
-  const int K = (S + W - 1) / W;  // rounds of W strips
-  float carry = 0.0f;             // strips of the rounds after this one
-  for (int k = K - 1; k >= 0; --k) {
-    const int s = k * W + lg;  // this lane's strip
-    float ts = 0.0f;
-    if (s >= 1 && s < S) {
-      const int i0 = s * W, i1 = min(i0 + W, R);
-      for (int i = i0; i < i1; ++i) {
-        const float px = ax[i], py = ay[i];
-        float m = pt_box(px, py, boxes[0]);
-        for (int t = 1; t < nbox; ++t) m = fminf(m, pt_box(px, py, boxes[t]));
-        ts += sqrt_lo(m);
-      }
-    }
-    // suffix scan: v = sum over strips s..(k+1)W-1 of this round, plus carry
-    float v = ts;
-#pragma unroll
-    for (int d = 1; d < W; d <<= 1) {
-      const float u = __shfl_down(v, d, W);
-      v += lg + d < W ? u : 0.0f;
-    }
-    v += carry;
-    if (s >= 1 && s < S) tbl[s - 1] = v;
-    carry = __shfl(v, 0, W);
+  float acc = 0.0f;
+  int i = (S - 1) * W + lg;
+  float px = ax[min(i, R - 1)], py = ay[min(i, R - 1)];
+  for (int s = S - 1; s >= 1; --s) {
+    const int in = i - W;
+    const float nx = ax[min(in, R - 1)], ny = ay[min(in, R - 1)];
+    float m = pt_box(px, py, boxes[0]);
+    for (int t = 1; t < nbox; ++t) m = fminf(m, pt_box(px, py, boxes[t]));
+    acc += group_sum_last<W>(i < R ? sqrt_lo(m) : 0.0f);
+    if (lg == W - 1) tbl[s - 1] = acc;
+    i = in;
+    px = nx;
+    py = ny;
   }
   __builtin_amdgcn_wave_barrier();
 }

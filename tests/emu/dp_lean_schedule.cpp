@@ -222,6 +222,7 @@ std::vector<float> emulate(const std::vector<Pair>& pairs, float eps, int cmax, 
             const std::vector<float>& A = n >= m ? p.a : p.b;
             const std::vector<float>& B = n >= m ? p.b : p.a;
             const int R = lz.fR, C = lz.fC;
+            // 64-point boxes of B (the set's superblocks)
             std::vector<std::array<float, 4>> box(nbox);
             for (int t = 0; t < nbox; ++t) {
               box[t] = {B[2 * 64 * t], B[2 * 64 * t], B[2 * 64 * t + 1], B[2 * 64 * t + 1]};
@@ -232,34 +233,29 @@ std::vector<float> emulate(const std::vector<Pair>& pairs, float eps, int cmax, 
             }
             // the boxes overwrite the halo row: tag it so no real cell reads it
             for (int i = 0; i < 4 * nbox; ++i) { gr.bnd[i] = {NAN, -3, -3}; gr.bnd_row[i] = -3; }
-            // lane lg sums the rows of strips lg, lg+W, ... in order; a
-            // suffix scan over the lanes per round of W strips, from the last
-            const int K = (S + W - 1) / W;
-            float carry = 0.0f;
-            for (int k = K - 1; k >= 0; --k) {
-              float v[W];
+            // strips from the last, each summed by the DPP reduction into lane W-1
+            float acc = 0.0f;
+            for (int s = S - 1; s >= 1; --s) {
+              float v[64];
               for (int lg = 0; lg < W; ++lg) {
-                const int s = k * W + lg;
-                float ts = 0.0f;
-                if (s >= 1 && s < S)
-                  for (int i = s * W; i < std::min(s * W + W, R); ++i) {
-                    float mb = pt_box(A[2 * i], A[2 * i + 1], box[0].data());
-                    for (int t = 1; t < nbox; ++t) mb = std::min(mb, pt_box(A[2 * i], A[2 * i + 1], box[t].data()));
-                    ts += sqrt_lo_max(mb);
-                  }
-                v[lg] = ts;
+                const int i = s * W + lg;
+                float mb = INFINITY;
+                if (i < R)
+                  for (int t = 0; t < nbox; ++t) mb = std::min(mb, pt_box(A[2 * i], A[2 * i + 1], box[t].data()));
+                v[lg] = i < R ? sqrt_lo_max(mb) : 0.0f;
               }
-              for (int d = 1; d < W; d <<= 1) {
-                float nv[W];
-                for (int lg = 0; lg < W; ++lg) nv[lg] = v[lg] + (lg + d < W ? v[lg + d] : 0.0f);
+              for (int d = 1; d < 16; d <<= 1) {
+                float nv[64];
+                for (int lg = 0; lg < W; ++lg) nv[lg] = v[lg] + ((lg & 15) >= d ? v[lg - d] : 0.0f);
                 for (int lg = 0; lg < W; ++lg) v[lg] = nv[lg];
               }
-              for (int lg = 0; lg < W; ++lg) {
-                v[lg] += carry;
-                const int s = k * W + lg;
-                if (s >= 1 && s < S) gr.sfxt[s - 1] = v[lg];
+              if (W >= 32) for (int lg = 16; lg < 32; ++lg) v[lg] += v[15];
+              if (W == 64) {
+                for (int lg = 48; lg < 64; ++lg) v[lg] += v[47];
+                for (int lg = 32; lg < 64; ++lg) v[lg] += v[31];
               }
-              carry = v[0];
+              acc += v[W - 1];
+              gr.sfxt[s - 1] = acc;
             }
             if (g_sfx1)  // entry s also counts strip s
               for (int s = S - 2; s >= 1; --s) gr.sfxt[s] = gr.sfxt[s - 1];
