@@ -98,13 +98,31 @@ __global__ void k_count_cells(const Item* items, const int32_t* nitems, int64_t 
 // the index, vals[i] = NaN); then cells and pairs per DP bucket as in
 // k_count_cells and, with outb, each item's place in its bucket's two-ended
 // list as in k_items_by_bucket. The list counters must be zero on entry.
+// With FROM_KEYS and `via`, the keys hold candidate slots instead of DB
+// indices: key i becomes a copy of the candidate via[slot] (its query, DB
+// index and value slot), the DB index goes to seed_idx[i], no value is
+// cleared, and mark[slot] is set to -inf (refined in this round). Without
+// FROM_KEYS and with `tb`, items already refined (tb[slot] == -inf) are
+// skipped, and items whose filter bound exceeds their query's threshold
+// tau[q - tau_q0] are skipped with NaN in pvals[slot]; a skipped item is
+// written back with d = -1.
+struct PrepExtra {
+  const Item* via = nullptr;
+  float* mark = nullptr;
+  const float* tb = nullptr;
+  const float* tau = nullptr;
+  int32_t tau_q0 = 0;
+  float* pvals = nullptr;
+};
+
 template <bool FROM_KEYS>
 __global__ void __launch_bounds__(256) k_refine_prep(const uint64_t* keys, int32_t s, int32_t q0, int32_t* seed_idx,
                                                      float* nan_vals, Item* items, const int32_t* nitems,
                                                      int64_t max_items, SetView q, SetView db, MeasureArgs ma,
                                                      int32_t by_bucket, unsigned long long* acc,
                                                      unsigned long long* acc_pairs, Item* outb, int64_t cap_each,
-                                                     int32_t* heads, int32_t* tails, BucketCells long_cells) {
+                                                     int32_t* heads, int32_t* tails, BucketCells long_cells,
+                                                     PrepExtra x) {
   __shared__ unsigned long long part[kDpBuckets], partp[kDpBuckets];
   if (threadIdx.x < kDpBuckets) {
     part[threadIdx.x] = 0;
@@ -118,12 +136,36 @@ __global__ void __launch_bounds__(256) k_refine_prep(const uint64_t* keys, int32
   if (i < n) {
     if (FROM_KEYS) {
       const uint64_t k = keys[i];
-      it = Item{q0 + (int32_t)(i / s), k == ~0ull ? -1 : (int32_t)(uint32_t)k, (int32_t)i, 0};
-      items[i] = it;
-      seed_idx[i] = it.d;
-      nan_vals[i] = __int_as_float(0x7fc00000);
+      if (x.via) {
+        if (k != ~0ull) {
+          const Item c = x.via[(uint32_t)k];
+          it = Item{c.q, c.d, c.slot, 0};
+          x.mark[c.slot] = __int_as_float(0xff800000);
+        } else {
+          it = Item{q0 + (int32_t)(i / s), -1, 0, 0};
+        }
+        items[i] = it;
+        seed_idx[i] = it.d;
+      } else {
+        it = Item{q0 + (int32_t)(i / s), k == ~0ull ? -1 : (int32_t)(uint32_t)k, (int32_t)i, 0};
+        items[i] = it;
+        seed_idx[i] = it.d;
+        nan_vals[i] = __int_as_float(0x7fc00000);
+      }
     } else {
       it = items[i];
+      if (x.tb && it.d >= 0) {
+        const float b = x.tb[it.slot];
+        bool skip = b == __int_as_float(0xff800000);
+        if (!skip && b > x.tau[it.q - x.tau_q0]) {
+          x.pvals[it.slot] = __int_as_float(0x7fc00000);
+          skip = true;
+        }
+        if (skip) {
+          it.d = -1;
+          items[i] = it;
+        }
+      }
     }
   }
   int b = -1;
@@ -173,7 +215,7 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 enum BufId {
   B_ENV, B_SCORES, B_KEYS, B_KEYS2, B_SCA, B_SCB, B_CAND, B_CANDIDX, B_ITEMS, B_ITEMS3, B_VALS, B_REGION,
   B_RVALS, B_COUNTS, B_CTR, B_TAU, B_TOP, B_OIDX, B_OVAL, B_CELLS, B_HOSTIN, B_ITEMS2, B_RITEMS, B_SEGC, B_NEED, B_BCNT,
-  B_BITS, B_BOUNDS, B_KQ, B_NBUF
+  B_BITS, B_BOUNDS, B_KQ, B_TB, B_TOP2, B_CANDIDX2, B_ITEMS5, B_MKEYS, B_TOPA, B_NBUF
 };
 
 struct Buf {
@@ -435,6 +477,8 @@ struct KeyItems {
   const uint64_t* keys;
   int32_t s, q0;
   int32_t* seed_idx;
+  const Item* via = nullptr;  // keys hold candidate slots of via (second seed round)
+  float* mark = nullptr;      // tb of the candidates: -inf where refined
 };
 
 // Refine a device item list into vals[slot]. nitems_dev may be null (all
@@ -442,10 +486,20 @@ struct KeyItems {
 // set of 32 DP counters the caller has already zeroed, which lets one
 // kernel prepare the refine (required with from_keys); otherwise the
 // refine's own set is zeroed by its cell count.
+// kind: 0 a main refine, 1 a seed refine, 2 the second seed round (launch
+// labels). prune: the main refine's candidate bounds tb[slot] and the
+// threshold they are tested against (items over it, or refined in the
+// second round, are skipped).
+struct Prune {
+  const float* tb;
+  const float* tau;
+};
 ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_set* qs, Item* items,
-                 const int32_t* nitems_dev, int64_t max_items, float* vals, bool seed = false,
+                 const int32_t* nitems_dev, int64_t max_items, float* vals, int kind = 0,
                  int32_t* ctr_zeroed = nullptr, const KeyItems* from_keys = nullptr,
-                 const RefineBufs* fixed = nullptr, const float* tau = nullptr, int32_t tau_q0 = 0) {
+                 const RefineBufs* fixed = nullptr, const float* tau = nullptr, int32_t tau_q0 = 0,
+                 const Prune* prune = nullptr) {
+  const bool seed = kind == 1;
   if (max_items == 0) return TS_OK;
   if (from_keys && !ctr_zeroed) return fail(TS_ERR_INVALID, "items from keys need zeroed counters");
   if (fixed && max_items > fixed->max_items) return fail(TS_ERR_INVALID, "refine larger than its fixed buffers");
@@ -490,18 +544,30 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
       long_cells.t[b] = cb * rmax / 2;
     }
   }
+  if (prune && (!ctr_zeroed || from_keys)) return fail(TS_ERR_INVALID, "pruning needs a prepared main refine");
   if (ctr_zeroed) {
     TS_TRY(run(e, "refine_prep", 0, 0, [&] {
       const dim3 grid((unsigned)((max_items + 255) / 256));
-      if (from_keys)
+      PrepExtra x;
+      if (from_keys) {
+        x.via = from_keys->via;
+        x.mark = from_keys->mark;
         hipLaunchKernelGGL(k_refine_prep<true>, grid, dim3(256), 0, e->st, from_keys->keys, from_keys->s,
                            from_keys->q0, from_keys->seed_idx, vals, items, nitems_dev, max_items, qv, dv, ma,
                            dp ? 1 : 0, cells + slot, cells + kCellSlots + slot, lists ? itemsb : nullptr, max_items,
-                           ctr + kDpBuckets, ctr + 2 * kDpBuckets, long_cells);
-      else
+                           ctr + kDpBuckets, ctr + 2 * kDpBuckets, long_cells, x);
+      } else {
+        if (prune) {
+          x.tb = prune->tb;
+          x.tau = prune->tau;
+          x.tau_q0 = tau_q0;
+          x.pvals = vals;
+        }
         hipLaunchKernelGGL(k_refine_prep<false>, grid, dim3(256), 0, e->st, nullptr, 1, 0, nullptr, nullptr, items,
                            nitems_dev, max_items, qv, dv, ma, dp ? 1 : 0, cells + slot, cells + kCellSlots + slot,
-                           lists ? itemsb : nullptr, max_items, ctr + kDpBuckets, ctr + 2 * kDpBuckets, long_cells);
+                           lists ? itemsb : nullptr, max_items, ctr + kDpBuckets, ctr + 2 * kDpBuckets, long_cells,
+                           x);
+      }
       return hipGetLastError();
     }));
   } else {
@@ -529,9 +595,10 @@ ts_status refine(ts_engine* e, const ts_params& p, const ts_set* db, const ts_se
     }, slot);
   }
   // Wavefront DP, one launch per bucket.
-  static const char* kNames[2][3] = {{"dtw_wavefront", "frechet_wavefront", "lcss_wavefront"},
-                                     {"dtw_wavefront_seed", "frechet_wavefront_seed", "lcss_wavefront_seed"}};
-  const char* nm = kNames[seed ? 1 : 0][p.measure == TS_DTW ? 0 : (p.measure == TS_FRECHET ? 1 : 2)];
+  static const char* kNames[3][3] = {{"dtw_wavefront", "frechet_wavefront", "lcss_wavefront"},
+                                     {"dtw_wavefront_seed", "frechet_wavefront_seed", "lcss_wavefront_seed"},
+                                     {"dtw_wavefront_seed2", "frechet_wavefront_seed2", "lcss_wavefront_seed2"}};
+  const char* nm = kNames[kind][p.measure == TS_DTW ? 0 : (p.measure == TS_FRECHET ? 1 : 2)];
   const DpTune* tune = e->tune[dp_table_of(max_items)][dp_variant(p)];
   if (b_lo == b_hi) {
     int sh = tune[b_lo].shape;
@@ -650,6 +717,12 @@ double exact_work(const ts_set* qs, const ts_set* db, int64_t nq) {
   return (double)nq * db->count * ql * dl;
 }
 
+// Second seed round: from this batch work (as exact_work), s2 = kSeeds2
+// candidates a query; the TS_TWO_ROUND environment variable overrides the
+// work rule ("0": never, "1": always).
+constexpr double kTwoRoundWork = 1099511627776.0;  // 2^40
+constexpr int32_t kSeeds2 = 32;
+
 // Largest batch (queries x DB trajectories) whose seed-pass bounds are kept
 // for the compaction (4 bytes a pair).
 constexpr int64_t kStoredBoundsMax = 1ll << 26;
@@ -660,7 +733,17 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   const int32_t N = db->count;
   const int32_t k = p.k;
   const bool desc = desc_measure(p.measure);
-  const int32_t s = exact_seeds(k, exact_work(qs, db, nq));
+  const double work = exact_work(qs, db, nq);
+  const int32_t s = exact_seeds(k, work);
+  // Second seed round (DTW and Frechet, batches in the throughput regime):
+  // after the tight filter, each query's s2 candidates of smallest bound are
+  // refined against the first threshold, and the k-th best of both rounds'
+  // values becomes the main refine's threshold. The first round's seeds
+  // leave the candidates; their values join the final top-k.
+  const char* two_env = std::getenv("TS_TWO_ROUND");  // "0" never, "1" always
+  const bool two = (p.measure == TS_DTW || p.measure == TS_FRECHET) &&
+                   (two_env && *two_env ? two_env[0] == '1' : work >= kTwoRoundWork);
+  const int32_t s2 = two ? kSeeds2 : 0;
   const int32_t nch = (N + bound_chunk_items(s) - 1) / bound_chunk_items(s);
   const SetView qv = qs->view(), dv = db->view();
   const MeasureArgs ma = margs(p);
@@ -680,7 +763,7 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   // DP counter sets of the seed refine [0, 32) and the main refine [32, 64),
   // zeroed by the seed pass
   int32_t* ctrs = nullptr;
-  TS_TRY(ensure(e, B_CTR, 64, &ctrs));
+  TS_TRY(ensure(e, B_CTR, 96, &ctrs));
   const double pairs = (double)nq * N;
   // The seed pass keeps every pair's bound for the compaction's count pass
   // when the batch is small enough; larger batches evaluate them again.
@@ -688,11 +771,11 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   if ((int64_t)nq * N <= kStoredBoundsMax) TS_TRY(ensure(e, B_BOUNDS, (int64_t)nq * N, &bvals));
   // 1. seeds: good candidates by bound, refined exactly
   TS_TRY(run(e, "bound_seed", pairs * 36.0, pairs, [&] {
-    return launch_bound_chunks(e->st, ma, k, qv, q0, nq, dv, s, bound_chunk_items(s), bkeys, bvals, ctrs, 64);
+    return launch_bound_chunks(e->st, ma, k, qv, q0, nq, dv, s, bound_chunk_items(s), bkeys, bvals, ctrs, 96);
   }));
   TS_TRY(topk(e, bkeys, (int64_t)nch * s, nch * s, nullptr, nq, s, seeds));
   const KeyItems seed_items{seeds, s, q0, seed_idx};
-  TS_TRY(refine(e, p, db, qs, items, nullptr, (int64_t)nq * s, vals, true, ctrs, &seed_items));
+  TS_TRY(refine(e, p, db, qs, items, nullptr, (int64_t)nq * s, vals, 1, ctrs, &seed_items));
   // 2. threshold: k-th best exact value among the seeds
   TS_TRY(run(e, "threshold", 0, 0, [&] {
     return launch_seed_tau(e->st, vals, seed_idx, nq, s, k, desc, tau, kq_dev ? kq_dev + q0 : nullptr);
@@ -764,20 +847,51 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   TS_TRY(run(e, "compact_write", 0, 0, [&] {
     return launch_bound_compact(e->st, ma, k, qv, q0, nq, dv, tau, d_off, nullptr, bcounts, bits, citems, cap);
   }));
-  // 3b. tighter O(n+m) bound on the survivors
+  // 3b. tighter O(n+m) bound on the survivors (with two rounds: every
+  // survivor's bound into tb, the first round's seeds rejected)
   int32_t* nitems2 = counts + nq;
+  float* tb = nullptr;
+  if (two) TS_TRY(ensure(e, B_TB, cap, &tb));
   // units: the pairs it tests
   TS_TRY(run(e, "tight_filter", 0, (double)total, [&] {
     return launch_tight_filter(e->st, ma, k, qv, q0, dv, tau, citems, ntot, total, items2, nitems2, rvals,
-                               device_cu_count() * 32);
+                               device_cu_count() * 32, tb, two ? seed_idx : nullptr, s);
   }));
+  const int32_t* seg_counts = dev_sized ? segc : counts;
+  if (two) {
+    // 3c. second seed round: each query's s2 candidates of smallest bound
+    uint64_t* top2 = nullptr;
+    int32_t* seed_idx2 = nullptr;
+    Item* items5 = nullptr;
+    TS_TRY(ensure(e, B_TOP2, (int64_t)nq * s2, &top2));
+    TS_TRY(ensure(e, B_CANDIDX2, (int64_t)nq * s2, &seed_idx2));
+    TS_TRY(ensure(e, B_ITEMS5, (int64_t)nq * s2, &items5));
+    TS_TRY(run(e, "keys", 0, 0, [&] { return launch_keys_from_slots(e->st, tb, citems, cap, rkeys); }));
+    TS_TRY(topk(e, rkeys, 0, seg_len, seg_counts, nq, s2, top2, d_off));
+    const KeyItems r2_items{top2, s2, q0, seed_idx2, citems, tb};
+    TS_TRY(refine(e, p, db, qs, items5, nullptr, (int64_t)nq * s2, rvals, 2, ctrs + 64, &r2_items, nullptr, tau, q0));
+    TS_TRY(run(e, "threshold", 0, 0, [&] {
+      return launch_tau2(e->st, vals, seed_idx, nq, s, top2, s2, rvals, citems, k, tau, kq_dev ? kq_dev + q0 : nullptr);
+    }));
+  }
   // the main refine abandons pairs whose value must exceed the threshold
-  TS_TRY(refine(e, p, db, qs, items2, nitems2, cap, rvals, false, ctrs + 32, nullptr, nullptr, tau, q0));
+  // (with two rounds it skips the second round's pairs and those whose
+  // bound exceeds the new threshold)
+  const Prune prune{tb, tau};
+  TS_TRY(refine(e, p, db, qs, items2, nitems2, cap, rvals, 0, ctrs + 32, nullptr, nullptr, tau, q0,
+                two ? &prune : nullptr));
   // 4. top-k over each query's refined candidates (slot = position in citems)
   TS_TRY(run(e, "keys", 0, 0, [&] { return launch_keys_from_items(e->st, rvals, citems, cap, desc, rkeys); }));
   // Region lengths: clamped to the capacity (device-sized), or the counted
   // totals.
-  return topk(e, rkeys, 0, seg_len, dev_sized ? segc : counts, nq, k, top, d_off);
+  if (!two) return topk(e, rkeys, 0, seg_len, seg_counts, nq, k, top, d_off);
+  // with two rounds, then over the candidates' top-k and the seeds
+  uint64_t *topa = nullptr, *mkeys = nullptr;
+  TS_TRY(ensure(e, B_TOPA, (int64_t)nq * k, &topa));
+  TS_TRY(ensure(e, B_MKEYS, (int64_t)nq * (k + s), &mkeys));
+  TS_TRY(topk(e, rkeys, 0, seg_len, seg_counts, nq, k, topa, d_off));
+  TS_TRY(run(e, "merge_seeds", 0, 0, [&] { return launch_merge_seed_keys(e->st, topa, k, vals, seed_idx, s, nq, mkeys); }));
+  return topk(e, mkeys, k + s, k + s, nullptr, nq, k, top);
 }
 
 // Run one batch through the graph cache: replay a recorded graph whose key

@@ -719,7 +719,8 @@ __device__ int stage_tiles(const float4* sblk, int n, float4* dst) {
 // partial sums of non-negative terms never exceed the full sums. The f32
 // sums are in a different order than a serial sum; disc covers it.
 __device__ bool tile_pass2(const float* ax, const float* ay, int na, const float* bx, const float* by, int nb,
-                           const float4* abox, int nsa, const float4* bbox, int nsb, float e, float disc, float t) {
+                           const float4* abox, int nsa, const float4* bbox, int nsb, float e, float disc, float t,
+                           float* bound) {
   const int lane = threadIdx.x & 63;
   const int L = max(na, nb);
   float sa = 0.0f, sb = 0.0f;
@@ -746,6 +747,7 @@ __device__ bool tile_pass2(const float* ax, const float* ay, int na, const float
     rx = nrx;
     ry = nry;
   }
+  *bound = (fmaxf(sa, sb) + e) * disc;
   return true;
 }
 
@@ -859,7 +861,8 @@ __device__ __forceinline__ float tiles_vs_mbr(const float4* boxes, int nb, int s
 template <int MEAS, bool TILED>
 __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k, SetView q, int32_t q0, SetView db,
                                                       const float* tau, const Item* items, const int32_t* nitems,
-                                                      int64_t max_items, Item* out, int32_t* nout, float* rej) {
+                                                      int64_t max_items, Item* out, int32_t* nout, float* rej,
+                                                      float* tb, const int32_t* seed_idx, int32_t s) {
   static_assert(!TILED || MEAS == TS_DTW, "the tiled pass is DTW's");
   __shared__ Item sbuf[kTightBuf];
   __shared__ int scnt, sbase;
@@ -873,7 +876,10 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
   for (int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); w < n_items; w += nw) {
     const Item it = items[w];
     if (it.d < 0) {
-      if (lane == 0) rej[it.slot] = __int_as_float(0x7fc00000);
+      if (lane == 0) {
+        rej[it.slot] = __int_as_float(0x7fc00000);
+        if (tb) tb[it.slot] = __int_as_float(0x7fc00000);
+      }
       continue;
     }
     const int n = q.len[it.q], m = db.len[it.d];
@@ -882,6 +888,7 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
     const float4 qe = q.ends[it.q], ce = db.ends[it.d];
     const float t = tau[it.q - q0];
     bool pass;
+    float bval = 0.0f;  // the bound compared with t (DTW and Frechet)
     if (MEAS == TS_LCSS) {
       int c1 = 0, c2 = 0;
       float f = 0.0f;
@@ -952,7 +959,7 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
       }
       if (pass)
         pass = tile_pass2(q.x + qo, q.y + qo, n, db.x + co, db.y + co, m, wbox, nsa, wbox + kTileBoxes, nsc, e,
-                          disc, t);
+                          disc, t, &bval);
       __builtin_amdgcn_wave_barrier();  // every lane's box reads before the next pair's writes
     } else if (MEAS == TS_DTW) {
       float a1 = 0.0f, a2 = 0.0f;
@@ -973,7 +980,8 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
         const float r2 = m >= 2 ? s2 + e0 + e1 : (n >= 2 ? e0 + e1 : e0);
         return fmaxf(r1, r2) * disc;
       };
-      pass = bound(a1, a2) <= t;
+      bval = bound(a1, a2);
+      pass = bval <= t;
       if (pass && ma.band != 0 && n >= 3 && m >= 3) {
         // The whole-MBR form passed: the banded form (row windows of the
         // band) is tighter and costs O(window / 64) per point. Each side alone
@@ -985,7 +993,8 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
         if (pass) {
           const float b2 = band_side_sum(db.x + co, db.y + co, m, q.blk + (qo >> 3), q.sblk + (qo >> 3), n, w * L,
                                          e0 + e1, disc, t);
-          pass = bound(b1, b2) <= t;
+          bval = fmaxf(bval, bound(b1, b2));
+          pass = bval <= t;
         }
       }
     } else {
@@ -1010,9 +1019,19 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
           const float c2 = tile_side_max(db.x + co, db.y + co, m, q.sblk + (qo >> 3), n, lim);
           c1 = fmaxf(c1, c2);
         }
-        pass = sqrtf(fmaxf(b2, c1)) <= t;
+        bval = sqrtf(fmaxf(b2, c1));
+        pass = bval <= t;
       }
     }
+    // Seed pairs (already refined exactly, their values merged into the
+    // top-k later) leave the candidates.
+    if (pass && seed_idx) {
+      bool hit = false;
+      for (int u = lane; u < s; u += 64) hit |= seed_idx[(int64_t)(it.q - q0) * s + u] == it.d;
+      pass = !__any(hit);
+    }
+    // tb[slot]: a survivor's bound, NaN for the others
+    if (tb && lane == 0) tb[it.slot] = pass ? bval : __int_as_float(0x7fc00000);
     // a rejected pair's value slot holds NaN (no refine writes it)
     if (!pass && lane == 0) rej[it.slot] = __int_as_float(0x7fc00000);
     if (pass && lane == 0) {
@@ -1033,7 +1052,7 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
 template <int MEAS, bool TILED>
 void launch_tight(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, SetView db, const float* tau,
                   const Item* items, const int32_t* nitems, int64_t max_items, Item* out, int32_t* nout, float* rej,
-                  int grid_waves) {
+                  int grid_waves, float* tb, const int32_t* seed_idx, int32_t s) {
   static int per_cu = 0;
   if (!per_cu && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tight_filter<MEAS, TILED>, 256, 0) !=
                       hipSuccess ||
@@ -1041,15 +1060,18 @@ void launch_tight(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q
     per_cu = 1;
   const dim3 grid(std::max(1, std::min((grid_waves + 3) / 4, per_cu * device_cu_count())));
   hipLaunchKernelGGL((k_tight_filter<MEAS, TILED>), grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems,
-                     max_items, out, nout, rej);
+                     max_items, out, nout, rej, tb, seed_idx, s);
 }
 
 hipError_t launch_tight_filter(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, SetView db,
                                const float* tau, const Item* items, const int32_t* nitems, int64_t max_items,
-                               Item* out, int32_t* nout, float* rej, int grid_waves) {
+                               Item* out, int32_t* nout, float* rej, int grid_waves, float* tb,
+                               const int32_t* seed_idx, int32_t s) {
+  if ((tb || seed_idx) && m.measure != TS_DTW && m.measure != TS_FRECHET) return hipErrorInvalidValue;
   switch (m.measure) {
-#define TS_TIGHT(MEAS, TILED) \
-  launch_tight<MEAS, TILED>(st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout, rej, grid_waves)
+#define TS_TIGHT(MEAS, TILED)                                                                              \
+  launch_tight<MEAS, TILED>(st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout, rej, grid_waves, tb, \
+                            seed_idx, s)
     case TS_DTW:
       if (m.band == 0) TS_TIGHT(TS_DTW, true);
       else TS_TIGHT(TS_DTW, false);

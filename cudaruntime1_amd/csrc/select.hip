@@ -306,6 +306,97 @@ hipError_t launch_seed_tau(hipStream_t st, const float* vals, const int32_t* see
   return hipGetLastError();
 }
 
+// Threshold of the second seed round: tau[q] = k-th best (kq[q]-th with kq)
+// of query q's s first-round seed values and its second-round values, the
+// value of the candidate slot in each of its s2 keys (rvals[slot], DB index
+// citems[slot].d; ~0 keys and dropped pairs count as missing). The two sets
+// are disjoint pairs.
+__global__ void __launch_bounds__(256) k_tau2(const float* vals, const int32_t* seed_idx, int32_t s,
+                                              const uint64_t* top2, int32_t s2, const float* rvals,
+                                              const Item* citems, int32_t k, const int32_t* kq, int32_t cs,
+                                              float* tau) {
+  __shared__ uint64_t sk[kSelChunk];
+  const int q = blockIdx.x;
+  for (int i = threadIdx.x; i < cs; i += 256) {
+    uint64_t key = ~0ull;
+    if (i < s) {
+      key = key_of(vals[(int64_t)q * s + i], seed_idx[(int64_t)q * s + i], false);
+    } else if (i < s + s2) {
+      const uint64_t t = top2[(int64_t)q * s2 + (i - s)];
+      if (t != ~0ull) {
+        const uint32_t slot = (uint32_t)t;
+        key = key_of(rvals[slot], citems[slot].d, false);
+      }
+    }
+    sk[i] = key;
+  }
+  __syncthreads();
+  for (int kk = 2; kk <= cs; kk <<= 1) {
+    for (int j = kk >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < cs; i += 256) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const uint64_t a = sk[i], b = sk[ixj];
+          const bool asc = (i & kk) == 0;
+          if ((a > b) == asc) { sk[i] = b; sk[ixj] = a; }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (threadIdx.x == 0) {
+    const uint64_t key = sk[(kq ? kq[q] : k) - 1];
+    tau[q] = key == ~0ull ? __int_as_float(0x7f800000) : value_of(key, false);
+  }
+}
+
+hipError_t launch_tau2(hipStream_t st, const float* vals, const int32_t* seed_idx, int32_t nq, int32_t s,
+                       const uint64_t* top2, int32_t s2, const float* rvals, const Item* citems, int32_t k,
+                       float* tau, const int32_t* kq) {
+  if (nq == 0) return hipSuccess;
+  if (s + s2 > kSelChunk || k < 1 || k > s) return hipErrorInvalidValue;
+  const int cs = chunk_for(s + s2);
+  hipLaunchKernelGGL(k_tau2, dim3(nq), dim3(256), 0, st, vals, seed_idx, s, top2, s2, rvals, citems, k, kq, cs, tau);
+  return hipGetLastError();
+}
+
+// Keys of candidate slots by their filter bound: (ordered bound, slot), ~0
+// for slots with no bound (NaN) or no pair. Slots are below 2^31.
+__global__ void __launch_bounds__(256) k_keys_from_slots(const float* tb, const Item* citems, int64_t n,
+                                                         uint64_t* keys) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float b = tb[i];
+  keys[i] = (b == b && citems[i].d >= 0) ? ((uint64_t)ord_bits_s(b) << 32) | (uint32_t)i : ~0ull;
+}
+
+hipError_t launch_keys_from_slots(hipStream_t st, const float* tb, const Item* citems, int64_t n, uint64_t* keys) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_keys_from_slots, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, tb, citems, n, keys);
+  return hipGetLastError();
+}
+
+// out[q][0, k) = top[q][0, k), out[q][k, k + s) = the keys of query q's s
+// first-round seed values, for a top-k over both.
+__global__ void __launch_bounds__(256) k_merge_seed_keys(const uint64_t* top, int32_t k, const float* vals,
+                                                         const int32_t* seed_idx, int32_t s, int32_t nq,
+                                                         uint64_t* out) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t w = k + s;
+  if (i >= (int64_t)nq * w) return;
+  const int64_t q = i / w, j = i % w;
+  out[i] = j < k ? top[q * k + j] : key_of(vals[q * s + (j - k)], seed_idx[q * s + (j - k)], false);
+}
+
+hipError_t launch_merge_seed_keys(hipStream_t st, const uint64_t* top, int32_t k, const float* vals,
+                                  const int32_t* seed_idx, int32_t s, int32_t nq, uint64_t* out) {
+  const int64_t n = (int64_t)nq * (k + s);
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_merge_seed_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, top, k, vals, seed_idx,
+                     s, nq, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_kth_threshold(hipStream_t st, const uint64_t* keys, int32_t nseg, int32_t s, int32_t k,
                                 int32_t descending, float* tau) {
   if (nseg == 0) return hipSuccess;

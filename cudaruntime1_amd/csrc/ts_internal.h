@@ -302,10 +302,14 @@ hipError_t launch_block_prefix(hipStream_t st, int32_t* bcounts, int32_t nq, int
 // Second, O(n+m) admissible bound over compacted pairs (one wave per pair);
 // survivors are appended to `out` with count *nout (zeroed by the caller:
 // launch_block_prefix clears counts[nq]); a rejected pair's rej[slot] is set
-// to NaN.
+// to NaN. DTW and Frechet only: with tb, tb[slot] = a survivor's bound (the
+// value compared with its threshold), NaN for every other pair; with
+// seed_idx (s DB indices per query, row q - q0), those pairs are rejected
+// too.
 hipError_t launch_tight_filter(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, SetView db,
                                const float* tau, const Item* items, const int32_t* nitems, int64_t max_items,
-                               Item* out, int32_t* nout, float* rej, int grid_waves);
+                               Item* out, int32_t* nout, float* rej, int grid_waves, float* tb = nullptr,
+                               const int32_t* seed_idx = nullptr, int32_t s = 0);
 
 // select.hip
 // Key order: ascending u64. enc_asc(v,i) = ordered(v)<<32|i, desc uses ~ordered(v).
@@ -338,6 +342,18 @@ hipError_t launch_seed_tau(hipStream_t st, const float* vals, const int32_t* see
                            int32_t k, int32_t descending, float* tau, const int32_t* kq = nullptr);
 hipError_t launch_kth_threshold(hipStream_t st, const uint64_t* keys, int32_t nseg, int32_t s,
                                 int32_t k, int32_t descending, float* tau);
+// Second seed round (ascending measures): tau[q] = k-th best (kq[q]-th) of
+// query q's s first-round values and the values rvals[slot] of the candidate
+// slots in its s2 keys top2[q][...] (low 32 bits: slot; ~0: none).
+hipError_t launch_tau2(hipStream_t st, const float* vals, const int32_t* seed_idx, int32_t nq, int32_t s,
+                       const uint64_t* top2, int32_t s2, const float* rvals, const Item* citems, int32_t k,
+                       float* tau, const int32_t* kq = nullptr);
+// keys[i] = (ordered tb[i], i) for the first n candidate slots, ~0 where tb
+// is NaN or the slot holds no pair.
+hipError_t launch_keys_from_slots(hipStream_t st, const float* tb, const Item* citems, int64_t n, uint64_t* keys);
+// out[q] = query q's k keys of top, then the keys of its s seed values.
+hipError_t launch_merge_seed_keys(hipStream_t st, const uint64_t* top, int32_t k, const float* vals,
+                                  const int32_t* seed_idx, int32_t s, int32_t nq, uint64_t* out);
 // Build Items (q0+row, d=decoded idx, slot=row*s+t) from per-row sorted keys;
 // with idx, also the decoded index (-1 for an empty key) per slot, and with
 // nan_vals, NaN into every value slot (the refine overwrites the live ones).
