@@ -717,9 +717,18 @@ double exact_work(const ts_set* qs, const ts_set* db, int64_t nq) {
   return (double)nq * db->count * ql * dl;
 }
 
-// Second seed round: from this batch work (as exact_work), s2 = kSeeds2
-// candidates a query; the TS_TWO_ROUND environment variable overrides the
-// work rule ("0": never, "1": always).
+// DP cells of a pair of mean lengths (the row width the DP visits times the
+// longer length).
+double mean_pair_cells(const ts_set* qs, const ts_set* db, const MeasureArgs& ma) {
+  const double ql = qs->count ? (double)qs->points / qs->count : 0.0;
+  const double dl = db->count ? (double)db->points / db->count : 0.0;
+  const int32_t a = (int32_t)std::min(ql, 2e9), b = (int32_t)std::min(dl, 2e9);
+  return (double)dp_width_of(a, b, ma) * std::max(ql, dl);
+}
+
+// Second seed round: from this batch work (as exact_work), kSeeds2 or twice
+// that candidates a query; the TS_TWO_ROUND environment variable overrides
+// the work rule ("0": never, "1": always), and TS_SEEDS2 the count.
 constexpr double kTwoRoundWork = 1099511627776.0;  // 2^40
 constexpr int32_t kSeeds2 = 32;
 
@@ -734,7 +743,6 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   const int32_t k = p.k;
   const bool desc = desc_measure(p.measure);
   const double work = exact_work(qs, db, nq);
-  const int32_t s = exact_seeds(k, work);
   // Second seed round (DTW and Frechet, batches in the throughput regime):
   // after the tight filter, each query's s2 candidates of smallest bound are
   // refined against the first threshold, and the k-th best of both rounds'
@@ -743,7 +751,17 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   const char* two_env = std::getenv("TS_TWO_ROUND");  // "0" never, "1" always
   const bool two = (p.measure == TS_DTW || p.measure == TS_FRECHET) &&
                    (two_env && *two_env ? two_env[0] == '1' : work >= kTwoRoundWork);
-  const int32_t s2 = two ? kSeeds2 : 0;
+  // Two rounds seed less at first (the second round tightens the
+  // threshold): s = max(2k, 32), and s2 = 64 candidates a query when a mean
+  // pair is under 2^20 DP cells, else 32 (long banded pairs: each extra
+  // refine costs more than the tighter threshold saves).
+  const int32_t s = exact_seeds(k, two ? 0.0 : work, two ? 32 : 0);
+  const char* s2_env = std::getenv("TS_SEEDS2");
+  const int32_t s2 = !two ? 0
+                          : std::max(1, std::min(1024, s2_env && *s2_env ? std::atoi(s2_env)
+                                                                         : (mean_pair_cells(qs, db, margs(p)) < 1048576.0
+                                                                                ? 2 * kSeeds2
+                                                                                : kSeeds2)));
   const int32_t nch = (N + bound_chunk_items(s) - 1) / bound_chunk_items(s);
   const SetView qv = qs->view(), dv = db->view();
   const MeasureArgs ma = margs(p);

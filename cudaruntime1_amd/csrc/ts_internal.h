@@ -159,25 +159,26 @@ __host__ __device__ inline bool dp_ranged(const MeasureArgs& m) {
   return (m.measure == TS_DTW && m.band != 0) || (m.measure == TS_LCSS && m.delta >= 0);
 }
 
-// Columns of a row the DP visits for an n x m pair: the shorter length C,
-// or for a band of half-width w (w = band, or ceil(-band * max(n,m) / 100)
-// for a negative band) or a window delta, at most 2w+1 or 2*delta+1.
 // Seeds per query of the EXACT pipeline (the k-th best of their exact
 // values is the pruning threshold), for a batch of `work` = queries x DB
 // trajectories x mean query length x mean DB length: max(2k, 64) from 2^40
 // on, where a tighter threshold saves more refine work than the extra seeds
 // cost (4096 x 1M DTW: 464 -> 356 ms; 256 x 20k banded DTW of 2048-8192
 // points: 314 -> 271 ms), max(2k, 16) below (64 x 100k of 64-256 points
-// is latency-bound: more seeds lengthen it). TS_EXACT_SEEDS overrides; at
-// least k, at most 1024.
-inline int32_t exact_seeds(int32_t k, double work) {
+// is latency-bound: more seeds lengthen it); lo_override replaces the 64
+// or 16 (a pipeline with a second seed round starts from 32).
+// TS_EXACT_SEEDS overrides; at least k, at most 1024.
+inline int32_t exact_seeds(int32_t k, double work, int32_t lo_override = 0) {
   const char* v = std::getenv("TS_EXACT_SEEDS");
   const int32_t env = v ? std::atoi(v) : 0;
-  const int32_t lo = work >= 1099511627776.0 ? 64 : 16;
+  const int32_t lo = lo_override > 0 ? lo_override : (work >= 1099511627776.0 ? 64 : 16);
   const int32_t s = env > 0 ? env : (2 * k > lo ? 2 * k : lo);
   return s < k ? k : (s > 1024 ? 1024 : s);
 }
 
+// Columns of a row the DP visits for an n x m pair: the shorter length C,
+// or for a band of half-width w (w = band, or ceil(-band * max(n,m) / 100)
+// for a negative band) or a window delta, at most 2w+1 or 2*delta+1.
 __host__ __device__ inline int32_t dp_width_of(int32_t n, int32_t m, const MeasureArgs& a) {
   const int32_t c = n < m ? n : m, r = n < m ? m : n;
   int64_t e = c;
