@@ -729,7 +729,12 @@ double mean_pair_cells(const ts_set* qs, const ts_set* db, const MeasureArgs& ma
 // Second seed round: from this batch work (as exact_work), kSeeds2 or twice
 // that candidates a query; the TS_TWO_ROUND environment variable overrides
 // the work rule ("0": never, "1": always), and TS_SEEDS2 the count.
-constexpr double kTwoRoundWork = 1099511627776.0;  // 2^40
+// The second round adds a DP launch round to the batch's critical path, so
+// it starts where refines are long: 2^44 keeps the serving loop's batches
+// of up to ~250 queries against 2M trajectories of 16-512 points on one
+// round (from 2^40, p99 at 5k QPS went 21 -> 33 ms), and 4096 queries x 1M
+// (2^48.6) or 256 x 20k of 2048-8192 points (2^46.9) on two.
+constexpr double kTwoRoundWork = 17592186044416.0;  // 2^44
 constexpr int32_t kSeeds2 = 32;
 
 // Largest batch (queries x DB trajectories) whose seed-pass bounds are kept
@@ -743,7 +748,8 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   const int32_t k = p.k;
   const bool desc = desc_measure(p.measure);
   const double work = exact_work(qs, db, nq);
-  // Second seed round (DTW and Frechet, batches in the throughput regime):
+  // Second seed round (DTW and Frechet, batches in the throughput regime,
+  // from kTwoRoundWork):
   // after the tight filter, each query's s2 candidates of smallest bound are
   // refined against the first threshold, and the k-th best of both rounds'
   // values becomes the main refine's threshold. The first round's seeds
