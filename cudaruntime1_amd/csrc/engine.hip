@@ -758,14 +758,17 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   const bool two = (p.measure == TS_DTW || p.measure == TS_FRECHET) &&
                    (two_env && *two_env ? two_env[0] == '1' : work >= kTwoRoundWork);
   // Two rounds seed less at first (the second round tightens the
-  // threshold): s = max(2k, 32), and s2 = 64 candidates a query when a mean
-  // pair is under 2^20 DP cells, else 32 (long banded pairs: each extra
-  // refine costs more than the tighter threshold saves).
+  // threshold): s = max(2k, 32), and s2 = 64 candidates a query for DTW
+  // when a mean pair is under 2^20 DP cells, else 32 (long banded pairs:
+  // each extra refine costs more than the tighter threshold saves; Frechet's
+  // max-based bounds leave it little to tighten).
   const int32_t s = exact_seeds(k, two ? 0.0 : work, two ? 32 : 0);
   const char* s2_env = std::getenv("TS_SEEDS2");
   const int32_t s2 = !two ? 0
                           : std::max(1, std::min(1024, s2_env && *s2_env ? std::atoi(s2_env)
-                                                                         : (mean_pair_cells(qs, db, margs(p)) < 1048576.0
+                                                                         : (p.measure == TS_DTW &&
+                                                                                    mean_pair_cells(qs, db, margs(p)) <
+                                                                                        1048576.0
                                                                                 ? 2 * kSeeds2
                                                                                 : kSeeds2)));
   const int32_t nch = (N + bound_chunk_items(s) - 1) / bound_chunk_items(s);
