@@ -202,6 +202,54 @@ __global__ void __launch_bounds__(256) k_refine_prep(const uint64_t* keys, int32
   }
 }
 
+// Upper bound of each seed pair's value from one warping path: along the
+// longer trajectory (L points), point t is matched with point t*S/L of the
+// shorter (S points), a monotone path from (0,0) to (L-1,S-1) with unit
+// steps that lies in every Sakoe-Chiba band of width w >= 1 (|i*m - j*n| is
+// then below max(n,m)). DTW's value is the least f32 running sum over its
+// paths, so at most this path's running sum, which the lanes' sum (another
+// order) times 1 + (L + 64) 2^-22 exceeds; Frechet's is at most the path's
+// largest cost (max never rounds). One wave per seed; seed_idx[i] gets the
+// seed's DB index (-1 and +inf for missing seeds).
+__global__ void __launch_bounds__(256) k_seed_ub(const uint64_t* keys, int32_t s, int32_t q0, int64_t n_seeds,
+                                                 SetView q, SetView db, int32_t measure, int32_t* seed_idx,
+                                                 float* ub) {
+  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n_seeds) return;
+  const uint64_t key = keys[i];
+  const int d = key == ~0ull ? -1 : (int32_t)(uint32_t)key;
+  if (d < 0) {
+    if (lane == 0) {
+      seed_idx[i] = -1;
+      ub[i] = __int_as_float(0x7f800000);
+    }
+    return;
+  }
+  const int qi = q0 + (int)(i / s);
+  const int n = q.len[qi], m = db.len[d];
+  const int64_t qo = q.off[qi], dof = db.off[d];
+  const bool qlong = n >= m;
+  const int L = qlong ? n : m, S = qlong ? m : n;
+  const float *Ax = qlong ? q.x + qo : db.x + dof, *Ay = qlong ? q.y + qo : db.y + dof;
+  const float *Bx = qlong ? db.x + dof : q.x + qo, *By = qlong ? db.y + dof : q.y + qo;
+  float acc = 0.0f;
+  for (int t = lane; t < L; t += 64) {
+    const int j = (int)(((int64_t)t * S) / L);
+    const float dx = Ax[t] - Bx[j], dy = Ay[t] - By[j];
+    const float c2 = dx * dx + dy * dy;
+    acc = measure == TS_DTW ? acc + sqrtf(c2) : fmaxf(acc, c2);
+  }
+  for (int sh = 32; sh > 0; sh >>= 1) {
+    const float o = __shfl_xor(acc, sh);
+    acc = measure == TS_DTW ? acc + o : fmaxf(acc, o);
+  }
+  if (lane == 0) {
+    seed_idx[i] = d;
+    ub[i] = measure == TS_DTW ? acc * (1.0f + (float)(L + 64) * 0x1p-22f) : sqrtf(acc);
+  }
+}
+
 __global__ void k_fill_f32(float* p, float v, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i < n) p[i] = v;
@@ -215,7 +263,7 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 enum BufId {
   B_ENV, B_SCORES, B_KEYS, B_KEYS2, B_SCA, B_SCB, B_CAND, B_CANDIDX, B_ITEMS, B_ITEMS3, B_VALS, B_REGION,
   B_RVALS, B_COUNTS, B_CTR, B_TAU, B_TOP, B_OIDX, B_OVAL, B_CELLS, B_HOSTIN, B_ITEMS2, B_RITEMS, B_SEGC, B_NEED, B_BCNT,
-  B_BITS, B_BOUNDS, B_KQ, B_TB, B_TOP2, B_CANDIDX2, B_ITEMS5, B_MKEYS, B_TOPA, B_NBUF
+  B_BITS, B_BOUNDS, B_KQ, B_TB, B_TOP2, B_CANDIDX2, B_ITEMS5, B_MKEYS, B_TOPA, B_UB, B_TAUUB, B_NBUF
 };
 
 struct Buf {
@@ -802,7 +850,26 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   }));
   TS_TRY(topk(e, bkeys, (int64_t)nch * s, nch * s, nullptr, nq, s, seeds));
   const KeyItems seed_items{seeds, s, q0, seed_idx};
-  TS_TRY(refine(e, p, db, qs, items, nullptr, (int64_t)nq * s, vals, 1, ctrs, &seed_items));
+  // The seeds stop early against the k-th smallest upper bound of their
+  // values (DTW, Frechet): a seed above it cannot be among the k best seeds,
+  // whose own bounds keep k of them below it, so the threshold the seeds
+  // give is unchanged.
+  float* tau_ub = nullptr;
+  if (p.measure == TS_DTW || p.measure == TS_FRECHET) {
+    float* ub = nullptr;
+    TS_TRY(ensure(e, B_UB, (int64_t)nq * s, &ub));
+    TS_TRY(ensure(e, B_TAUUB, nq, &tau_ub));
+    TS_TRY(run(e, "seed_ub", 0, 0, [&] {
+      const int64_t ns = (int64_t)nq * s;
+      hipLaunchKernelGGL(k_seed_ub, dim3((unsigned)((ns + 3) / 4)), dim3(256), 0, e->st, seeds, s, q0, ns, qv, dv,
+                         p.measure, seed_idx, ub);
+      return hipGetLastError();
+    }));
+    TS_TRY(run(e, "threshold", 0, 0, [&] {
+      return launch_seed_tau(e->st, ub, seed_idx, nq, s, k, 0, tau_ub, kq_dev ? kq_dev + q0 : nullptr);
+    }));
+  }
+  TS_TRY(refine(e, p, db, qs, items, nullptr, (int64_t)nq * s, vals, 1, ctrs, &seed_items, nullptr, tau_ub, q0));
   // 2. threshold: k-th best exact value among the seeds
   TS_TRY(run(e, "threshold", 0, 0, [&] {
     return launch_seed_tau(e->st, vals, seed_idx, nq, s, k, desc, tau, kq_dev ? kq_dev + q0 : nullptr);
