@@ -93,6 +93,42 @@ __device__ __forceinline__ float from_prev_lane(float v, float first0, int lg) {
   }
 }
 
+// from_prev_lane over a step's U values. W = 32 takes one instruction per
+// value: v_cndmask_b32 with a DPP (wave_shr:1, out-of-range lanes read 0)
+// first operand, VCC holding lanes 0 and 32, selects first0 there and the
+// shifted value elsewhere. The s_nop gives the 2 wait states a DPP read
+// needs after a VALU write of its source.
+#define TS_CND_DPP(D, S) "v_cndmask_b32_dpp %" #D ", %" #S ", %" #D ", vcc wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+#define TS_CND_HEAD "s_mov_b32 vcc_lo, 1\n\ts_mov_b32 vcc_hi, 1\n\ts_nop 1\n\t"
+template <int W, int U>
+__device__ __forceinline__ void from_prev_lanes(float (&up)[U], const float (&o)[U], const float (&first0)[U], int lg) {
+  if constexpr (W == 32 && U == 8) {
+#pragma unroll
+    for (int j = 0; j < U; ++j) up[j] = first0[j];
+    asm(TS_CND_HEAD TS_CND_DPP(0, 8) TS_CND_DPP(1, 9) TS_CND_DPP(2, 10) TS_CND_DPP(3, 11) TS_CND_DPP(4, 12)
+            TS_CND_DPP(5, 13) TS_CND_DPP(6, 14) TS_CND_DPP(7, 15)
+        : "+v"(up[0]), "+v"(up[1]), "+v"(up[2]), "+v"(up[3]), "+v"(up[4]), "+v"(up[5]), "+v"(up[6]), "+v"(up[7])
+        : "v"(o[0]), "v"(o[1]), "v"(o[2]), "v"(o[3]), "v"(o[4]), "v"(o[5]), "v"(o[6]), "v"(o[7])
+        : "vcc");
+  } else if constexpr (W == 32 && U == 4) {
+#pragma unroll
+    for (int j = 0; j < U; ++j) up[j] = first0[j];
+    asm(TS_CND_HEAD TS_CND_DPP(0, 4) TS_CND_DPP(1, 5) TS_CND_DPP(2, 6) TS_CND_DPP(3, 7)
+        : "+v"(up[0]), "+v"(up[1]), "+v"(up[2]), "+v"(up[3])
+        : "v"(o[0]), "v"(o[1]), "v"(o[2]), "v"(o[3])
+        : "vcc");
+  } else if constexpr (W == 32 && U == 2) {
+    up[0] = first0[0];
+    up[1] = first0[1];
+    asm(TS_CND_HEAD TS_CND_DPP(0, 2) TS_CND_DPP(1, 3) : "+v"(up[0]), "+v"(up[1]) : "v"(o[0]), "v"(o[1]) : "vcc");
+  } else {
+#pragma unroll
+    for (int j = 0; j < U; ++j) up[j] = from_prev_lane<W>(o[j], first0[j], lg);
+  }
+}
+#undef TS_CND_DPP
+#undef TS_CND_HEAD
+
 template <int U>
 __device__ __forceinline__ void load_vec(const float* p, float (&v)[U]) {
   if constexpr (U == 8) {
@@ -452,7 +488,9 @@ __global__ void __launch_bounds__(256) k_dp(DpArgs a) {
 //    D+W-1+k, after the read;
 //  * row changes are selects. Rows are single-buffered: a lane uses a row
 //    point read in the step before a strip starts, and every lane's last
-//    strip of a pair starts before lane 0 refills the buffer.
+//    strip of a pair starts before lane 0 refills the buffer. Rows read
+//    from global memory need no select: the point read in the step before
+//    is that of the lane's current row (its own pair's, set at the start).
 template <int U>
 __device__ __forceinline__ void load_pts(const float2* p, float (&x)[U], float (&y)[U]) {
   if constexpr (U == 1) {
@@ -571,7 +609,7 @@ __device__ __forceinline__ int next_event(int rem, int lg, int res_rem) {
 // barriers keep the reads below the point terms and above the rest.
 #define TS_LEAN_STEP(TINY)                                                                          \
   do {                                                                                              \
-    if (wrapped) {                                                                                  \
+    if (!SR || wrapped) {                                                                           \
       ax = rp.x;                                                                                    \
       ay = rp.y;                                                                                    \
     }                                                                                               \
@@ -610,8 +648,7 @@ __device__ __forceinline__ int next_event(int rem, int lg, int res_rem) {
                                                                                                     \
     /* ---- U cells */                                                                              \
     float up[U], on[U];                                                                             \
-    _Pragma("unroll")                                                                               \
-    for (int j = 0; j < U; ++j) up[j] = from_prev_lane<W>(o[j], hv[j], lg);                         \
+    from_prev_lanes<W, U>(up, o, hv, lg);                                                           \
     const bool first = kh == 0;                                                                     \
     float d = first ? dfirst : up_last;                                                             \
     float l = first ? BND : o[U - 1];                                                               \
@@ -845,6 +882,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
           } else {
             ax = Ax[min(lg, R - 1)];
             ay = Ay[min(lg, R - 1)];
+            rp = make_float2(ax, ay);
           }
           load_vec<U>(brow, hv);
           dfirst = (MEAS == TS_LCSS || lg == 0) ? 0.0f : BND;
