@@ -819,7 +819,8 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
                                                                                         1048576.0
                                                                                 ? 2 * kSeeds2
                                                                                 : kSeeds2)));
-  const int32_t nch = (N + bound_chunk_items(s) - 1) / bound_chunk_items(s);
+  const int32_t chunk = bound_chunk_items(s, nq, N);
+  const int32_t nch = (N + chunk - 1) / chunk;
   const SetView qv = qs->view(), dv = db->view();
   const MeasureArgs ma = margs(p);
   uint64_t *bkeys = nullptr, *seeds = nullptr;
@@ -846,7 +847,7 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   if ((int64_t)nq * N <= kStoredBoundsMax) TS_TRY(ensure(e, B_BOUNDS, (int64_t)nq * N, &bvals));
   // 1. seeds: good candidates by bound, refined exactly
   TS_TRY(run(e, "bound_seed", pairs * 36.0, pairs, [&] {
-    return launch_bound_chunks(e->st, ma, k, qv, q0, nq, dv, s, bound_chunk_items(s), bkeys, bvals, ctrs, 96);
+    return launch_bound_chunks(e->st, ma, k, qv, q0, nq, dv, s, chunk, bkeys, bvals, ctrs, 96);
   }));
   TS_TRY(topk(e, bkeys, (int64_t)nch * s, nch * s, nullptr, nq, s, seeds));
   const KeyItems seed_items{seeds, s, q0, seed_idx};

@@ -288,11 +288,69 @@ __device__ __forceinline__ float pair_bound_m(const MeasureArgs& ma, const QSum&
   return pair_bound(mc, qs, q, cm, ce, m);
 }
 
+// Seed ranking, where no pair's bound is kept: seeds are refined exactly and
+// any exact value bounds the k-th best from above, so their order need not
+// come from an admissible bound. DTW: the bound's terms with plain square
+// roots and no rounding discount; Frechet: the largest squared term (the
+// square root is monotone).
+template <int MEAS>
+__device__ __forceinline__ float seed_rank(const QSum& qs, const float4 cm, const float4 ce, int m) {
+  const float4 a = qs.mbr;
+  const float gx = fmaxf(0.0f, fmaxf(cm.x - a.y, a.x - cm.y));
+  const float gy = fmaxf(0.0f, fmaxf(cm.z - a.w, a.z - cm.w));
+  const float g2 = gx * gx + gy * gy;
+  const float dx = qs.ends.x - ce.x, dy = qs.ends.y - ce.y, ex = qs.ends.z - ce.z, ey = qs.ends.w - ce.w;
+  if (MEAS == TS_FRECHET) return fmaxf(g2, fmaxf(dx * dx + dy * dy, ex * ex + ey * ey));
+  const int L = max(qs.n, m);
+  return __builtin_amdgcn_sqrtf(dx * dx + dy * dy) + (L >= 2 ? __builtin_amdgcn_sqrtf(ex * ex + ey * ey) : 0.0f) +
+         (float)max(0, L - 2) * __builtin_amdgcn_sqrtf(g2);
+}
+
+// The compaction's test of a distance bound, b <= tq. DTW and Frechet first
+// try to reject on the MBR-gap term alone, compared squared (no square
+// root): the bound is at least (L-2) sqrt_lo(g2) disc (DTW, L = max(n, m))
+// or sqrt_lo(g2) (Frechet). With g2 >= 2^-96 (where sqrt_lo is not 0) and
+// tq >= 2^-40 (tq^2 normal), a pair whose bound is at most tq has
+// ((L-2) disc)^2 g2 (DTW) or g2 (Frechet) at most tq^2 (1 + 2^-17) as f32
+// evaluates it (sqrt_lo(g2) >= sqrt(g2) (1 - 2^-20), and each product or sum
+// rounds by at most 2^-24), which is below r2 = tq^2 (1 + 2^-16) in f32: the
+// shortcut rejects only pairs the bound rejects. r2 = +inf turns it off.
+template <int MEAS>
+__device__ __forceinline__ bool bound_passes(const MeasureArgs& ma, const QSum& qs, const SetView& q, const float4 cm,
+                                             const float4 ce, int m, float tq, float r2) {
+  if (MEAS == TS_DTW || MEAS == TS_FRECHET) {
+    const float4 a = qs.mbr;
+    const float gx = fmaxf(0.0f, fmaxf(cm.x - a.y, a.x - cm.y));
+    const float gy = fmaxf(0.0f, fmaxf(cm.z - a.w, a.z - cm.w));
+    const float g2 = gx * gx + gy * gy;
+    float lhs = g2;
+    if (MEAS == TS_DTW) {
+      const int L = max(qs.n, m);
+      const float c = (float)max(0, L - 2) * (1.0f - (float)(L + 8) * 1.1920929e-7f);
+      lhs = c * c * g2;
+    }
+    if (g2 >= 0x1p-96f && lhs > r2) return false;
+  }
+  const float b = pair_bound_m<MEAS>(ma, qs, q, cm, ce, m);
+  return isfinite(b) && (b <= tq);
+}
+
+// Queries per seeding block (the keep-the-best form).
+constexpr int kSeedQB = 8;
+
 // DB trajectories per seeding block and thread: 16 when the block keeps the
 // best key of each thread (s <= 256: the chunk's blocks fit the chip in one
-// round at the 64x100k scale), 8 when it sorts every key of the chunk.
-constexpr int kSeedPerThread = 16, kSeedPerThreadWhole = 8;
+// round at the 64x100k scale), 64 when the batch still gives 4 blocks per CU
+// at that (a quarter of the blocks, so of the per-block sorts: the 4096 x 1M
+// DTW seeding went 8.4 -> 5.5 ms), 8 when it sorts every key of the chunk.
+constexpr int kSeedPerThread = 16, kSeedPerThreadWide = 64, kSeedPerThreadWhole = 8;
 int32_t bound_chunk_items(int32_t s) { return 256 * (s <= 256 ? kSeedPerThread : kSeedPerThreadWhole); }
+int32_t bound_chunk_items(int32_t s, int32_t nq, int32_t n_db) {
+  if (s > 256) return 256 * kSeedPerThreadWhole;
+  const int64_t wide_blocks = (int64_t)((n_db + 256 * kSeedPerThreadWide - 1) / (256 * kSeedPerThreadWide)) *
+                              ((nq + kSeedQB - 1) / kSeedQB);
+  return 256 * (wide_blocks >= 4ll * device_cu_count() ? kSeedPerThreadWide : kSeedPerThread);
+}
 
 // Bitonic sort of `nseg` consecutive segments of `seg` keys each (seg a
 // power of two), every segment ascending: compare partners and directions
@@ -321,12 +379,12 @@ __device__ void bitonic_sort_segments(uint64_t* s, int seg, int nseg, int tid, i
 // key of its kSeedPerThread trajectories and the block keeps the best `s` of
 // those (s <= 256), for QB queries at once: every DB summary (36 B) is read
 // once per QB queries. Larger s sorts the whole chunk, one query per block.
-template <bool WHOLE, int MEAS, int QB>
+// RANK: seed_rank instead of the bound (DTW, Frechet; no bstore).
+template <bool WHOLE, int MEAS, int QB, int PER, bool RANK>
 __global__ void __launch_bounds__(256) k_bound_chunks(MeasureArgs ma, SetView q, int32_t q0, int32_t nq, SetView db,
                                                       int32_t s, uint64_t* keys, float* bstore, int32_t* zero,
                                                       int32_t nzero) {
   static_assert(!WHOLE || QB == 1, "whole-chunk sorts take one query per block");
-  constexpr int PER = WHOLE ? kSeedPerThreadWhole : kSeedPerThread;
   constexpr int NS = WHOLE ? 256 * PER : 256;
   __shared__ uint64_t sk[NS * QB];
   // counters the batch's later kernels expect to start at zero
@@ -358,8 +416,8 @@ __global__ void __launch_bounds__(256) k_bound_chunks(MeasureArgs ma, SetView q,
       for (int j = 0; j < QB; ++j) {
         uint64_t key = ~0ull;
         if (ok) {
-          const float b = pair_bound_m<MEAS>(ma, qs[j], q, cm, ce, m);
-          if (bstore && qr0 + j < nq) bstore[(int64_t)(qr0 + j) * db.count + jd] = b;  // for the count pass
+          const float b = RANK ? seed_rank<MEAS>(qs[j], cm, ce, m) : pair_bound_m<MEAS>(ma, qs[j], q, cm, ce, m);
+          if (!RANK && bstore && qr0 + j < nq) bstore[(int64_t)(qr0 + j) * db.count + jd] = b;  // for the count pass
           key = make_key(b, jd, desc);
         }
         if (WHOLE) sk[i] = key;
@@ -381,23 +439,34 @@ __global__ void __launch_bounds__(256) k_bound_chunks(MeasureArgs ma, SetView q,
   }
 }
 
-// Queries per seeding block (the keep-the-best form).
-constexpr int kSeedQB = 8;
-
 hipError_t launch_bound_chunks(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, int32_t nq,
                                SetView db, int32_t s, int32_t chunk, uint64_t* keys, float* bstore, int32_t* zero,
                                int32_t nzero) {
   (void)k;
-  (void)chunk;
-  const int32_t items = bound_chunk_items(s);
-  const int32_t gx = (db.count + items - 1) / items;
-#define TS_CHUNKS(MEAS)                                                                                         \
-  if (s <= 256)                                                                                                 \
-    hipLaunchKernelGGL((k_bound_chunks<false, MEAS, kSeedQB>), dim3(gx, (nq + kSeedQB - 1) / kSeedQB), dim3(256), \
-                       0, st, m, q, q0, nq, db, s, keys, bstore, zero, nzero);                                   \
-  else                                                                                                          \
-    hipLaunchKernelGGL((k_bound_chunks<true, MEAS, 1>), dim3(gx, nq), dim3(256), 0, st, m, q, q0, nq, db, s, keys, \
-                       bstore, zero, nzero);
+  if (chunk != bound_chunk_items(s, nq, db.count)) return hipErrorInvalidValue;
+  const int32_t gx = (db.count + chunk - 1) / chunk;
+  const dim3 grid_qb(gx, (nq + kSeedQB - 1) / kSeedQB);
+  const bool rank = !bstore && (m.measure == TS_DTW || m.measure == TS_FRECHET);
+#define TS_CHUNKS_R(MEAS, RANK)                                                                                  \
+  if (s > 256)                                                                                                   \
+    hipLaunchKernelGGL((k_bound_chunks<true, MEAS, 1, kSeedPerThreadWhole, RANK>), dim3(gx, nq), dim3(256), 0, st, \
+                       m, q, q0, nq, db, s, keys, bstore, zero, nzero);                                          \
+  else if (chunk == 256 * kSeedPerThreadWide)                                                                    \
+    hipLaunchKernelGGL((k_bound_chunks<false, MEAS, kSeedQB, kSeedPerThreadWide, RANK>), grid_qb, dim3(256), 0,  \
+                       st, m, q, q0, nq, db, s, keys, bstore, zero, nzero);                                      \
+  else                                                                                                           \
+    hipLaunchKernelGGL((k_bound_chunks<false, MEAS, kSeedQB, kSeedPerThread, RANK>), grid_qb, dim3(256), 0, st,  \
+                       m, q, q0, nq, db, s, keys, bstore, zero, nzero);
+#define TS_CHUNKS(MEAS)                   \
+  if constexpr (MEAS == TS_DTW || MEAS == TS_FRECHET) { \
+    if (rank) {                           \
+      TS_CHUNKS_R(MEAS, true)             \
+    } else {                              \
+      TS_CHUNKS_R(MEAS, false)            \
+    }                                     \
+  } else {                                \
+    TS_CHUNKS_R(MEAS, false)              \
+  }
   switch (m.measure) {
     case TS_DTW: TS_CHUNKS(TS_DTW) break;
     case TS_FRECHET: TS_CHUNKS(TS_FRECHET) break;
@@ -407,10 +476,11 @@ hipError_t launch_bound_chunks(hipStream_t st, MeasureArgs m, int32_t k, SetView
     default: return hipErrorInvalidValue;
   }
 #undef TS_CHUNKS
+#undef TS_CHUNKS_R
   return hipGetLastError();
 }
 
-// Threshold compaction, kCompactQ queries per block and kCompactJ DB
+// Threshold compaction, QB queries per block and kCompactJ DB
 // trajectories per thread: each DB summary is loaded once and tested against
 // every query of the block. A pair survives when its bound can still reach
 // the query's k-th value tau: distances bound <= tau; LCSS ub >= tau and,
@@ -422,27 +492,31 @@ hipError_t launch_bound_chunks(hipStream_t st, MeasureArgs m, int32_t k, SetView
 // the counts into exclusive offsets within each query's region, WRITE reads
 // the masks back (no bound is evaluated twice) and places the block's
 // survivors at off[q] + bcounts[...] + an LDS cursor (slot = position).
-constexpr int kCompactQ = 8, kCompactJ = 2;
+// The count pass takes 16 queries per block (half the DB summary reads of 8:
+// 4096 x 1M DTW 4.97 -> 4.47 ms), the write pass 8 (2.83 -> 2.53 ms).
+constexpr int kCompactQCount = 16, kCompactQWrite = 8, kCompactJ = 2;
 
 int32_t compact_blocks(int32_t n_db) { return (n_db + 256 * kCompactJ - 1) / (256 * kCompactJ); }
 int64_t compact_mask_words(int32_t n_db) { return (int64_t)compact_blocks(n_db) * kCompactJ * 4; }
 
-template <bool WRITE, int MEAS>
+template <bool WRITE, int MEAS, int QB = WRITE ? kCompactQWrite : kCompactQCount>
 __global__ void __launch_bounds__(256) k_bound_compact(MeasureArgs ma, int32_t k, SetView q, int32_t q0, int32_t nq,
                                                        SetView db, const float* tau, const int64_t* off,
                                                        const float* bvals, int32_t* bcounts, uint64_t* bits,
                                                        Item* items, int64_t cap) {
-  __shared__ QSum sq[kCompactQ];
-  __shared__ float st[kCompactQ];
-  __shared__ int bc[kCompactQ];
-  __shared__ int64_t bbase[kCompactQ];
-  const int qb = blockIdx.y * kCompactQ;
-  const int nqb = min(kCompactQ, nq - qb);
+  __shared__ QSum sq[QB];
+  __shared__ float st[QB], sr2[QB];
+  __shared__ int bc[QB];
+  __shared__ int64_t bbase[QB];
+  const int qb = blockIdx.y * QB;
+  const int nqb = min(QB, nq - qb);
   const int gx = gridDim.x, bx = blockIdx.x;
   if (threadIdx.x < nqb) {
     const int qi = q0 + qb + threadIdx.x;
     sq[threadIdx.x] = QSum{q.mbr[qi], q.ends[qi], q.len[qi], q.off[qi]};
-    st[threadIdx.x] = tau[qb + threadIdx.x];
+    const float tq = tau[qb + threadIdx.x];
+    st[threadIdx.x] = tq;
+    sr2[threadIdx.x] = tq >= 0x1p-40f ? tq * tq * (1.0f + 0x1p-16f) : __int_as_float(0x7f800000);  // see bound_passes
     bc[threadIdx.x] = 0;
     if (WRITE) bbase[threadIdx.x] = off[qb + threadIdx.x] + bcounts[(int64_t)(qb + threadIdx.x) * gx + bx];
   }
@@ -463,15 +537,15 @@ __global__ void __launch_bounds__(256) k_bound_compact(MeasureArgs ma, int32_t k
     }
     // every load of the block's queries up front (independent, so their
     // latencies overlap)
-    uint64_t wbits[kCompactQ];
-    float sb[kCompactQ];
+    uint64_t wbits[QB];
+    float sb[QB];
 #pragma unroll
-    for (int u = 0; u < kCompactQ; ++u) {
+    for (int u = 0; u < QB; ++u) {
       if (WRITE) wbits[u] = u < nqb ? bits[(int64_t)(qb + u) * words + word] : 0ull;
       else sb[u] = (bvals && valid && u < nqb) ? bvals[(int64_t)(qb + u) * db.count + j] : 0.0f;
     }
 #pragma unroll
-    for (int u = 0; u < kCompactQ; ++u) {
+    for (int u = 0; u < QB; ++u) {
       if (u >= nqb) break;
       uint64_t bal;
       bool pass;
@@ -481,10 +555,15 @@ __global__ void __launch_bounds__(256) k_bound_compact(MeasureArgs ma, int32_t k
       } else {
         pass = false;
         if (valid) {
-          const float b = bvals ? sb[u] : pair_bound_m<MEAS>(ma, sq[u], q, cm, ce, m);
           const float tq = st[u];
-          if (MEAS == TS_LCSS) pass = (b >= tq) && (b > 0.0f || j < k);
-          else pass = isfinite(b) && (b <= tq);
+          if (MEAS == TS_LCSS) {
+            const float b = bvals ? sb[u] : pair_bound_m<MEAS>(ma, sq[u], q, cm, ce, m);
+            pass = (b >= tq) && (b > 0.0f || j < k);
+          } else if (bvals) {
+            pass = isfinite(sb[u]) && (sb[u] <= tq);
+          } else {
+            pass = bound_passes<MEAS>(ma, sq[u], q, cm, ce, m, tq, sr2[u]);
+          }
         }
         bal = __ballot(pass);
         if (lane == 0) bits[(int64_t)(qb + u) * words + word] = bal;
@@ -545,13 +624,14 @@ hipError_t launch_block_prefix(hipStream_t st, int32_t* bcounts, int32_t nq, int
 hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, int32_t nq,
                                 SetView db, const float* tau, const int64_t* off, const float* bvals, int32_t* counts,
                                 uint64_t* bits, Item* items, int64_t cap) {
-  dim3 grid(compact_blocks(db.count), (nq + kCompactQ - 1) / kCompactQ);
-#define TS_COMPACT(MEAS)                                                                                       \
-  if (items)                                                                                                   \
-    hipLaunchKernelGGL((k_bound_compact<true, MEAS>), grid, dim3(256), 0, st, m, k, q, q0, nq, db, tau, off, bvals, \
-                       counts, bits, items, cap);                                                                     \
-  else                                                                                                         \
-    hipLaunchKernelGGL((k_bound_compact<false, MEAS>), grid, dim3(256), 0, st, m, k, q, q0, nq, db, tau, off,  \
+  const dim3 grid_w(compact_blocks(db.count), (nq + kCompactQWrite - 1) / kCompactQWrite);
+  const dim3 grid_c(compact_blocks(db.count), (nq + kCompactQCount - 1) / kCompactQCount);
+#define TS_COMPACT(MEAS)                                                                                          \
+  if (items)                                                                                                      \
+    hipLaunchKernelGGL((k_bound_compact<true, MEAS>), grid_w, dim3(256), 0, st, m, k, q, q0, nq, db, tau, off,    \
+                       bvals, counts, bits, items, cap);                                                          \
+  else                                                                                                            \
+    hipLaunchKernelGGL((k_bound_compact<false, MEAS>), grid_c, dim3(256), 0, st, m, k, q, q0, nq, db, tau, off,   \
                        bvals, counts, bits, items, cap);
   switch (m.measure) {
     case TS_DTW: TS_COMPACT(TS_DTW) break;

@@ -276,9 +276,11 @@ hipError_t launch_envelope(hipStream_t st, SetView q, int32_t max_len, int32_t r
 hipError_t launch_filter_ref(hipStream_t st, int32_t measure, SetView q, int32_t q0, int32_t nq,
                              const float4* env, SetView db, const TileView* tiles, float eps, float* scores);
 // Admissible summary bounds (EXACT mode) of queries [q0,q0+nq) against the
-// whole DB, reduced per DB chunk of bound_chunk_items(s) trajectories to the
-// best `s` keys per query: keys[(q * nchunks + chunk) * s + t].
+// whole DB, reduced per DB chunk of `chunk` = bound_chunk_items(s, nq, N)
+// trajectories to the best `s` keys per query: keys[(q * nchunks + c) * s + t].
+// bound_chunk_items(s) is the smallest chunk any batch takes (for sizing).
 int32_t bound_chunk_items(int32_t s);
+int32_t bound_chunk_items(int32_t s, int32_t nq, int32_t n_db);
 // With bstore, also stores every pair's bound at bstore[q * N + j]; zeroes
 // zero[0, nzero) (nzero <= 256) for the batch's later kernels.
 hipError_t launch_bound_chunks(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0,

@@ -3,8 +3,9 @@ kernels' schedules, modelled lane by lane against the oracle
 (tests/emu/dp_lean_schedule.cpp, unbanded; tests/emu/dp_lean_ranged_schedule.cpp,
 banded DTW and windowed LCSS: lane stagger, pair fetch and start, strip table,
 halo and ring timing, result countdown, staged operands' provenance, LDS
-bounds), and the neighbour fixup that makes v_sqrt_f32 a correctly rounded
-square root (tests/emu/sqrt_fixup.c)."""
+bounds), the neighbour fixup that makes v_sqrt_f32 a correctly rounded
+square root (tests/emu/sqrt_fixup.c), and the threshold compaction's
+squared shortcut (tests/emu/compact_shortcut.c)."""
 import os
 import shutil
 import subprocess
@@ -98,3 +99,19 @@ def test_sqrt_neighbour_fixup_is_correctly_rounded(tmp_path):
     r = subprocess.run([exe, "all"], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0 and " 0 mismatches" in r.stdout, r.stdout + r.stderr
     assert int(r.stdout.split()[0]) == 3 * ((0x7f0 - 0x0f8) << 20)
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
+def test_compaction_shortcut_rejects_only_what_the_bound_rejects(tmp_path):
+    """The compaction's squared gap-term test (DTW and Frechet) never rejects
+    a pair whose f32 bound is at most the threshold, over 40M random pairs
+    with the threshold at the bound or a few ulps above it and the hardware
+    square root one ulp either way; with the margin cut from 2^-16 to 2^-21
+    (a plant) it does."""
+    exe = str(tmp_path / "compact_shortcut")
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-o", exe, os.path.join(HERE, "emu", "compact_shortcut.c"),
+                    "-lm"], check=True, timeout=120)
+    r = subprocess.run([exe, "20000000"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.strip() == "40000000 pairs, 0 violations", r.stdout + r.stderr
+    r = subprocess.run([exe, "20000000", "21"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and not r.stdout.strip().endswith(" 0 violations"), r.stdout
