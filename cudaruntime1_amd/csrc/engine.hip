@@ -263,7 +263,7 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 enum BufId {
   B_ENV, B_SCORES, B_KEYS, B_KEYS2, B_SCA, B_SCB, B_CAND, B_CANDIDX, B_ITEMS, B_ITEMS3, B_VALS, B_REGION,
   B_RVALS, B_COUNTS, B_CTR, B_TAU, B_TOP, B_OIDX, B_OVAL, B_CELLS, B_HOSTIN, B_ITEMS2, B_RITEMS, B_SEGC, B_NEED, B_BCNT,
-  B_BITS, B_BOUNDS, B_KQ, B_TB, B_TOP2, B_CANDIDX2, B_ITEMS5, B_MKEYS, B_TOPA, B_UB, B_TAUUB, B_NBUF
+  B_BITS, B_BOUNDS, B_KQ, B_TB, B_TOP2, B_CANDIDX2, B_ITEMS5, B_MKEYS, B_TOPA, B_UB, B_TAUUB, B_PRE, B_NBUF
 };
 
 struct Buf {
@@ -834,7 +834,7 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   TS_TRY(ensure(e, B_TAU, nq, &tau));
   TS_TRY(ensure(e, B_VALS, (int64_t)nq * s, &vals));
   TS_TRY(ensure(e, B_ITEMS, (int64_t)nq * s, &items));
-  TS_TRY(ensure(e, B_COUNTS, nq + 1, &counts));
+  TS_TRY(ensure(e, B_COUNTS, nq + 2, &counts));
   TS_TRY(ensure(e, B_REGION, 2 * (int64_t)(nq + 1), (int32_t**)&d_off));
   // DP counter sets of the seed refine [0, 32) and the main refine [32, 64),
   // zeroed by the seed pass
@@ -947,9 +947,22 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   int32_t* nitems2 = counts + nq;
   float* tb = nullptr;
   if (two) TS_TRY(ensure(e, B_TB, cap, &tb));
-  // units: the pairs it tests
+  // Unbanded DTW and Frechet: a lane-per-pair pre-check on superblock boxes
+  // first, the tight filter on what it keeps. Units: the pairs each tests.
+  const Item* tight_in = citems;
+  const int32_t* tight_n = ntot;
+  if ((p.measure == TS_DTW && p.band == 0) || p.measure == TS_FRECHET) {
+    Item* pre = nullptr;
+    TS_TRY(ensure(e, B_PRE, cap, &pre));
+    int32_t* npre = counts + nq + 1;
+    TS_TRY(run(e, "tight_pre", 0, (double)total, [&] {
+      return launch_tight_pre(e->st, ma, qv, q0, dv, tau, citems, ntot, total, pre, npre, rvals, tb);
+    }));
+    tight_in = pre;
+    tight_n = npre;
+  }
   TS_TRY(run(e, "tight_filter", 0, (double)total, [&] {
-    return launch_tight_filter(e->st, ma, k, qv, q0, dv, tau, citems, ntot, total, items2, nitems2, rvals,
+    return launch_tight_filter(e->st, ma, k, qv, q0, dv, tau, tight_in, tight_n, total, items2, nitems2, rvals,
                                device_cu_count() * 32, tb, two ? seed_idx : nullptr, s);
   }));
   const int32_t* seg_counts = dev_sized ? segc : counts;

@@ -611,8 +611,9 @@ __global__ void __launch_bounds__(1024) k_block_prefix(int32_t* bcounts, int32_t
     run += c;
   }
   if (threadIdx.x == 1023) counts[blockIdx.x] = part[1023];
-  // counts[nq] is the tight filter's survivor counter, cleared here
-  if (blockIdx.x == 0 && threadIdx.x == 0) counts[gridDim.x] = 0;
+  // counts[nq] and counts[nq + 1] are the tight filter's and its
+  // pre-check's survivor counters, cleared here
+  if (blockIdx.x == 0 && threadIdx.x < 2) counts[gridDim.x + threadIdx.x] = 0;
 }
 
 hipError_t launch_block_prefix(hipStream_t st, int32_t* bcounts, int32_t nq, int32_t gx, int32_t* counts) {
@@ -1125,6 +1126,109 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
   if (threadIdx.x == 0) sbase = n > 0 ? atomicAdd(nout, n) : 0;
   __syncthreads();
   for (int i = threadIdx.x; i < n; i += blockDim.x) out[sbase + i] = sbuf[i];
+}
+
+// Lane-per-pair pre-check ahead of the tight filter (unbanded DTW, Frechet),
+// which reads no point: each lane takes one candidate and tests both
+// trajectories' 64-point superblock boxes (every 8th entry of a set's sblk)
+// against the other trajectory's MBR. Every point lies in its superblock's
+// box, so the box's gap to the MBR bounds the point's distance to the MBR
+// from below (f32 monotone), and that in turn bounds its distance to the
+// nearest tile box the tight filter uses:
+//   DTW:     the sum over a side's superblocks of (its interior points) x
+//            sqrt_lo(gap^2), plus the endpoint costs e, discounted by
+//            dtw_disc (a serial sum of at most kPreBoxes terms, inside the
+//            discount's (L/64 + 9) 2^-24 allowance for the bound's sums);
+//   Frechet: sqrtf of the largest squared gap of either side's superblocks
+//            and of the endpoint costs (max does not round).
+// A pair either form puts above its query's threshold is rejected as the
+// tight filter would (NaN into rej[slot], and into tb[slot] when given); the
+// others, dead items, short pairs (DTW: n or m < 3, whose interior the forms
+// leave out) and pairs of more than kPreBoxes superblocks go on to `out`, in
+// order within each 256-item round of a block (one atomic per round).
+constexpr int kPreBoxes = 48;
+template <int MEAS>
+__global__ void __launch_bounds__(256) k_tight_pre(SetView q, int32_t q0, SetView db, const float* tau,
+                                                   const Item* items, const int32_t* nitems, int64_t max_items,
+                                                   Item* out, int32_t* nout, float* rej, float* tb) {
+  __shared__ int wcnt[4];
+  __shared__ int sbase;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t n_items = nitems ? min<int64_t>(max_items, (int64_t)*nitems) : max_items;
+  for (int64_t r0 = (int64_t)blockIdx.x * 256; r0 < n_items; r0 += (int64_t)gridDim.x * 256) {
+    const int64_t i = r0 + threadIdx.x;
+    bool keep = false;
+    Item it{0, -1, 0, 0};
+    if (i < n_items) {
+      it = items[i];
+      keep = true;
+      if (it.d >= 0) {
+        const int n = q.len[it.q], m = db.len[it.d];
+        const int nsa = (n + 63) >> 6, nsc = (m + 63) >> 6;
+        if (nsa + nsc <= kPreBoxes && (MEAS == TS_FRECHET || (n >= 3 && m >= 3))) {
+          const int64_t qo = q.off[it.q], co = db.off[it.d];
+          const float4 qb = q.mbr[it.q], cb = db.mbr[it.d];
+          const float4 qe = q.ends[it.q], ce = db.ends[it.d];
+          const float t = tau[it.q - q0];
+          const float4* qs = q.sblk + (qo >> 3);
+          const float4* cs = db.sblk + (co >> 3);
+          const float dx0 = qe.x - ce.x, dy0 = qe.y - ce.y, dx1 = qe.z - ce.z, dy1 = qe.w - ce.w;
+          if (MEAS == TS_DTW) {
+            float ta = 0.0f, tc = 0.0f;
+            for (int u = 0; u < nsa; ++u) {
+              const int lo = max(64 * u, 1), hi = min(64 * u + 64, n - 1);
+              ta += (float)(hi - lo) * sqrt_lo(box_gap2(qs[8 * u], cb));
+            }
+            for (int u = 0; u < nsc; ++u) {
+              const int lo = max(64 * u, 1), hi = min(64 * u + 64, m - 1);
+              tc += (float)(hi - lo) * sqrt_lo(box_gap2(cs[8 * u], qb));
+            }
+            const float e = sqrt_lo(dx0 * dx0 + dy0 * dy0) + sqrt_lo(dx1 * dx1 + dy1 * dy1);
+            keep = (fmaxf(ta, tc) + e) * dtw_disc(max(n, m)) <= t;
+          } else {
+            float g2 = fmaxf(dx0 * dx0 + dy0 * dy0, dx1 * dx1 + dy1 * dy1);
+            for (int u = 0; u < nsa; ++u) g2 = fmaxf(g2, box_gap2(qs[8 * u], cb));
+            for (int u = 0; u < nsc; ++u) g2 = fmaxf(g2, box_gap2(cs[8 * u], qb));
+            keep = sqrtf(g2) <= t;
+          }
+          if (!keep) {
+            rej[it.slot] = __int_as_float(0x7fc00000);
+            if (tb) tb[it.slot] = __int_as_float(0x7fc00000);
+          }
+        }
+      }
+    }
+    const uint64_t bal = __ballot(keep);
+    if (lane == 0) wcnt[wv] = __popcll(bal);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+      sbase = tot > 0 ? atomicAdd(nout, tot) : 0;
+    }
+    __syncthreads();
+    if (keep) {
+      int pos = sbase + __popcll(bal & ((1ull << lane) - 1ull));
+      for (int w = 0; w < wv; ++w) pos += wcnt[w];
+      out[pos] = it;
+    }
+    __syncthreads();  // wcnt and sbase are rewritten next round
+  }
+}
+
+hipError_t launch_tight_pre(hipStream_t st, MeasureArgs m, SetView q, int32_t q0, SetView db, const float* tau,
+                            const Item* items, const int32_t* nitems, int64_t max_items, Item* out, int32_t* nout,
+                            float* rej, float* tb) {
+  const int64_t rounds = (max_items + 255) / 256;
+  const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>(rounds, 16ll * device_cu_count())));
+  if (m.measure == TS_DTW && m.band == 0)
+    hipLaunchKernelGGL(k_tight_pre<TS_DTW>, grid, dim3(256), 0, st, q, q0, db, tau, items, nitems, max_items, out,
+                       nout, rej, tb);
+  else if (m.measure == TS_FRECHET)
+    hipLaunchKernelGGL(k_tight_pre<TS_FRECHET>, grid, dim3(256), 0, st, q, q0, db, tau, items, nitems, max_items,
+                       out, nout, rej, tb);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
 }
 
 // At most grid_waves waves, and no more blocks than fit on the device at once

@@ -27,9 +27,12 @@ __device__ __forceinline__ float sqrt_lo(float x) {
 // Squared distance from a point to a box (minx, maxx, miny, maxy). For any
 // point b of the box, the f32 squared distance from p to b is at least this
 // value: f32 subtraction, squaring of magnitudes and addition are monotone.
+// Per axis, p minus p clamped into the box (v_med3_f32): 0 inside, else
+// exactly the negated or plain f32 gap max(lo - p, p - hi), with one
+// instruction fewer.
 __device__ __forceinline__ float pt_box(float px, float py, float4 b) {
-  const float gx = fmaxf(0.0f, fmaxf(b.x - px, px - b.y));
-  const float gy = fmaxf(0.0f, fmaxf(b.z - py, py - b.w));
+  const float gx = px - __builtin_amdgcn_fmed3f(px, b.x, b.y);
+  const float gy = py - __builtin_amdgcn_fmed3f(py, b.z, b.w);
   return gx * gx + gy * gy;
 }
 
@@ -302,6 +305,14 @@ hipError_t launch_bound_compact(hipStream_t st, MeasureArgs m, int32_t k, SetVie
                                 SetView db, const float* tau, const int64_t* off, const float* bvals,
                                 int32_t* bcounts, uint64_t* bits, Item* items, int64_t cap);
 hipError_t launch_block_prefix(hipStream_t st, int32_t* bcounts, int32_t nq, int32_t gx, int32_t* counts);
+// Pre-check of the tight filter's input for unbanded DTW and Frechet (one
+// lane per pair, superblock boxes against the other MBR, no point read):
+// rejected pairs get NaN in rej[slot] (and tb[slot]); the rest are appended
+// to `out`, *nout counting them (zeroed by the caller: launch_block_prefix
+// clears counts[nq + 1]).
+hipError_t launch_tight_pre(hipStream_t st, MeasureArgs m, SetView q, int32_t q0, SetView db, const float* tau,
+                            const Item* items, const int32_t* nitems, int64_t max_items, Item* out, int32_t* nout,
+                            float* rej, float* tb);
 // Second, O(n+m) admissible bound over compacted pairs (one wave per pair);
 // survivors are appended to `out` with count *nout (zeroed by the caller:
 // launch_block_prefix clears counts[nq]); a rejected pair's rej[slot] is set
