@@ -250,6 +250,15 @@ __global__ void __launch_bounds__(256) k_seed_ub(const uint64_t* keys, int32_t s
   }
 }
 
+// Keys (upper bound, DB index) of a seed pool, for picking the seeds of
+// smallest upper bound: missing seeds (index < 0) sort last.
+__global__ void k_ub_keys(const float* ub, const int32_t* idx, int64_t n, uint64_t* keys) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int32_t d = idx[i];
+  keys[i] = d < 0 ? ~0ull : ((uint64_t)__float_as_uint(ub[i]) << 32) | (uint32_t)d;
+}
+
 __global__ void k_fill_f32(float* p, float v, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i < n) p[i] = v;
@@ -263,7 +272,8 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 enum BufId {
   B_ENV, B_SCORES, B_KEYS, B_KEYS2, B_SCA, B_SCB, B_CAND, B_CANDIDX, B_ITEMS, B_ITEMS3, B_VALS, B_REGION,
   B_RVALS, B_COUNTS, B_CTR, B_TAU, B_TOP, B_OIDX, B_OVAL, B_CELLS, B_HOSTIN, B_ITEMS2, B_RITEMS, B_SEGC, B_NEED, B_BCNT,
-  B_BITS, B_BOUNDS, B_KQ, B_TB, B_TOP2, B_CANDIDX2, B_ITEMS5, B_MKEYS, B_TOPA, B_UB, B_TAUUB, B_PRE, B_NBUF
+  B_BITS, B_BOUNDS, B_KQ, B_TB, B_TOP2, B_CANDIDX2, B_ITEMS5, B_MKEYS, B_TOPA, B_UB, B_TAUUB, B_PRE, B_POOL,
+  B_POOLK, B_POOLIDX, B_NBUF
 };
 
 struct Buf {
@@ -785,6 +795,24 @@ double mean_pair_cells(const ts_set* qs, const ts_set* db, const MeasureArgs& ma
 constexpr double kTwoRoundWork = 17592186044416.0;  // 2^44
 constexpr int32_t kSeeds2 = 32;
 
+// Seed pool (DTW, Frechet): the bound pass keeps pool x s candidates a
+// query, each gets its upper bound (one in-band warping path), and the s of
+// smallest upper bound are the seeds; the k-th smallest upper bound of the
+// whole pool is their threshold (its k pairs are among the seeds). The
+// summary bound ranks long pairs poorly; the path's cost ranks them by
+// their actual alignment. Factor 8 for DTW, 4 for Frechet (4096 x 1M: DTW
+// 135.2 -> 120.2 ms, Frechet 43.4 -> 40.9 ms; 256 x 20k banded DTW of
+// 2048-8192 points 189.7 -> 160.2 ms), at most 256 / s, and only with the
+// second seed round (from kTwoRoundWork: smaller batches lose more to its
+// launches than they gain, 64 x 100k DTW went 0.74 -> 0.81 ms a step).
+// TS_SEED_POOL overrides the factor (1: no pool).
+int32_t seed_pool(const ts_params& p, int32_t s, bool two) {
+  if (!two || (p.measure != TS_DTW && p.measure != TS_FRECHET)) return 1;
+  const char* env = std::getenv("TS_SEED_POOL");
+  const int32_t f = env && *env ? std::max(1, std::atoi(env)) : (p.measure == TS_DTW ? 8 : 4);
+  return std::max(1, std::min(f, 256 / std::max(1, s)));
+}
+
 // Largest batch (queries x DB trajectories) whose seed-pass bounds are kept
 // for the compaction (4 bytes a pair).
 constexpr int64_t kStoredBoundsMax = 1ll << 26;
@@ -819,7 +847,8 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
                                                                                         1048576.0
                                                                                 ? 2 * kSeeds2
                                                                                 : kSeeds2)));
-  const int32_t chunk = bound_chunk_items(s, nq, N);
+  const int32_t pool = seed_pool(p, s, two), sp = pool * s;  // candidates a query from the bound pass
+  const int32_t chunk = bound_chunk_items(sp, nq, N);
   const int32_t nch = (N + chunk - 1) / chunk;
   const SetView qv = qs->view(), dv = db->view();
   const MeasureArgs ma = margs(p);
@@ -828,7 +857,7 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   int64_t* d_off = nullptr;
   Item* items = nullptr;
   float *vals = nullptr, *tau = nullptr;
-  TS_TRY(ensure(e, B_KEYS, (int64_t)nq * nch * s, &bkeys));
+  TS_TRY(ensure(e, B_KEYS, (int64_t)nq * nch * sp, &bkeys));
   TS_TRY(ensure(e, B_CAND, (int64_t)nq * s, &seeds));
   TS_TRY(ensure(e, B_CANDIDX, (int64_t)nq * s, &seed_idx));
   TS_TRY(ensure(e, B_TAU, nq, &tau));
@@ -847,16 +876,44 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   if ((int64_t)nq * N <= kStoredBoundsMax) TS_TRY(ensure(e, B_BOUNDS, (int64_t)nq * N, &bvals));
   // 1. seeds: good candidates by bound, refined exactly
   TS_TRY(run(e, "bound_seed", pairs * 36.0, pairs, [&] {
-    return launch_bound_chunks(e->st, ma, k, qv, q0, nq, dv, s, chunk, bkeys, bvals, ctrs, 96);
+    return launch_bound_chunks(e->st, ma, k, qv, q0, nq, dv, sp, chunk, bkeys, bvals, ctrs, 96);
   }));
-  TS_TRY(topk(e, bkeys, (int64_t)nch * s, nch * s, nullptr, nq, s, seeds));
+  float* tau_ub = nullptr;
+  if (pool > 1) {
+    // the pool: sp candidates a query by bound, their upper bounds, the
+    // threshold from all of them, and the s of smallest upper bound
+    uint64_t *pk = nullptr, *pkeys = nullptr;
+    int32_t* pidx = nullptr;
+    float* ub = nullptr;
+    const int64_t np = (int64_t)nq * sp;
+    TS_TRY(ensure(e, B_POOL, np, &pk));
+    TS_TRY(ensure(e, B_POOLK, np, &pkeys));
+    TS_TRY(ensure(e, B_POOLIDX, np, &pidx));
+    TS_TRY(ensure(e, B_UB, np, &ub));
+    TS_TRY(ensure(e, B_TAUUB, nq, &tau_ub));
+    TS_TRY(topk(e, bkeys, (int64_t)nch * sp, nch * sp, nullptr, nq, sp, pk));
+    TS_TRY(run(e, "seed_ub", 0, 0, [&] {
+      hipLaunchKernelGGL(k_seed_ub, dim3((unsigned)((np + 3) / 4)), dim3(256), 0, e->st, pk, sp, q0, np, qv, dv,
+                         p.measure, pidx, ub);
+      return hipGetLastError();
+    }));
+    TS_TRY(run(e, "threshold", 0, 0, [&] {
+      return launch_seed_tau(e->st, ub, pidx, nq, sp, k, 0, tau_ub, kq_dev ? kq_dev + q0 : nullptr);
+    }));
+    TS_TRY(run(e, "keys", 0, 0, [&] {
+      hipLaunchKernelGGL(k_ub_keys, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, e->st, ub, pidx, np, pkeys);
+      return hipGetLastError();
+    }));
+    TS_TRY(topk(e, pkeys, sp, sp, nullptr, nq, s, seeds));
+  } else {
+    TS_TRY(topk(e, bkeys, (int64_t)nch * s, nch * s, nullptr, nq, s, seeds));
+  }
   const KeyItems seed_items{seeds, s, q0, seed_idx};
   // The seeds stop early against the k-th smallest upper bound of their
   // values (DTW, Frechet): a seed above it cannot be among the k best seeds,
   // whose own bounds keep k of them below it, so the threshold the seeds
   // give is unchanged.
-  float* tau_ub = nullptr;
-  if (p.measure == TS_DTW || p.measure == TS_FRECHET) {
+  if (pool == 1 && (p.measure == TS_DTW || p.measure == TS_FRECHET)) {
     float* ub = nullptr;
     TS_TRY(ensure(e, B_UB, (int64_t)nq * s, &ub));
     TS_TRY(ensure(e, B_TAUUB, nq, &tau_ub));
@@ -1129,8 +1186,9 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
     TS_HIP(hipEventCreate(&e->call_a));
     TS_HIP(hipEventCreate(&e->call_b));
   }
-  // Queries per batch: bounded by the dense REF score matrix or by the
-  // EXACT candidate regions.
+  // Queries per batch: bounded by the dense REF score matrix, or by the
+  // EXACT seed pass's kept keys (8 bytes each, DB chunks x seed candidates a
+  // query) within 2 GiB.
   int32_t bq;
   if (plain) {
     bq = 1;
@@ -1138,8 +1196,9 @@ ts_status knn_impl(ts_engine* e, const ts_set* db, const ts_set* qs, const ts_pa
     bq = (int32_t)std::max<int64_t>(1, std::min<int64_t>(Q, (int64_t)(256ll << 20) / (8ll * std::max(1, N))));
   } else {
     int64_t s = exact_seeds(k, exact_work(qs, db, Q));
+    s *= seed_pool(p, (int32_t)s, true);  // at most this many
     int64_t nch = (N + bound_chunk_items((int32_t)s) - 1) / bound_chunk_items((int32_t)s);
-    bq = (int32_t)std::max<int64_t>(1, std::min<int64_t>({(int64_t)Q, 4096, (int64_t)(512ll << 20) / (8 * nch * s)}));
+    bq = (int32_t)std::max<int64_t>(1, std::min<int64_t>({(int64_t)Q, 4096, (int64_t)(2048ll << 20) / (8 * nch * s)}));
   }
   // Small EXACT problems refine every pair (Euclidean keeps its filter,
   // which also drops pairs of unequal length).
