@@ -78,6 +78,7 @@ _sig = {
     "ts_engine_set_timing": (ctypes.c_int, [_P, ctypes.c_int32]),
     "ts_engine_set_graphs": (ctypes.c_int, [_P, ctypes.c_int32]),
     "ts_dp_shape_count": (ctypes.c_int32, []),
+    "ts_dp_cells": (ctypes.c_int64, [_P, ctypes.c_int32, ctypes.c_int32]),
     "ts_dp_shape": (ctypes.c_int, [ctypes.c_int32, _P, _P]),
     "ts_engine_load_tuning": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "ts_engine_dp_fallbacks": (ctypes.c_int, [_P, _P]),
@@ -117,6 +118,12 @@ for _name, (_res, _args) in _sig.items():
 # width the DP visits in each).
 DP_VARIANTS = ["dtw", "dtw_band", "frechet", "lcss", "lcss_window"]
 DP_BUCKET_CAPS = [31, 63, 127, 255, 1023, 1 << 24]
+
+
+def dp_cells(p, n, m):
+    """DP cells of an n x m pair under params p (band or window included), as
+    the engine counts them."""
+    return int(_lib.ts_dp_cells(ctypes.byref(p), int(n), int(m)))
 
 
 def dp_shapes():

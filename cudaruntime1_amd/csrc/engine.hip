@@ -36,29 +36,6 @@ namespace ts {
 // DP cells a pair needs: all of them, or those inside a Sakoe-Chiba band
 // (|i*C - j*R| <= w*R, R the longer length) or an LCSS index window
 // (|i - j| <= delta), counted row by row.
-__device__ unsigned long long pair_cells(int64_t a, int64_t b, int32_t measure, int32_t band, int32_t delta) {
-  if (measure == TS_EUCLIDEAN) return (unsigned long long)a;
-  const int64_t R = max(a, b), C = min(a, b);
-  if (measure == TS_DTW && band != 0) {
-    const int64_t bw = (band > 0 ? band : (-band * R + 99) / 100) * R;
-    unsigned long long c = 0;
-    for (int64_t r = 0; r < R; ++r) {
-      const int64_t x = r * C - bw;
-      const int64_t lo = x <= 0 ? 0 : (x + R - 1) / R;
-      const int64_t hi = min(C - 1, (r * C + bw) / R);
-      if (hi >= lo) c += (unsigned long long)(hi - lo + 1);
-    }
-    return c;
-  }
-  if (measure == TS_LCSS && delta >= 0) {
-    unsigned long long c = 0;
-    for (int64_t r = 0; r < R && r - delta <= C - 1; ++r)
-      c += (unsigned long long)(min(C - 1, r + delta) - max<int64_t>(0, r - delta) + 1);
-    return c;
-  }
-  return (unsigned long long)(a * b);
-}
-
 // Cells and pairs of an item list, per DP bucket (acc[b], acc_pairs[b]), so
 // that each bucket's launch is credited with its own work. Measures without
 // DP count into bucket 0.
@@ -836,17 +813,18 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   // Two rounds seed less at first (the second round tightens the
   // threshold): s = max(2k, 32), and s2 = 64 candidates a query for DTW
   // when a mean pair is under 2^20 DP cells, else 32 (long banded pairs:
-  // each extra refine costs more than the tighter threshold saves; Frechet's
-  // max-based bounds leave it little to tighten).
+  // each extra refine costs more than the tighter threshold saves), and 16
+  // for Frechet (its max-based values leave little to tighten after the
+  // seed pool: 4096 x 1M Frechet 40.5 ms at 32, 38.7 ms at 16).
   const int32_t s = exact_seeds(k, two ? 0.0 : work, two ? 32 : 0);
   const char* s2_env = std::getenv("TS_SEEDS2");
   const int32_t s2 = !two ? 0
                           : std::max(1, std::min(1024, s2_env && *s2_env ? std::atoi(s2_env)
-                                                                         : (p.measure == TS_DTW &&
-                                                                                    mean_pair_cells(qs, db, margs(p)) <
-                                                                                        1048576.0
-                                                                                ? 2 * kSeeds2
-                                                                                : kSeeds2)));
+                                                                         : (p.measure == TS_FRECHET ? kSeeds2 / 2
+                                                                            : mean_pair_cells(qs, db, margs(p)) <
+                                                                                      1048576.0
+                                                                                  ? 2 * kSeeds2
+                                                                                  : kSeeds2)));
   const int32_t pool = seed_pool(p, s, two), sp = pool * s;  // candidates a query from the bound pass
   const int32_t chunk = bound_chunk_items(sp, nq, N);
   const int32_t nch = (N + chunk - 1) / chunk;
@@ -1539,6 +1517,11 @@ ts_status ts_engine_get_dp_shape(ts_engine* e, int32_t table, int32_t variant, i
 }
 
 int32_t ts_dp_shape_count(void) { return kDpShapes; }
+
+int64_t ts_dp_cells(const ts_params* p, int32_t n, int32_t m) {
+  if (!p || n < 1 || m < 1) return -1;
+  return (int64_t)pair_cells(n, m, p->measure, p->band, p->delta);
+}
 
 ts_status ts_engine_dp_fallbacks(ts_engine* e, int64_t* n) {
   if (!e || !n) return fail(TS_ERR_INVALID, "null argument");

@@ -179,6 +179,66 @@ inline int32_t exact_seeds(int32_t k, double work, int32_t lo_override = 0) {
   return s < k ? k : (s > 1024 ? 1024 : s);
 }
 
+// Sum over i in [0, n) of floor((a i + b) / m), for a, b >= 0 and m > 0, by
+// the usual reduction: take a and b modulo m (summing what they carry),
+// then solve the problem with m and a exchanged; O(log m) rounds. Every
+// intermediate stays below max(n, m)^2 + m.
+__host__ __device__ inline uint64_t floor_sum(uint64_t n, uint64_t m, uint64_t a, uint64_t b) {
+  uint64_t s = 0;
+  while (n > 0) {
+    if (a >= m) {
+      s += n * (n - 1) / 2 * (a / m);
+      a %= m;
+    }
+    if (b >= m) {
+      s += n * (b / m);
+      b %= m;
+    }
+    const uint64_t y = a * n + b;
+    if (y < m) break;
+    n = y / m;
+    b = y % m;
+    const uint64_t t = m;
+    m = a;
+    a = t;
+  }
+  return s;
+}
+
+// DP cells of an a x b pair as the DP visits them (R = max, C = min): every
+// cell; a band of w = band (or ceil(-band R / 100)) and bw = w R, the cells
+// with |r C - j R| <= bw, i.e. rows r from max(0, ceil((r C - bw) / R)) to
+// min(C - 1, floor((r C + bw) / R)) (never empty: bw >= R); a window delta,
+// the cells with |r - j| <= delta. Closed forms over the rows: the sums of
+// the clamped ends split where the clamps start to bind, floor sums between.
+__host__ __device__ inline uint64_t pair_cells(int64_t a, int64_t b, int32_t measure, int32_t band, int32_t delta) {
+  if (measure == TS_EUCLIDEAN) return (uint64_t)a;
+  const int64_t R = a > b ? a : b, C = a < b ? a : b;
+  if (measure == TS_DTW && band != 0) {
+    const int64_t bw = (band > 0 ? (int64_t)band : (-(int64_t)band * R + 99) / 100) * R;
+    // sum of the upper ends: floor((r C + bw) / R) until it reaches C - 1
+    const int64_t t1 = (C - 1) * R - bw;
+    int64_t r1 = t1 <= 0 ? 0 : (t1 + C - 1) / C;
+    if (r1 > R) r1 = R;
+    const uint64_t hi = floor_sum((uint64_t)r1, (uint64_t)R, (uint64_t)C, (uint64_t)bw) + (uint64_t)(R - r1) * (uint64_t)(C - 1);
+    // sum of the lower ends: ceil((r C - bw) / R) from the first r with r C > bw
+    const int64_t r0 = bw / C + 1;
+    const uint64_t lo = r0 < R ? floor_sum((uint64_t)(R - r0), (uint64_t)R, (uint64_t)C, (uint64_t)(r0 * C + R - 1 - bw)) : 0;
+    return hi - lo + (uint64_t)R;
+  }
+  if (measure == TS_LCSS && delta >= 0) {
+    // rows r < min(R, C + delta); upper ends r + delta up to C - 1, lower
+    // ends r - delta from r = delta + 1
+    const int64_t d = delta, nr = R < C + d ? R : C + d;
+    int64_t t = C - d;
+    t = t < 0 ? 0 : (t > nr ? nr : t);
+    const uint64_t hi = (uint64_t)(t * (t - 1) / 2 + t * d) + (uint64_t)(nr - t) * (uint64_t)(C - 1);
+    const int64_t u = nr - 1 - d > 0 ? nr - 1 - d : 0;
+    return hi - (uint64_t)(u * (u + 1) / 2) + (uint64_t)nr;
+  }
+  return (uint64_t)a * (uint64_t)b;
+}
+
 // Columns of a row the DP visits for an n x m pair: the shorter length C,
 // or for a band of half-width w (w = band, or ceil(-band * max(n,m) / 100)
 // for a negative band) or a window delta, at most 2w+1 or 2*delta+1.

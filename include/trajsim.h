@@ -187,6 +187,14 @@ ts_status ts_engine_graph_info(ts_engine* e, int64_t* captures, int64_t* replays
  * before). ts_engine_set_dp_shape sets table 0, 1, 3 or 4, or all of them
  * (2); ts_engine_get_dp_shape reads table 0, 1, 3 or 4. Results never
  * depend on the shape. */
+/* DP cells of an n x m pair under p's measure, band and window, as the
+ * engine's cell counters and the bench count them: n*m, the in-band cells
+ * of a DTW band (w = band, or ceil(-band * max(n,m) / 100) for a negative
+ * band: cells with |r*C - j*R| <= w*R, R = max(n,m), C = min(n,m)), or the
+ * cells within an LCSS window; n for Euclidean. Host only (no GPU); -1 for
+ * a null p or a length below 1. No counterpart in the reference (whose
+ * cores visit every cell). */
+int64_t ts_dp_cells(const ts_params* p, int32_t n, int32_t m);
 int32_t ts_dp_shape_count(void);
 ts_status ts_dp_shape(int32_t i, int32_t* lanes, int32_t* cells);
 ts_status ts_engine_load_tuning(ts_engine* e, const char* path);

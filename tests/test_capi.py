@@ -97,3 +97,38 @@ def test_committed_tuning_table_is_well_formed():
         assert table in ("", "lat", "mid", "few"), (n, line)
         seen.add((v, b, table))
     assert len(seen) == len(ts.DP_VARIANTS) * len(ts.DP_BUCKET_CAPS) * 4
+
+
+def _cells_by_rows(measure, band, delta, n, m):
+    """DP cells counted row by row: every cell, the rows' in-band column
+    ranges of a DTW band (|r*C - j*R| <= w*R), or the window |r - j| <= delta."""
+    R, C = max(n, m), min(n, m)
+    if measure == "dtw" and band != 0:
+        bw = (band if band > 0 else (-band * R + 99) // 100) * R
+        tot = 0
+        for r in range(R):
+            lo = max(0, -((bw - r * C) // R))  # ceil((r*C - bw) / R), clamped
+            hi = min(C - 1, (r * C + bw) // R)
+            tot += max(0, hi - lo + 1)
+        return tot
+    if measure == "lcss" and delta >= 0:
+        return sum(min(C - 1, r + delta) - max(0, r - delta) + 1 for r in range(min(R, C + delta)))
+    return n * m
+
+
+def test_dp_cells_closed_forms_match_row_counts():
+    """ts_dp_cells (closed forms over the rows, as the engine's cell counters
+    use them) equals the row-by-row count of in-band DTW cells and in-window
+    LCSS cells, over random lengths, bands and windows and their edges."""
+    rng = np.random.default_rng(11)
+    cases = [(1, 1), (1, 7), (7, 1), (2, 2), (3, 1000), (1000, 3), (2048, 8192), (8192, 8191), (5000, 4999)]
+    cases += [(int(a), int(b)) for a, b in rng.integers(1, 3000, (300, 2))]
+    for n, m in cases:
+        for band in (1, 2, 7, 100, -1, -10, -37, -100, -250):
+            p = ts.params("dtw", band=band)
+            assert ts.dp_cells(p, n, m) == _cells_by_rows("dtw", band, -1, n, m), (n, m, band)
+        for delta in (0, 1, 16, 64, 5000):
+            p = ts.params("lcss", delta=delta)
+            assert ts.dp_cells(p, n, m) == _cells_by_rows("lcss", 0, delta, n, m), (n, m, delta)
+        assert ts.dp_cells(ts.params("dtw"), n, m) == n * m
+    assert ts.dp_cells(ts.params("dtw"), 0, 5) == -1
