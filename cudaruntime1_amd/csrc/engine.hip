@@ -960,7 +960,8 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
       e->exact_cap = std::max<int64_t>(e->exact_cap, std::min<int64_t>(total + total / 4 + 4096, (1ll << 31) - 1));
       cap = e->exact_cap;
       TS_TRY(ensure(e, B_SEGC, nq + 2, &segc));
-      const int64_t need = select_scratch_elems(N, nq, k);
+      // the selections over each query's region: k, and s2 with two rounds
+      const int64_t need = select_scratch_elems(N, nq, std::max(k, s2));
       uint64_t* sc = nullptr;
       TS_TRY(ensure(e, B_SCA, need, &sc));
       TS_TRY(ensure(e, B_SCB, need, &sc));

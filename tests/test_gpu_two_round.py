@@ -67,10 +67,16 @@ def test_two_round_knn_matches_oracle(engines, monkeypatch, force):
         else:
             prm = oracle.params(CODES[meas], ts.EXACT, k=k, kc=p.kc, band=p.band)
             want = oracle.knn(prm, dx, do, qx, qo)
+            before = engines[graphs].graph_info()
             for rep in range(3 if graphs else 1):
                 idx, val, _ = engines[graphs].knn(db, qs, p)
                 assert np.array_equal(idx, want[0]), (t, meas, k, kw, graphs, rep)
                 assert np.array_equal(val.view(np.int32), want[1].view(np.int32)), (t, meas, k, kw, graphs, rep)
+            if graphs:
+                # the second call records and the third replays (a recording
+                # that fails runs eagerly, with the same results)
+                after = engines[graphs].graph_info()
+                assert after["captures"] > before["captures"] and after["replays"] > before["replays"], (t, after)
 
 
 def test_two_round_runs_its_second_round(monkeypatch):
