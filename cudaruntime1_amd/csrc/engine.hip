@@ -188,22 +188,10 @@ __global__ void __launch_bounds__(256) k_refine_prep(const uint64_t* keys, int32
 // order) times 1 + (L + 64) 2^-22 exceeds; Frechet's is at most the path's
 // largest cost (max never rounds). One wave per seed; seed_idx[i] gets the
 // seed's DB index (-1 and +inf for missing seeds).
-__global__ void __launch_bounds__(256) k_seed_ub(const uint64_t* keys, int32_t s, int32_t q0, int64_t n_seeds,
-                                                 SetView q, SetView db, int32_t measure, int32_t* seed_idx,
-                                                 float* ub) {
-  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+// Upper bound of pair (query qi, DB trajectory d) by one wave (see k_seed_ub),
+// in every lane.
+__device__ float path_ub(const SetView& q, const SetView& db, int qi, int d, int32_t measure) {
   const int lane = threadIdx.x & 63;
-  if (i >= n_seeds) return;
-  const uint64_t key = keys[i];
-  const int d = key == ~0ull ? -1 : (int32_t)(uint32_t)key;
-  if (d < 0) {
-    if (lane == 0) {
-      seed_idx[i] = -1;
-      ub[i] = __int_as_float(0x7f800000);
-    }
-    return;
-  }
-  const int qi = q0 + (int)(i / s);
   const int n = q.len[qi], m = db.len[d];
   const int64_t qo = q.off[qi], dof = db.off[d];
   const bool qlong = n >= m;
@@ -221,10 +209,46 @@ __global__ void __launch_bounds__(256) k_seed_ub(const uint64_t* keys, int32_t s
     const float o = __shfl_xor(acc, sh);
     acc = measure == TS_DTW ? acc + o : fmaxf(acc, o);
   }
+  return measure == TS_DTW ? acc * (1.0f + (float)(L + 64) * 0x1p-22f) : sqrtf(acc);
+}
+
+__global__ void __launch_bounds__(256) k_seed_ub(const uint64_t* keys, int32_t s, int32_t q0, int64_t n_seeds,
+                                                 SetView q, SetView db, int32_t measure, int32_t* seed_idx,
+                                                 float* ub) {
+  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n_seeds) return;
+  const uint64_t key = keys[i];
+  const int d = key == ~0ull ? -1 : (int32_t)(uint32_t)key;
+  if (d < 0) {
+    if (lane == 0) {
+      seed_idx[i] = -1;
+      ub[i] = __int_as_float(0x7f800000);
+    }
+    return;
+  }
+  const float u = path_ub(q, db, q0 + (int)(i / s), d, measure);
   if (lane == 0) {
     seed_idx[i] = d;
-    ub[i] = measure == TS_DTW ? acc * (1.0f + (float)(L + 64) * 0x1p-22f) : sqrtf(acc);
+    ub[i] = u;
   }
+}
+
+// Second-round pool: keys of candidate slots (low 32 bits) to keys of
+// (upper bound, slot), one wave per key; missing keys stay last.
+__global__ void __launch_bounds__(256) k_slot_ub_keys(const uint64_t* keys, int64_t n, const Item* citems, SetView q,
+                                                      SetView db, int32_t measure, uint64_t* out) {
+  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;
+  const uint64_t key = keys[i];
+  if (key == ~0ull) {
+    if ((threadIdx.x & 63) == 0) out[i] = ~0ull;
+    return;
+  }
+  const uint32_t slot = (uint32_t)key;
+  const Item it = citems[slot];
+  const float u = path_ub(q, db, it.q, it.d, measure);
+  if ((threadIdx.x & 63) == 0) out[i] = ((uint64_t)__float_as_uint(u) << 32) | slot;
 }
 
 // Keys (upper bound, DB index) of a seed pool, for picking the seeds of
@@ -250,7 +274,7 @@ enum BufId {
   B_ENV, B_SCORES, B_KEYS, B_KEYS2, B_SCA, B_SCB, B_CAND, B_CANDIDX, B_ITEMS, B_ITEMS3, B_VALS, B_REGION,
   B_RVALS, B_COUNTS, B_CTR, B_TAU, B_TOP, B_OIDX, B_OVAL, B_CELLS, B_HOSTIN, B_ITEMS2, B_RITEMS, B_SEGC, B_NEED, B_BCNT,
   B_BITS, B_BOUNDS, B_KQ, B_TB, B_TOP2, B_CANDIDX2, B_ITEMS5, B_MKEYS, B_TOPA, B_UB, B_TAUUB, B_PRE, B_POOL,
-  B_POOLK, B_POOLIDX, B_NBUF
+  B_POOLK, B_POOLIDX, B_TOP2P, B_KEYS2P, B_NBUF
 };
 
 struct Buf {
@@ -826,6 +850,14 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
                                                                                   ? 2 * kSeeds2
                                                                                   : kSeeds2)));
   const int32_t pool = seed_pool(p, s, two), sp = pool * s;  // candidates a query from the bound pass
+  // Second-round pool: DTW with short pairs (s2 = 64) picks its s2 from the
+  // 2 s2 candidates of smallest bound by their path upper bounds (4096 x 1M
+  // DTW 122.0 -> 118.1 ms; Frechet and the long banded pairs of 256 x 20k
+  // measured no gain); TS_SEED2_POOL overrides the factor.
+  const char* p2_env = std::getenv("TS_SEED2_POOL");
+  const int32_t pool2 = !two ? 1
+                             : (p2_env && *p2_env ? std::max(1, std::min(16, std::atoi(p2_env)))
+                                                  : (p.measure == TS_DTW && s2 == 2 * kSeeds2 ? 2 : 1));
   const int32_t chunk = bound_chunk_items(sp, nq, N);
   const int32_t nch = (N + chunk - 1) / chunk;
   const SetView qv = qs->view(), dv = db->view();
@@ -960,8 +992,9 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
       e->exact_cap = std::max<int64_t>(e->exact_cap, std::min<int64_t>(total + total / 4 + 4096, (1ll << 31) - 1));
       cap = e->exact_cap;
       TS_TRY(ensure(e, B_SEGC, nq + 2, &segc));
-      // the selections over each query's region: k, and s2 with two rounds
-      const int64_t need = select_scratch_elems(N, nq, std::max(k, s2));
+      // the selections over each query's region: k, and s2 (times its pool)
+      // with two rounds
+      const int64_t need = select_scratch_elems(N, nq, std::max(k, s2 * pool2));
       uint64_t* sc = nullptr;
       TS_TRY(ensure(e, B_SCA, need, &sc));
       TS_TRY(ensure(e, B_SCB, need, &sc));
@@ -1011,7 +1044,23 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
     TS_TRY(ensure(e, B_CANDIDX2, (int64_t)nq * s2, &seed_idx2));
     TS_TRY(ensure(e, B_ITEMS5, (int64_t)nq * s2, &items5));
     TS_TRY(run(e, "keys", 0, 0, [&] { return launch_keys_from_slots(e->st, tb, citems, cap, rkeys); }));
-    TS_TRY(topk(e, rkeys, 0, seg_len, seg_counts, nq, s2, top2, d_off));
+    if (pool2 > 1) {
+      // pool2 x s2 candidates by bound, then the s2 of smallest path upper bound
+      const int32_t s2p = pool2 * s2;
+      uint64_t *top2p = nullptr, *keys2p = nullptr;
+      TS_TRY(ensure(e, B_TOP2P, (int64_t)nq * s2p, &top2p));
+      TS_TRY(ensure(e, B_KEYS2P, (int64_t)nq * s2p, &keys2p));
+      TS_TRY(topk(e, rkeys, 0, seg_len, seg_counts, nq, s2p, top2p, d_off));
+      TS_TRY(run(e, "seed_ub", 0, 0, [&] {
+        const int64_t n2 = (int64_t)nq * s2p;
+        hipLaunchKernelGGL(k_slot_ub_keys, dim3((unsigned)((n2 + 3) / 4)), dim3(256), 0, e->st, top2p, n2, citems, qv,
+                           dv, p.measure, keys2p);
+        return hipGetLastError();
+      }));
+      TS_TRY(topk(e, keys2p, s2p, s2p, nullptr, nq, s2, top2));
+    } else {
+      TS_TRY(topk(e, rkeys, 0, seg_len, seg_counts, nq, s2, top2, d_off));
+    }
     const KeyItems r2_items{top2, s2, q0, seed_idx2, citems, tb};
     TS_TRY(refine(e, p, db, qs, items5, nullptr, (int64_t)nq * s2, rvals, 2, ctrs + 64, &r2_items, nullptr, tau, q0));
     TS_TRY(run(e, "threshold", 0, 0, [&] {
