@@ -1032,7 +1032,7 @@ ts_status knn_exact_batch(ts_engine* e, const ts_params& p, const ts_set* db, co
   }
   TS_TRY(run(e, "tight_filter", 0, (double)total, [&] {
     return launch_tight_filter(e->st, ma, k, qv, q0, dv, tau, tight_in, tight_n, total, items2, nitems2, rvals,
-                               device_cu_count() * 32, tb, two ? seed_idx : nullptr, s);
+                               device_cu_count() * 32, tb, two ? seed_idx : nullptr, s, tight_in != citems);
   }));
   const int32_t* seg_counts = dev_sized ? segc : counts;
   if (two) {

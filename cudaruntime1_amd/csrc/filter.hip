@@ -938,12 +938,18 @@ __device__ __forceinline__ float tiles_vs_mbr(const float4* boxes, int nb, int s
   return wave_sum_dpp(v);
 }
 
-// TILED: unbanded DTW, every pair by the one-pass tiled form.
+// Largest superblock count (both trajectories) of a pair that k_tight_pre
+// tests (see there).
+constexpr int kPreBoxes = 48;
+
+// TILED: unbanded DTW, every pair by the one-pass tiled form. pre: the pairs
+// have passed k_tight_pre, whose superblock test equals the staged-tile test
+// below for pairs it tested (tiles are superblocks up to 4096 points).
 template <int MEAS, bool TILED>
 __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k, SetView q, int32_t q0, SetView db,
                                                       const float* tau, const Item* items, const int32_t* nitems,
                                                       int64_t max_items, Item* out, int32_t* nout, float* rej,
-                                                      float* tb, const int32_t* seed_idx, int32_t s) {
+                                                      float* tb, const int32_t* seed_idx, int32_t s, bool pre) {
   static_assert(!TILED || MEAS == TS_DTW, "the tiled pass is DTW's");
   __shared__ Item sbuf[kTightBuf];
   __shared__ int scnt, sbase;
@@ -1029,9 +1035,11 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
         // 4096 x 1M DTW call they leave about 25% and 14% of the pairs to
         // the point pass. Rows 0 and n-1 hold the cells (0,0) and
         // (n-1,m-1), priced by e.
-        const float ta = tiles_vs_mbr(wbox, nsa, 64 * tile_group(n), n, cb);
-        const float tc = tiles_vs_mbr(wbox + kTileBoxes, nsc, 64 * tile_group(m), m, qb);
-        pass = (fmaxf(ta, tc) + e) * disc <= t;
+        if (!(pre && ((n + 63) >> 6) + ((m + 63) >> 6) <= kPreBoxes)) {
+          const float ta = tiles_vs_mbr(wbox, nsa, 64 * tile_group(n), n, cb);
+          const float tc = tiles_vs_mbr(wbox + kTileBoxes, nsc, 64 * tile_group(m), m, qb);
+          pass = (fmaxf(ta, tc) + e) * disc <= t;
+        }
         if (pass) {
           const float ba = tiles_vs_mbr(q.blk + (qo >> 3), (n + 7) >> 3, 8, n, cb);
           const float bc = tiles_vs_mbr(db.blk + (co >> 3), (m + 7) >> 3, 8, m, qb);
@@ -1146,7 +1154,6 @@ __global__ void __launch_bounds__(256) k_tight_filter(MeasureArgs ma, int32_t k,
 // others, dead items, short pairs (DTW: n or m < 3, whose interior the forms
 // leave out) and pairs of more than kPreBoxes superblocks go on to `out`, in
 // order within each 256-item round of a block (one atomic per round).
-constexpr int kPreBoxes = 48;
 template <int MEAS>
 __global__ void __launch_bounds__(256) k_tight_pre(SetView q, int32_t q0, SetView db, const float* tau,
                                                    const Item* items, const int32_t* nitems, int64_t max_items,
@@ -1236,7 +1243,7 @@ hipError_t launch_tight_pre(hipStream_t st, MeasureArgs m, SetView q, int32_t q0
 template <int MEAS, bool TILED>
 void launch_tight(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, SetView db, const float* tau,
                   const Item* items, const int32_t* nitems, int64_t max_items, Item* out, int32_t* nout, float* rej,
-                  int grid_waves, float* tb, const int32_t* seed_idx, int32_t s) {
+                  int grid_waves, float* tb, const int32_t* seed_idx, int32_t s, bool pre) {
   static int per_cu = 0;
   if (!per_cu && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tight_filter<MEAS, TILED>, 256, 0) !=
                       hipSuccess ||
@@ -1244,18 +1251,18 @@ void launch_tight(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q
     per_cu = 1;
   const dim3 grid(std::max(1, std::min((grid_waves + 3) / 4, per_cu * device_cu_count())));
   hipLaunchKernelGGL((k_tight_filter<MEAS, TILED>), grid, dim3(256), 0, st, m, k, q, q0, db, tau, items, nitems,
-                     max_items, out, nout, rej, tb, seed_idx, s);
+                     max_items, out, nout, rej, tb, seed_idx, s, pre);
 }
 
 hipError_t launch_tight_filter(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, SetView db,
                                const float* tau, const Item* items, const int32_t* nitems, int64_t max_items,
                                Item* out, int32_t* nout, float* rej, int grid_waves, float* tb,
-                               const int32_t* seed_idx, int32_t s) {
+                               const int32_t* seed_idx, int32_t s, bool pre) {
   if ((tb || seed_idx) && m.measure != TS_DTW && m.measure != TS_FRECHET) return hipErrorInvalidValue;
   switch (m.measure) {
 #define TS_TIGHT(MEAS, TILED)                                                                              \
   launch_tight<MEAS, TILED>(st, m, k, q, q0, db, tau, items, nitems, max_items, out, nout, rej, grid_waves, tb, \
-                            seed_idx, s)
+                            seed_idx, s, pre)
     case TS_DTW:
       if (m.band == 0) TS_TIGHT(TS_DTW, true);
       else TS_TIGHT(TS_DTW, false);

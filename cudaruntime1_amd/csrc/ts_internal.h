@@ -379,11 +379,12 @@ hipError_t launch_tight_pre(hipStream_t st, MeasureArgs m, SetView q, int32_t q0
 // to NaN. DTW and Frechet only: with tb, tb[slot] = a survivor's bound (the
 // value compared with its threshold), NaN for every other pair; with
 // seed_idx (s DB indices per query, row q - q0), those pairs are rejected
-// too.
+// too. pre: the items come from launch_tight_pre (unbanded DTW then skips
+// the staged-tile test the pre-check already made).
 hipError_t launch_tight_filter(hipStream_t st, MeasureArgs m, int32_t k, SetView q, int32_t q0, SetView db,
                                const float* tau, const Item* items, const int32_t* nitems, int64_t max_items,
                                Item* out, int32_t* nout, float* rej, int grid_waves, float* tb = nullptr,
-                               const int32_t* seed_idx = nullptr, int32_t s = 0);
+                               const int32_t* seed_idx = nullptr, int32_t s = 0, bool pre = false);
 
 // select.hip
 // Key order: ascending u64. enc_asc(v,i) = ordered(v)<<32|i, desc uses ~ordered(v).
